@@ -1,0 +1,675 @@
+// capi.cpp -- host runtime behind include/cffc_euler2d.h.
+//
+// Owns the device state of one multi-block mesh, orders the gfx950 kernels on
+// one HIP stream and maps each C entry point onto the reference call it
+// replaces (the comments in cffc_euler2d.h cite them).  No torch, no CPU
+// fallback: without a HIP device cffc_create fails with CFFC_E_NODEVICE.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cffc_euler2d.h"
+#include "euler2d_device.hpp"
+#include "euler2d_kernels.hpp"
+
+using namespace cffc;
+
+static thread_local std::string g_err;
+
+static int fail(int code, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+static int fail(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+#define HIPCHK(x)                                                                              \
+    do {                                                                                       \
+        hipError_t e_ = (x);                                                                   \
+        if (e_ != hipSuccess) return fail(CFFC_E_HIP, "%s: %s", #x, hipGetErrorString(e_));    \
+    } while (0)
+
+static const unsigned long long MILLION_BITS = 0x412E848000000000ull;
+static const unsigned long long ERR_NONE = ~0ull;
+
+struct cffc_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    cffc_scheme_desc s{};
+    int nblk = 0;
+    int64_t ncp = 0, ncell = 0;
+    int max_int = 0, max_cells = 0;
+    std::vector<BlockMeta> meta_h;
+    BlockMeta *meta = nullptr;
+    int32_t *nbr = nullptr, *bc = nullptr;
+    double *wo = nullptr, *geom = nullptr;
+    double *W[2] = {nullptr, nullptr};
+    int wcur = 0;
+    double *U = nullptr, *Uo = nullptr, *dUdt[3] = {nullptr, nullptr, nullptr};
+    double *dt = nullptr, *phi = nullptr;
+    // tiles
+    int ti = 16, tj = 16, ntiles = 0;
+    TileDesc *tiles = nullptr;
+    // scalars: [0],[1] dt slots, [2] error key
+    unsigned long long *scal = nullptr;
+    double *norm_part = nullptr, *norm_small = nullptr;
+    unsigned int *counter = nullptr;
+    int32_t *step_ctr = nullptr, *stop_flag = nullptr;
+    double *norm_out = nullptr, *dt_out = nullptr;
+    int max_steps = 0;
+    double *staging = nullptr;
+    int freeze = 0;      // limiters frozen (PHI_LOAD)
+    int phi_valid = 0;   // phi array holds the last evaluated limiters
+    int store_phi = 0;   // keep phi stored every step (freeze may follow)
+    bool granular_ready = false;
+};
+
+// ---------------------------------------------------------------------------
+static void free_ctx(cffc_ctx *c) {
+    if (!c) return;
+    void *ptrs[] = {c->meta, c->nbr, c->bc, c->wo, c->geom, c->W[0], c->W[1], c->U, c->Uo, c->dUdt[0],
+                    c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
+                    c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging};
+    for (void *p : ptrs)
+        if (p) (void)hipFree(p);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+template <typename T> static int dalloc(T **p, size_t n) {
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) return fail(CFFC_E_NOMEM, "hipMalloc of %zu bytes failed", n * sizeof(T));
+    return 0;
+}
+
+static int build_tiles(cffc_ctx *c, int ti, int tj) {
+    std::vector<TileDesc> t;
+    for (int b = 0; b < c->nblk; ++b) {
+        const BlockMeta &m = c->meta_h[b];
+        for (int j0 = 0; j0 < m.nj; j0 += tj)
+            for (int i0 = 0; i0 < m.ni; i0 += ti) t.push_back(TileDesc{b, i0, j0, 0});
+    }
+    if (c->tiles) (void)hipFree(c->tiles);
+    if (c->norm_part) (void)hipFree(c->norm_part);
+    c->tiles = nullptr;
+    c->norm_part = nullptr;
+    c->ntiles = (int)t.size();
+    if (int e = dalloc(&c->tiles, t.size())) return e;
+    if (int e = dalloc(&c->norm_part, 3 * t.size())) return e;
+    HIPCHK(hipMemcpy(c->tiles, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
+    c->ti = ti;
+    c->tj = tj;
+    return 0;
+}
+
+static bool tile_supported(int ti, int tj) {
+#define T(A, B) if (ti == A && tj == B) return true;
+    CFFC_TILE_LIST(T)
+#undef T
+    return false;
+}
+
+static int check_scheme(const cffc_scheme_desc *s) {
+    const int ti = s->i_time_integration;
+    if (ti != CFFC_TIME_STEPPING_EXPLICIT_EULER && ti != CFFC_TIME_STEPPING_EXPLICIT_PREDICTOR_CORRECTOR &&
+        ti != CFFC_TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING && ti != CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA)
+        return fail(CFFC_E_ARG, "unsupported i_time_integration %d", ti);
+    if (s->n_stage < 1 || s->n_stage > 5) return fail(CFFC_E_ARG, "unsupported N_Stage %d", s->n_stage);
+    if (s->i_reconstruction != CFFC_RECONSTRUCTION_LEAST_SQUARES)
+        return fail(CFFC_E_ARG, "only least-squares reconstruction is implemented");
+    if (s->i_flux_function != CFFC_FLUX_FUNCTION_ROE) return fail(CFFC_E_ARG, "only the Roe flux is implemented");
+    if (s->i_limiter != CFFC_LIMITER_VENKATAKRISHNAN && s->i_limiter != CFFC_LIMITER_ONE &&
+        s->i_limiter != CFFC_LIMITER_ZERO)
+        return fail(CFFC_E_ARG, "unsupported limiter %d", s->i_limiter);
+    if (s->local_time_stepping != CFFC_GLOBAL_TIME_STEPPING && s->local_time_stepping != CFFC_SCALAR_LOCAL_TIME_STEPPING)
+        return fail(CFFC_E_ARG, "unsupported Local_Time_Stepping %d", s->local_time_stepping);
+    if (s->flux_precision != CFFC_FLUX_FP64 && s->flux_precision != CFFC_FLUX_FP32)
+        return fail(CFFC_E_ARG, "unsupported flux precision %d", s->flux_precision);
+    return 0;
+}
+
+// Runge_Kutta (CFD.cc:901-939) and MultiStage_Optimally_Smoothing (CFD.cc:953-1030)
+static double runge_kutta(int i, int n) {
+    double beta = 1.0;
+    switch (n) {
+    case 2: if (i == 2) beta = 0.5 * beta; break;
+    case 4: if (i == 1 || i == 2) beta = 0.5 * beta; if (i == 4) beta = beta / 6.0; break;
+    case 5: if (i == 1) beta *= 0.059; if (i == 2) beta *= 0.14; if (i == 3) beta *= 0.273; if (i == 4) beta *= 0.5; break;
+    default: break;
+    }
+    return beta;
+}
+static double optimal_smoothing(int i, int n, int lim) {
+    // van Leer, Tai & Powell (1989) coefficient table (first-order / limited)
+    static const double base0[6] = {0, 1.0, 1.0, 1.50, 2.00, 2.50};
+    static const double base1[6] = {0, 1.0, 0.4693, 0.6936, 0.9214, 1.1508};
+    static const double a0[6][5] = {{0}, {0}, {0.3333}, {0.1481, 0.4000}, {0.0833, 0.2069, 0.4265},
+                                    {0.0533, 0.1263, 0.2375, 0.4414}};
+    static const double a1[6][5] = {{0}, {0}, {0.4242}, {0.1918, 0.4929}, {0.1084, 0.2602, 0.5052},
+                                    {0.0695, 0.1602, 0.2898, 0.5060}};
+    if (n < 1 || n > 5) return 1.0;
+    double beta = (lim == CFFC_LIMITER_ZERO) ? base0[n] : base1[n];
+    if (i < n) beta = ((lim == CFFC_LIMITER_ZERO) ? a0[n][i - 1] : a1[n][i - 1]) * beta;
+    return beta;
+}
+static double stage_omega(const cffc_scheme_desc &s, int i) {
+    if (s.i_time_integration == CFFC_TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING)
+        return optimal_smoothing(i, s.n_stage, s.i_limiter);
+    return runge_kutta(i, s.n_stage);
+}
+static int k_residual(const cffc_scheme_desc &s, int i) {
+    if (s.i_time_integration == CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA && s.n_stage == 4) return i == 4 ? 0 : i - 1;
+    return 0;
+}
+// init of the residual storage at stage i (Euler2DQuad.cc:1306-1343)
+static int granular_init(const cffc_scheme_desc &s, int i) {
+    if (i == 1) return GINIT_ZERO;
+    if (s.i_time_integration == CFFC_TIME_STEPPING_EXPLICIT_PREDICTOR_CORRECTOR) return GINIT_KEEP;
+    if (s.i_time_integration == CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA) {
+        if (s.n_stage == 2) return GINIT_KEEP;
+        if (s.n_stage == 4 && i == 4) return GINIT_RK4;
+    }
+    return GINIT_ZERO;
+}
+// the same, folded into the single accumulator of the fused path
+static int fused_mode(const cffc_scheme_desc &s, int i) {
+    const int g = granular_init(s, i);
+    if (g == GINIT_KEEP || g == GINIT_RK4) return MODE_FROM_S;
+    if (s.i_time_integration == CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA && s.n_stage == 4 && (i == 2 || i == 3))
+        return MODE_ZERO_ACC2;
+    return MODE_ZERO;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" const char *cffc_last_error(void) { return g_err.c_str(); }
+
+extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd, cffc_ctx **out) {
+    if (!md || !sd || !out) return fail(CFFC_E_ARG, "null argument");
+    *out = nullptr;
+    if (md->nblk <= 0 || md->nghost != 2) return fail(CFFC_E_ARG, "nblk must be > 0 and nghost == 2");
+    if (int e = check_scheme(sd)) return e;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(CFFC_E_NODEVICE, "no HIP device");
+    cffc_ctx *c = new cffc_ctx();
+    c->s = *sd;
+    c->nblk = md->nblk;
+    HIPCHK(hipGetDevice(&c->device));
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        free_ctx(c);
+        return fail(CFFC_E_HIP, "hipStreamCreate failed");
+    }
+    // block metadata and topology checks
+    int64_t coff = 0, ioff = 0, noff = 0;
+    int32_t bcoff = 0;
+    c->meta_h.resize(md->nblk);
+    for (int b = 0; b < md->nblk; ++b) {
+        BlockMeta &m = c->meta_h[b];
+        m.ni = md->ni[b];
+        m.nj = md->nj[b];
+        if (m.ni < 2 || m.nj < 2 || m.ni > 4096 || m.nj > 4096) {
+            free_ctx(c);
+            return fail(CFFC_E_ARG, "block %d: ni, nj must be in [2, 4096]", b);
+        }
+        m.nci = m.ni + 4;
+        m.ncj = m.nj + 4;
+        m.coff = coff;
+        m.ioff = ioff;
+        m.noff = noff;
+        m.bcoff = bcoff;
+        m.pad = 0;
+        coff += (int64_t)m.nci * m.ncj;
+        ioff += (int64_t)m.ni * m.nj;
+        noff += (int64_t)(m.nci + 1) * (m.ncj + 1);
+        bcoff += 2 * m.ncj + 2 * m.nci;
+        c->max_int = std::max(c->max_int, m.ni * m.nj);
+        c->max_cells = std::max(c->max_cells, m.nci * m.ncj);
+    }
+    for (int b = 0; b < md->nblk; ++b) {
+        const BlockMeta &m = c->meta_h[b];
+        for (int s = 0; s < 8; ++s) {
+            const int nb = md->nbr[8 * b + s];
+            if (nb < -1 || nb >= md->nblk) {
+                free_ctx(c);
+                return fail(CFFC_E_TOPOLOGY, "block %d slot %d: neighbour %d out of range", b, s, nb);
+            }
+            if (nb < 0) continue;
+            const BlockMeta &n = c->meta_h[nb];
+            if ((s == CFFC_NBR_N || s == CFFC_NBR_S) && n.ni != m.ni) {
+                free_ctx(c);
+                return fail(CFFC_E_TOPOLOGY, "block %d: N/S neighbour %d has different ni", b, nb);
+            }
+            if ((s == CFFC_NBR_E || s == CFFC_NBR_W) && n.nj != m.nj) {
+                free_ctx(c);
+                return fail(CFFC_E_TOPOLOGY, "block %d: E/W neighbour %d has different nj", b, nb);
+            }
+        }
+        const int32_t *bcb = md->bc + m.bcoff;
+        for (int q = 0; q < 2 * m.ncj + 2 * m.nci; ++q) {
+            const int v = bcb[q];
+            if (v != CFFC_BC_NONE && v != CFFC_BC_FIXED && v != CFFC_BC_CONSTANT_EXTRAPOLATION &&
+                v != CFFC_BC_REFLECTION && v != CFFC_BC_PERIODIC && v != CFFC_BC_CHARACTERISTIC) {
+                free_ctx(c);
+                return fail(CFFC_E_ARG, "block %d: unsupported boundary condition %d", b, v);
+            }
+        }
+    }
+    c->ncp = coff;
+    c->ncell = ioff;
+    const int64_t ncp = c->ncp;
+    int e = 0;
+    if ((e = dalloc(&c->meta, md->nblk)) || (e = dalloc(&c->nbr, 8 * (size_t)md->nblk)) ||
+        (e = dalloc(&c->bc, (size_t)bcoff)) || (e = dalloc(&c->wo, 4 * (size_t)bcoff)) ||
+        (e = dalloc(&c->geom, GEO_COUNT * (size_t)ncp)) || (e = dalloc(&c->W[0], 4 * (size_t)ncp)) ||
+        (e = dalloc(&c->W[1], 4 * (size_t)ncp)) || (e = dalloc(&c->U, 4 * (size_t)ncp)) ||
+        (e = dalloc(&c->dUdt[0], 4 * (size_t)ncp)) || (e = dalloc(&c->dt, (size_t)ncp)) ||
+        (e = dalloc(&c->scal, 4)) || (e = dalloc(&c->counter, 4)) || (e = dalloc(&c->step_ctr, 4)) ||
+        (e = dalloc(&c->stop_flag, 4)) || (e = dalloc(&c->norm_small, 3))) {
+        free_ctx(c);
+        return e;
+    }
+    HIPCHK(hipMemcpy(c->meta, c->meta_h.data(), md->nblk * sizeof(BlockMeta), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->nbr, md->nbr, 8 * md->nblk * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->bc, md->bc, bcoff * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->wo, md->wo, 4 * (size_t)bcoff * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(c->W[0], 0, 4 * ncp * sizeof(double)));
+    HIPCHK(hipMemset(c->W[1], 0, 4 * ncp * sizeof(double)));
+    HIPCHK(hipMemset(c->U, 0, 4 * ncp * sizeof(double)));
+    HIPCHK(hipMemset(c->dUdt[0], 0, 4 * ncp * sizeof(double)));
+    HIPCHK(hipMemset(c->dt, 0, ncp * sizeof(double)));
+    HIPCHK(hipMemset(c->counter, 0, 4 * sizeof(unsigned)));
+    // geometry from the nodes
+    double *xn = nullptr, *yn = nullptr;
+    unsigned int *ndeg = nullptr;
+    if ((e = dalloc(&xn, (size_t)noff)) || (e = dalloc(&yn, (size_t)noff)) || (e = dalloc(&ndeg, 1))) {
+        if (xn) (void)hipFree(xn);
+        if (yn) (void)hipFree(yn);
+        free_ctx(c);
+        return e;
+    }
+    HIPCHK(hipMemcpy(xn, md->xnode, noff * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(yn, md->ynode, noff * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemset(ndeg, 0, sizeof(unsigned)));
+    HIPCHK(cffc_launch_geometry(c->meta, c->nblk, c->max_cells, xn, yn, c->geom, ncp, ndeg, c->stream));
+    unsigned nd = 0;
+    HIPCHK(hipMemcpyAsync(&nd, ndeg, sizeof nd, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    (void)hipFree(xn);
+    (void)hipFree(yn);
+    (void)hipFree(ndeg);
+    if (nd) {
+        free_ctx(c);
+        return fail(CFFC_E_ARG, "mesh has %u degenerate faces (length <= 1e-9 x cell perimeter)", nd);
+    }
+    // default tile: the largest supported square-ish tile that the blocks fill
+    int ti = 16, tj = 16;
+    int minn = 1 << 30;
+    for (auto &m : c->meta_h) minn = std::min(minn, std::min(m.ni, m.nj));
+    if (minn < 16) ti = tj = 8;
+    if ((e = build_tiles(c, ti, tj))) {
+        free_ctx(c);
+        return e;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" void cffc_destroy(cffc_ctx *c) {
+    if (c && c->stream) (void)hipStreamSynchronize(c->stream);
+    free_ctx(c);
+}
+
+extern "C" int64_t cffc_num_cells(const cffc_ctx *c) { return c ? c->ncell : 0; }
+extern "C" void *cffc_stream(cffc_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+extern "C" int cffc_set_tile(cffc_ctx *c, int ti, int tj) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    if (!tile_supported(ti, tj)) return fail(CFFC_E_ARG, "tile %dx%d is not instantiated", ti, tj);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return build_tiles(c, ti, tj);
+}
+extern "C" int cffc_get_tile(const cffc_ctx *c, int *ti, int *tj) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    *ti = c->ti;
+    *tj = c->tj;
+    return 0;
+}
+
+static int ensure_staging(cffc_ctx *c) {
+    if (!c->staging) return dalloc(&c->staging, 4 * (size_t)c->ncell);
+    return 0;
+}
+
+static int set_state(cffc_ctx *c, const double *host, int64_t n, int kind) {
+    if (!c || !host) return fail(CFFC_E_ARG, "null argument");
+    if (n != c->ncell) return fail(CFFC_E_ARG, "state has %lld cells, mesh has %lld", (long long)n, (long long)c->ncell);
+    if (int e = ensure_staging(c)) return e;
+    HIPCHK(hipMemcpyAsync(c->staging, host, 4 * n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(cffc_launch_scatter(c->meta, c->nblk, c->max_int, c->staging, c->W[c->wcur], c->ncp, kind, c->U, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    c->phi_valid = 0;
+    return 0;
+}
+extern "C" int cffc_set_W(cffc_ctx *c, const double *W, int64_t n) { return set_state(c, W, n, 0); }
+extern "C" int cffc_set_U(cffc_ctx *c, const double *U, int64_t n) { return set_state(c, U, n, 1); }
+
+static int get_field(cffc_ctx *c, const double *src, double *host, int64_t n, int nvar) {
+    if (!c || !host) return fail(CFFC_E_ARG, "null argument");
+    if (n != c->ncell) return fail(CFFC_E_ARG, "buffer has %lld cells, mesh has %lld", (long long)n, (long long)c->ncell);
+    if (int e = ensure_staging(c)) return e;
+    HIPCHK(cffc_launch_gather(c->meta, c->nblk, c->max_int, src, c->staging, c->ncp, nvar, c->stream));
+    HIPCHK(hipMemcpyAsync(host, c->staging, nvar * n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+extern "C" int cffc_get_W(cffc_ctx *c, double *W, int64_t n) { return c ? get_field(c, c->W[c->wcur], W, n, 4) : CFFC_E_ARG; }
+extern "C" int cffc_get_U(cffc_ctx *c, double *U, int64_t n) { return c ? get_field(c, c->U, U, n, 4) : CFFC_E_ARG; }
+extern "C" int cffc_get_dt(cffc_ctx *c, double *dt, int64_t n) { return c ? get_field(c, c->dt, dt, n, 1) : CFFC_E_ARG; }
+extern "C" int cffc_get_dUdt(cffc_ctx *c, int k, double *d, int64_t n) {
+    if (!c || k < 0 || k > 2) return fail(CFFC_E_ARG, "bad residual index");
+    if (!c->dUdt[k]) return fail(CFFC_E_ARG, "dUdt[%d] has not been computed", k);
+    return get_field(c, c->dUdt[k], d, n, 4);
+}
+
+extern "C" int cffc_synchronize(cffc_ctx *c) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long long *reset_slot) {
+    ExchangeParams p;
+    p.meta = c->meta;
+    p.nbr = c->nbr;
+    p.bc = c->bc;
+    p.wo = c->wo;
+    p.geom = c->geom;
+    p.ncp = c->ncp;
+    p.W = c->W[c->wcur];
+    p.U = c->U;
+    p.from_U = from_U;
+    p.stop_flag = nullptr;
+    p.reset_slot = reset_slot;
+    return p;
+}
+
+// ---- granular entry points (one reference call each) -----------------------
+static int ensure_granular(cffc_ctx *c) {
+    if (c->granular_ready) return 0;
+    const size_t n = 4 * (size_t)c->ncp;
+    int e;
+    if (!c->Uo && (e = dalloc(&c->Uo, n))) return e;
+    for (int k = 1; k < 3; ++k)
+        if (!c->dUdt[k]) {
+            if ((e = dalloc(&c->dUdt[k], n))) return e;
+            HIPCHK(hipMemsetAsync(c->dUdt[k], 0, n * sizeof(double), c->stream));
+        }
+    c->granular_ready = true;
+    return 0;
+}
+
+extern "C" int cffc_exchange_and_bcs(cffc_ctx *c) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    // the granular path keeps U current, so ghosts are always W(U)
+    ExchangeParams p = exchange_params(c, 1, nullptr);
+    HIPCHK(cffc_launch_exchange(&p, c->nblk, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int cffc_cfl(cffc_ctx *c, double *dt_min, int set_global) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, 0, 1, c->stream));
+    HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
+                           c->stream));
+    if (set_global) HIPCHK(cffc_launch_set_global_dt(c->meta, c->nblk, c->max_int, c->dt, c->scal, c->stream));
+    unsigned long long bits = 0;
+    HIPCHK(hipMemcpyAsync(&bits, c->scal, sizeof bits, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    double d;
+    memcpy(&d, &bits, sizeof d);
+    if (dt_min) *dt_min = d;
+    return 0;
+}
+
+static StageParams base_stage_params(cffc_ctx *c) {
+    StageParams p;
+    memset(&p, 0, sizeof p);
+    p.meta = c->meta;
+    p.tiles = c->tiles;
+    p.ntiles = c->ntiles;
+    p.ncp = c->ncp;
+    p.geom = c->geom;
+    p.bc = c->bc;
+    p.wo = c->wo;
+    p.U = c->U;
+    p.dt = c->dt;
+    p.phi = c->phi;
+    p.cfl = c->s.cfl_number;
+    p.limiter = c->s.i_limiter;
+    p.lts = c->s.local_time_stepping == CFFC_SCALAR_LOCAL_TIME_STEPPING;
+    p.err = c->scal + 2;
+    p.norm_part = c->norm_part;
+    p.counter = c->counter;
+    p.step_ctr = c->step_ctr;
+    p.stop_flag = nullptr;
+    p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE;
+    return p;
+}
+
+static hipError_t launch_stage(cffc_ctx *c, const StageParams &p, int fused) {
+    if (c->s.flux_precision == CFFC_FLUX_FP32) return cffc_launch_stage_mixed(&p, c->ti, c->tj, fused, c->stream);
+    return cffc_launch_stage_fp64(&p, c->ti, c->tj, fused, c->stream);
+}
+
+static int ensure_phi(cffc_ctx *c) {
+    if (!c->phi) {
+        if (int e = dalloc(&c->phi, 4 * (size_t)c->ncp)) return e;
+        HIPCHK(hipMemsetAsync(c->phi, 0, 4 * c->ncp * sizeof(double), c->stream));
+    }
+    return 0;
+}
+
+extern "C" int cffc_freeze_limiters(cffc_ctx *c, int freeze) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    if (freeze && !c->phi_valid) return fail(CFFC_E_ARG, "no evaluated limiters to freeze (run a stage first)");
+    c->freeze = freeze ? 1 : 0;
+    return 0;
+}
+
+extern "C" int cffc_dudt_stage(cffc_ctx *c, int i_stage) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    if (i_stage < 1 || i_stage > c->s.n_stage) return fail(CFFC_E_ARG, "bad stage %d", i_stage);
+    if (int e = ensure_granular(c)) return e;
+    if (int e = ensure_phi(c)) return e;
+    if (i_stage == 1)
+        HIPCHK(cffc_launch_copy_interior(c->meta, c->nblk, c->max_int, c->U, c->Uo, c->ncp, c->stream));
+    StageParams p = base_stage_params(c);
+    p.Win = c->W[c->wcur];
+    p.Wout = c->W[c->wcur];
+    const int k = k_residual(c->s, i_stage);
+    p.S = c->dUdt[k];
+    p.S1 = c->dUdt[1];
+    p.S2 = c->dUdt[2];
+    p.mode = granular_init(c->s, i_stage);
+    p.dtmin_in = c->scal;
+    p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE_STORE;
+    HIPCHK(launch_stage(c, p, 0));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (!c->freeze) c->phi_valid = 1;
+    return 0;
+}
+
+extern "C" int cffc_update_stage(cffc_ctx *c, int i_stage, int64_t *bad_cell) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    if (i_stage < 1 || i_stage > c->s.n_stage) return fail(CFFC_E_ARG, "bad stage %d", i_stage);
+    if (int e = ensure_granular(c)) return e;
+    const int k = k_residual(c->s, i_stage);
+    HIPCHK(cffc_launch_set_u64(c->scal + 2, ERR_NONE, 0, 1, c->stream));
+    HIPCHK(cffc_launch_update(c->meta, c->nblk, c->max_int, c->W[c->wcur], c->U, c->Uo, c->dUdt[k], c->dt, c->ncp,
+                              stage_omega(c->s, i_stage),
+                              c->s.local_time_stepping == CFFC_SCALAR_LOCAL_TIME_STEPPING, c->scal + 2, c->stream));
+    unsigned long long key = 0;
+    HIPCHK(hipMemcpyAsync(&key, c->scal + 2, sizeof key, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (key != ERR_NONE) {
+        if (bad_cell) *bad_cell = (int64_t)key;
+        return (int)(key & 0xFFFF);  // the reference returns the i-index
+    }
+    if (bad_cell) *bad_cell = -1;
+    return 0;
+}
+
+extern "C" int cffc_residual_norms(cffc_ctx *c, int var, double out[3]) {
+    if (!c || !out || var < 1 || var > 4) return fail(CFFC_E_ARG, "bad argument");
+    HIPCHK(cffc_launch_norms(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->dUdt[0], c->ncp, var, c->norm_part,
+                             c->norm_small, c->stream));
+    HIPCHK(hipMemcpyAsync(out, c->norm_small, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+// ---- the fused time-marching loop ------------------------------------------
+static int ensure_steps(cffc_ctx *c, int nsteps) {
+    if (nsteps <= c->max_steps) return 0;
+    if (c->norm_out) (void)hipFree(c->norm_out);
+    if (c->dt_out) (void)hipFree(c->dt_out);
+    c->norm_out = c->dt_out = nullptr;
+    int cap = std::max(nsteps, 1024);
+    if (int e = dalloc(&c->norm_out, 3 * (size_t)cap)) return e;
+    if (int e = dalloc(&c->dt_out, (size_t)cap)) return e;
+    c->max_steps = cap;
+    return 0;
+}
+
+// enqueue one step (all stages) with step parity par
+static int enqueue_step(cffc_ctx *c, int par, double tol) {
+    const int N = c->s.n_stage;
+    for (int st = 1; st <= N; ++st) {
+        ExchangeParams ex = exchange_params(c, st == 1, st == 1 ? c->scal + (1 - par) : nullptr);
+        ex.stop_flag = c->stop_flag;
+        HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
+        StageParams p = base_stage_params(c);
+        p.Win = c->W[c->wcur];
+        p.Wout = c->W[1 - c->wcur];
+        p.S = c->dUdt[0];
+        p.mode = fused_mode(c->s, st);
+        p.omega = stage_omega(c->s, st);
+        p.last_stage = (st == N);
+        p.dtmin_in = c->scal + par;
+        p.dtmin_out = c->scal + (1 - par);
+        p.norm_out = c->norm_out;
+        p.dt_out = c->dt_out;
+        p.stop_flag = c->stop_flag;
+        p.tol = tol;
+        if (c->freeze) p.phi_mode = PHI_LOAD;
+        else if (c->store_phi && st == N) p.phi_mode = PHI_COMPUTE_STORE;
+        else p.phi_mode = PHI_COMPUTE;
+        HIPCHK(launch_stage(c, p, 1));
+        c->wcur = 1 - c->wcur;
+    }
+    return 0;
+}
+
+static int begin_run(cffc_ctx *c, int nsteps) {
+    if (int e = ensure_steps(c, nsteps)) return e;
+    if (c->store_phi || c->freeze)
+        if (int e = ensure_phi(c)) return e;
+    int32_t zero[2] = {0, 0};
+    HIPCHK(hipMemcpyAsync(c->step_ctr, zero, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->stop_flag, zero, sizeof(int32_t), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(c->counter, 0, sizeof(unsigned), c->stream));
+    HIPCHK(cffc_launch_set_u64(c->scal + 2, ERR_NONE, 0, 1, c->stream));
+    // CFL of the first step from the current state (slot 0)
+    HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, MILLION_BITS, 2, c->stream));
+    HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
+                           c->stream));
+    return 0;
+}
+
+static int end_run(cffc_ctx *c, int wstart, int nsteps, double *norms, double *dt_out, int *steps_done) {
+    int32_t done = 0;
+    unsigned long long key = 0;
+    HIPCHK(hipMemcpyAsync(&done, c->step_ctr, sizeof done, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(&key, c->scal + 2, sizeof key, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (done > nsteps) done = nsteps;
+    if (norms && done > 0) HIPCHK(hipMemcpy(norms, c->norm_out, 3 * (size_t)done * sizeof(double), hipMemcpyDeviceToHost));
+    if (dt_out && done > 0) HIPCHK(hipMemcpy(dt_out, c->dt_out, (size_t)done * sizeof(double), hipMemcpyDeviceToHost));
+    if (steps_done) *steps_done = done;
+    // kernels after a stop are skipped: the newest W is the one after `done` steps
+    c->wcur = (wstart + done * c->s.n_stage) & 1;
+    if (c->store_phi) c->phi_valid = 1;
+    if (key != ERR_NONE) {
+        g_err = "non-positive density or energy";
+        return (int)(key & 0xFFFF);
+    }
+    return 0;
+}
+
+extern "C" int cffc_run_steps(cffc_ctx *c, int nsteps, double tol, double *norms, double *dt_out, int *steps_done) {
+    if (!c || nsteps < 0) return fail(CFFC_E_ARG, "bad argument");
+    if (int e = begin_run(c, nsteps)) return e;
+    const int wstart = c->wcur;
+    int32_t *hflag = nullptr;
+    HIPCHK(hipHostMalloc((void **)&hflag, 2 * sizeof(int32_t), hipHostMallocDefault));
+    hflag[0] = 0;
+    for (int n = 0; n < nsteps; ++n) {
+        if (int e = enqueue_step(c, n & 1, tol)) {
+            (void)hipHostFree(hflag);
+            return e;
+        }
+        // poll for convergence / failure now and then without draining the queue
+        if ((n & 63) == 63) {
+            if (hflag[0]) break;
+            (void)hipMemcpyAsync(hflag, c->stop_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+        }
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipHostFree(hflag);
+    return end_run(c, wstart, nsteps, norms, dt_out, steps_done);
+}
+
+extern "C" int cffc_run_steps_graph(cffc_ctx *c, int nsteps, double *norms, double *dt_out) {
+    if (!c || nsteps < 0) return fail(CFFC_E_ARG, "bad argument");
+    if (int e = begin_run(c, nsteps)) return e;
+    const int wstart = c->wcur;
+    const int pairs = nsteps / 2;
+    if (pairs > 0) {
+        hipGraph_t g = nullptr;
+        hipGraphExec_t ge = nullptr;
+        const int w0 = c->wcur;
+        HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        int e = enqueue_step(c, 0, 0.0);
+        if (!e) e = enqueue_step(c, 1, 0.0);
+        hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (e) return e;
+        if (ce != hipSuccess) return fail(CFFC_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+        HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        for (int q = 0; q < pairs; ++q) HIPCHK(hipGraphLaunch(ge, c->stream));
+        c->wcur = w0;  // every replay ends where it began (2*N_Stage buffer flips)
+        HIPCHK(hipStreamSynchronize(c->stream));
+        (void)hipGraphExecDestroy(ge);
+        (void)hipGraphDestroy(g);
+    }
+    if (nsteps & 1)
+        if (int e = enqueue_step(c, 0, 0.0)) return e;
+    (void)hipStreamSynchronize(c->stream);
+    int done = 0;
+    int rc = end_run(c, wstart, nsteps, norms, dt_out, &done);
+    return rc;
+}
+
+// keep the last stage's limiters stored so that a freeze can follow a run
+extern "C" int cffc_set_store_limiters(cffc_ctx *c, int on) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    c->store_phi = on ? 1 : 0;
+    return 0;
+}

@@ -1,0 +1,51 @@
+// euler2d_device.hpp -- device data layout shared by the gfx950 kernels and
+// the host runtime.
+//
+// HBM layout (one context):
+//   * every block is stored padded with its two ghost layers,
+//     nci = ni+4 by ncj = nj+4 cells, i fastest; blocks are concatenated, so a
+//     padded cell has one global index c (block offset coff + j*nci + i);
+//   * every field is SoA: variable v of field F lives at F + v*ncp, where ncp
+//     is the total number of padded cells (coalesced 8-byte lanes along i);
+//   * W (primitive, FP64) is double-buffered: a stage reads W^(s) with filled
+//     ghosts and writes W^(s+1) for the interior; U holds the conserved state
+//     U^n; dUdt[0] is the residual accumulator; geometry is precomputed once
+//     per context from the nodes (GEO_* below).
+#pragma once
+
+#include <stdint.h>
+
+namespace cffc {
+
+enum {
+    GEO_XC = 0, GEO_YC, GEO_A,                  // Cell[i][j].Xc, .A
+    GEO_NEX, GEO_NEY, GEO_LE, GEO_XFE, GEO_YFE, // nfaceE, lfaceE, xfaceE
+    GEO_NNX, GEO_NNY, GEO_LN, GEO_XFN, GEO_YFN, // nfaceN, lfaceN, xfaceN
+    GEO_COUNT
+};
+
+struct BlockMeta {
+    int32_t ni, nj, nci, ncj;
+    int64_t coff;    // first padded cell of the block
+    int64_t ioff;    // first interior cell of the block in the host state arrays
+    int64_t noff;    // first node of the block in the node arrays
+    int32_t bcoff;   // first entry of the block in the bc / wo arrays
+    int32_t pad;
+};
+
+struct TileDesc {
+    int32_t b, i0, j0, pad;  // block, first interior cell of the tile
+};
+
+// stage modes of the residual accumulator (dUdt[0] in the fused path)
+enum StageMode {
+    MODE_ZERO = 0,       // dU starts at zero, store S = dU
+    MODE_ZERO_ACC2 = 1,  // dU starts at zero, store S = S + 2*dU (RK4 stages 2, 3)
+    MODE_FROM_S = 2,     // dU starts at S, store S = dU (RK4 stage 4, RK2 stage 2)
+};
+
+enum PhiMode { PHI_COMPUTE = 0, PHI_COMPUTE_STORE = 1, PHI_LOAD = 2 };
+
+constexpr double MILLION = 1000000.00;
+
+}  // namespace cffc

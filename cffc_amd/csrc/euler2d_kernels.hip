@@ -1,0 +1,475 @@
+// euler2d_kernels.hip -- FP64 gfx950 kernels of the 2D Euler explicit path.
+//
+// Built with -ffp-contract=off so that every FP64 operation rounds exactly as
+// in the reference's strict-IEEE CPU build.  Contents:
+//   geometry_kernel    cell centroid/area and face geometry from the nodes
+//                      (HO_Grid2DQuad.h:1403-1410, 1782-1980; Math.h:311-350)
+//   exchange_bc_kernel Send_All_Messages (same-level faces and corners,
+//                      AdaptiveBlock2D_MessagePassing.h:32-1183, 5680-6282,
+//                      7883-8020) fused with BCs (Euler2DQuadSingleBlock.cc:
+//                      2354-3010), one workgroup per block
+//   cfl_kernel         CFL + global minimum (SingleBlock.cc:3019-3080)
+//   update_kernel      Update_Solution_Multistage_Explicit (SingleBlock.cc:5300)
+//   norm kernels       L1/L2/Max_Norm_Residual (SingleBlock.cc:3111-3167)
+//   stage kernels      the fused residual kernel (euler2d_stage.hpp), FP64
+#include <hip/hip_runtime.h>
+
+#include "euler2d_device.hpp"
+#include "euler2d_kernels.hpp"
+#include "euler2d_stage.hpp"
+#include "euler2d_state.hpp"
+
+namespace cffc {
+
+// ---------------------------------------------------------------------------
+// geometry: one thread per padded cell, all blocks (blockIdx.y = block)
+__global__ void geometry_kernel(const BlockMeta *meta, const double *xn, const double *yn, double *geom,
+                                int64_t ncp, unsigned int *ndegenerate) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int ncell = m.nci * m.ncj;
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= ncell) return;
+    const int i = q % m.nci, j = q / m.nci;
+    const int nn = m.nci + 1;
+    auto X = [&](int a, int b) { return xn[m.noff + (int64_t)b * nn + a]; };
+    auto Y = [&](int a, int b) { return yn[m.noff + (int64_t)b * nn + a]; };
+    const int64_t c = m.coff + q;
+    // quadCentroid with SW, SE, NE, NW
+    const double X0x = X(i, j), X0y = Y(i, j), X1x = X(i + 1, j), X1y = Y(i + 1, j);
+    const double X2x = X(i + 1, j + 1), X2y = Y(i + 1, j + 1), X3x = X(i, j + 1), X3y = Y(i, j + 1);
+    double ai, atmp = 0, xtmp = 0, ytmp = 0;
+    ai = X3x * X0y - X0x * X3y; atmp += ai; xtmp += (X0x + X3x) * ai; ytmp += (X0y + X3y) * ai;
+    ai = X0x * X1y - X1x * X0y; atmp += ai; xtmp += (X1x + X0x) * ai; ytmp += (X1y + X0y) * ai;
+    ai = X1x * X2y - X2x * X1y; atmp += ai; xtmp += (X2x + X1x) * ai; ytmp += (X2y + X1y) * ai;
+    ai = X2x * X3y - X3x * X2y; atmp += ai; xtmp += (X3x + X2x) * ai; ytmp += (X3y + X2y) * ai;
+    geom[GEO_A * ncp + c] = 0.5 * atmp;
+    geom[GEO_XC * ncp + c] = atmp != 0 ? xtmp / (3 * atmp) : 0.0;
+    geom[GEO_YC * ncp + c] = atmp != 0 ? ytmp / (3 * atmp) : 0.0;
+    // face lengths: E = |N(i+1,j)-N(i+1,j+1)|, W = |N(i,j)-N(i,j+1)|,
+    //               N = |N(i,j+1)-N(i+1,j+1)|, S = |N(i,j)-N(i+1,j)|
+    const double lE = sqrt((X1x - X2x) * (X1x - X2x) + (X1y - X2y) * (X1y - X2y));
+    const double lW = sqrt((X0x - X3x) * (X0x - X3x) + (X0y - X3y) * (X0y - X3y));
+    const double lN = sqrt((X3x - X2x) * (X3x - X2x) + (X3y - X2y) * (X3y - X2y));
+    const double lS = sqrt((X0x - X1x) * (X0x - X1x) + (X0y - X1y) * (X0y - X1y));
+    const double perim = lE + lN + lW + lS;
+    geom[GEO_LE * ncp + c] = lE;
+    geom[GEO_LN * ncp + c] = lN;
+    geom[GEO_XFE * ncp + c] = 0.5 * (X1x + X2x);
+    geom[GEO_YFE * ncp + c] = 0.5 * (X1y + X2y);
+    geom[GEO_XFN * ncp + c] = 0.5 * (X3x + X2x);
+    geom[GEO_YFN * ncp + c] = 0.5 * (X3y + X2y);
+    // nfaceE = (dy, -dx)/|d| with d = N(i+1,j+1) - N(i+1,j)
+    if (lE > 0.000000001 * perim) {
+        const double dx = X2x - X1x, dy = X2y - X1y, l = sqrt(dx * dx + dy * dy);
+        geom[GEO_NEX * ncp + c] = dy / l;
+        geom[GEO_NEY * ncp + c] = -dx / l;
+    } else {
+        geom[GEO_NEX * ncp + c] = 1.0;
+        geom[GEO_NEY * ncp + c] = 0.0;
+        atomicAdd(ndegenerate, 1u);
+    }
+    // nfaceN = (a.y-b.y, -(a.x-b.x))/|a-b| with a = N(i,j+1), b = N(i+1,j+1)
+    if (lN > 0.000000001 * perim) {
+        const double dx = X3x - X2x, dy = X3y - X2y, l = sqrt(dx * dx + dy * dy);
+        geom[GEO_NNX * ncp + c] = dy / l;
+        geom[GEO_NNY * ncp + c] = -dx / l;
+    } else {
+        geom[GEO_NNX * ncp + c] = 1.0;
+        geom[GEO_NNY * ncp + c] = 0.0;
+        atomicAdd(ndegenerate, 1u);
+    }
+    // a W or S face that is degenerate seen from this cell but not from its
+    // neighbour would break nfaceW(i) = -nfaceE(i-1); count it too
+    if (!(lW > 0.000000001 * perim) || !(lS > 0.000000001 * perim)) atomicAdd(ndegenerate, 1u);
+}
+
+// ---------------------------------------------------------------------------
+// state scatter / gather between host AoS interior order and device SoA padded
+__global__ void scatter_state_kernel(const BlockMeta *meta, const double *aos, double *dst, int64_t ncp, int kind,
+                                     double *Ucons) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.ni * m.nj) return;
+    const int i = q % m.ni + 2, j = q / m.ni + 2;
+    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    const double *a = aos + 4 * (m.ioff + q);
+    if (kind == 0) {  // W given: W, and U = U(W)
+        PS<double> W = {a[0], a[1], a[2], a[3]};
+        CS<double> U = U_of_W(W);
+        dst[c] = W.d; dst[ncp + c] = W.vx; dst[2 * ncp + c] = W.vy; dst[3 * ncp + c] = W.p;
+        Ucons[c] = U.d; Ucons[ncp + c] = U.mx; Ucons[2 * ncp + c] = U.my; Ucons[3 * ncp + c] = U.E;
+    } else if (kind == 1) {  // U given: U, and W = W(U)
+        CS<double> U = {a[0], a[1], a[2], a[3]};
+        PS<double> W = W_of_U(U);
+        dst[c] = W.d; dst[ncp + c] = W.vx; dst[2 * ncp + c] = W.vy; dst[3 * ncp + c] = W.p;
+        Ucons[c] = U.d; Ucons[ncp + c] = U.mx; Ucons[2 * ncp + c] = U.my; Ucons[3 * ncp + c] = U.E;
+    }
+}
+
+__global__ void gather_state_kernel(const BlockMeta *meta, const double *src, double *aos, int64_t ncp,
+                                    int nvar) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.ni * m.nj) return;
+    const int i = q % m.ni + 2, j = q / m.ni + 2;
+    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    double *a = aos + (int64_t)nvar * (m.ioff + q);
+    for (int v = 0; v < nvar; ++v) a[v] = src[v * ncp + c];
+}
+
+// ---------------------------------------------------------------------------
+// ghost exchange + physical BCs.  One workgroup per block.
+__device__ __forceinline__ PS<double> ldW(const double *W, int64_t ncp, int64_t c) {
+    PS<double> r = {W[c], W[ncp + c], W[2 * ncp + c], W[3 * ncp + c]};
+    return r;
+}
+__device__ __forceinline__ void stW(double *W, int64_t ncp, int64_t c, const PS<double> &r) {
+    W[c] = r.d; W[ncp + c] = r.vx; W[2 * ncp + c] = r.vy; W[3 * ncp + c] = r.p;
+}
+
+__global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
+    if (p.stop_flag && *p.stop_flag) return;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && p.reset_slot) *p.reset_slot = 0x412E848000000000ull;  // MILLION
+    const int b = blockIdx.x;
+    const BlockMeta m = p.meta[b];
+    const int64_t ncp = p.ncp;
+    double *W = p.W;
+    const int ICl = 2, ICu = m.ni + 1, JCl = 2, JCu = m.nj + 1;
+    const int tid = threadIdx.x, NT = blockDim.x;
+    // phase 0: same-level neighbours, faces (2 layers over the interior span)
+    // and corners (2 x 2); U is transferred and W = W(U) recomputed on stage 1
+    // (Euler2DQuad.h:1643-1647); later stages copy W, which equals W(U).
+    for (int s = 0; s < 8; ++s) {
+        const int nb = p.nbr[8 * b + s];
+        if (nb < 0) continue;
+        const BlockMeta S = p.meta[nb];
+        int i0, i1, j0, j1, di, dj;
+        switch (s) {
+        case 0: i0 = ICl; i1 = ICu; j0 = JCu + 1; j1 = JCu + 2; di = 2 - ICl; dj = 2 - (JCu + 1); break;
+        case 1: i0 = ICl; i1 = ICu; j0 = JCl - 2; j1 = JCl - 1; di = 2 - ICl; dj = (S.nj + 1) - (JCl - 1); break;
+        case 2: i0 = ICu + 1; i1 = ICu + 2; j0 = JCl; j1 = JCu; di = 2 - (ICu + 1); dj = 2 - JCl; break;
+        case 3: i0 = ICl - 2; i1 = ICl - 1; j0 = JCl; j1 = JCu; di = (S.ni + 1) - (ICl - 1); dj = 2 - JCl; break;
+        case 4: i0 = ICu + 1; i1 = ICu + 2; j0 = JCu + 1; j1 = JCu + 2; di = 2 - (ICu + 1); dj = 2 - (JCu + 1); break;
+        case 5: i0 = ICl - 2; i1 = ICl - 1; j0 = JCu + 1; j1 = JCu + 2; di = (S.ni + 1) - (ICl - 1); dj = 2 - (JCu + 1); break;
+        case 6: i0 = ICu + 1; i1 = ICu + 2; j0 = JCl - 2; j1 = JCl - 1; di = 2 - (ICu + 1); dj = (S.nj + 1) - (JCl - 1); break;
+        default: i0 = ICl - 2; i1 = ICl - 1; j0 = JCl - 2; j1 = JCl - 1; di = (S.ni + 1) - (ICl - 1); dj = (S.nj + 1) - (JCl - 1); break;
+        }
+        const int w = i1 - i0 + 1, h = j1 - j0 + 1;
+        for (int q = tid; q < w * h; q += NT) {
+            const int i = i0 + q % w, j = j0 + q / w;
+            const int64_t cd = m.coff + (int64_t)j * m.nci + i;
+            const int64_t cs = S.coff + (int64_t)(j + dj) * S.nci + (i + di);
+            PS<double> Wg;
+            if (p.from_U) {
+                const CS<double> U = {p.U[cs], p.U[ncp + cs], p.U[2 * ncp + cs], p.U[3 * ncp + cs]};
+                Wg = W_of_U(U);
+            } else {
+                Wg = ldW(W, ncp, cs);
+            }
+            stW(W, ncp, cd, Wg);
+        }
+    }
+    __syncthreads();
+    const int32_t *bcW = p.bc + m.bcoff, *bcE = bcW + m.ncj, *bcS = bcE + m.ncj, *bcN = bcS + m.nci;
+    const double *woW = p.wo + 4 * m.bcoff, *woE = woW + 4 * m.ncj, *woS = woE + 4 * m.ncj, *woN = woS + 4 * m.nci;
+    auto cidx = [&](int i, int j) { return m.coff + (int64_t)j * m.nci + i; };
+    auto wo = [&](const double *a, int k) { PS<double> r = {a[4 * k], a[4 * k + 1], a[4 * k + 2], a[4 * k + 3]}; return r; };
+    // phase 1: West and East boundaries, j over all rows (corner rows when the
+    // South/North BC of the ghost column is BC_NONE), SingleBlock.cc:2362-2676
+    for (int q = tid; q < 2 * m.ncj; q += NT) {
+        const int j = q >> 1;
+        const bool west = (q & 1) == 0;
+        const int bcv = west ? bcW[j] : bcE[j];
+        if (bcv == 0) continue;
+        const int icol = west ? ICl - 1 : ICu + 1;
+        const bool apply = (j >= JCl && j <= JCu) || (j < JCl && bcS[icol] == 0) || (j > JCu && bcN[icol] == 0);
+        if (!apply) continue;
+        const int in = west ? ICl : ICu;       // adjacent interior column
+        const int sg = west ? -1 : 1;          // ghost direction
+        // normal of the boundary face: nfaceW(ICl) = -nfaceE(ICl-1), nfaceE(ICu)
+        const int64_t cf = west ? cidx(ICl - 1, j) : cidx(ICu, j);
+        const double nx = west ? -p.geom[GEO_NEX * ncp + cf] : p.geom[GEO_NEX * ncp + cf];
+        const double ny = west ? -p.geom[GEO_NEY * ncp + cf] : p.geom[GEO_NEY * ncp + cf];
+        switch (bcv) {
+        case 11000: {  // FIXED
+            const PS<double> w0 = wo(west ? woW : woE, j);
+            stW(W, ncp, cidx(in + sg, j), w0); stW(W, ncp, cidx(in + 2 * sg, j), w0);
+        } break;
+        case 7001:  // REFLECTION: ghost g mirrors cell in-(g-1)*sg
+            stW(W, ncp, cidx(in + sg, j), reflect(ldW(W, ncp, cidx(in, j)), nx, ny));
+            stW(W, ncp, cidx(in + 2 * sg, j), reflect(ldW(W, ncp, cidx(in - sg, j)), nx, ny));
+            break;
+        case 6000:  // PERIODIC: W ghosts take ICu-g+1 (E ghosts take ICl+g-1)
+            stW(W, ncp, cidx(in + sg, j), ldW(W, ncp, cidx(west ? ICu : ICl, j)));
+            stW(W, ncp, cidx(in + 2 * sg, j), ldW(W, ncp, cidx(west ? ICu - 1 : ICl + 1, j)));
+            break;
+        case 8001: {  // CHARACTERISTIC: first layer from BC_Characteristic_Pressure, second copied
+            const PS<double> g1 = bc_characteristic_pressure(ldW(W, ncp, cidx(in, j)), wo(west ? woW : woE, j), nx, ny);
+            stW(W, ncp, cidx(in + sg, j), g1); stW(W, ncp, cidx(in + 2 * sg, j), g1);
+        } break;
+        default: {  // CONSTANT_EXTRAPOLATION (and the reference's default)
+            const PS<double> w0 = ldW(W, ncp, cidx(in, j));
+            stW(W, ncp, cidx(in + sg, j), w0); stW(W, ncp, cidx(in + 2 * sg, j), w0);
+        } break;
+        }
+    }
+    __syncthreads();
+    // phase 2: South and North boundaries, i over all columns, :2678-2989
+    for (int q = tid; q < 2 * m.nci; q += NT) {
+        const int i = q >> 1;
+        const bool south = (q & 1) == 0;
+        const int bcv = south ? bcS[i] : bcN[i];
+        if (bcv == 0) continue;
+        const int in = south ? JCl : JCu;
+        const int sg = south ? -1 : 1;
+        const int64_t cf = south ? cidx(i, JCl - 1) : cidx(i, JCu);
+        const double nx = south ? -p.geom[GEO_NNX * ncp + cf] : p.geom[GEO_NNX * ncp + cf];
+        const double ny = south ? -p.geom[GEO_NNY * ncp + cf] : p.geom[GEO_NNY * ncp + cf];
+        switch (bcv) {
+        case 11000: {
+            const PS<double> w0 = wo(south ? woS : woN, i);
+            stW(W, ncp, cidx(i, in + sg), w0); stW(W, ncp, cidx(i, in + 2 * sg), w0);
+        } break;
+        case 7001:
+            stW(W, ncp, cidx(i, in + sg), reflect(ldW(W, ncp, cidx(i, in)), nx, ny));
+            stW(W, ncp, cidx(i, in + 2 * sg), reflect(ldW(W, ncp, cidx(i, in - sg)), nx, ny));
+            break;
+        case 6000:
+            stW(W, ncp, cidx(i, in + sg), ldW(W, ncp, cidx(i, south ? JCu : JCl)));
+            stW(W, ncp, cidx(i, in + 2 * sg), ldW(W, ncp, cidx(i, south ? JCu - 1 : JCl + 1)));
+            break;
+        case 8001: {
+            const PS<double> g1 = bc_characteristic_pressure(ldW(W, ncp, cidx(i, in)), wo(south ? woS : woN, i), nx, ny);
+            stW(W, ncp, cidx(i, in + sg), g1); stW(W, ncp, cidx(i, in + 2 * sg), g1);
+        } break;
+        default: {
+            const PS<double> w0 = ldW(W, ncp, cidx(i, in));
+            stW(W, ncp, cidx(i, in + sg), w0); stW(W, ncp, cidx(i, in + 2 * sg), w0);
+        } break;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// CFL over the interior cells of each tile; global minimum with atomicMin on
+// the bit pattern (positive doubles order like their bits).
+__global__ __launch_bounds__(256) void cfl_kernel(const BlockMeta *meta, const TileDesc *tiles, int ti, int tj,
+                                                  const double *W, const double *geom, int64_t ncp, double *dt,
+                                                  unsigned long long *dtmin) {
+    __shared__ double sm[4];
+    const TileDesc t = tiles[blockIdx.x];
+    const BlockMeta m = meta[t.b];
+    double dmn = MILLION;
+    for (int q = threadIdx.x; q < ti * tj; q += blockDim.x) {
+        const int ip = t.i0 + 2 + q % ti, jp = t.j0 + 2 + q / ti;
+        if (ip > m.ni + 1 || jp > m.nj + 1) continue;
+        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+        const double *G = geom;
+        const double A = G[GEO_A * ncp + c];
+        const double lW = G[GEO_LE * ncp + c - 1], lE = G[GEO_LE * ncp + c];
+        const double lS = G[GEO_LN * ncp + c - m.nci], lN = G[GEO_LN * ncp + c];
+        const double d_i = 2.0 * (A / (lE + lW));
+        const double d_j = 2.0 * (A / (lN + lS));
+        const PS<double> Wc = ldW(W, ncp, c);
+        const double nWx = -G[GEO_NEX * ncp + c - 1], nWy = -G[GEO_NEY * ncp + c - 1];
+        const double nSx = -G[GEO_NNX * ncp + c - m.nci], nSy = -G[GEO_NNY * ncp + c - m.nci];
+        const double v_i = 0.5 * (Wc.vx * (G[GEO_NEX * ncp + c] - nWx) + Wc.vy * (G[GEO_NEY * ncp + c] - nWy));
+        const double v_j = 0.5 * (Wc.vx * (G[GEO_NNX * ncp + c] - nSx) + Wc.vy * (G[GEO_NNY * ncp + c] - nSy));
+        const double a = W_a(Wc);
+        const double d = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
+        dt[c] = d;
+        dmn = dmin(dmn, d);
+    }
+    for (int off = 32; off > 0; off >>= 1) dmn = dmin(dmn, __shfl_xor(dmn, off));
+    if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = dmn;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) dmn = dmin(dmn, sm[w]);
+        dmn = dmin(sm[0], dmn);
+        atomicMin(dtmin, dbits(dmn));
+    }
+}
+
+// Set_Global_TimeStep over the interior cells
+__global__ void set_global_dt_kernel(const BlockMeta *meta, double *dt, const unsigned long long *dtmin) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.ni * m.nj) return;
+    const int i = q % m.ni + 2, j = q / m.ni + 2;
+    dt[m.coff + (int64_t)j * m.nci + i] = bitsd(*dtmin);
+}
+
+// ---------------------------------------------------------------------------
+// Update_Solution_Multistage_Explicit on its own (granular path), in place.
+__global__ void update_kernel(const BlockMeta *meta, double *W, double *U, const double *Uo, double *dU_k,
+                              double *dt, int64_t ncp, double omega, int lts, unsigned long long *err) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.ni * m.nj) return;
+    const int i = q % m.ni + 2, j = q / m.ni + 2;
+    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    double d[4] = {dU_k[c], dU_k[ncp + c], dU_k[2 * ncp + c], dU_k[3 * ncp + c]};
+    const double U0[4] = {Uo[c], Uo[ncp + c], Uo[2 * ncp + c], Uo[3 * ncp + c]};
+    CS<double> Un = {U0[0] + omega * d[0], U0[1] + omega * d[1], U0[2] + omega * d[2], U0[3] + omega * d[3]};
+    bool bad = (Un.d <= 0.0 || Un.E <= 0.0 || U_e(Un) <= 0.0);
+    if (bad && lts) {
+        double f = 1.0;
+        double dtl = dt[c];
+        for (int n = 1; n <= 10; ++n) {
+            f = 0.5 * f;
+            dtl = f * dtl;
+            for (int v = 0; v < 4; ++v) d[v] = f * d[v];
+            Un.d = U0[0] + omega * d[0]; Un.mx = U0[1] + omega * d[1];
+            Un.my = U0[2] + omega * d[2]; Un.E = U0[3] + omega * d[3];
+            if (Un.d > 0.0 && Un.E > 0.0 && U_e(Un) > 0.0) break;
+        }
+        dt[c] = dtl;
+        for (int v = 0; v < 4; ++v) dU_k[v * ncp + c] = d[v];
+        bad = (Un.d <= 0.0 || Un.E <= 0.0 || U_e(Un) <= 0.0);
+    }
+    U[c] = Un.d; U[ncp + c] = Un.mx; U[2 * ncp + c] = Un.my; U[3 * ncp + c] = Un.E;
+    if (bad) {
+        atomicMin(err, ((unsigned long long)blockIdx.y << 32) | ((unsigned long long)j << 16) | (unsigned long long)i);
+        return;
+    }
+    stW(W, ncp, c, W_of_U(Un));
+}
+
+// Uo = U over the interior (the stage-1 copy of dUdt_Multistage_Explicit)
+__global__ void copy_interior_kernel(const BlockMeta *meta, const double *src, double *dst, int64_t ncp) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.ni * m.nj) return;
+    const int i = q % m.ni + 2, j = q / m.ni + 2;
+    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    for (int v = 0; v < 4; ++v) dst[v * ncp + c] = src[v * ncp + c];
+}
+
+// residual norms: per-tile partials, then one workgroup sums them in order
+__global__ __launch_bounds__(256) void norm_partial_kernel(const BlockMeta *meta, const TileDesc *tiles, int ti,
+                                                           int tj, const double *dU, int64_t ncp, int var,
+                                                           double *part) {
+    __shared__ double sm[4][3];
+    const TileDesc t = tiles[blockIdx.x];
+    const BlockMeta m = meta[t.b];
+    double a = 0.0, b = 0.0, mx = 0.0;
+    for (int q = threadIdx.x; q < ti * tj; q += blockDim.x) {
+        const int ip = t.i0 + 2 + q % ti, jp = t.j0 + 2 + q / ti;
+        if (ip > m.ni + 1 || jp > m.nj + 1) continue;
+        const double v = dU[(int64_t)(var - 1) * ncp + m.coff + (int64_t)jp * m.nci + ip];
+        a += fabs(v);
+        b += v * v;
+        mx = dmax(mx, fabs(v));
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off); b += __shfl_xor(b, off); mx = dmax(mx, __shfl_xor(mx, off));
+    }
+    if ((threadIdx.x & 63) == 0) { sm[threadIdx.x >> 6][0] = a; sm[threadIdx.x >> 6][1] = b; sm[threadIdx.x >> 6][2] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double A = sm[0][0], B = sm[0][1], C = sm[0][2];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { A += sm[w][0]; B += sm[w][1]; C = dmax(C, sm[w][2]); }
+        part[3 * blockIdx.x] = A; part[3 * blockIdx.x + 1] = B; part[3 * blockIdx.x + 2] = C;
+    }
+}
+
+__global__ __launch_bounds__(256) void norm_final_kernel(const double *part, int n, double *out) {
+    __shared__ double sm[4][3];
+    double a = 0.0, b = 0.0, mx = 0.0;
+    for (int q = threadIdx.x; q < n; q += blockDim.x) { a += part[3 * q]; b += part[3 * q + 1]; mx = dmax(mx, part[3 * q + 2]); }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off); b += __shfl_xor(b, off); mx = dmax(mx, __shfl_xor(mx, off));
+    }
+    if ((threadIdx.x & 63) == 0) { sm[threadIdx.x >> 6][0] = a; sm[threadIdx.x >> 6][1] = b; sm[threadIdx.x >> 6][2] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double A = sm[0][0], B = sm[0][1], C = sm[0][2];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { A += sm[w][0]; B += sm[w][1]; C = dmax(C, sm[w][2]); }
+        out[0] = A; out[1] = sqrt(B); out[2] = C;
+    }
+}
+
+// small device-side scalar writes (graph-safe, no memset nodes)
+__global__ void set_u64_kernel(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n) {
+    if (threadIdx.x == 0) { dst[0] = v0; if (n > 1) dst[1] = v1; }
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+}  // namespace cffc
+
+using namespace cffc;
+
+extern "C" {
+
+hipError_t cffc_launch_geometry(const BlockMeta *meta, int nblk, int max_cells, const double *xn, const double *yn,
+                                double *geom, int64_t ncp, unsigned int *ndeg, hipStream_t st) {
+    dim3 grid((max_cells + 255) / 256, nblk);
+    hipLaunchKernelGGL(geometry_kernel, grid, dim3(256), 0, st, meta, xn, yn, geom, ncp, ndeg);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_scatter(const BlockMeta *meta, int nblk, int max_int, const double *aos, double *W, int64_t ncp,
+                               int kind, double *U, hipStream_t st) {
+    dim3 grid((max_int + 255) / 256, nblk);
+    hipLaunchKernelGGL(scatter_state_kernel, grid, dim3(256), 0, st, meta, aos, W, ncp, kind, U);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_gather(const BlockMeta *meta, int nblk, int max_int, const double *src, double *aos,
+                              int64_t ncp, int nvar, hipStream_t st) {
+    dim3 grid((max_int + 255) / 256, nblk);
+    hipLaunchKernelGGL(gather_state_kernel, grid, dim3(256), 0, st, meta, src, aos, ncp, nvar);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_exchange(const ExchangeParams *p, int nblk, hipStream_t st) {
+    hipLaunchKernelGGL(exchange_bc_kernel, dim3(nblk), dim3(256), 0, st, *p);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_cfl(const BlockMeta *meta, const TileDesc *tiles, int ntiles, int ti, int tj, const double *W,
+                           const double *geom, int64_t ncp, double *dt, unsigned long long *dtmin, hipStream_t st) {
+    hipLaunchKernelGGL(cfl_kernel, dim3(ntiles), dim3(256), 0, st, meta, tiles, ti, tj, W, geom, ncp, dt, dtmin);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_set_global_dt(const BlockMeta *meta, int nblk, int max_int, double *dt,
+                                     const unsigned long long *dtmin, hipStream_t st) {
+    dim3 grid((max_int + 255) / 256, nblk);
+    hipLaunchKernelGGL(set_global_dt_kernel, grid, dim3(256), 0, st, meta, dt, dtmin);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_update(const BlockMeta *meta, int nblk, int max_int, double *W, double *U, const double *Uo,
+                              double *dU, double *dt, int64_t ncp, double omega, int lts, unsigned long long *err,
+                              hipStream_t st) {
+    dim3 grid((max_int + 255) / 256, nblk);
+    hipLaunchKernelGGL(update_kernel, grid, dim3(256), 0, st, meta, W, U, Uo, dU, dt, ncp, omega, lts, err);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_copy_interior(const BlockMeta *meta, int nblk, int max_int, const double *src, double *dst,
+                                     int64_t ncp, hipStream_t st) {
+    dim3 grid((max_int + 255) / 256, nblk);
+    hipLaunchKernelGGL(copy_interior_kernel, grid, dim3(256), 0, st, meta, src, dst, ncp);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_norms(const BlockMeta *meta, const TileDesc *tiles, int ntiles, int ti, int tj,
+                             const double *dU, int64_t ncp, int var, double *part, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(norm_partial_kernel, dim3(ntiles), dim3(256), 0, st, meta, tiles, ti, tj, dU, ncp, var, part);
+    hipLaunchKernelGGL(norm_final_kernel, dim3(1), dim3(256), 0, st, part, ntiles, out);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n,
+                               hipStream_t st) {
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(64), 0, st, dst, v0, v1, n);
+    return hipGetLastError();
+}
+
+hipError_t cffc_launch_stage_fp64(const StageParams *p, int ti, int tj, int fused, hipStream_t st) {
+#define L(TI, TJ)                                                                                          \
+    if (ti == TI && tj == TJ) {                                                                            \
+        if (fused) hipLaunchKernelGGL((stage_kernel<double, TI, TJ, true>), dim3(p->ntiles), dim3(TI * TJ), 0, st, *p); \
+        else hipLaunchKernelGGL((stage_kernel<double, TI, TJ, false>), dim3(p->ntiles), dim3(TI * TJ), 0, st, *p); \
+        return hipGetLastError();                                                                          \
+    }
+    CFFC_TILE_LIST(L)
+#undef L
+    return hipErrorInvalidValue;
+}
+
+}  // extern "C"

@@ -1,0 +1,55 @@
+// euler2d_kernels.hpp -- launch interface between the host runtime (capi.cpp)
+// and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "euler2d_device.hpp"
+#include "euler2d_stage.hpp"
+
+namespace cffc {
+
+struct ExchangeParams {
+    const BlockMeta *meta;
+    const int32_t *nbr;        // 8 per block
+    const int32_t *bc;
+    const double *wo;
+    const double *geom;
+    int64_t ncp;
+    double *W;                 // W buffer whose ghosts are filled
+    const double *U;           // conserved state (stage 1: ghosts = W(U))
+    int32_t from_U;
+    int32_t *stop_flag;
+    unsigned long long *reset_slot;  // set to MILLION (next-step dt slot), may be null
+};
+
+}  // namespace cffc
+
+// tile shapes the stage kernel is instantiated for (TI x TJ interior cells)
+#define CFFC_TILE_LIST(X) X(16, 16) X(32, 8) X(8, 32) X(16, 8) X(8, 8) X(32, 4)
+
+extern "C" {
+hipError_t cffc_launch_geometry(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *xn,
+                                const double *yn, double *geom, int64_t ncp, unsigned int *ndeg, hipStream_t st);
+hipError_t cffc_launch_scatter(const cffc::BlockMeta *meta, int nblk, int max_int, const double *aos, double *W,
+                               int64_t ncp, int kind, double *U, hipStream_t st);
+hipError_t cffc_launch_gather(const cffc::BlockMeta *meta, int nblk, int max_int, const double *src, double *aos,
+                              int64_t ncp, int nvar, hipStream_t st);
+hipError_t cffc_launch_exchange(const cffc::ExchangeParams *p, int nblk, hipStream_t st);
+hipError_t cffc_launch_cfl(const cffc::BlockMeta *meta, const cffc::TileDesc *tiles, int ntiles, int ti, int tj,
+                           const double *W, const double *geom, int64_t ncp, double *dt, unsigned long long *dtmin,
+                           hipStream_t st);
+hipError_t cffc_launch_set_global_dt(const cffc::BlockMeta *meta, int nblk, int max_int, double *dt,
+                                     const unsigned long long *dtmin, hipStream_t st);
+hipError_t cffc_launch_update(const cffc::BlockMeta *meta, int nblk, int max_int, double *W, double *U,
+                              const double *Uo, double *dU, double *dt, int64_t ncp, double omega, int lts,
+                              unsigned long long *err, hipStream_t st);
+hipError_t cffc_launch_copy_interior(const cffc::BlockMeta *meta, int nblk, int max_int, const double *src,
+                                     double *dst, int64_t ncp, hipStream_t st);
+hipError_t cffc_launch_norms(const cffc::BlockMeta *meta, const cffc::TileDesc *tiles, int ntiles, int ti, int tj,
+                             const double *dU, int64_t ncp, int var, double *part, double *out, hipStream_t st);
+hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n,
+                               hipStream_t st);
+hipError_t cffc_launch_stage_fp64(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
+hipError_t cffc_launch_stage_mixed(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
+}

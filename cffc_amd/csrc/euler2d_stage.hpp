@@ -1,0 +1,485 @@
+// euler2d_stage.hpp -- the fused per-stage residual kernel for gfx950.
+//
+// One workgroup owns a TI x TJ tile of interior cells of one block and, for
+// that tile, performs what the reference does per block and stage in
+//   Linear_Reconstruction_LeastSquares   Euler2DQuadSingleBlock.cc:3678-4087
+//   Limiter_Venkatakrishnan              CFD.cc:728-752
+//   dUdt_Multistage_Explicit             Euler2DQuad.cc:1235-1727
+//   Update_Solution_Multistage_Explicit  Euler2DQuadSingleBlock.cc:5300-5507
+//   CFL (next step) and the norms        Euler2DQuadSingleBlock.cc:3019-3167
+// in four LDS-staged phases:
+//   1. load W and the cell centroids of the (TI+4) x (TJ+4) window;
+//   2. least-squares gradient + limiter for the (TI+2) x (TJ+2) cells whose
+//      reconstructions the tile's faces need (the tile plus one ring);
+//   3. Roe fluxes of the (TI+1) x TJ i-faces and TI x (TJ+1) j-faces;
+//   4. per cell: accumulate +W -E +S -N (the reference's order), update.
+// FT is the arithmetic of phases 2-3 (double: FP64 path, bitwise with the
+// reference when compiled with -ffp-contract=off; float: mixed path).  The
+// accumulation, update, time step and norms are always FP64.
+#pragma once
+
+#include "euler2d_device.hpp"
+#include "euler2d_state.hpp"
+
+namespace cffc {
+
+struct StageParams {
+    const BlockMeta *meta;
+    const TileDesc *tiles;
+    int32_t ntiles;
+    int64_t ncp;            // padded cells of all blocks
+    const double *geom;     // GEO_COUNT x ncp
+    const int32_t *bc;      // concatenated BCtypeW/E/S/N
+    const double *wo;       // reference states of the bc entries (AoS, 4)
+    const double *Win;      // W^(s), 4 x ncp, ghosts filled
+    double *Wout;           // W^(s+1) interior (fused) -- may alias Win when !fused
+    double *U;              // conserved U^n (fused path)
+    double *S;              // residual accumulator dUdt[k]
+    const double *S1, *S2;  // dUdt[1], dUdt[2] (granular RK4 stage 4)
+    double *dt;             // per-cell time step (padded)
+    const unsigned long long *dtmin_in;  // global dt (bit pattern) of this step
+    unsigned long long *dtmin_out;       // next step's global dt (atomicMin)
+    double *phi;            // stored limiter values (freeze), 4 x ncp
+    double cfl, omega;
+    int32_t mode;           // StageMode (fused) / granular init mode
+    int32_t phi_mode;       // PhiMode
+    int32_t limiter;        // CFFC_LIMITER_*
+    int32_t lts;            // scalar local time stepping
+    int32_t last_stage;     // fused: produce norms and the next time step
+    unsigned long long *err;   // first failing cell key (atomicMin)
+    double *norm_part;      // [ntiles][3]
+    unsigned int *counter;  // last-workgroup counter
+    double *norm_out;       // [max_steps][3]
+    double *dt_out;         // [max_steps]
+    int32_t *step_ctr;      // step index for norm_out/dt_out
+    int32_t *stop_flag;     // nonzero: converged or failed, skip work
+    double tol;             // convergence threshold on the L2 norm (<= 0: none)
+};
+
+// granular init modes (dUdt_Multistage_Explicit called on its own)
+enum GranularInit { GINIT_ZERO = 0, GINIT_KEEP = 1, GINIT_RK4 = 2 };
+
+// Stencil of Linear_Reconstruction_LeastSquares (SingleBlock.cc:3691-3906)
+// as a mask over the neighbours in the reference's listing order
+// (i-1,j-1) (i,j-1) (i+1,j-1) (i-1,j) (i+1,j) (i-1,j+1) (i,j+1) (i+1,j+1);
+// every reduced stencil of the reference lists its points in that order.
+__device__ __forceinline__ bool bc_extrap(int bc) {
+    return bc == 6000 || bc == 4001 || bc == 4002 || bc == 8001;
+}
+__device__ __forceinline__ unsigned stencil_mask(int i, int j, int ICu, int JCu, int bcw, int bce, int bcs,
+                                                 int bcn) {
+    const int ICl = 2, JCl = 2;
+    if (i == 0 || i == ICu + 2 || j == 0 || j == JCu + 2) return 0u;
+    if (i == 1 && bcw != 0) {
+        if (j == 1 || j == JCu + 1) return 0u;
+        if (bc_extrap(bcw)) return j == JCl ? 0xF8u : j == JCu ? 0x1Fu : 0xFFu;
+        return j == JCl ? 0xD0u : j == JCu ? 0x16u : 0xD6u;
+    }
+    if (i == ICu + 1 && bce != 0) {
+        if (j == 1 || j == JCu + 1) return 0u;
+        if (bc_extrap(bce)) return j == JCl ? 0xF8u : j == JCu ? 0x1Fu : 0xFFu;
+        return j == JCl ? 0x68u : j == JCu ? 0x0Bu : 0x6Bu;
+    }
+    if (j == 1 && bcs != 0) {
+        if (i == 1 || i == ICu + 1) return 0u;
+        if (bc_extrap(bcs)) return i == ICl ? 0xD6u : i == ICu ? 0x6Bu : 0xFFu;
+        return i == ICl ? 0xD0u : i == ICu ? 0x68u : 0xF8u;
+    }
+    if (j == JCu + 1 && bcn != 0) {
+        if (i == 1 || i == ICu + 1) return 0u;
+        if (bc_extrap(bcn)) return i == ICl ? 0xD6u : i == ICu ? 0x6Bu : 0xFFu;
+        return i == ICl ? 0x16u : i == ICu ? 0x0Bu : 0x1Fu;
+    }
+    return 0xFFu;
+}
+
+__device__ __forceinline__ bool face_bc(int bc) { return bc == 7001 || bc == 8001; }
+
+__device__ __forceinline__ unsigned long long dbits(double x) { return (unsigned long long)__double_as_longlong(x); }
+__device__ __forceinline__ double bitsd(unsigned long long u) { return __longlong_as_double((long long)u); }
+
+// W + (phi^dWdx)*dX.x + (phi^dWdy)*dX.y  (Euler2DQuad.cc:1338-1340)
+template <typename FT>
+__device__ __forceinline__ PS<FT> face_state(const PS<FT> &W, const PS<FT> &px, const PS<FT> &py, FT dx, FT dy) {
+    PS<FT> r;
+    r.d = W.d + px.d * dx + py.d * dy;
+    r.vx = W.vx + px.vx * dx + py.vx * dy;
+    r.vy = W.vy + px.vy * dx + py.vy * dy;
+    r.p = W.p + px.p * dx + py.p * dy;
+    return r;
+}
+
+template <typename FT, int TI, int TJ, bool FUSED>
+__global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
+    constexpr int NT = TI * TJ;
+    constexpr int LI = TI + 4, LJ = TJ + 4;
+    constexpr int RI = TI + 2, RJ = TJ + 2;
+    __shared__ FT sW[4][LJ][LI];
+    __shared__ double sX[LJ][LI], sY[LJ][LI];
+    __shared__ FT sPx[4][RJ][RI], sPy[4][RJ][RI];
+    __shared__ FT sFi[4][TJ][TI + 1], sFj[4][TJ + 1][TI];
+    __shared__ double sRed[NT / 64][4];
+    __shared__ int sLast;
+
+    if (p.stop_flag && *p.stop_flag) return;
+    const TileDesc t = p.tiles[blockIdx.x];
+    const BlockMeta m = p.meta[t.b];
+    const int tid = threadIdx.x;
+    const int64_t ncp = p.ncp;
+    const double *G = p.geom;
+    const int ICu = m.ni + 1, JCu = m.nj + 1;
+    const int pi0 = t.i0, pj0 = t.j0;   // padded index of the LDS window origin
+    const int32_t *bcW = p.bc + m.bcoff, *bcE = bcW + m.ncj, *bcS = bcE + m.ncj, *bcN = bcS + m.nci;
+
+    // ---- phase 1: W and centroids of the window ---------------------------
+    for (int q = tid; q < LI * LJ; q += NT) {
+        const int il = q % LI, jl = q / LI;
+        const int ip = pi0 + il, jp = pj0 + jl;
+        if (ip < m.nci && jp < m.ncj) {
+            const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sW[v][jl][il] = (FT)p.Win[v * ncp + c];
+            sX[jl][il] = G[GEO_XC * ncp + c];
+            sY[jl][il] = G[GEO_YC * ncp + c];
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: least-squares reconstruction + limiter -------------------
+    for (int q = tid; q < RI * RJ; q += NT) {
+        const int ir = q % RI, jr = q / RI;
+        const int il = ir + 1, jl = jr + 1;
+        const int ip = pi0 + il, jp = pj0 + jl;
+        if (ip > m.nci - 2 || jp > m.ncj - 2) continue;
+        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+        const unsigned mask = stencil_mask(ip, jp, ICu, JCu, bcW[jp], bcE[jp], bcS[ip], bcN[ip]);
+        const PS<FT> W0 = {sW[0][jl][il], sW[1][jl][il], sW[2][jl][il], sW[3][jl][il]};
+        PS<FT> gx = {FT(0), FT(0), FT(0), FT(0)}, gy = gx, ph = gx;
+        if (mask) {
+            PS<FT> Dx = {FT(0), FT(0), FT(0), FT(0)}, Dy = Dx;
+            FT DxDx = FT(0), DxDy = FT(0), DyDy = FT(0);
+            const double xc0 = sX[jl][il], yc0 = sY[jl][il];
+            int n_pts = 0;
+#pragma unroll
+            for (int s = 0; s < 8; ++s) {
+                if (!(mask & (1u << s))) continue;
+                const int kk = s < 4 ? s : s + 1;  // skip the centre (1,1)
+                const int a = il + (kk % 3) - 1, b = jl + (kk / 3) - 1;
+                const FT dX = (FT)(sX[b][a] - xc0), dY = (FT)(sY[b][a] - yc0);
+                const PS<FT> DU = {sW[0][b][a] - W0.d, sW[1][b][a] - W0.vx, sW[2][b][a] - W0.vy,
+                                   sW[3][b][a] - W0.p};
+                Dx.d += DU.d * dX; Dx.vx += DU.vx * dX; Dx.vy += DU.vy * dX; Dx.p += DU.p * dX;
+                Dy.d += DU.d * dY; Dy.vx += DU.vx * dY; Dy.vy += DU.vy * dY; Dy.p += DU.p * dY;
+                DxDx += dX * dX; DxDy += dX * dY; DyDy += dY * dY;
+                ++n_pts;
+            }
+            const FT np = (FT)n_pts;
+            Dx.d /= np; Dx.vx /= np; Dx.vy /= np; Dx.p /= np;
+            Dy.d /= np; Dy.vx /= np; Dy.vy /= np; Dy.p /= np;
+            DxDx /= np; DxDy /= np; DyDy /= np;
+            const FT det = DxDx * DyDy - DxDy * DxDy;
+            gx.d = (Dx.d * DyDy - Dy.d * DxDy) / det;
+            gx.vx = (Dx.vx * DyDy - Dy.vx * DxDy) / det;
+            gx.vy = (Dx.vy * DyDy - Dy.vy * DxDy) / det;
+            gx.p = (Dx.p * DyDy - Dy.p * DxDy) / det;
+            gy.d = (Dy.d * DxDx - Dx.d * DxDy) / det;
+            gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) / det;
+            gy.vy = (Dy.vy * DxDx - Dx.vy * DxDy) / det;
+            gy.p = (Dy.p * DxDx - Dx.p * DxDy) / det;
+            if (p.phi_mode == PHI_LOAD) {
+                ph.d = (FT)p.phi[c]; ph.vx = (FT)p.phi[ncp + c];
+                ph.vy = (FT)p.phi[2 * ncp + c]; ph.p = (FT)p.phi[3 * ncp + c];
+            } else if (p.limiter == -1) {
+                ph.d = ph.vx = ph.vy = ph.p = FT(1);
+            } else if (p.limiter == 0) {
+                ph.d = ph.vx = ph.vy = ph.p = FT(0);
+            } else {
+                // face-midpoint offsets: E, W (= E of i-1), N, S (= N of j-1)
+                const FT dxE = (FT)(G[GEO_XFE * ncp + c] - xc0), dyE = (FT)(G[GEO_YFE * ncp + c] - yc0);
+                const FT dxW = (FT)(G[GEO_XFE * ncp + c - 1] - xc0), dyW = (FT)(G[GEO_YFE * ncp + c - 1] - yc0);
+                const FT dxN = (FT)(G[GEO_XFN * ncp + c] - xc0), dyN = (FT)(G[GEO_YFN * ncp + c] - yc0);
+                const FT dxS = (FT)(G[GEO_XFN * ncp + c - m.nci] - xc0),
+                         dyS = (FT)(G[GEO_YFN * ncp + c - m.nci] - yc0);
+#pragma unroll
+                for (int n = 0; n < 4; ++n) {
+                    const FT u0 = n == 0 ? W0.d : n == 1 ? W0.vx : n == 2 ? W0.vy : W0.p;
+                    FT umin = u0, umax = u0;
+#pragma unroll
+                    for (int s = 0; s < 8; ++s) {
+                        if (!(mask & (1u << s))) continue;
+                        const int kk = s < 4 ? s : s + 1;
+                        const FT v = sW[n][jl + (kk / 3) - 1][il + (kk % 3) - 1];
+                        umin = dmin(umin, v);
+                        umax = dmax(umax, v);
+                    }
+                    const FT gxn = n == 0 ? gx.d : n == 1 ? gx.vx : n == 2 ? gx.vy : gx.p;
+                    const FT gyn = n == 0 ? gy.d : n == 1 ? gy.vx : n == 2 ? gy.vy : gy.p;
+                    FT phi = FT(1);
+                    phi = venkat_fold(phi, u0 + gxn * dxE + gyn * dyE, u0, umin, umax);
+                    phi = venkat_fold(phi, u0 + gxn * dxW + gyn * dyW, u0, umin, umax);
+                    phi = venkat_fold(phi, u0 + gxn * dxN + gyn * dyN, u0, umin, umax);
+                    phi = venkat_fold(phi, u0 + gxn * dxS + gyn * dyS, u0, umin, umax);
+                    if (n == 0) ph.d = phi; else if (n == 1) ph.vx = phi; else if (n == 2) ph.vy = phi; else ph.p = phi;
+                }
+            }
+            if (p.phi_mode == PHI_COMPUTE_STORE) {
+                const bool own_i = (ip >= pi0 + 2 && ip < pi0 + 2 + TI) || ip == 1 || ip == m.nci - 2;
+                const bool own_j = (jp >= pj0 + 2 && jp < pj0 + 2 + TJ) || jp == 1 || jp == m.ncj - 2;
+                if (own_i && own_j) {
+                    p.phi[c] = ph.d; p.phi[ncp + c] = ph.vx; p.phi[2 * ncp + c] = ph.vy; p.phi[3 * ncp + c] = ph.p;
+                }
+            }
+        } else if (p.phi_mode == PHI_COMPUTE_STORE) {
+            const bool own_i = (ip >= pi0 + 2 && ip < pi0 + 2 + TI) || ip == 1 || ip == m.nci - 2;
+            const bool own_j = (jp >= pj0 + 2 && jp < pj0 + 2 + TJ) || jp == 1 || jp == m.ncj - 2;
+            if (own_i && own_j) { p.phi[c] = 0; p.phi[ncp + c] = 0; p.phi[2 * ncp + c] = 0; p.phi[3 * ncp + c] = 0; }
+        }
+        // phi^dWdx, phi^dWdy (zero stencil: both products are 0*0)
+        sPx[0][jr][ir] = ph.d * gx.d; sPx[1][jr][ir] = ph.vx * gx.vx;
+        sPx[2][jr][ir] = ph.vy * gx.vy; sPx[3][jr][ir] = ph.p * gx.p;
+        sPy[0][jr][ir] = ph.d * gy.d; sPy[1][jr][ir] = ph.vx * gy.vx;
+        sPy[2][jr][ir] = ph.vy * gy.vy; sPy[3][jr][ir] = ph.p * gy.p;
+    }
+    __syncthreads();
+
+    // ---- phase 3: Roe fluxes ------------------------------------------------
+    constexpr int NFI = (TI + 1) * TJ, NFJ = TI * (TJ + 1);
+    for (int q = tid; q < NFI + NFJ; q += NT) {
+        const bool idir = q < NFI;
+        int ipL, jpL, rl_i, rl_j, rr_i, rr_j;  // left cell (padded) and recon coords
+        if (idir) {
+            const int fi = q % (TI + 1), tr = q / (TI + 1);
+            ipL = pi0 + 1 + fi; jpL = pj0 + 2 + tr;
+            if (jpL > JCu || ipL > ICu) continue;
+            rl_i = fi; rl_j = tr + 1; rr_i = fi + 1; rr_j = tr + 1;
+        } else {
+            const int qq = q - NFI;
+            const int tc = qq % TI, fj = qq / TI;
+            ipL = pi0 + 2 + tc; jpL = pj0 + 1 + fj;
+            if (ipL > ICu || jpL > JCu) continue;
+            rl_i = tc + 1; rl_j = fj; rr_i = tc + 1; rr_j = fj + 1;
+        }
+        const int64_t cL = m.coff + (int64_t)jpL * m.nci + ipL;
+        const double nxd = idir ? G[GEO_NEX * ncp + cL] : G[GEO_NNX * ncp + cL];
+        const double nyd = idir ? G[GEO_NEY * ncp + cL] : G[GEO_NNY * ncp + cL];
+        const double xf = idir ? G[GEO_XFE * ncp + cL] : G[GEO_XFN * ncp + cL];
+        const double yf = idir ? G[GEO_YFE * ncp + cL] : G[GEO_YFN * ncp + cL];
+        const FT nx = (FT)nxd, ny = (FT)nyd;
+        auto load_state = [&](int ri, int rj, PS<FT> &W) {
+            const int il = ri + 1, jl = rj + 1;
+            W.d = sW[0][jl][il]; W.vx = sW[1][jl][il]; W.vy = sW[2][jl][il]; W.p = sW[3][jl][il];
+            const PS<FT> px = {sPx[0][rj][ri], sPx[1][rj][ri], sPx[2][rj][ri], sPx[3][rj][ri]};
+            const PS<FT> py = {sPy[0][rj][ri], sPy[1][rj][ri], sPy[2][rj][ri], sPy[3][rj][ri]};
+            W = face_state(W, px, py, (FT)(xf - sX[jl][il]), (FT)(yf - sY[jl][il]));
+        };
+        PS<FT> Wl, Wr;
+        int bcL, bcR, slotL, slotR;
+        if (idir) {
+            bcL = (ipL == 1) ? bcW[jpL] : 0;
+            bcR = (ipL == ICu) ? bcE[jpL] : 0;
+            slotL = jpL; slotR = m.ncj + jpL;
+        } else {
+            bcL = (jpL == 1) ? bcS[ipL] : 0;
+            bcR = (jpL == JCu) ? bcN[ipL] : 0;
+            slotL = 2 * m.ncj + ipL; slotR = 2 * m.ncj + m.nci + ipL;
+        }
+        if (face_bc(bcL)) {
+            // boundary face on the low side: the outside state comes from the
+            // inside face state, normal nfaceW(i+1) = -nfaceE(i) (resp. S, N)
+            load_state(rr_i, rr_j, Wr);
+            if (bcL == 7001) {
+                Wl = reflect(Wr, -nx, -ny);
+            } else {
+                const double *wo = p.wo + 4 * (m.bcoff + slotL);
+                const PS<FT> Wo = {(FT)wo[0], (FT)wo[1], (FT)wo[2], (FT)wo[3]};
+                Wl = bc_characteristic_pressure(Wr, Wo, -nx, -ny);
+            }
+        } else if (face_bc(bcR)) {
+            load_state(rl_i, rl_j, Wl);
+            if (bcR == 7001) {
+                Wr = reflect(Wl, nx, ny);
+            } else {
+                const double *wo = p.wo + 4 * (m.bcoff + slotR);
+                const PS<FT> Wo = {(FT)wo[0], (FT)wo[1], (FT)wo[2], (FT)wo[3]};
+                Wr = bc_characteristic_pressure(Wl, Wo, nx, ny);
+            }
+        } else {
+            load_state(rl_i, rl_j, Wl);
+            load_state(rr_i, rr_j, Wr);
+        }
+        const CS<FT> F = flux_roe_n(Wl, Wr, nx, ny);
+        if (idir) {
+            const int fi = q % (TI + 1), tr = q / (TI + 1);
+            sFi[0][tr][fi] = F.d; sFi[1][tr][fi] = F.mx; sFi[2][tr][fi] = F.my; sFi[3][tr][fi] = F.E;
+        } else {
+            const int qq = q - NFI;
+            const int tc = qq % TI, fj = qq / TI;
+            sFj[0][fj][tc] = F.d; sFj[1][fj][tc] = F.mx; sFj[2][fj][tc] = F.my; sFj[3][fj][tc] = F.E;
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 4: accumulate (+W -E +S -N) and update -------------------------
+    const int tc = tid % TI, tr = tid / TI;
+    const int ip = pi0 + 2 + tc, jp = pj0 + 2 + tr;
+    const bool valid = (ip <= ICu) && (jp <= JCu);
+    double l1 = 0.0, l2 = 0.0, mx = 0.0, dtn = MILLION;
+    if (valid) {
+        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+        const double A = G[GEO_A * ncp + c];
+        const double lW = G[GEO_LE * ncp + c - 1], lE = G[GEO_LE * ncp + c];
+        const double lS = G[GEO_LN * ncp + c - m.nci], lN = G[GEO_LN * ncp + c];
+        const double dtc = p.lts ? p.dt[c] : bitsd(*p.dtmin_in);
+        const double cdt = p.cfl * dtc;
+        double dU[4];
+        if (FUSED) {
+            if (p.mode == MODE_FROM_S) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dU[v] = p.S[v * ncp + c];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dU[v] = 0.0;
+            }
+        } else {
+            if (p.mode == GINIT_KEEP) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dU[v] = p.S[v * ncp + c];
+            } else if (p.mode == GINIT_RK4) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    dU[v] = p.S[v * ncp + c] + 2.0 * p.S1[v * ncp + c] + 2.0 * p.S2[v * ncp + c];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dU[v] = 0.0;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const double FW = (double)sFi[v][tr][tc], FE = (double)sFi[v][tr][tc + 1];
+            const double FS = (double)sFj[v][tr][tc], FN = (double)sFj[v][tr + 1][tc];
+            double d = dU[v];
+            d += cdt * FW * lW / A;
+            d -= cdt * FE * lE / A;
+            d += cdt * FS * lS / A;
+            d -= cdt * FN * lN / A;
+            dU[v] = d;
+        }
+        if (!FUSED) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = dU[v];
+        } else {
+            const double om = p.omega;
+            CS<double> Un;
+            const double U0[4] = {p.U[c], p.U[ncp + c], p.U[2 * ncp + c], p.U[3 * ncp + c]};
+            Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
+            Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
+            bool bad = (Un.d <= 0.0 || Un.E <= 0.0 || U_e(Un) <= 0.0);
+            if (bad && p.lts) {
+                // scalar local time stepping: halve dt and dUdt up to 10 times
+                double f = 1.0, dtl = dtc;
+                for (int n = 1; n <= 10; ++n) {
+                    f = 0.5 * f;
+                    dtl = f * dtl;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) dU[v] = f * dU[v];
+                    Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
+                    Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
+                    if (Un.d > 0.0 && Un.E > 0.0 && U_e(Un) > 0.0) break;
+                }
+                p.dt[c] = dtl;
+                bad = (Un.d <= 0.0 || Un.E <= 0.0 || U_e(Un) <= 0.0);
+            }
+            if (bad) {
+                const unsigned long long key =
+                    ((unsigned long long)t.b << 32) | ((unsigned long long)jp << 16) | (unsigned long long)ip;
+                atomicMin(p.err, key);
+                // the reference returns from the failing stage; later kernels skip
+                if (p.stop_flag) atomicExch(p.stop_flag, 1);
+            }
+            const PS<double> Wn = W_of_U(Un);
+            p.Wout[c] = Wn.d; p.Wout[ncp + c] = Wn.vx; p.Wout[2 * ncp + c] = Wn.vy; p.Wout[3 * ncp + c] = Wn.p;
+            if (p.mode == MODE_ZERO_ACC2) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = p.S[v * ncp + c] + 2.0 * dU[v];
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = dU[v];
+            }
+            if (p.last_stage) {
+                p.U[c] = Un.d; p.U[ncp + c] = Un.mx; p.U[2 * ncp + c] = Un.my; p.U[3 * ncp + c] = Un.E;
+                l1 = fabs(dU[0]);
+                l2 = dU[0] * dU[0];
+                mx = l1;
+                // CFL of the next step, SingleBlock.cc:3036-3053
+                const double d_i = 2.0 * (A / (lE + lW));
+                const double d_j = 2.0 * (A / (lN + lS));
+                const double nWx = -G[GEO_NEX * ncp + c - 1], nWy = -G[GEO_NEY * ncp + c - 1];
+                const double nSx = -G[GEO_NNX * ncp + c - m.nci], nSy = -G[GEO_NNY * ncp + c - m.nci];
+                const double v_i = 0.5 * (Wn.vx * (G[GEO_NEX * ncp + c] - nWx) + Wn.vy * (G[GEO_NEY * ncp + c] - nWy));
+                const double v_j = 0.5 * (Wn.vx * (G[GEO_NNX * ncp + c] - nSx) + Wn.vy * (G[GEO_NNY * ncp + c] - nSy));
+                const double a = W_a(Wn);
+                dtn = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
+                if (p.lts) p.dt[c] = dtn;
+            }
+        }
+    }
+    if (!(FUSED) || !p.last_stage) return;
+
+    // ---- last stage: workgroup partials of the norms and the next dt --------
+    // wavefront reductions with DPP/permute shuffles, then across the 4 waves
+    for (int off = 32; off > 0; off >>= 1) {
+        l1 += __shfl_xor(l1, off);
+        l2 += __shfl_xor(l2, off);
+        mx = dmax(mx, __shfl_xor(mx, off));
+        dtn = dmin(dtn, __shfl_xor(dtn, off));
+    }
+    const int wv = tid >> 6;
+    if ((tid & 63) == 0) { sRed[wv][0] = l1; sRed[wv][1] = l2; sRed[wv][2] = mx; sRed[wv][3] = dtn; }
+    __syncthreads();
+    if (tid == 0) {
+        double a = sRed[0][0], b = sRed[0][1], cmx = sRed[0][2], dmn = sRed[0][3];
+        for (int w = 1; w < NT / 64; ++w) {
+            a += sRed[w][0]; b += sRed[w][1]; cmx = dmax(cmx, sRed[w][2]); dmn = dmin(dmn, sRed[w][3]);
+        }
+        p.norm_part[3 * blockIdx.x + 0] = a;
+        p.norm_part[3 * blockIdx.x + 1] = b;
+        p.norm_part[3 * blockIdx.x + 2] = cmx;
+        atomicMin(p.dtmin_out, dbits(dmn));  // positive doubles order like their bits
+        __threadfence();
+        const unsigned prev = atomicAdd(p.counter, 1u);
+        sLast = (prev == (unsigned)(p.ntiles - 1));
+    }
+    __syncthreads();
+    if (!sLast) return;
+    // the last workgroup to finish reduces all partials in tile order
+    __threadfence();
+    double a = 0.0, b = 0.0, cmx = 0.0;
+    for (int q = tid; q < p.ntiles; q += NT) {
+        a += p.norm_part[3 * q + 0];
+        b += p.norm_part[3 * q + 1];
+        cmx = dmax(cmx, p.norm_part[3 * q + 2]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off);
+        b += __shfl_xor(b, off);
+        cmx = dmax(cmx, __shfl_xor(cmx, off));
+    }
+    __syncthreads();
+    if ((tid & 63) == 0) { sRed[wv][0] = a; sRed[wv][1] = b; sRed[wv][2] = cmx; }
+    __syncthreads();
+    if (tid == 0) {
+        double A0 = sRed[0][0], B0 = sRed[0][1], C0 = sRed[0][2];
+        for (int w = 1; w < NT / 64; ++w) { A0 += sRed[w][0]; B0 += sRed[w][1]; C0 = dmax(C0, sRed[w][2]); }
+        const int s = *p.step_ctr;
+        const double l2n = sqrt(B0);
+        p.norm_out[3 * s + 0] = A0;
+        p.norm_out[3 * s + 1] = l2n;
+        p.norm_out[3 * s + 2] = C0;
+        p.dt_out[s] = bitsd(*p.dtmin_in);
+        *p.step_ctr = s + 1;
+        *p.counter = 0u;
+        if ((p.tol > 0.0 && l2n <= p.tol) || *p.err != ~0ull) *p.stop_flag = 1;
+    }
+}
+
+}  // namespace cffc

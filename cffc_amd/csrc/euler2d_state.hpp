@@ -1,0 +1,241 @@
+// euler2d_state.hpp -- ideal-gas 2D Euler state algebra for the gfx950 kernels.
+//
+// Templated on the arithmetic type T (double for the FP64 path, float for the
+// FP32 reconstruction / limiter / Roe flux of the mixed path).  Every formula
+// keeps the reference's operation order so that, built with
+// -ffp-contract=off, the FP64 instantiation reproduces the reference's
+// arithmetic bit for bit:
+//   state functions          src_2D/Euler2D/Euler2DState.h:1033-1200, 1511-1560,
+//                            1748-1860, 2153-2161, 2702-2720, 2968-2986, 3429-3435
+//   RoeAverage, Reflect,     src_2D/Euler2D/Euler2DState.cc:654-683, 751-787,
+//   BC_Characteristic_Pressure, HartenFix, FluxRoe_x/_n
+//                            1383-1465, 1872-1905, 2311-2353, 2446-2489
+//   HartenFixPos/Neg, Limiter_Venkatakrishnan
+//                            src_2D/CFD/CFD.cc:605-643, 728-752
+// min/max are the reference's ternaries (Math.h:73-92), not fmin/fmax, so
+// signed zeros and NaNs propagate the same way.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#define CFFC_HD __host__ __device__ __forceinline__
+
+namespace cffc {
+
+template <typename T> struct Consts {
+    static constexpr T ZERO = T(0), ONE = T(1), TWO = T(2), HALF = T(0.5);
+    static constexpr T TOLER = T(0.0000001);
+};
+
+// gamma = 1.4; gm1 and gm1i exactly as Euler2DState.cc:13-15 evaluates them
+template <typename T> CFFC_HD T gam() { return T(1.40); }
+template <typename T> CFFC_HD T gm1() { return T(1.40 - 1.0); }
+template <typename T> CFFC_HD T gm1i() { return T(1.0 / (1.40 - 1.0)); }
+
+template <typename T> CFFC_HD T dmax(T x, T y) { return (x > y) ? x : y; }
+template <typename T> CFFC_HD T dmin(T x, T y) { return (x < y) ? x : y; }
+template <typename T> CFFC_HD T sqr(T x) { return x * x; }
+template <typename T> CFFC_HD T tabs(T x) { return fabs(x); }
+template <typename T> CFFC_HD T tsqrt(T x) { return sqrt(x); }
+template <> CFFC_HD float tsqrt<float>(float x) { return sqrtf(x); }
+template <> CFFC_HD float tabs<float>(float x) { return fabsf(x); }
+
+template <typename T> struct PS { T d, vx, vy, p; };   // Euler2D_pState
+template <typename T> struct CS { T d, mx, my, E; };   // Euler2D_cState
+
+template <typename T> CFFC_HD T get(const PS<T> &w, int n) {
+    return n == 1 ? w.d : n == 2 ? w.vx : n == 3 ? w.vy : w.p;
+}
+
+template <typename T> CFFC_HD PS<T> W_of_U(const CS<T> &U) {
+    PS<T> W;
+    W.d = U.d;
+    W.vx = U.mx / U.d;
+    W.vy = U.my / U.d;
+    W.p = gm1<T>() * (U.E - Consts<T>::HALF * (U.mx * U.mx + U.my * U.my) / U.d);
+    return W;
+}
+template <typename T> CFFC_HD CS<T> U_of_W(const PS<T> &W) {
+    CS<T> U;
+    U.d = W.d;
+    U.mx = W.d * W.vx;
+    U.my = W.d * W.vy;
+    U.E = W.p * gm1i<T>() + Consts<T>::HALF * W.d * (W.vx * W.vx + W.vy * W.vy);
+    return U;
+}
+template <typename T> CFFC_HD T W_h(const PS<T> &W) {
+    return gam<T>() * W.p / (gm1<T>() * W.d) + Consts<T>::HALF * (W.vx * W.vx + W.vy * W.vy);
+}
+template <typename T> CFFC_HD T W_H(const PS<T> &W) {
+    return gam<T>() * gm1i<T>() * W.p + Consts<T>::HALF * W.d * (W.vx * W.vx + W.vy * W.vy);
+}
+template <typename T> CFFC_HD T W_a(const PS<T> &W) { return tsqrt(gam<T>() * W.p / W.d); }
+template <typename T> CFFC_HD T W_a2(const PS<T> &W) { return gam<T>() * W.p / W.d; }
+template <typename T> CFFC_HD T U_e(const CS<T> &U) {
+    return U.E / U.d - Consts<T>::HALF * (U.mx * U.mx + U.my * U.my) / sqr(U.d);
+}
+template <typename T> CFFC_HD CS<T> W_Fx(const PS<T> &W) {
+    CS<T> F;
+    F.d = W.d * W.vx;
+    F.mx = W.d * sqr(W.vx) + W.p;
+    F.my = W.d * W.vx * W.vy;
+    F.E = W.vx * W_H(W);
+    return F;
+}
+
+// HartenFixPos / HartenFixNeg, CFD.cc:605-643
+template <typename T> CFFC_HD T harten_pos(T la, T ll, T lr) {
+    typedef Consts<T> K;
+    T delta = dmax(K::ZERO, K::TWO * (lr - ll));
+    if (tabs(la) > delta || delta < K::TOLER) delta = K::ZERO;
+    else delta = K::HALF * (sqr(la) / delta + delta) - tabs(la);
+    return K::HALF * (la + tabs(la) + delta);
+}
+template <typename T> CFFC_HD T harten_neg(T la, T ll, T lr) {
+    typedef Consts<T> K;
+    T delta = dmax(K::ZERO, K::TWO * (lr - ll));
+    if (tabs(la) > delta || delta < K::TOLER) delta = K::ZERO;
+    else delta = K::HALF * (sqr(la) / delta + delta) - tabs(la);
+    return K::HALF * (la - tabs(la) - delta);
+}
+
+// RoeAverage, Euler2DState.cc:654-683
+template <typename T> CFFC_HD PS<T> roe_average(const PS<T> &Wl, const PS<T> &Wr) {
+    typedef Consts<T> K;
+    T hl = W_h(Wl), hr = W_h(Wr);
+    T sdl = tsqrt(Wl.d), sdr = tsqrt(Wr.d);
+    PS<T> Wa;
+    Wa.d = sdl * sdr;
+    Wa.vx = (sdl * Wl.vx + sdr * Wr.vx) / (sdl + sdr);
+    Wa.vy = (sdl * Wl.vy + sdr * Wr.vy) / (sdl + sdr);
+    T ha = (sdl * hl + sdr * hr) / (sdl + sdr);
+    T aa2 = gm1<T>() * (ha - K::HALF * (sqr(Wa.vx) + sqr(Wa.vy)));
+    Wa.p = Wa.d * aa2 / gam<T>();
+    return Wa;
+}
+
+// FluxRoe_x, Euler2DState.cc:2311-2353.  The wave loop is unrolled; the
+// left/right eigenvectors are lp_x/rc_x (Euler2DState.h:2702-2720,
+// 2968-2986) and the wave strengths are the pState inner product
+// (Euler2DState.h:1222-1224), all in the reference's order.
+template <typename T> CFFC_HD CS<T> flux_roe_x(const PS<T> &Wl, const PS<T> &Wr) {
+    typedef Consts<T> K;
+    PS<T> Wa = roe_average(Wl, Wr);
+    PS<T> dW = {Wr.d - Wl.d, Wr.vx - Wl.vx, Wr.vy - Wl.vy, Wr.p - Wl.p};
+    T cl = W_a(Wl), cr = W_a(Wr), ca = W_a(Wa);
+    // lambdas (vx-c, vx, vx, vx+c) of the left, right and average states
+    T la1 = Wa.vx - ca, la4 = Wa.vx + ca;
+    T ll1 = Wl.vx - cl, ll4 = Wl.vx + cl;
+    T lr1 = Wr.vx - cr, lr4 = Wr.vx + cr;
+    const bool left = (Wa.vx >= K::ZERO);
+    CS<T> F = left ? W_Fx(Wl) : W_Fx(Wr);
+    T w1, w2, w4;
+    if (left) {
+        w1 = harten_neg(la1, ll1, lr1);
+        w2 = K::HALF * (Wa.vx - tabs(Wa.vx));
+        w4 = harten_neg(la4, ll4, lr4);
+    } else {
+        w1 = harten_pos(la1, ll1, lr1);
+        w2 = K::HALF * (Wa.vx + tabs(Wa.vx));
+        w4 = harten_pos(la4, ll4, lr4);
+    }
+    const T sg = left ? T(1) : T(-1);
+    // wave 1: lp = (0, -HALF*d/c, 0, HALF/c^2), rc = (1, u-c, v, h-u*c)
+    if (left ? (w1 < K::ZERO) : (w1 > K::ZERO)) {
+        T lp2 = -K::HALF * Wa.d / ca, lp4 = K::HALF / sqr(ca);
+        T a = w1 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p);
+        T h = W_h(Wa);
+        F.d += sg * (a * K::ONE);   // sg = +/-1 flips exactly (Flux += / Flux -=)
+        F.mx += sg * (a * (Wa.vx - ca));
+        F.my += sg * (a * Wa.vy);
+        F.E += sg * (a * (h - Wa.vx * ca));
+    }
+    // waves 2 and 3 share the eigenvalue u
+    if (left ? (w2 < K::ZERO) : (w2 > K::ZERO)) {
+        T lp4 = -K::ONE / W_a2(Wa);
+        T a = w2 * (K::ONE * dW.d + (K::ZERO * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p);
+        F.d += sg * (a * K::ONE);
+        F.mx += sg * (a * Wa.vx);
+        F.my += sg * (a * Wa.vy);
+        F.E += sg * (a * (K::HALF * (Wa.vx * Wa.vx + Wa.vy * Wa.vy)));
+        T b = w2 * (K::ZERO * dW.d + (K::ZERO * dW.vx + K::ONE * dW.vy) + K::ZERO * dW.p);
+        F.d += sg * (b * K::ZERO);
+        F.mx += sg * (b * K::ZERO);
+        F.my += sg * (b * Wa.d);
+        F.E += sg * (b * (Wa.d * Wa.vy));
+    }
+    if (left ? (w4 < K::ZERO) : (w4 > K::ZERO)) {
+        T lp2 = K::HALF * Wa.d / ca, lp4 = K::HALF / sqr(ca);
+        T a = w4 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p);
+        T h = W_h(Wa);
+        F.d += sg * (a * K::ONE);
+        F.mx += sg * (a * (Wa.vx + ca));
+        F.my += sg * (a * Wa.vy);
+        F.E += sg * (a * (h + Wa.vx * ca));
+    }
+    return F;
+}
+
+// FluxRoe_n, Euler2DState.cc:2446-2489
+template <typename T> CFFC_HD CS<T> flux_roe_n(const PS<T> &Wl, const PS<T> &Wr, T nx, T ny) {
+    PS<T> a = {Wl.d, Wl.vx * nx + Wl.vy * ny, -Wl.vx * ny + Wl.vy * nx, Wl.p};
+    PS<T> b = {Wr.d, Wr.vx * nx + Wr.vy * ny, -Wr.vx * ny + Wr.vy * nx, Wr.p};
+    CS<T> Fr = flux_roe_x(a, b);
+    CS<T> F = {Fr.d, Fr.mx * nx - Fr.my * ny, Fr.mx * ny + Fr.my * nx, Fr.E};
+    return F;
+}
+
+// Reflect, Euler2DState.cc:751-787
+template <typename T> CFFC_HD PS<T> reflect(const PS<T> &W, T nx, T ny) {
+    T ur = W.vx * nx + W.vy * ny;
+    T vr = -W.vx * ny + W.vy * nx;
+    ur = -ur;
+    PS<T> r = {W.d, ur * nx - vr * ny, ur * ny + vr * nx, W.p};
+    return r;
+}
+
+// BC_Characteristic_Pressure, Euler2DState.cc:1383-1465
+template <typename T> CFFC_HD PS<T> bc_characteristic_pressure(const PS<T> &Wi, const PS<T> &Wo, T nx, T ny) {
+    typedef Consts<T> K;
+    PS<T> Wir = {Wi.d, Wi.vx * nx + Wi.vy * ny, -Wi.vx * ny + Wi.vy * nx, Wi.p};
+    PS<T> Wor = {Wo.d, Wo.vx * nx + Wo.vy * ny, -Wo.vx * ny + Wo.vy * nx, Wo.p};
+    T mi = Wir.vx / W_a(Wir);
+    T db, pb, ab, ubr, vbr;
+    if (mi >= K::ONE) {
+        db = Wir.d; ubr = Wir.vx; vbr = Wir.vy; pb = Wir.p;
+    } else if (mi >= K::ZERO) {
+        pb = Wor.p;
+        db = Wir.d * pow(pb / Wir.p, K::ONE / gam<T>());
+        ab = tsqrt(gam<T>() * pb / db);
+        ubr = Wir.vx + K::TWO * (W_a(Wir) - ab) * gm1i<T>();
+        vbr = Wir.vy;
+    } else if (mi >= -K::ONE) {
+        pb = Wor.p;
+        db = Wor.d;
+        vbr = Wor.vy;
+        ab = tsqrt(gam<T>() * pb / db);
+        ubr = Wir.vx + K::TWO * (W_a(Wir) - ab) * gm1i<T>();
+    } else {
+        db = Wor.d; ubr = Wor.vx; vbr = Wor.vy; pb = Wor.p;
+    }
+    PS<T> r = {db, ubr * nx - vbr * ny, ubr * ny + vbr * nx, pb};
+    return r;
+}
+
+// Limiter_Venkatakrishnan for one quadrature value, CFD.cc:728-752.  The
+// caller folds phi = min(phi, f) over the four face midpoints.  When
+// y >= 2.5 (and not huge) the rational term is provably >= 1 in floating
+// point and cannot lower phi <= 1, so its division is skipped; the result is
+// unchanged bit for bit.
+template <typename T> CFFC_HD T venkat_fold(T phi, T uq, T u0, T u0Min, T u0Max) {
+    typedef Consts<T> K;
+    T du = uq - u0;
+    T y;
+    if (du > K::TOLER) y = (u0Max - u0) / du;
+    else if (du < -K::TOLER) y = (u0Min - u0) / du;
+    else return dmin(phi, K::ONE);
+    if (y >= T(2.5) && y <= T(1.0e6)) return phi;
+    return dmin(phi, (sqr(y) + K::TWO * y) / (sqr(y) + y + K::TWO));
+}
+
+}  // namespace cffc

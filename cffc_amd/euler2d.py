@@ -1,0 +1,221 @@
+"""Python host interface of the MI355X 2D Euler path, over the C ABI of
+libcffc_hip.so (include/cffc_euler2d.h).
+
+Euler2DQuadBlocks mirrors the reference's multi-block call surface
+(src_2D/Euler2D/Euler2DQuadMultiBlock.cc): the methods carry the names of the
+reference functions they replace and return the reference's int codes.
+Everything runs on the GPU; there is no CPU fallback -- constructing a solver
+without the built library or without a HIP device raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcffc_hip.so")
+
+_lib = None
+
+
+class CffcError(RuntimeError):
+    pass
+
+
+def library_path():
+    return LIB_PATH
+
+
+def load():
+    """Load libcffc_hip.so (built in-tree by __graft_entry__.build() or
+    `make -C cffc_amd/csrc`)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise CffcError("libcffc_hip.so is not built (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double
+    pd = ctypes.POINTER(ctypes.c_double)
+    L.cffc_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    L.cffc_create.restype = i32
+    L.cffc_destroy.argtypes = [vp]
+    L.cffc_destroy.restype = None
+    L.cffc_num_cells.argtypes = [vp]
+    L.cffc_num_cells.restype = i64
+    for n in ("cffc_set_W", "cffc_set_U", "cffc_get_W", "cffc_get_U", "cffc_get_dt"):
+        getattr(L, n).argtypes = [vp, pd, i64]
+        getattr(L, n).restype = i32
+    L.cffc_get_dUdt.argtypes = [vp, i32, pd, i64]
+    L.cffc_exchange_and_bcs.argtypes = [vp]
+    L.cffc_cfl.argtypes = [vp, pd, i32]
+    L.cffc_dudt_stage.argtypes = [vp, i32]
+    L.cffc_update_stage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64)]
+    L.cffc_residual_norms.argtypes = [vp, i32, pd]
+    L.cffc_freeze_limiters.argtypes = [vp, i32]
+    L.cffc_set_store_limiters.argtypes = [vp, i32]
+    L.cffc_run_steps.argtypes = [vp, i32, d, pd, pd, ctypes.POINTER(ctypes.c_int)]
+    L.cffc_run_steps_graph.argtypes = [vp, i32, pd, pd]
+    L.cffc_synchronize.argtypes = [vp]
+    L.cffc_stream.argtypes = [vp]
+    L.cffc_stream.restype = vp
+    L.cffc_set_tile.argtypes = [vp, i32, i32]
+    L.cffc_get_tile.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.cffc_last_error.argtypes = []
+    L.cffc_last_error.restype = ctypes.c_char_p
+    _lib = L
+    return L
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _check(rc, what):
+    if rc < 0:
+        raise CffcError("%s failed (%d): %s" % (what, rc, load().cffc_last_error().decode()))
+    return rc
+
+
+class Euler2DQuadBlocks:
+    """All solution blocks of one mesh resident on the GPU (the reference's
+    Euler2D_Quad_Block array plus its AdaptiveBlock2D_List)."""
+
+    def __init__(self, mesh, scheme=None):
+        L = load()
+        self.mesh = mesh
+        self.scheme = scheme or abi.Scheme()
+        self._keep = []
+        self._md = abi.mesh_desc(mesh, self._keep)
+        self._sd = abi.scheme_desc(self.scheme)
+        h = ctypes.c_void_p()
+        _check(L.cffc_create(ctypes.byref(self._md), ctypes.byref(self._sd), ctypes.byref(h)), "cffc_create")
+        self.h = h
+        self.ncell = L.cffc_num_cells(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().cffc_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- state -----------------------------------------------------------------
+    def set_W(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.ncell, 4)
+        _check(load().cffc_set_W(self.h, _p(W), self.ncell), "cffc_set_W")
+
+    def set_U(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.ncell, 4)
+        _check(load().cffc_set_U(self.h, _p(U), self.ncell), "cffc_set_U")
+
+    def get_W(self):
+        out = np.empty((self.ncell, 4))
+        _check(load().cffc_get_W(self.h, _p(out), self.ncell), "cffc_get_W")
+        return out
+
+    def get_U(self):
+        out = np.empty((self.ncell, 4))
+        _check(load().cffc_get_U(self.h, _p(out), self.ncell), "cffc_get_U")
+        return out
+
+    def get_dUdt(self, k=0):
+        out = np.empty((self.ncell, 4))
+        _check(load().cffc_get_dUdt(self.h, k, _p(out), self.ncell), "cffc_get_dUdt")
+        return out
+
+    def get_dt(self):
+        out = np.empty(self.ncell)
+        _check(load().cffc_get_dt(self.h, _p(out), self.ncell), "cffc_get_dt")
+        return out
+
+    # ---- the reference's multi-block calls ---------------------------------------
+    def Send_All_Messages_and_BCs(self):
+        """Send_All_Messages(..., NUM_VAR_EULER2D, OFF) followed by BCs()."""
+        return _check(load().cffc_exchange_and_bcs(self.h), "cffc_exchange_and_bcs")
+
+    def CFL(self, set_global=True):
+        """CFL() + CFFC_Minimum_MPI() (+ Set_Global_TimeStep()); returns dTime."""
+        d = ctypes.c_double()
+        _check(load().cffc_cfl(self.h, ctypes.byref(d), 1 if set_global else 0), "cffc_cfl")
+        return d.value
+
+    def dUdt_Multistage_Explicit(self, i_stage):
+        return _check(load().cffc_dudt_stage(self.h, i_stage), "cffc_dudt_stage")
+
+    def Update_Solution_Multistage_Explicit(self, i_stage):
+        """Returns the reference's code: 0, or the i-index of the first cell
+        with non-positive density/energy (self.bad_cell then holds
+        (block, j, i))."""
+        bad = ctypes.c_int64(-1)
+        rc = _check(load().cffc_update_stage(self.h, i_stage, ctypes.byref(bad)), "cffc_update_stage")
+        v = bad.value
+        self.bad_cell = None if v < 0 else (v >> 32, (v >> 16) & 0xFFFF, v & 0xFFFF)
+        return rc
+
+    def Residual_Norms(self, var=1):
+        """(L1, L2, max) of dUdt[..][0][var] (L1/L2/Max_Norm_Residual)."""
+        out = np.zeros(3)
+        _check(load().cffc_residual_norms(self.h, var, _p(out)), "cffc_residual_norms")
+        return out
+
+    def Freeze_Limiters(self, freeze=True):
+        return _check(load().cffc_freeze_limiters(self.h, 1 if freeze else 0), "cffc_freeze_limiters")
+
+    def store_limiters(self, on=True):
+        return _check(load().cffc_set_store_limiters(self.h, 1 if on else 0), "cffc_set_store_limiters")
+
+    # ---- the fused time-marching loop (Euler2DQuadSolver explicit branch) --------
+    def run_steps(self, nsteps, tol=0.0, graph=False):
+        """Returns (rc, steps_done, norms[steps,3], dt[steps])."""
+        L = load()
+        norms = np.zeros((max(nsteps, 1), 3))
+        dts = np.zeros(max(nsteps, 1))
+        if graph:
+            rc = _check(L.cffc_run_steps_graph(self.h, nsteps, _p(norms), _p(dts)), "cffc_run_steps_graph")
+            n = nsteps
+        else:
+            done = ctypes.c_int(0)
+            rc = _check(L.cffc_run_steps(self.h, nsteps, tol, _p(norms), _p(dts), ctypes.byref(done)),
+                        "cffc_run_steps")
+            n = done.value
+        return rc, n, norms[:n], dts[:n]
+
+    def synchronize(self):
+        _check(load().cffc_synchronize(self.h), "cffc_synchronize")
+
+    def stream(self):
+        return load().cffc_stream(self.h)
+
+    def set_tile(self, ti, tj):
+        _check(load().cffc_set_tile(self.h, ti, tj), "cffc_set_tile")
+
+    def get_tile(self):
+        a, b = ctypes.c_int(), ctypes.c_int()
+        load().cffc_get_tile(self.h, ctypes.byref(a), ctypes.byref(b))
+        return a.value, b.value
+
+    def step_reference_order(self, nsteps):
+        """The reference's loop (Euler2DQuadSolvers.cc:549-898) driven call by
+        call through the granular entry points; returns (rc, norms)."""
+        s = self.scheme
+        norms = []
+        for n in range(nsteps + 1):
+            self.CFL(set_global=(s.local_time_stepping == abi.GLOBAL_TIME_STEPPING))
+            if n > 0:
+                norms.append(self.Residual_Norms(1))
+            if n == nsteps:
+                break
+            for st in range(1, s.n_stage + 1):
+                self.Send_All_Messages_and_BCs()
+                self.dUdt_Multistage_Explicit(st)
+                rc = self.Update_Solution_Multistage_Explicit(st)
+                if rc:
+                    return rc, np.array(norms)
+        return 0, np.array(norms)

@@ -1,0 +1,219 @@
+/*
+ * cffc_euler2d.h -- C ABI of the MI355X (gfx950) implementation of CFFC's
+ * explicit finite-volume residual/update path for the 2D Euler equations on
+ * quadrilateral multi-block meshes.
+ *
+ * The reference calls this path through C++ free functions that loop over the
+ * blocks of a processor (src_2D/Euler2D/Euler2DQuadMultiBlock.cc) and through
+ * the message-passing template Send_All_Messages
+ * (src_2D/AMR/AdaptiveBlock2D_MessagePassing.h:7883).  Each entry point below
+ * replaces one of those calls; its comment cites the reference interface.
+ * All arguments are plain pointers and sizes: no C++ types cross this
+ * boundary and no exception escapes it.
+ *
+ * Threading: one host thread per context.  All work is ordered on the
+ * context's HIP stream; entry points that return host data synchronise it.
+ *
+ * Return codes: 0 = success; > 0 = a failure the reference reports through
+ * its int return (e.g. the i-index of the first cell whose density, total
+ * energy or internal energy became non-positive, as returned by
+ * Update_Solution_Multistage_Explicit, Euler2DQuadSingleBlock.cc:5375-5385);
+ * < 0 = a CFFC_E_* runtime error below.
+ */
+#ifndef CFFC_EULER2D_H
+#define CFFC_EULER2D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- runtime error codes (negative returns) ---------------------------- */
+#define CFFC_E_ARG        (-1)  /* invalid argument / unsupported option      */
+#define CFFC_E_HIP        (-2)  /* a HIP runtime call failed                  */
+#define CFFC_E_NOMEM      (-3)  /* device allocation failed                   */
+#define CFFC_E_TOPOLOGY   (-4)  /* inconsistent neighbour table (the reference
+                                   reports these as 2200-2205 from
+                                   Load_Send_Message_Buffers_NoResChange)     */
+#define CFFC_E_NODEVICE   (-5)  /* no HIP device visible                      */
+
+/* ---- reference constants (src_2D/CFD/CFD.h) ---------------------------- */
+/* boundary-condition codes, CFD.h:273-351 */
+#define CFFC_BC_NONE                    0
+#define CFFC_BC_CONSTANT_EXTRAPOLATION  4001
+#define CFFC_BC_PERIODIC                6000
+#define CFFC_BC_REFLECTION              7001
+#define CFFC_BC_CHARACTERISTIC          8001
+#define CFFC_BC_FIXED                   11000
+/* time integration, CFD.h:651-659 */
+#define CFFC_TIME_STEPPING_EXPLICIT_EULER                 0
+#define CFFC_TIME_STEPPING_EXPLICIT_PREDICTOR_CORRECTOR   3
+#define CFFC_TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING   7
+#define CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA           8
+/* limiters, CFD.h:749-761 */
+#define CFFC_LIMITER_ONE             (-1)
+#define CFFC_LIMITER_ZERO              0
+#define CFFC_LIMITER_VENKATAKRISHNAN   9
+/* flux function, CFD.h:769 */
+#define CFFC_FLUX_FUNCTION_ROE         2
+/* reconstruction, CFD.h */
+#define CFFC_RECONSTRUCTION_LEAST_SQUARES 2
+/* time-step type, CFD.h:801-802 */
+#define CFFC_GLOBAL_TIME_STEPPING        0
+#define CFFC_SCALAR_LOCAL_TIME_STEPPING  1
+
+/* arithmetic of the reconstruction / limiter / Roe flux (the state, the
+   update, the time-step minimum and the residual norms are always FP64) */
+#define CFFC_FLUX_FP64 0
+#define CFFC_FLUX_FP32 1
+
+/* neighbour slots in cffc_mesh_desc.nbr */
+enum { CFFC_NBR_N = 0, CFFC_NBR_S, CFFC_NBR_E, CFFC_NBR_W,
+       CFFC_NBR_NE, CFFC_NBR_NW, CFFC_NBR_SE, CFFC_NBR_SW, CFFC_NBR_COUNT };
+
+/*
+ * Multi-block mesh: the data the reference keeps in Grid2D_Quad_Block_HO
+ * (nodes, BCtypeN/S/E/W; HO_Grid2DQuad.h:264-330), in the solution block's
+ * boundary reference states WoN/S/E/W (Euler2DQuad.h:140-205) and in the
+ * same-level neighbour fields nN/nS/.../infoX[0].blknum of
+ * AdaptiveBlock2D_List (AdaptiveBlock.h:820).
+ *
+ * Block b has ni[b] x nj[b] interior cells and nghost (= 2) ghost layers, so
+ * nci = ni+2*nghost, ncj = nj+2*nghost cells including ghosts and
+ * (nci+1) x (ncj+1) nodes.  Arrays of all blocks are concatenated in block
+ * order; within a block the i index runs fastest.
+ *   xnode/ynode : (nci+1)*(ncj+1) node coordinates per block, ghost nodes
+ *                 included (node (i,j) is the SW corner of cell (i,j)).
+ *   nbr         : 8 ints per block (CFFC_NBR_*), the same-level neighbour
+ *                 block index or -1.  Blocks must share orientation.
+ *   bc          : per block BCtypeW[ncj], BCtypeE[ncj], BCtypeS[nci],
+ *                 BCtypeN[nci] (ghost entries included).
+ *   wo          : the reference state (rho,u,v,p) for each bc entry, same
+ *                 order, 4 doubles per entry (used by FIXED/CHARACTERISTIC).
+ */
+typedef struct cffc_mesh_desc {
+    int32_t        nblk;
+    int32_t        nghost;
+    const int32_t *ni;
+    const int32_t *nj;
+    const double  *xnode;
+    const double  *ynode;
+    const int32_t *nbr;
+    const int32_t *bc;
+    const double  *wo;
+} cffc_mesh_desc;
+
+/* The scheme fields of Euler2D_Input_Parameters read on this path
+   (Euler2DInput.h:57-118). */
+typedef struct cffc_scheme_desc {
+    int32_t i_time_integration;   /* CFFC_TIME_STEPPING_*                   */
+    int32_t n_stage;              /* N_Stage                                */
+    double  cfl_number;           /* CFL_Number                             */
+    int32_t i_reconstruction;     /* CFFC_RECONSTRUCTION_LEAST_SQUARES      */
+    int32_t i_limiter;            /* CFFC_LIMITER_*                         */
+    int32_t i_flux_function;      /* CFFC_FLUX_FUNCTION_ROE                 */
+    int32_t local_time_stepping;  /* CFFC_GLOBAL/SCALAR_LOCAL_TIME_STEPPING */
+    int32_t flux_precision;       /* CFFC_FLUX_FP64 / CFFC_FLUX_FP32        */
+} cffc_scheme_desc;
+
+typedef struct cffc_ctx cffc_ctx;
+
+/* Allocate the device state for a mesh and scheme (the reference's
+   Allocate + Create_Initial_Solution_Blocks, Euler2DQuadMultiBlock.cc:22-33,
+   AMR.h:39).  Geometry is derived from the nodes on the device side. */
+int  cffc_create(const cffc_mesh_desc *mesh, const cffc_scheme_desc *scheme,
+                 cffc_ctx **out);
+void cffc_destroy(cffc_ctx *ctx);
+
+/* Number of interior cells over all blocks (the unit of the state arrays). */
+int64_t cffc_num_cells(const cffc_ctx *ctx);
+
+/* Host <-> device copies of the interior state, AoS, 4 doubles per cell,
+   blocks in order, i fastest.  set_W sets W then U = U(W) (the reference ICs,
+   Euler2DQuadSingleBlock.cc:1689); set_U sets U then W = W(U) (the restart
+   path, operator>> Euler2DQuad.h:1340-1399). */
+int  cffc_set_W(cffc_ctx *ctx, const double *W, int64_t ncell);
+int  cffc_set_U(cffc_ctx *ctx, const double *U, int64_t ncell);
+int  cffc_get_W(cffc_ctx *ctx, double *W, int64_t ncell);
+int  cffc_get_U(cffc_ctx *ctx, double *U, int64_t ncell);
+/* Residual storage dUdt[k] of the interior cells (AoS, 4 doubles/cell).
+   k follows the reference's k_residual (Euler2DQuad.cc:1249-1279). */
+int  cffc_get_dUdt(cffc_ctx *ctx, int k, double *dUdt, int64_t ncell);
+/* Per-cell time step dt of the interior cells. */
+int  cffc_get_dt(cffc_ctx *ctx, double *dt, int64_t ncell);
+
+/* == Send_All_Messages(blocks, list, NUM_VAR_EULER2D, OFF)
+      (AdaptiveBlock2D_MessagePassing.h:7883-8020) followed by
+      BCs(blocks, list, IP) (Euler2DQuadMultiBlock.cc:937): fills both ghost
+      layers of every block from its same-level neighbours (faces and
+      corners), then applies the physical boundary conditions. */
+int  cffc_exchange_and_bcs(cffc_ctx *ctx);
+
+/* == CFL(blocks, list, IP) (Euler2DQuadMultiBlock.cc:963) followed by
+      CFFC_Minimum_MPI (MPI.h:184) and, when set_global != 0,
+      Set_Global_TimeStep (Euler2DQuadMultiBlock.cc:994). */
+int  cffc_cfl(cffc_ctx *ctx, double *dt_min, int set_global);
+
+/* == dUdt_Multistage_Explicit(blocks, global, list, IP, i_stage)
+      (Euler2DQuadMultiBlock.cc:1346 -> Euler2DQuad.cc:1235). */
+int  cffc_dudt_stage(cffc_ctx *ctx, int i_stage);
+
+/* == Update_Solution_Multistage_Explicit(blocks, list, IP, i_stage)
+      (Euler2DQuadMultiBlock.cc:1383 -> Euler2DQuadSingleBlock.cc:5300).
+      Returns the reference's code (> 0: i-index of the first failing cell);
+      bad_cell, if not NULL, receives block*2^32 + j*2^16 + i of that cell. */
+int  cffc_update_stage(cffc_ctx *ctx, int i_stage, int64_t *bad_cell);
+
+/* == L1_Norm_Residual, L2_Norm_Residual, Max_Norm_Residual
+      (Euler2DQuadMultiBlock.cc:1017-1104) of dUdt[..][0][var], var in 1..4.
+      out = {L1, L2, max}. */
+int  cffc_residual_norms(cffc_ctx *ctx, int var, double out[3]);
+
+/* == Freeze_Limiters / Evaluate_Limiters (Euler2DQuadMultiBlock.cc:1190-1218).
+      Freezing reuses the limiter values of the last evaluated stage. */
+int  cffc_freeze_limiters(cffc_ctx *ctx, int freeze);
+/* Keep the last stage's limiter values stored during cffc_run_steps so that
+   a later cffc_freeze_limiters (the reference freezes once the residual falls
+   below Freeze_Limiter_Residual_Level, Euler2DQuadSolvers.cc:796-803) has
+   them; costs one extra 32-byte write per cell on the last stage. */
+int  cffc_set_store_limiters(cffc_ctx *ctx, int on);
+
+/* The explicit time-marching loop of Euler2DQuadSolver
+   (Euler2DQuadSolvers.cc:549-898) for a fixed number of steps, with the
+   stages fused (exchange/BC kernel + one fused residual/update kernel per
+   stage; the next step's time step and the residual norms are produced by
+   the last stage).  norms (may be NULL) receives {L1,L2,max} of the density
+   residual per step (nsteps*3 doubles, the reference's lagged values: entry
+   n is the norm printed at the top of step n+1).  dt_out (may be NULL)
+   receives the global dt of each step.  Stops early (returning the number of
+   steps taken in *steps_done) when the L2 norm drops to tol (tol <= 0: never). */
+int  cffc_run_steps(cffc_ctx *ctx, int nsteps, double tol, double *norms,
+                    double *dt_out, int *steps_done);
+
+/* Same as cffc_run_steps but without host synchronisation per step: the
+   steps are captured once into a hipGraph and replayed.  norms/dt_out are
+   copied back at the end.  tol is ignored. */
+int  cffc_run_steps_graph(cffc_ctx *ctx, int nsteps, double *norms,
+                          double *dt_out);
+
+/* Wait for all work on the context's stream. */
+int  cffc_synchronize(cffc_ctx *ctx);
+
+/* HIP stream of the context (a hipStream_t), for callers that time kernels
+   with their own events on the stream the kernels run on. */
+void *cffc_stream(cffc_ctx *ctx);
+
+/* Tile configuration of the fused stage kernel, chosen by the autotuner or
+   set here: tile_i x tile_j interior cells per workgroup. */
+int  cffc_set_tile(cffc_ctx *ctx, int tile_i, int tile_j);
+int  cffc_get_tile(const cffc_ctx *ctx, int *tile_i, int *tile_j);
+
+/* Last error message of this thread (static storage). */
+const char *cffc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CFFC_EULER2D_H */

@@ -1,0 +1,259 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes bindings of the CPU oracle.
+
+The oracle (oracle/euler2d_oracle.c) is a plain-C restatement of CFFC's 2D
+Euler explicit residual/update path.  Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg may import this module; the product path
+(cffc_amd) never does.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_KAT = os.path.join(HERE, "_ref", "ref_kat")
+
+_lib = None
+
+
+def build(force=False):
+    """Compile liboracle.so (gcc, -O2 -ffp-contract=off)."""
+    if force or not os.path.exists(LIB_PATH):
+        subprocess.run(["make", "-C", HERE, "-B" if force else "-s", "liboracle.so"], check=True,
+                       stdout=subprocess.DEVNULL)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        d = ctypes.c_double
+        pd = ctypes.POINTER(ctypes.c_double)
+        vp = ctypes.c_void_p
+        L.orc_flux_roe_n.argtypes = [pd, pd, d, d, pd]
+        L.orc_flux_roe_x.argtypes = [pd, pd, pd]
+        L.orc_roe_average.argtypes = [pd, pd, pd]
+        L.orc_limiter_venkatakrishnan.argtypes = [pd, d, d, d]
+        L.orc_limiter_venkatakrishnan.restype = d
+        L.orc_harten_fix_pos.argtypes = [d, d, d]
+        L.orc_harten_fix_pos.restype = d
+        L.orc_harten_fix_neg.argtypes = [d, d, d]
+        L.orc_harten_fix_neg.restype = d
+        L.orc_reflect.argtypes = [pd, d, d, pd]
+        L.orc_bc_characteristic_pressure.argtypes = [pd, pd, d, d, pd]
+        L.orc_W_of_U.argtypes = [pd, pd]
+        L.orc_U_of_W.argtypes = [pd, pd]
+        L.orc_Fx.argtypes = [pd, pd]
+        L.orc_runge_kutta.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_runge_kutta.restype = d
+        L.orc_multistage_optimal_smoothing.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.orc_multistage_optimal_smoothing.restype = d
+        L.orc_quad_centroid.argtypes = [pd, pd, pd]
+        L.orc_create.argtypes = [vp, vp, ctypes.POINTER(vp)]
+        L.orc_destroy.argtypes = [vp]
+        L.orc_num_cells.argtypes = [vp]
+        L.orc_num_cells.restype = ctypes.c_int64
+        for name in ("orc_set_W", "orc_set_U", "orc_get_W", "orc_get_U", "orc_get_dt"):
+            getattr(L, name).argtypes = [vp, pd]
+        L.orc_get_dUdt.argtypes = [vp, ctypes.c_int, pd]
+        L.orc_get_recon.argtypes = [vp, ctypes.c_int, pd]
+        L.orc_get_block_W_padded.argtypes = [vp, ctypes.c_int, pd]
+        L.orc_get_block_geometry.argtypes = [vp, ctypes.c_int, pd, pd, pd]
+        L.orc_exchange_and_bcs.argtypes = [vp]
+        L.orc_cfl.argtypes = [vp, pd, ctypes.c_int]
+        L.orc_dudt_stage.argtypes = [vp, ctypes.c_int]
+        L.orc_update_stage.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        L.orc_residual_norms.argtypes = [vp, ctypes.c_int, pd]
+        L.orc_freeze_limiters.argtypes = [vp, ctypes.c_int]
+        L.orc_run.argtypes = [vp, ctypes.c_int, d, ctypes.c_int, pd, pd, ctypes.POINTER(ctypes.c_int)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _arr(x, n=4):
+    a = np.ascontiguousarray(np.asarray(x, dtype=np.float64))
+    assert a.size == n
+    return a
+
+
+# ---- scalar functions ------------------------------------------------------
+def flux_roe_n(Wl, Wr, nx, ny):
+    out = np.zeros(4)
+    lib().orc_flux_roe_n(_p(_arr(Wl)), _p(_arr(Wr)), nx, ny, _p(out))
+    return out
+
+
+def flux_roe_x(Wl, Wr):
+    out = np.zeros(4)
+    lib().orc_flux_roe_x(_p(_arr(Wl)), _p(_arr(Wr)), _p(out))
+    return out
+
+
+def roe_average(Wl, Wr):
+    out = np.zeros(4)
+    lib().orc_roe_average(_p(_arr(Wl)), _p(_arr(Wr)), _p(out))
+    return out
+
+
+def limiter_venkatakrishnan(uq, u0, umin, umax):
+    return lib().orc_limiter_venkatakrishnan(_p(_arr(uq)), u0, umin, umax)
+
+
+def harten_fix_pos(la, ll, lr):
+    return lib().orc_harten_fix_pos(la, ll, lr)
+
+
+def harten_fix_neg(la, ll, lr):
+    return lib().orc_harten_fix_neg(la, ll, lr)
+
+
+def reflect(W, nx, ny):
+    out = np.zeros(4)
+    lib().orc_reflect(_p(_arr(W)), nx, ny, _p(out))
+    return out
+
+
+def bc_characteristic_pressure(Wi, Wo, nx, ny):
+    out = np.zeros(4)
+    lib().orc_bc_characteristic_pressure(_p(_arr(Wi)), _p(_arr(Wo)), nx, ny, _p(out))
+    return out
+
+
+def W_of_U(U):
+    out = np.zeros(4)
+    lib().orc_W_of_U(_p(_arr(U)), _p(out))
+    return out
+
+
+def U_of_W(W):
+    out = np.zeros(4)
+    lib().orc_U_of_W(_p(_arr(W)), _p(out))
+    return out
+
+
+def Fx(W):
+    out = np.zeros(4)
+    lib().orc_Fx(_p(_arr(W)), _p(out))
+    return out
+
+
+def runge_kutta(i, n):
+    return lib().orc_runge_kutta(i, n)
+
+
+def multistage_optimal_smoothing(i, n, limiter):
+    return lib().orc_multistage_optimal_smoothing(i, n, limiter)
+
+
+def quad_centroid(x4, y4):
+    out = np.zeros(3)
+    lib().orc_quad_centroid(_p(_arr(x4)), _p(_arr(y4)), _p(out))
+    return out
+
+
+# ---- multi-block solver -----------------------------------------------------
+class Oracle:
+    """The oracle's multi-block solver on a cffc_amd.mesh.Mesh."""
+
+    def __init__(self, mesh, scheme):
+        from cffc_amd import abi  # ctypes mirrors of the descriptor structs
+        self._keep = []
+        md = abi.mesh_desc(mesh, self._keep)
+        sd = abi.scheme_desc(scheme)
+        self._md, self._sd = md, sd
+        h = ctypes.c_void_p()
+        rc = lib().orc_create(ctypes.byref(md), ctypes.byref(sd), ctypes.byref(h))
+        if rc != 0:
+            raise ValueError("orc_create failed: %d" % rc)
+        self.h = h
+        self.ncell = lib().orc_num_cells(h)
+        self.mesh = mesh
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_destroy(self.h)
+            self.h = None
+
+    def set_W(self, W):
+        W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.ncell, 4)
+        lib().orc_set_W(self.h, _p(W))
+
+    def set_U(self, U):
+        U = np.ascontiguousarray(U, dtype=np.float64).reshape(self.ncell, 4)
+        lib().orc_set_U(self.h, _p(U))
+
+    def _get(self, fn, *args):
+        out = np.zeros((self.ncell, 4))
+        fn(self.h, *args, _p(out))
+        return out
+
+    def get_W(self):
+        return self._get(lib().orc_get_W)
+
+    def get_U(self):
+        return self._get(lib().orc_get_U)
+
+    def get_dUdt(self, k=0):
+        return self._get(lib().orc_get_dUdt, k)
+
+    def get_recon(self, which):
+        """which: 0 = dWdx, 1 = dWdy, 2 = phi (interior cells)."""
+        return self._get(lib().orc_get_recon, which)
+
+    def get_dt(self):
+        out = np.zeros(self.ncell)
+        lib().orc_get_dt(self.h, _p(out))
+        return out
+
+    def block_W_padded(self, b):
+        nci, ncj = self.mesh.ni[b] + 4, self.mesh.nj[b] + 4
+        out = np.zeros((ncj, nci, 4))
+        lib().orc_get_block_W_padded(self.h, b, _p(out))
+        return out
+
+    def block_geometry(self, b):
+        nci, ncj = self.mesh.ni[b] + 4, self.mesh.nj[b] + 4
+        xc, yc, A = (np.zeros((ncj, nci)) for _ in range(3))
+        lib().orc_get_block_geometry(self.h, b, _p(xc), _p(yc), _p(A))
+        return xc, yc, A
+
+    def exchange_and_bcs(self):
+        return lib().orc_exchange_and_bcs(self.h)
+
+    def cfl(self, set_global=True):
+        d = ctypes.c_double()
+        lib().orc_cfl(self.h, ctypes.byref(d), 1 if set_global else 0)
+        return d.value
+
+    def dudt_stage(self, i_stage):
+        return lib().orc_dudt_stage(self.h, i_stage)
+
+    def update_stage(self, i_stage):
+        bad = ctypes.c_int64(0)
+        rc = lib().orc_update_stage(self.h, i_stage, ctypes.byref(bad))
+        return rc, bad.value
+
+    def residual_norms(self, var=1):
+        out = np.zeros(3)
+        lib().orc_residual_norms(self.h, var, _p(out))
+        return out
+
+    def freeze_limiters(self, freeze=True):
+        lib().orc_freeze_limiters(self.h, 1 if freeze else 0)
+
+    def run(self, nsteps, tol=0.0, nthreads=1):
+        """Explicit time-marching loop; returns (rc, steps_done, norms[n,3], dt[n])."""
+        norms = np.zeros((max(nsteps, 1), 3))
+        dts = np.zeros(max(nsteps, 1))
+        done = ctypes.c_int(0)
+        rc = lib().orc_run(self.h, nsteps, tol, nthreads, _p(norms), _p(dts), ctypes.byref(done))
+        n = done.value
+        return rc, n, norms[:n], dts[:n]

@@ -1,0 +1,1364 @@
+/*
+ * euler2d_oracle.c -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement, in plain C, of CFFC's explicit finite-volume residual and
+ * multistage update for 2D Euler on quadrilateral multi-block meshes.  It is
+ * the checker for the HIP implementation and nothing else: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path never calls it.
+ *
+ * Every function follows the reference's operation order so that, compiled
+ * with -O2 -ffp-contract=off (strict IEEE double, no FMA contraction), it
+ * reproduces the reference's arithmetic bit for bit.  Each function cites the
+ * reference file:line it restates.  Pinning (see tests/):
+ *   - state conversions, Roe flux, Harten fix, Venkatakrishnan limiter,
+ *     Reflect, BC_Characteristic_Pressure, Runge_Kutta: bitwise against
+ *     known-answer vectors written by the reference's own Euler2DState.cc /
+ *     CFD.cc compiled from source (oracle/_ref) and committed under
+ *     tests/golden/;
+ *   - block-level functions (LSQ reconstruction, residual, update, CFL,
+ *     norms, BCs, ghost exchange): against closed forms and invariants
+ *     (linear-field exactness of the LSQ gradient, free-stream preservation
+ *     on jittered meshes, discrete conservation, the diagonal mirror symmetry
+ *     of the 4-quadrant problem, the uniform-mesh CFL closed form, the RK4
+ *     combination, brute-force ghost copies, multi-block vs single-block
+ *     equivalence).  The full reference binary needs LAPACK/Fortran
+ *     libraries the image lacks, so no end-to-end reference trajectory pins
+ *     these; the residual loop's accumulation order is pinned only by the
+ *     single-block equivalence and conservation tests.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#include <pthread.h>
+
+#include "../include/cffc_euler2d.h"
+
+/* ---- constants: src_2D/Math/Math.h:40-53, Physics/GasConstants.h:66 ---- */
+#define ZERO 0.0
+#define ONE 1.0
+#define TWO 2.0
+#define SIX 6.0
+#define HALF 0.50
+#define MILLION 1000000.00
+#define TOLER 0.0000001
+#define NANO 0.000000001
+#define GAMMA_AIR 1.40
+
+/* Euler2DState.cc:13-15 */
+static const double g = GAMMA_AIR;
+static const double gm1 = GAMMA_AIR - ONE;
+static const double gm1i = ONE / (GAMMA_AIR - ONE);
+
+/* Math.h:73-92 */
+static inline double dmax(double x, double y) { return (x > y) ? x : y; }
+static inline double dmin(double x, double y) { return (x < y) ? x : y; }
+static inline double sqr(double x) { return x * x; }
+
+typedef struct { double d, vx, vy, p; } pst; /* Euler2D_pState */
+typedef struct { double d, mx, my, E; } cst; /* Euler2D_cState */
+
+static inline double pget(const pst *w, int n) { return ((const double *)w)[n - 1]; }
+static inline double *pref(pst *w, int n) { return ((double *)w) + (n - 1); }
+
+/* ---- state functions: Euler2DState.h ------------------------------------ */
+/* Euler2D_cState::W() Euler2DState.h:3429-3435, v() :1511, p() :1530 */
+static inline pst W_of_U(cst U) {
+    pst W;
+    W.d = U.d;
+    W.vx = U.mx / U.d;
+    W.vy = U.my / U.d;
+    W.p = gm1 * (U.E - HALF * (U.mx * U.mx + U.my * U.my) / U.d);
+    return W;
+}
+/* Euler2D_pState::U() Euler2DState.h:1748-1762, dv() :1132, E() :1059 */
+static inline cst U_of_W(pst W) {
+    cst U;
+    U.d = W.d;
+    U.mx = W.d * W.vx;
+    U.my = W.d * W.vy;
+    U.E = W.p * gm1i + HALF * W.d * (W.vx * W.vx + W.vy * W.vy);
+    return U;
+}
+/* h() Euler2DState.h:1070 */
+static inline double W_h(pst W) { return g * W.p / (gm1 * W.d) + HALF * (W.vx * W.vx + W.vy * W.vy); }
+/* H() :1081 */
+static inline double W_H(pst W) { return g * gm1i * W.p + HALF * W.d * (W.vx * W.vx + W.vy * W.vy); }
+/* a() :1092, a2() :1103 */
+static inline double W_a(pst W) { return sqrt(g * W.p / W.d); }
+static inline double W_a2(pst W) { return g * W.p / W.d; }
+/* cState e() :1556 */
+static inline double U_e(cst U) { return U.E / U.d - HALF * (U.mx * U.mx + U.my * U.my) / sqr(U.d); }
+
+/* Fx() Euler2DState.h:1845-1860 */
+static inline cst W_Fx(pst W) {
+    cst F;
+    F.d = W.d * W.vx;
+    F.mx = W.d * sqr(W.vx) + W.p;
+    F.my = W.d * W.vx * W.vy;
+    F.E = W.vx * W_H(W);
+    return F;
+}
+/* lambda_x() Euler2DState.h:2158-2161 */
+static inline pst W_lambda_x(pst W) {
+    double c = W_a(W);
+    pst l = {W.vx - c, W.vx, W.vx, W.vx + c};
+    return l;
+}
+/* lp_x(index) Euler2DState.h:2968-2986 */
+static inline pst W_lp_x(pst W, int index) {
+    double c;
+    pst r;
+    switch (index) {
+    case 2: r.d = ONE; r.vx = ZERO; r.vy = ZERO; r.p = -ONE / W_a2(W); return r;
+    case 3: r.d = ZERO; r.vx = ZERO; r.vy = ONE; r.p = ZERO; return r;
+    case 4: c = W_a(W); r.d = ZERO; r.vx = HALF * W.d / c; r.vy = ZERO; r.p = HALF / sqr(c); return r;
+    default: c = W_a(W); r.d = ZERO; r.vx = -HALF * W.d / c; r.vy = ZERO; r.p = HALF / sqr(c); return r;
+    }
+}
+/* rc_x(index) Euler2DState.h:2702-2720 */
+static inline cst W_rc_x(pst W, int index) {
+    double c;
+    cst r;
+    switch (index) {
+    case 2: r.d = ONE; r.mx = W.vx; r.my = W.vy; r.E = HALF * (W.vx * W.vx + W.vy * W.vy); return r;
+    case 3: r.d = ZERO; r.mx = ZERO; r.my = W.d; r.E = W.d * W.vy; return r;
+    case 4: c = W_a(W); r.d = ONE; r.mx = W.vx + c; r.my = W.vy; r.E = W_h(W) + W.vx * c; return r;
+    default: c = W_a(W); r.d = ONE; r.mx = W.vx - c; r.my = W.vy; r.E = W_h(W) - W.vx * c; return r;
+    }
+}
+/* pState inner product, Euler2DState.h:1222-1224 (Vector2D * :362) */
+static inline double pdot(pst a, pst b) { return a.d * b.d + (a.vx * b.vx + a.vy * b.vy) + a.p * b.p; }
+
+/* ---- CFD.cc ------------------------------------------------------------- */
+/* HartenFixPos CFD.cc:605-619 */
+static double harten_fix_pos(double la, double ll, double lr) {
+    double delta = dmax(ZERO, TWO * (lr - ll));
+    if (fabs(la) > delta || delta < TOLER) {
+        delta = ZERO;
+    } else {
+        delta = HALF * (sqr(la) / delta + delta) - fabs(la);
+    }
+    return HALF * (la + fabs(la) + delta);
+}
+/* HartenFixNeg CFD.cc:629-643 */
+static double harten_fix_neg(double la, double ll, double lr) {
+    double delta = dmax(ZERO, TWO * (lr - ll));
+    if (fabs(la) > delta || delta < TOLER) {
+        delta = ZERO;
+    } else {
+        delta = HALF * (sqr(la) / delta + delta) - fabs(la);
+    }
+    return HALF * (la - fabs(la) - delta);
+}
+/* Limiter_Venkatakrishnan CFD.cc:728-752 */
+static double limiter_venkatakrishnan(const double *uQuad, double u0, double u0Min, double u0Max, int nQuad) {
+    double phi = ONE, y;
+    for (int i = 0; i <= nQuad - 1; i++) {
+        if (uQuad[i] - u0 > TOLER) {
+            y = (u0Max - u0) / (uQuad[i] - u0);
+            phi = dmin(phi, (sqr(y) + TWO * y) / (sqr(y) + y + TWO));
+        } else if (uQuad[i] - u0 < -TOLER) {
+            y = (u0Min - u0) / (uQuad[i] - u0);
+            phi = dmin(phi, (sqr(y) + TWO * y) / (sqr(y) + y + TWO));
+        } else {
+            phi = dmin(phi, ONE);
+        }
+    }
+    return phi;
+}
+/* Runge_Kutta CFD.cc:901-939 */
+static double runge_kutta(int I_stage, int N_stage) {
+    double beta;
+    switch (N_stage) {
+    case 1: beta = ONE; break;
+    case 2: beta = ONE; if (I_stage == 2) beta = HALF * beta; break;
+    case 4:
+        beta = ONE;
+        if (I_stage == 1) beta = HALF * beta;
+        if (I_stage == 2) beta = HALF * beta;
+        if (I_stage == 4) beta = beta / SIX;
+        break;
+    case 5:
+        beta = ONE;
+        if (I_stage == 1) beta *= 0.059;
+        if (I_stage == 2) beta *= 0.14;
+        if (I_stage == 3) beta *= 0.273;
+        if (I_stage == 4) beta *= HALF;
+        break;
+    default: beta = ONE; break;
+    }
+    return beta;
+}
+/* MultiStage_Optimally_Smoothing CFD.cc:953-1030: van Leer, Tai & Powell
+   (1989) coefficients for LIMITER_ZERO (first order) and limited schemes. */
+static double multistage_optimal_smoothing(int I_stage, int N_stage, int Limiter_Type) {
+    double beta;
+    switch (N_stage) {
+    case 1: beta = ONE; break;
+    case 2:
+        if (Limiter_Type == CFFC_LIMITER_ZERO) { beta = ONE; if (I_stage == 1) beta = 0.3333 * beta; }
+        else { beta = 0.4693; if (I_stage == 1) beta = 0.4242 * beta; }
+        break;
+    case 3:
+        if (Limiter_Type == CFFC_LIMITER_ZERO) {
+            beta = 1.50; if (I_stage == 1) beta = 0.1481 * beta; if (I_stage == 2) beta = 0.4000 * beta;
+        } else {
+            beta = 0.6936; if (I_stage == 1) beta = 0.1918 * beta; if (I_stage == 2) beta = 0.4929 * beta;
+        }
+        break;
+    case 4:
+        if (Limiter_Type == CFFC_LIMITER_ZERO) {
+            beta = 2.00; if (I_stage == 1) beta = 0.0833 * beta; if (I_stage == 2) beta = 0.2069 * beta;
+            if (I_stage == 3) beta = 0.4265 * beta;
+        } else {
+            beta = 0.9214; if (I_stage == 1) beta = 0.1084 * beta; if (I_stage == 2) beta = 0.2602 * beta;
+            if (I_stage == 3) beta = 0.5052 * beta;
+        }
+        break;
+    case 5:
+        if (Limiter_Type == CFFC_LIMITER_ZERO) {
+            beta = 2.50; if (I_stage == 1) beta = 0.0533 * beta; if (I_stage == 2) beta = 0.1263 * beta;
+            if (I_stage == 3) beta = 0.2375 * beta; if (I_stage == 4) beta = 0.4414 * beta;
+        } else {
+            beta = 1.1508; if (I_stage == 1) beta = 0.0695 * beta; if (I_stage == 2) beta = 0.1602 * beta;
+            if (I_stage == 3) beta = 0.2898 * beta; if (I_stage == 4) beta = 0.5060 * beta;
+        }
+        break;
+    default: beta = ONE; break;
+    }
+    return beta;
+}
+
+/* ---- Euler2DState.cc ---------------------------------------------------- */
+/* RoeAverage Euler2DState.cc:654-683 */
+static pst roe_average(pst Wl, pst Wr) {
+    double hl = W_h(Wl), hr = W_h(Wr);
+    double sdl = sqrt(Wl.d), sdr = sqrt(Wr.d);
+    double da = sdl * sdr;
+    double ua = (sdl * Wl.vx + sdr * Wr.vx) / (sdl + sdr);
+    double va = (sdl * Wl.vy + sdr * Wr.vy) / (sdl + sdr);
+    double ha = (sdl * hl + sdr * hr) / (sdl + sdr);
+    double aa2 = gm1 * (ha - HALF * (sqr(ua) + sqr(va)));
+    double pa = da * aa2 / g;
+    pst Wa = {da, ua, va, pa};
+    return Wa;
+}
+/* HartenFixNeg/Pos (state versions) Euler2DState.cc:1872-1905 */
+static pst harten_fix_neg_state(pst la, pst ll, pst lr) {
+    pst r = {harten_fix_neg(la.d, ll.d, lr.d), HALF * (la.vx - fabs(la.vx)), HALF * (la.vy - fabs(la.vy)),
+             harten_fix_neg(la.p, ll.p, lr.p)};
+    return r;
+}
+static pst harten_fix_pos_state(pst la, pst ll, pst lr) {
+    pst r = {harten_fix_pos(la.d, ll.d, lr.d), HALF * (la.vx + fabs(la.vx)), HALF * (la.vy + fabs(la.vy)),
+             harten_fix_pos(la.p, ll.p, lr.p)};
+    return r;
+}
+/* FluxRoe_x Euler2DState.cc:2311-2353 */
+static cst flux_roe_x(pst Wl, pst Wr) {
+    pst Wa = roe_average(Wl, Wr);
+    pst dWrl = {Wr.d - Wl.d, Wr.vx - Wl.vx, Wr.vy - Wl.vy, Wr.p - Wl.p};
+    pst lambdas_l = W_lambda_x(Wl), lambdas_r = W_lambda_x(Wr), lambdas_a = W_lambda_x(Wa);
+    cst Flux;
+    if (Wa.vx >= ZERO) {
+        Flux = W_Fx(Wl);
+        pst ws = harten_fix_neg_state(lambdas_a, lambdas_l, lambdas_r);
+        for (int i = 1; i <= 4; ++i) {
+            double w = pget(&ws, i);
+            if (w < ZERO) {
+                double s = w * pdot(W_lp_x(Wa, i), dWrl);
+                cst rc = W_rc_x(Wa, i);
+                Flux.d += s * rc.d; Flux.mx += s * rc.mx; Flux.my += s * rc.my; Flux.E += s * rc.E;
+            }
+        }
+    } else {
+        Flux = W_Fx(Wr);
+        pst ws = harten_fix_pos_state(lambdas_a, lambdas_l, lambdas_r);
+        for (int i = 1; i <= 4; ++i) {
+            double w = pget(&ws, i);
+            if (w > ZERO) {
+                double s = w * pdot(W_lp_x(Wa, i), dWrl);
+                cst rc = W_rc_x(Wa, i);
+                Flux.d -= s * rc.d; Flux.mx -= s * rc.mx; Flux.my -= s * rc.my; Flux.E -= s * rc.E;
+            }
+        }
+    }
+    return Flux;
+}
+/* FluxRoe_n Euler2DState.cc:2446-2489 */
+static cst flux_roe_n(pst Wl, pst Wr, double nx, double ny) {
+    double cos_angle = nx, sin_angle = ny;
+    pst Wlr = {Wl.d, Wl.vx * cos_angle + Wl.vy * sin_angle, -Wl.vx * sin_angle + Wl.vy * cos_angle, Wl.p};
+    pst Wrr = {Wr.d, Wr.vx * cos_angle + Wr.vy * sin_angle, -Wr.vx * sin_angle + Wr.vy * cos_angle, Wr.p};
+    cst Fr = flux_roe_x(Wlr, Wrr);
+    cst F = {Fr.d, Fr.mx * cos_angle - Fr.my * sin_angle, Fr.mx * sin_angle + Fr.my * cos_angle, Fr.E};
+    return F;
+}
+/* Reflect Euler2DState.cc:751-787 */
+static pst reflect(pst W, double nx, double ny) {
+    double cos_angle = nx, sin_angle = ny;
+    double dr = W.d;
+    double ur = W.vx * cos_angle + W.vy * sin_angle;
+    double vr = -W.vx * sin_angle + W.vy * cos_angle;
+    double pr = W.p;
+    ur = -ur;
+    pst r = {dr, ur * cos_angle - vr * sin_angle, ur * sin_angle + vr * cos_angle, pr};
+    return r;
+}
+/* BC_Characteristic_Pressure Euler2DState.cc:1383-1465 */
+static pst bc_characteristic_pressure(pst Wi, pst Wo, double nx, double ny) {
+    double cos_angle = nx, sin_angle = ny;
+    pst Wir = {Wi.d, Wi.vx * cos_angle + Wi.vy * sin_angle, -Wi.vx * sin_angle + Wi.vy * cos_angle, Wi.p};
+    pst Wor = {Wo.d, Wo.vx * cos_angle + Wo.vy * sin_angle, -Wo.vx * sin_angle + Wo.vy * cos_angle, Wo.p};
+    double mi = Wir.vx / W_a(Wir);
+    double db, ub, vb, pb, ab, ubr, vbr;
+    if (mi >= ONE) {
+        db = Wir.d; ubr = Wir.vx; vbr = Wir.vy; pb = Wir.p;
+    } else if (mi >= ZERO) {
+        pb = Wor.p;
+        db = Wir.d * pow(pb / Wir.p, ONE / g);
+        ab = sqrt(g * pb / db);
+        ubr = Wir.vx + TWO * (W_a(Wir) - ab) * gm1i;
+        vbr = Wir.vy;
+    } else if (mi >= -ONE) {
+        pb = Wor.p;
+        db = Wor.d;
+        vbr = Wor.vy;
+        ab = sqrt(g * pb / db);
+        ubr = Wir.vx + TWO * (W_a(Wir) - ab) * gm1i;
+    } else {
+        db = Wor.d; ubr = Wor.vx; vbr = Wor.vy; pb = Wor.p;
+    }
+    ub = ubr * cos_angle - vbr * sin_angle;
+    vb = ubr * sin_angle + vbr * cos_angle;
+    pst r = {db, ub, vb, pb};
+    return r;
+}
+
+/* ====================================================================== */
+/* Multi-block data                                                        */
+/* ====================================================================== */
+typedef struct {
+    int ni, nj, ng, nci, ncj;
+    int ICl, ICu, JCl, JCu;
+    const double *xn, *yn; /* (nci+1)*(ncj+1) nodes */
+    double *xc, *yc, *A;   /* per cell (Cell[i][j].Xc, .A) */
+    int *bcW, *bcE, *bcS, *bcN;
+    pst *WoW, *WoE, *WoS, *WoN;
+    pst *W, *dWdx, *dWdy, *phi;
+    cst *U, *Uo, *dUdt[3];
+    double *dt;
+    int nbr[8];
+    int freeze_limiter;
+} oblk;
+
+typedef struct orc_ctx {
+    int nblk;
+    oblk *b;
+    cffc_scheme_desc s;
+    int64_t ncell;
+    double *xnode, *ynode; /* owned copies */
+} orc_ctx;
+
+#define CI(B, i, j) ((j) * (B)->nci + (i))
+#define NI(B, i, j) ((j) * ((B)->nci + 1) + (i))
+
+/* HO_Grid2DQuad.h: face midpoints :1820-1840, lengths :1884-1900,
+   normals :1972-2000 (computed from the nodes on every access) */
+static inline void nodeX(const oblk *B, int i, int j, double *x, double *y) {
+    *x = B->xn[NI(B, i, j)];
+    *y = B->yn[NI(B, i, j)];
+}
+static inline void xfaceE(const oblk *B, int i, int j, double *x, double *y) {
+    double x1, y1, x2, y2;
+    nodeX(B, i + 1, j, &x1, &y1); nodeX(B, i + 1, j + 1, &x2, &y2);
+    *x = HALF * (x1 + x2); *y = HALF * (y1 + y2);
+}
+static inline void xfaceW(const oblk *B, int i, int j, double *x, double *y) {
+    double x1, y1, x2, y2;
+    nodeX(B, i, j, &x1, &y1); nodeX(B, i, j + 1, &x2, &y2);
+    *x = HALF * (x1 + x2); *y = HALF * (y1 + y2);
+}
+static inline void xfaceN(const oblk *B, int i, int j, double *x, double *y) {
+    double x1, y1, x2, y2;
+    nodeX(B, i, j + 1, &x1, &y1); nodeX(B, i + 1, j + 1, &x2, &y2);
+    *x = HALF * (x1 + x2); *y = HALF * (y1 + y2);
+}
+static inline void xfaceS(const oblk *B, int i, int j, double *x, double *y) {
+    double x1, y1, x2, y2;
+    nodeX(B, i, j, &x1, &y1); nodeX(B, i + 1, j, &x2, &y2);
+    *x = HALF * (x1 + x2); *y = HALF * (y1 + y2);
+}
+static inline double vabs(double x, double y) { return sqrt(x * x + y * y); }
+/* lfaceN = abs(Node[i][j+1]-Node[i+1][j+1]), S = abs(Node[i][j]-Node[i+1][j]),
+   E = abs(Node[i+1][j]-Node[i+1][j+1]), W = abs(Node[i][j]-Node[i][j+1]) */
+static inline double lfaceE(const oblk *B, int i, int j) {
+    double x1, y1, x2, y2;
+    nodeX(B, i + 1, j, &x1, &y1); nodeX(B, i + 1, j + 1, &x2, &y2);
+    return vabs(x1 - x2, y1 - y2);
+}
+static inline double lfaceW(const oblk *B, int i, int j) {
+    double x1, y1, x2, y2;
+    nodeX(B, i, j, &x1, &y1); nodeX(B, i, j + 1, &x2, &y2);
+    return vabs(x1 - x2, y1 - y2);
+}
+static inline double lfaceN(const oblk *B, int i, int j) {
+    double x1, y1, x2, y2;
+    nodeX(B, i, j + 1, &x1, &y1); nodeX(B, i + 1, j + 1, &x2, &y2);
+    return vabs(x1 - x2, y1 - y2);
+}
+static inline double lfaceS(const oblk *B, int i, int j) {
+    double x1, y1, x2, y2;
+    nodeX(B, i, j, &x1, &y1); nodeX(B, i + 1, j, &x2, &y2);
+    return vabs(x1 - x2, y1 - y2);
+}
+/* cell_perimeter HO_Grid2DQuad.h:1909-1911 */
+static inline double cell_perimeter(const oblk *B, int i, int j) {
+    return lfaceE(B, i, j) + lfaceN(B, i, j) + lfaceW(B, i, j) + lfaceS(B, i, j);
+}
+static inline void nfaceE(const oblk *B, int i, int j, double *nx, double *ny) {
+    if (lfaceE(B, i, j) > NANO * cell_perimeter(B, i, j)) {
+        double x1, y1, x2, y2;
+        nodeX(B, i + 1, j + 1, &x2, &y2); nodeX(B, i + 1, j, &x1, &y1);
+        double l = vabs(x2 - x1, y2 - y1);
+        *nx = (y2 - y1) / l; *ny = -(x2 - x1) / l;
+    } else { *nx = ONE; *ny = ZERO; }
+}
+static inline void nfaceW(const oblk *B, int i, int j, double *nx, double *ny) {
+    if (lfaceW(B, i, j) > NANO * cell_perimeter(B, i, j)) {
+        double x1, y1, x2, y2;
+        nodeX(B, i, j, &x1, &y1); nodeX(B, i, j + 1, &x2, &y2);
+        double l = vabs(x1 - x2, y1 - y2);
+        *nx = (y1 - y2) / l; *ny = -(x1 - x2) / l;
+    } else { *nx = ONE; *ny = ZERO; }
+}
+static inline void nfaceN(const oblk *B, int i, int j, double *nx, double *ny) {
+    if (lfaceN(B, i, j) > NANO * cell_perimeter(B, i, j)) {
+        double x1, y1, x2, y2;
+        nodeX(B, i, j + 1, &x1, &y1); nodeX(B, i + 1, j + 1, &x2, &y2);
+        double l = vabs(x1 - x2, y1 - y2);
+        *nx = (y1 - y2) / l; *ny = -(x1 - x2) / l;
+    } else { *nx = ONE; *ny = ZERO; }
+}
+static inline void nfaceS(const oblk *B, int i, int j, double *nx, double *ny) {
+    if (lfaceS(B, i, j) > NANO * cell_perimeter(B, i, j)) {
+        double x1, y1, x2, y2;
+        nodeX(B, i + 1, j, &x2, &y2); nodeX(B, i, j, &x1, &y1);
+        double l = vabs(x2 - x1, y2 - y1);
+        *nx = (y2 - y1) / l; *ny = -(x2 - x1) / l;
+    } else { *nx = ONE; *ny = ZERO; }
+}
+/* quadCentroid Math/Math.h:311-350 with vertices SW, SE, NE, NW
+   (quadAreaAndCentroid HO_Grid2DQuad.h:1403-1410, global coordinates since
+   Minimize_Error_Calculation_Of_Geometric_Properties defaults to OFF,
+   HO_Grid2DQuad.cc:80) */
+static void quad_centroid(const oblk *B, int i, int j, double *xc, double *yc, double *A) {
+    double X0x, X0y, X1x, X1y, X2x, X2y, X3x, X3y;
+    nodeX(B, i, j, &X0x, &X0y); nodeX(B, i + 1, j, &X1x, &X1y);
+    nodeX(B, i + 1, j + 1, &X2x, &X2y); nodeX(B, i, j + 1, &X3x, &X3y);
+    double ai, atmp = 0, xtmp = 0, ytmp = 0;
+    ai = X3x * X0y - X0x * X3y; atmp += ai; xtmp += (X0x + X3x) * ai; ytmp += (X0y + X3y) * ai;
+    ai = X0x * X1y - X1x * X0y; atmp += ai; xtmp += (X1x + X0x) * ai; ytmp += (X1y + X0y) * ai;
+    ai = X1x * X2y - X2x * X1y; atmp += ai; xtmp += (X2x + X1x) * ai; ytmp += (X2y + X1y) * ai;
+    ai = X2x * X3y - X3x * X2y; atmp += ai; xtmp += (X3x + X2x) * ai; ytmp += (X3y + X2y) * ai;
+    *A = 0.5 * atmp;
+    if (atmp != 0) { *xc = xtmp / (3 * atmp); *yc = ytmp / (3 * atmp); }
+    else { *xc = 0; *yc = 0; }
+}
+
+/* ---- BCs: Euler2DQuadSingleBlock.cc:2354-3010 (supported types only) ---- */
+static int bc_supported(int bc) {
+    return bc == CFFC_BC_NONE || bc == CFFC_BC_FIXED || bc == CFFC_BC_CONSTANT_EXTRAPOLATION ||
+           bc == CFFC_BC_REFLECTION || bc == CFFC_BC_PERIODIC || bc == CFFC_BC_CHARACTERISTIC;
+}
+
+static void set_ghost_W(oblk *B, int i, int j, pst W) {
+    B->W[CI(B, i, j)] = W;
+    B->U[CI(B, i, j)] = U_of_W(W);
+}
+static void copy_cell(oblk *B, int id, int jd, int is, int js) {
+    B->W[CI(B, id, jd)] = B->W[CI(B, is, js)];
+    B->U[CI(B, id, jd)] = B->U[CI(B, is, js)];
+}
+
+static void block_bcs(oblk *B) {
+    int ICl = B->ICl, ICu = B->ICu, JCl = B->JCl, JCu = B->JCu, Ng = B->ng;
+    double nx, ny;
+    for (int j = JCl - Ng; j <= JCu + Ng; ++j) {
+        /* West :2363-2519 */
+        if ((j >= JCl && j <= JCu) || (j < JCl && B->bcS[ICl - 1] == CFFC_BC_NONE) ||
+            (j > JCu && B->bcN[ICl - 1] == CFFC_BC_NONE)) {
+            switch (B->bcW[j]) {
+            case CFFC_BC_NONE: break;
+            case CFFC_BC_FIXED:
+                for (int gh = 1; gh <= Ng; ++gh) set_ghost_W(B, ICl - gh, j, B->WoW[j]);
+                break;
+            case CFFC_BC_CONSTANT_EXTRAPOLATION:
+                for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, ICl - gh, j, ICl, j);
+                break;
+            case CFFC_BC_REFLECTION:
+                nfaceW(B, ICl, j, &nx, &ny);
+                for (int gh = 1; gh <= Ng; ++gh)
+                    set_ghost_W(B, ICl - gh, j, reflect(B->W[CI(B, ICl + gh - 1, j)], nx, ny));
+                break;
+            case CFFC_BC_PERIODIC:
+                for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, ICl - gh, j, ICu - gh + 1, j);
+                break;
+            case CFFC_BC_CHARACTERISTIC:
+                nfaceW(B, ICl, j, &nx, &ny);
+                set_ghost_W(B, ICl - 1, j, bc_characteristic_pressure(B->W[CI(B, ICl, j)], B->WoW[j], nx, ny));
+                for (int gh = 2; gh <= Ng; ++gh) copy_cell(B, ICl - gh, j, ICl - 1, j);
+                break;
+            default:
+                for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, ICl - gh, j, ICl, j);
+                break;
+            }
+        }
+        /* East :2520-2675 */
+        if ((j >= JCl && j <= JCu) || (j < JCl && B->bcS[ICu + 1] == CFFC_BC_NONE) ||
+            (j > JCu && B->bcN[ICu + 1] == CFFC_BC_NONE)) {
+            switch (B->bcE[j]) {
+            case CFFC_BC_NONE: break;
+            case CFFC_BC_FIXED:
+                for (int gh = 1; gh <= Ng; ++gh) set_ghost_W(B, ICu + gh, j, B->WoE[j]);
+                break;
+            case CFFC_BC_CONSTANT_EXTRAPOLATION:
+                for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, ICu + gh, j, ICu, j);
+                break;
+            case CFFC_BC_REFLECTION:
+                nfaceE(B, ICu, j, &nx, &ny);
+                for (int gh = 1; gh <= Ng; ++gh)
+                    set_ghost_W(B, ICu + gh, j, reflect(B->W[CI(B, ICu - gh + 1, j)], nx, ny));
+                break;
+            case CFFC_BC_PERIODIC:
+                for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, ICu + gh, j, ICl + gh - 1, j);
+                break;
+            case CFFC_BC_CHARACTERISTIC:
+                nfaceE(B, ICu, j, &nx, &ny);
+                set_ghost_W(B, ICu + 1, j, bc_characteristic_pressure(B->W[CI(B, ICu, j)], B->WoE[j], nx, ny));
+                for (int gh = 2; gh <= Ng; ++gh) copy_cell(B, ICu + gh, j, ICu + 1, j);
+                break;
+            default:
+                for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, ICu + gh, j, ICu, j);
+                break;
+            }
+        }
+    }
+    for (int i = ICl - Ng; i <= ICu + Ng; ++i) {
+        /* South :2681-2837 */
+        switch (B->bcS[i]) {
+        case CFFC_BC_NONE: break;
+        case CFFC_BC_FIXED:
+            for (int gh = 1; gh <= Ng; ++gh) set_ghost_W(B, i, JCl - gh, B->WoS[i]);
+            break;
+        case CFFC_BC_CONSTANT_EXTRAPOLATION:
+            for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, i, JCl - gh, i, JCl);
+            break;
+        case CFFC_BC_REFLECTION:
+            nfaceS(B, i, JCl, &nx, &ny);
+            for (int gh = 1; gh <= Ng; ++gh)
+                set_ghost_W(B, i, JCl - gh, reflect(B->W[CI(B, i, JCl + gh - 1)], nx, ny));
+            break;
+        case CFFC_BC_PERIODIC:
+            for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, i, JCl - gh, i, JCu - gh + 1);
+            break;
+        case CFFC_BC_CHARACTERISTIC:
+            nfaceS(B, i, JCl, &nx, &ny);
+            set_ghost_W(B, i, JCl - 1, bc_characteristic_pressure(B->W[CI(B, i, JCl)], B->WoS[i], nx, ny));
+            for (int gh = 2; gh <= Ng; ++gh) copy_cell(B, i, JCl - gh, i, JCl - 1);
+            break;
+        default:
+            for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, i, JCl - gh, i, JCl);
+            break;
+        }
+        /* North :2838-2989 */
+        switch (B->bcN[i]) {
+        case CFFC_BC_NONE: break;
+        case CFFC_BC_FIXED:
+            for (int gh = 1; gh <= Ng; ++gh) set_ghost_W(B, i, JCu + gh, B->WoN[i]);
+            break;
+        case CFFC_BC_CONSTANT_EXTRAPOLATION:
+            for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, i, JCu + gh, i, JCu);
+            break;
+        case CFFC_BC_REFLECTION:
+            nfaceN(B, i, JCu, &nx, &ny);
+            for (int gh = 1; gh <= Ng; ++gh)
+                set_ghost_W(B, i, JCu + gh, reflect(B->W[CI(B, i, JCu - gh + 1)], nx, ny));
+            break;
+        case CFFC_BC_PERIODIC:
+            for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, i, JCu + gh, i, JCl + gh - 1);
+            break;
+        case CFFC_BC_CHARACTERISTIC:
+            nfaceN(B, i, JCu, &nx, &ny);
+            set_ghost_W(B, i, JCu + 1, bc_characteristic_pressure(B->W[CI(B, i, JCu)], B->WoN[i], nx, ny));
+            for (int gh = 2; gh <= Ng; ++gh) copy_cell(B, i, JCu + gh, i, JCu + 1);
+            break;
+        default:
+            for (int gh = 1; gh <= Ng; ++gh) copy_cell(B, i, JCu + gh, i, JCu);
+            break;
+        }
+    }
+}
+
+/* ---- same-level ghost exchange ----------------------------------------
+   Load_Send_Message_Buffers_NoResChange (AdaptiveBlock2D_MessagePassing.h:32),
+   Exchange_Messages_NoResChange (AdaptiveBlock_NoMPI.cc:28),
+   Unload_Receive_Message_Buffers_NoResChange (:5680); LoadSendBuffer /
+   UnloadReceiveBuffer Euler2DQuad.h:1570-1651 (U transferred, then W=W(U)).
+   Faces carry Nghost layers over the interior span, corners Nghost x Nghost.
+   Neighbours share orientation (dimen.i, dimen.j > 0). */
+static void exchange_block(oblk *all, int b) {
+    oblk *B = &all[b];
+    int Ng = B->ng;
+    for (int slot = 0; slot < 8; ++slot) {
+        int nb = B->nbr[slot];
+        if (nb < 0) continue;
+        const oblk *S = &all[nb];
+        int i0, i1, j0, j1; /* destination ranges in B */
+        int di, dj;         /* source index = dest index + (di,dj) */
+        switch (slot) {
+        case CFFC_NBR_N: i0 = B->ICl; i1 = B->ICu; j0 = B->JCu + 1; j1 = B->JCu + Ng; di = S->ICl - B->ICl; dj = S->JCl - (B->JCu + 1); break;
+        case CFFC_NBR_S: i0 = B->ICl; i1 = B->ICu; j0 = B->JCl - Ng; j1 = B->JCl - 1; di = S->ICl - B->ICl; dj = S->JCu - (B->JCl - 1); break;
+        case CFFC_NBR_E: i0 = B->ICu + 1; i1 = B->ICu + Ng; j0 = B->JCl; j1 = B->JCu; di = S->ICl - (B->ICu + 1); dj = S->JCl - B->JCl; break;
+        case CFFC_NBR_W: i0 = B->ICl - Ng; i1 = B->ICl - 1; j0 = B->JCl; j1 = B->JCu; di = S->ICu - (B->ICl - 1); dj = S->JCl - B->JCl; break;
+        case CFFC_NBR_NE: i0 = B->ICu + 1; i1 = B->ICu + Ng; j0 = B->JCu + 1; j1 = B->JCu + Ng; di = S->ICl - (B->ICu + 1); dj = S->JCl - (B->JCu + 1); break;
+        case CFFC_NBR_NW: i0 = B->ICl - Ng; i1 = B->ICl - 1; j0 = B->JCu + 1; j1 = B->JCu + Ng; di = S->ICu - (B->ICl - 1); dj = S->JCl - (B->JCu + 1); break;
+        case CFFC_NBR_SE: i0 = B->ICu + 1; i1 = B->ICu + Ng; j0 = B->JCl - Ng; j1 = B->JCl - 1; di = S->ICl - (B->ICu + 1); dj = S->JCu - (B->JCl - 1); break;
+        default: /* SW */ i0 = B->ICl - Ng; i1 = B->ICl - 1; j0 = B->JCl - Ng; j1 = B->JCl - 1; di = S->ICu - (B->ICl - 1); dj = S->JCu - (B->JCl - 1); break;
+        }
+        for (int j = j0; j <= j1; ++j)
+            for (int i = i0; i <= i1; ++i) {
+                cst U = S->U[CI(S, i + di, j + dj)];
+                B->U[CI(B, i, j)] = U;
+                B->W[CI(B, i, j)] = W_of_U(U);
+            }
+    }
+}
+
+/* ---- LSQ reconstruction: Euler2DQuadSingleBlock.cc:3678-4087 ---------- */
+static void lsq_reconstruct(oblk *B, int i, int j, int Limiter) {
+    int n_pts, ii[8], jj[8];
+    int Ng = B->ng, ICl = B->ICl, ICu = B->ICu, JCl = B->JCl, JCu = B->JCu;
+#define SET(k, a, c) do { ii[k] = (a); jj[k] = (c); } while (0)
+#define FULL8() do { n_pts = 8; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i+1, j-1); SET(3, i-1, j); \
+                     SET(4, i+1, j); SET(5, i-1, j+1); SET(6, i, j+1); SET(7, i+1, j+1); } while (0)
+    int bcw = B->bcW[j], bce = B->bcE[j], bcs = B->bcS[i], bcn = B->bcN[i];
+#define EXTRAP(bc) ((bc) == CFFC_BC_PERIODIC || (bc) == CFFC_BC_CONSTANT_EXTRAPOLATION || \
+                    (bc) == 4002 /* LINEAR_EXTRAPOLATION */ || (bc) == CFFC_BC_CHARACTERISTIC)
+    if (i == ICl - Ng || i == ICu + Ng || j == JCl - Ng || j == JCu + Ng) {
+        n_pts = 0;
+    } else if (i == ICl - Ng + 1 && bcw != CFFC_BC_NONE) {
+        if (j == JCl - Ng + 1 || j == JCu + Ng - 1) n_pts = 0;
+        else if (EXTRAP(bcw)) {
+            if (j == JCl) { n_pts = 5; SET(0, i-1, j); SET(1, i+1, j); SET(2, i-1, j+1); SET(3, i, j+1); SET(4, i+1, j+1); }
+            else if (j == JCu) { n_pts = 5; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i+1, j-1); SET(3, i-1, j); SET(4, i+1, j); }
+            else FULL8();
+        } else {
+            if (j == JCl) { n_pts = 3; SET(0, i+1, j); SET(1, i, j+1); SET(2, i+1, j+1); }
+            else if (j == JCu) { n_pts = 3; SET(0, i, j-1); SET(1, i+1, j-1); SET(2, i+1, j); }
+            else { n_pts = 5; SET(0, i, j-1); SET(1, i+1, j-1); SET(2, i+1, j); SET(3, i, j+1); SET(4, i+1, j+1); }
+        }
+    } else if (i == ICu + Ng - 1 && bce != CFFC_BC_NONE) {
+        if (j == JCl - Ng + 1 || j == JCu + Ng - 1) n_pts = 0;
+        else if (EXTRAP(bce)) {
+            if (j == JCl) { n_pts = 5; SET(0, i-1, j); SET(1, i+1, j); SET(2, i-1, j+1); SET(3, i, j+1); SET(4, i+1, j+1); }
+            else if (j == JCu) { n_pts = 5; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i+1, j-1); SET(3, i-1, j); SET(4, i+1, j); }
+            else FULL8();
+        } else {
+            if (j == JCl) { n_pts = 3; SET(0, i-1, j); SET(1, i-1, j+1); SET(2, i, j+1); }
+            else if (j == JCu) { n_pts = 3; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i-1, j); }
+            else { n_pts = 5; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i-1, j); SET(3, i-1, j+1); SET(4, i, j+1); }
+        }
+    } else if (j == JCl - Ng + 1 && bcs != CFFC_BC_NONE) {
+        if (i == ICl - Ng + 1 || i == ICu + Ng - 1) n_pts = 0;
+        else if (EXTRAP(bcs)) {
+            if (i == ICl) { n_pts = 5; SET(0, i, j-1); SET(1, i+1, j-1); SET(2, i+1, j); SET(3, i, j+1); SET(4, i+1, j+1); }
+            else if (i == ICu) { n_pts = 5; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i-1, j); SET(3, i-1, j+1); SET(4, i, j+1); }
+            else FULL8();
+        } else {
+            if (i == ICl) { n_pts = 3; SET(0, i+1, j); SET(1, i, j+1); SET(2, i+1, j+1); }
+            else if (i == ICu) { n_pts = 3; SET(0, i-1, j); SET(1, i-1, j+1); SET(2, i, j+1); }
+            else { n_pts = 5; SET(0, i-1, j); SET(1, i+1, j); SET(2, i-1, j+1); SET(3, i, j+1); SET(4, i+1, j+1); }
+        }
+    } else if (j == JCu + Ng - 1 && bcn != CFFC_BC_NONE) {
+        if (i == ICl - Ng + 1 || i == ICu + Ng - 1) n_pts = 0;
+        else if (EXTRAP(bcn)) {
+            if (i == ICl) { n_pts = 5; SET(0, i, j-1); SET(1, i+1, j-1); SET(2, i+1, j); SET(3, i, j+1); SET(4, i+1, j+1); }
+            else if (i == ICu) { n_pts = 5; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i-1, j); SET(3, i-1, j+1); SET(4, i, j+1); }
+            else FULL8();
+        } else {
+            if (i == ICl) { n_pts = 3; SET(0, i, j-1); SET(1, i+1, j-1); SET(2, i+1, j); }
+            else if (i == ICu) { n_pts = 3; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i-1, j); }
+            else { n_pts = 5; SET(0, i-1, j-1); SET(1, i, j-1); SET(2, i+1, j-1); SET(3, i-1, j); SET(4, i+1, j); }
+        }
+    } else {
+        FULL8();
+    }
+#undef SET
+#undef FULL8
+#undef EXTRAP
+    int c = CI(B, i, j);
+    if (n_pts > 0) {
+        pst DUDx = {0, 0, 0, 0}, DUDy = {0, 0, 0, 0};
+        double DxDx = ZERO, DxDy = ZERO, DyDy = ZERO;
+        pst W0 = B->W[c];
+        for (int n2 = 0; n2 <= n_pts - 1; ++n2) {
+            int cn = CI(B, ii[n2], jj[n2]);
+            double dXx = B->xc[cn] - B->xc[c], dXy = B->yc[cn] - B->yc[c];
+            pst Wn = B->W[cn];
+            pst DU = {Wn.d - W0.d, Wn.vx - W0.vx, Wn.vy - W0.vy, Wn.p - W0.p};
+            DUDx.d += DU.d * dXx; DUDx.vx += DU.vx * dXx; DUDx.vy += DU.vy * dXx; DUDx.p += DU.p * dXx;
+            DUDy.d += DU.d * dXy; DUDy.vx += DU.vx * dXy; DUDy.vy += DU.vy * dXy; DUDy.p += DU.p * dXy;
+            DxDx += dXx * dXx; DxDy += dXx * dXy; DyDy += dXy * dXy;
+        }
+        double np = (double)n_pts;
+        DUDx.d /= np; DUDx.vx /= np; DUDx.vy /= np; DUDx.p /= np;
+        DUDy.d /= np; DUDy.vx /= np; DUDy.vy /= np; DUDy.p /= np;
+        DxDx /= np; DxDy /= np; DyDy /= np;
+        double det = DxDx * DyDy - DxDy * DxDy;
+        pst gx, gy;
+        /* (DUDx*DyDy - DUDy*DxDy)/det : pState*double, pState-pState, pState/double */
+        gx.d = (DUDx.d * DyDy - DUDy.d * DxDy) / det;
+        gx.vx = (DUDx.vx * DyDy - DUDy.vx * DxDy) / det;
+        gx.vy = (DUDx.vy * DyDy - DUDy.vy * DxDy) / det;
+        gx.p = (DUDx.p * DyDy - DUDy.p * DxDy) / det;
+        gy.d = (DUDy.d * DxDx - DUDx.d * DxDy) / det;
+        gy.vx = (DUDy.vx * DxDx - DUDx.vx * DxDy) / det;
+        gy.vy = (DUDy.vy * DxDx - DUDx.vy * DxDy) / det;
+        gy.p = (DUDy.p * DxDx - DUDx.p * DxDy) / det;
+        B->dWdx[c] = gx;
+        B->dWdy[c] = gy;
+        if (!B->freeze_limiter) {
+            double xE, yE, xW, yW, xN, yN, xS, yS;
+            xfaceE(B, i, j, &xE, &yE); xfaceW(B, i, j, &xW, &yW);
+            xfaceN(B, i, j, &xN, &yN); xfaceS(B, i, j, &xS, &yS);
+            double xc0 = B->xc[c], yc0 = B->yc[c];
+            for (int n = 1; n <= 4; ++n) {
+                double u0 = pget(&W0, n), u0Min = u0, u0Max = u0;
+                for (int n2 = 0; n2 <= n_pts - 1; ++n2) {
+                    double v = pget(&B->W[CI(B, ii[n2], jj[n2])], n);
+                    u0Min = dmin(u0Min, v);
+                    u0Max = dmax(u0Max, v);
+                }
+                double gxn = pget(&gx, n), gyn = pget(&gy, n), uQuad[4], phi;
+                uQuad[0] = u0 + gxn * (xE - xc0) + gyn * (yE - yc0);
+                uQuad[1] = u0 + gxn * (xW - xc0) + gyn * (yW - yc0);
+                uQuad[2] = u0 + gxn * (xN - xc0) + gyn * (yN - yc0);
+                uQuad[3] = u0 + gxn * (xS - xc0) + gyn * (yS - yc0);
+                switch (Limiter) {
+                case CFFC_LIMITER_ONE: phi = ONE; break;
+                case CFFC_LIMITER_ZERO: phi = ZERO; break;
+                default: phi = limiter_venkatakrishnan(uQuad, u0, u0Min, u0Max, 4); break;
+                }
+                *pref(&B->phi[c], n) = phi;
+            }
+        }
+    } else {
+        pst z = {0, 0, 0, 0};
+        B->dWdx[c] = z;
+        B->dWdy[c] = z;
+        B->phi[c] = z;
+    }
+}
+
+/* W + (phi^dWdx)*dX.x + (phi^dWdy)*dX.y  (Euler2DQuad.cc:1338-1340) */
+static inline pst face_state(const oblk *B, int c, double dXx, double dXy) {
+    pst W = B->W[c], P = B->phi[c], gx = B->dWdx[c], gy = B->dWdy[c], r;
+    r.d = W.d + (P.d * gx.d) * dXx + (P.d * gy.d) * dXy;
+    r.vx = W.vx + (P.vx * gx.vx) * dXx + (P.vx * gy.vx) * dXy;
+    r.vy = W.vy + (P.vy * gx.vy) * dXx + (P.vy * gy.vy) * dXy;
+    r.p = W.p + (P.p * gx.p) * dXx + (P.p * gy.p) * dXy;
+    return r;
+}
+
+static int stage_k_residual(const cffc_scheme_desc *s, int i_stage) {
+    if (s->i_time_integration == CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA && s->n_stage == 4)
+        return (i_stage == 4) ? 0 : i_stage - 1;
+    return 0;
+}
+static double stage_omega(const cffc_scheme_desc *s, int i_stage) {
+    if (s->i_time_integration == CFFC_TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING)
+        return multistage_optimal_smoothing(i_stage, s->n_stage, s->i_limiter);
+    return runge_kutta(i_stage, s->n_stage);
+}
+
+/* dUdt -= ((CFL*dt)*Flux*l)/A  and  += variant (Euler2DQuad.cc:1502-1509) */
+static inline void acc_sub(cst *d, double cdt, cst F, double l, double A) {
+    d->d -= cdt * F.d * l / A; d->mx -= cdt * F.mx * l / A; d->my -= cdt * F.my * l / A; d->E -= cdt * F.E * l / A;
+}
+static inline void acc_add(cst *d, double cdt, cst F, double l, double A) {
+    d->d += cdt * F.d * l / A; d->mx += cdt * F.mx * l / A; d->my += cdt * F.my * l / A; d->E += cdt * F.E * l / A;
+}
+
+static int is_face_bc(int bc) { return bc == CFFC_BC_REFLECTION || bc == CFFC_BC_CHARACTERISTIC; }
+
+/* Euler2D_Quad_Block::dUdt_Multistage_Explicit, Euler2DQuad.cc:1235-1727 */
+static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
+    int ICl = B->ICl, ICu = B->ICu, JCl = B->JCl, JCu = B->JCu, Ng = B->ng;
+    int k = stage_k_residual(s, i_stage);
+    const cst Z = {0, 0, 0, 0};
+    double CFL = s->cfl_number;
+    /* Linear_Reconstruction_LeastSquares, SingleBlock.cc:4467-4482 */
+    for (int j = JCl - Ng + 1; j <= JCu + Ng - 1; ++j)
+        for (int i = ICl - Ng + 1; i <= ICu + Ng - 1; ++i) lsq_reconstruct(B, i, j, s->i_limiter);
+    /* i-direction fluxes :1304-1534 */
+    for (int j = JCl - 1; j <= JCu + 1; ++j) {
+        if (i_stage == 1) B->Uo[CI(B, ICl - 1, j)] = B->U[CI(B, ICl - 1, j)];
+        B->dUdt[k][CI(B, ICl - 1, j)] = Z;
+        for (int i = ICl - 1; i <= ICu; ++i) {
+            int cL = CI(B, i, j), cR = CI(B, i + 1, j);
+            if (i_stage == 1) {
+                B->Uo[cR] = B->U[cR];
+                B->dUdt[k][cR] = Z;
+            } else if (j > JCl - 1 && j < JCu + 1) {
+                int ti = s->i_time_integration;
+                if (ti == CFFC_TIME_STEPPING_EXPLICIT_PREDICTOR_CORRECTOR) {
+                } else if (ti == CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA) {
+                    if (s->n_stage == 2) {
+                    } else if (s->n_stage == 4 && i_stage == 4) {
+                        cst a = B->dUdt[0][cR], b = B->dUdt[1][cR], c2 = B->dUdt[2][cR], r;
+                        r.d = a.d + TWO * b.d + TWO * c2.d;
+                        r.mx = a.mx + TWO * b.mx + TWO * c2.mx;
+                        r.my = a.my + TWO * b.my + TWO * c2.my;
+                        r.E = a.E + TWO * b.E + TWO * c2.E;
+                        B->dUdt[k][cR] = r;
+                    } else {
+                        B->dUdt[k][cR] = Z;
+                    }
+                } else {
+                    B->dUdt[k][cR] = Z;
+                }
+            }
+            if (j > JCl - 1 && j < JCu + 1) {
+                pst Wl, Wr;
+                double nx, ny, xf, yf;
+                if (i == ICl - 1 && is_face_bc(B->bcW[j])) {
+                    xfaceW(B, i + 1, j, &xf, &yf);
+                    Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+                    nfaceW(B, i + 1, j, &nx, &ny);
+                    if (B->bcW[j] == CFFC_BC_REFLECTION) Wl = reflect(Wr, nx, ny);
+                    else Wl = bc_characteristic_pressure(Wr, B->WoW[j], nx, ny);
+                } else if (i == ICu && is_face_bc(B->bcE[j])) {
+                    xfaceE(B, i, j, &xf, &yf);
+                    Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                    nfaceE(B, i, j, &nx, &ny);
+                    if (B->bcE[j] == CFFC_BC_REFLECTION) Wr = reflect(Wl, nx, ny);
+                    else Wr = bc_characteristic_pressure(Wl, B->WoE[j], nx, ny);
+                } else {
+                    xfaceE(B, i, j, &xf, &yf);
+                    Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                    xfaceW(B, i + 1, j, &xf, &yf);
+                    Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+                }
+                nfaceE(B, i, j, &nx, &ny);
+                cst F = flux_roe_n(Wl, Wr, nx, ny);
+                acc_sub(&B->dUdt[k][cL], CFL * B->dt[cL], F, lfaceE(B, i, j), B->A[cL]);
+                acc_add(&B->dUdt[k][cR], CFL * B->dt[cR], F, lfaceW(B, i + 1, j), B->A[cR]);
+            }
+        }
+        if (j > JCl - 1 && j < JCu + 1) {
+            B->dUdt[k][CI(B, ICl - 1, j)] = Z;
+            B->dUdt[k][CI(B, ICu + 1, j)] = Z;
+        }
+    }
+    /* j-direction fluxes :1537-1721 */
+    for (int i = ICl; i <= ICu; ++i) {
+        for (int j = JCl - 1; j <= JCu; ++j) {
+            int cL = CI(B, i, j), cR = CI(B, i, j + 1);
+            pst Wl, Wr;
+            double nx, ny, xf, yf;
+            if (j == JCl - 1 && is_face_bc(B->bcS[i])) {
+                xfaceS(B, i, j + 1, &xf, &yf);
+                Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+                nfaceS(B, i, j + 1, &nx, &ny);
+                if (B->bcS[i] == CFFC_BC_REFLECTION) Wl = reflect(Wr, nx, ny);
+                else Wl = bc_characteristic_pressure(Wr, B->WoS[i], nx, ny);
+            } else if (j == JCu && is_face_bc(B->bcN[i])) {
+                xfaceN(B, i, j, &xf, &yf);
+                Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                nfaceN(B, i, j, &nx, &ny);
+                if (B->bcN[i] == CFFC_BC_REFLECTION) Wr = reflect(Wl, nx, ny);
+                else Wr = bc_characteristic_pressure(Wl, B->WoN[i], nx, ny);
+            } else {
+                xfaceN(B, i, j, &xf, &yf);
+                Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                xfaceS(B, i, j + 1, &xf, &yf);
+                Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+            }
+            nfaceN(B, i, j, &nx, &ny);
+            cst F = flux_roe_n(Wl, Wr, nx, ny);
+            acc_sub(&B->dUdt[k][cL], CFL * B->dt[cL], F, lfaceN(B, i, j), B->A[cL]);
+            acc_add(&B->dUdt[k][cR], CFL * B->dt[cR], F, lfaceS(B, i, j + 1), B->A[cR]);
+        }
+        B->dUdt[k][CI(B, i, JCl - 1)] = Z;
+        B->dUdt[k][CI(B, i, JCu + 1)] = Z;
+    }
+}
+
+/* Update_Solution_Multistage_Explicit, Euler2DQuadSingleBlock.cc:5300-5507
+   (GLOBAL and SCALAR_LOCAL time stepping).  Returns 0 or the i-index of the
+   first failing cell; *bad_j receives its j-index. */
+static int block_update(oblk *B, const cffc_scheme_desc *s, int i_stage, int *bad_j) {
+    int k = stage_k_residual(s, i_stage);
+    double omega = stage_omega(s, i_stage);
+    for (int j = B->JCl; j <= B->JCu; ++j)
+        for (int i = B->ICl; i <= B->ICu; ++i) {
+            int c = CI(B, i, j);
+            cst *U = &B->U[c], Uo = B->Uo[c], *dU = &B->dUdt[k][c];
+            U->d = Uo.d + omega * dU->d; U->mx = Uo.mx + omega * dU->mx;
+            U->my = Uo.my + omega * dU->my; U->E = Uo.E + omega * dU->E;
+            int bad = (U->d <= ZERO || U->E <= ZERO || U_e(*U) <= ZERO);
+            if (s->local_time_stepping == CFFC_GLOBAL_TIME_STEPPING && bad) {
+                *bad_j = j;
+                return i;
+            } else if (s->local_time_stepping == CFFC_SCALAR_LOCAL_TIME_STEPPING && bad) {
+                double f = ONE;
+                for (int n = 1; n <= 10; ++n) {
+                    f = HALF * f;
+                    B->dt[c] = f * B->dt[c];
+                    dU->d = f * dU->d; dU->mx = f * dU->mx; dU->my = f * dU->my; dU->E = f * dU->E;
+                    U->d = Uo.d + omega * dU->d; U->mx = Uo.mx + omega * dU->mx;
+                    U->my = Uo.my + omega * dU->my; U->E = Uo.E + omega * dU->E;
+                    if (U->d > ZERO && U->E > ZERO && U_e(*U) > ZERO) break;
+                }
+                if (U->d <= ZERO || U->E <= ZERO || U_e(*U) <= ZERO) {
+                    *bad_j = j;
+                    return i;
+                }
+            }
+            B->W[c] = W_of_U(*U);
+        }
+    return 0;
+}
+
+/* CFL, Euler2DQuadSingleBlock.cc:3019-3080 (non-Weiss-Smith branch) */
+static double block_cfl(oblk *B) {
+    double dtMin = MILLION;
+    int Ng = B->ng;
+    for (int j = B->JCl - Ng; j <= B->JCu + Ng; ++j)
+        for (int i = B->ICl - Ng; i <= B->ICu + Ng; ++i) {
+            int c = CI(B, i, j);
+            if (i < B->ICl || i > B->ICu || j < B->JCl || j > B->JCu) {
+                B->dt[c] = ZERO;
+            } else {
+                double d_i = TWO * (B->A[c] / (lfaceE(B, i, j) + lfaceW(B, i, j)));
+                double d_j = TWO * (B->A[c] / (lfaceN(B, i, j) + lfaceS(B, i, j)));
+                double nEx, nEy, nWx, nWy, nNx, nNy, nSx, nSy;
+                nfaceE(B, i, j, &nEx, &nEy); nfaceW(B, i, j, &nWx, &nWy);
+                nfaceN(B, i, j, &nNx, &nNy); nfaceS(B, i, j, &nSx, &nSy);
+                pst W = B->W[c];
+                double v_i = HALF * (W.vx * (nEx - nWx) + W.vy * (nEy - nWy));
+                double v_j = HALF * (W.vx * (nNx - nSx) + W.vy * (nNy - nSy));
+                double a = W_a(W);
+                B->dt[c] = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
+                dtMin = dmin(dtMin, B->dt[c]);
+            }
+        }
+    for (int j = B->JCl - Ng; j <= B->JCu + Ng; ++j)
+        for (int i = B->ICl - Ng; i <= B->ICu + Ng; ++i)
+            if (i < B->ICl || i > B->ICu || j < B->JCl || j > B->JCu) B->dt[CI(B, i, j)] = dtMin;
+    return dtMin;
+}
+/* Set_Global_TimeStep :3089-3100 */
+static void block_set_global_dt(oblk *B, double dt) {
+    for (int j = B->JCl; j <= B->JCu; ++j)
+        for (int i = B->ICl; i <= B->ICu; ++i) B->dt[CI(B, i, j)] = dt;
+}
+
+/* ====================================================================== */
+/* exported API (ctypes)                                                   */
+/* ====================================================================== */
+static void blk_free(oblk *B) {
+    free(B->xc); free(B->yc); free(B->A);
+    free(B->bcW); free(B->bcE); free(B->bcS); free(B->bcN);
+    free(B->WoW); free(B->WoE); free(B->WoS); free(B->WoN);
+    free(B->W); free(B->dWdx); free(B->dWdy); free(B->phi);
+    free(B->U); free(B->Uo); free(B->dUdt[0]); free(B->dUdt[1]); free(B->dUdt[2]);
+    free(B->dt);
+}
+
+void orc_destroy(orc_ctx *ctx) {
+    if (!ctx) return;
+    for (int b = 0; b < ctx->nblk; ++b) blk_free(&ctx->b[b]);
+    free(ctx->b);
+    free(ctx->xnode);
+    free(ctx->ynode);
+    free(ctx);
+}
+
+int orc_create(const cffc_mesh_desc *m, const cffc_scheme_desc *s, orc_ctx **out) {
+    if (!m || !s || !out || m->nblk <= 0 || m->nghost != 2) return CFFC_E_ARG;
+    if (s->i_time_integration != CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA &&
+        s->i_time_integration != CFFC_TIME_STEPPING_EXPLICIT_EULER &&
+        s->i_time_integration != CFFC_TIME_STEPPING_EXPLICIT_PREDICTOR_CORRECTOR &&
+        s->i_time_integration != CFFC_TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING)
+        return CFFC_E_ARG;
+    orc_ctx *ctx = (orc_ctx *)calloc(1, sizeof(orc_ctx));
+    ctx->nblk = m->nblk;
+    ctx->s = *s;
+    ctx->b = (oblk *)calloc(m->nblk, sizeof(oblk));
+    int64_t nn = 0, nbc = 0;
+    for (int b = 0; b < m->nblk; ++b) {
+        nn += (int64_t)(m->ni[b] + 5) * (m->nj[b] + 5);
+        nbc += 2 * (m->nj[b] + 4) + 2 * (m->ni[b] + 4);
+    }
+    ctx->xnode = (double *)malloc(nn * sizeof(double));
+    ctx->ynode = (double *)malloc(nn * sizeof(double));
+    memcpy(ctx->xnode, m->xnode, nn * sizeof(double));
+    memcpy(ctx->ynode, m->ynode, nn * sizeof(double));
+    int64_t noff = 0, boff = 0;
+    ctx->ncell = 0;
+    for (int b = 0; b < m->nblk; ++b) {
+        oblk *B = &ctx->b[b];
+        B->ni = m->ni[b]; B->nj = m->nj[b]; B->ng = 2;
+        B->nci = B->ni + 4; B->ncj = B->nj + 4;
+        B->ICl = 2; B->ICu = B->ni + 1; B->JCl = 2; B->JCu = B->nj + 1;
+        B->xn = ctx->xnode + noff; B->yn = ctx->ynode + noff;
+        noff += (int64_t)(B->nci + 1) * (B->ncj + 1);
+        int nc = B->nci * B->ncj;
+        B->xc = (double *)malloc(nc * sizeof(double));
+        B->yc = (double *)malloc(nc * sizeof(double));
+        B->A = (double *)malloc(nc * sizeof(double));
+        for (int j = 0; j < B->ncj; ++j)
+            for (int i = 0; i < B->nci; ++i)
+                quad_centroid(B, i, j, &B->xc[CI(B, i, j)], &B->yc[CI(B, i, j)], &B->A[CI(B, i, j)]);
+        B->bcW = (int *)malloc(B->ncj * sizeof(int)); B->bcE = (int *)malloc(B->ncj * sizeof(int));
+        B->bcS = (int *)malloc(B->nci * sizeof(int)); B->bcN = (int *)malloc(B->nci * sizeof(int));
+        B->WoW = (pst *)malloc(B->ncj * sizeof(pst)); B->WoE = (pst *)malloc(B->ncj * sizeof(pst));
+        B->WoS = (pst *)malloc(B->nci * sizeof(pst)); B->WoN = (pst *)malloc(B->nci * sizeof(pst));
+        int *dst[4] = {B->bcW, B->bcE, B->bcS, B->bcN};
+        pst *wdst[4] = {B->WoW, B->WoE, B->WoS, B->WoN};
+        int len[4] = {B->ncj, B->ncj, B->nci, B->nci};
+        for (int f = 0; f < 4; ++f) {
+            for (int q = 0; q < len[f]; ++q) {
+                dst[f][q] = m->bc[boff + q];
+                if (!bc_supported(dst[f][q])) { orc_destroy(ctx); return CFFC_E_ARG; }
+                memcpy(&wdst[f][q], &m->wo[4 * (boff + q)], sizeof(pst));
+            }
+            boff += len[f];
+        }
+        for (int q = 0; q < 8; ++q) B->nbr[q] = m->nbr[8 * b + q];
+        B->W = (pst *)calloc(nc, sizeof(pst)); B->dWdx = (pst *)calloc(nc, sizeof(pst));
+        B->dWdy = (pst *)calloc(nc, sizeof(pst)); B->phi = (pst *)calloc(nc, sizeof(pst));
+        B->U = (cst *)calloc(nc, sizeof(cst)); B->Uo = (cst *)calloc(nc, sizeof(cst));
+        for (int q = 0; q < 3; ++q) B->dUdt[q] = (cst *)calloc(nc, sizeof(cst));
+        B->dt = (double *)calloc(nc, sizeof(double));
+        ctx->ncell += (int64_t)B->ni * B->nj;
+    }
+    *out = ctx;
+    return 0;
+}
+
+int64_t orc_num_cells(const orc_ctx *ctx) { return ctx->ncell; }
+
+/* interior cells, blocks in order, i fastest, 4 doubles per cell */
+int orc_set_W(orc_ctx *ctx, const double *W) {
+    int64_t q = 0;
+    for (int b = 0; b < ctx->nblk; ++b) {
+        oblk *B = &ctx->b[b];
+        for (int j = B->JCl; j <= B->JCu; ++j)
+            for (int i = B->ICl; i <= B->ICu; ++i, ++q) {
+                pst w; memcpy(&w, W + 4 * q, sizeof(pst));
+                B->W[CI(B, i, j)] = w;
+                B->U[CI(B, i, j)] = U_of_W(w);
+            }
+    }
+    return 0;
+}
+int orc_set_U(orc_ctx *ctx, const double *U) {
+    int64_t q = 0;
+    for (int b = 0; b < ctx->nblk; ++b) {
+        oblk *B = &ctx->b[b];
+        for (int j = B->JCl; j <= B->JCu; ++j)
+            for (int i = B->ICl; i <= B->ICu; ++i, ++q) {
+                cst u; memcpy(&u, U + 4 * q, sizeof(cst));
+                B->U[CI(B, i, j)] = u;
+                B->W[CI(B, i, j)] = W_of_U(u);
+            }
+    }
+    return 0;
+}
+static int get_field(orc_ctx *ctx, int which, int k, double *out) {
+    int64_t q = 0;
+    for (int b = 0; b < ctx->nblk; ++b) {
+        oblk *B = &ctx->b[b];
+        for (int j = B->JCl; j <= B->JCu; ++j)
+            for (int i = B->ICl; i <= B->ICu; ++i, ++q) {
+                int c = CI(B, i, j);
+                const void *src;
+                switch (which) {
+                case 0: src = &B->W[c]; break;
+                case 1: src = &B->U[c]; break;
+                case 2: src = &B->dUdt[k][c]; break;
+                case 3: src = &B->dWdx[c]; break;
+                case 4: src = &B->dWdy[c]; break;
+                default: src = &B->phi[c]; break;
+                }
+                memcpy(out + 4 * q, src, 4 * sizeof(double));
+            }
+    }
+    return 0;
+}
+int orc_get_W(orc_ctx *ctx, double *W) { return get_field(ctx, 0, 0, W); }
+int orc_get_U(orc_ctx *ctx, double *U) { return get_field(ctx, 1, 0, U); }
+int orc_get_dUdt(orc_ctx *ctx, int k, double *d) { return get_field(ctx, 2, k, d); }
+int orc_get_recon(orc_ctx *ctx, int which, double *d) { return get_field(ctx, 3 + which, 0, d); }
+int orc_get_dt(orc_ctx *ctx, double *dt) {
+    int64_t q = 0;
+    for (int b = 0; b < ctx->nblk; ++b) {
+        oblk *B = &ctx->b[b];
+        for (int j = B->JCl; j <= B->JCu; ++j)
+            for (int i = B->ICl; i <= B->ICu; ++i, ++q) dt[q] = B->dt[CI(B, i, j)];
+    }
+    return 0;
+}
+/* the whole padded W array of one block (ghosts included), i fastest */
+int orc_get_block_W_padded(orc_ctx *ctx, int b, double *W) {
+    oblk *B = &ctx->b[b];
+    memcpy(W, B->W, sizeof(pst) * B->nci * B->ncj);
+    return 0;
+}
+/* cell geometry of one block (padded): xc, yc, A */
+int orc_get_block_geometry(orc_ctx *ctx, int b, double *xc, double *yc, double *A) {
+    oblk *B = &ctx->b[b];
+    int nc = B->nci * B->ncj;
+    memcpy(xc, B->xc, nc * sizeof(double)); memcpy(yc, B->yc, nc * sizeof(double));
+    memcpy(A, B->A, nc * sizeof(double));
+    return 0;
+}
+
+/* ---- multi-block wrappers (Euler2DQuadMultiBlock.cc) -------------------- */
+typedef struct {
+    orc_ctx *ctx;
+    int b0, b1;
+} range_t;
+
+int orc_exchange_and_bcs(orc_ctx *ctx) {
+    for (int b = 0; b < ctx->nblk; ++b) exchange_block(ctx->b, b);
+    for (int b = 0; b < ctx->nblk; ++b) block_bcs(&ctx->b[b]);
+    return 0;
+}
+int orc_cfl(orc_ctx *ctx, double *dt_min, int set_global) {
+    double dtMin = MILLION;
+    for (int b = 0; b < ctx->nblk; ++b) dtMin = dmin(dtMin, block_cfl(&ctx->b[b]));
+    if (set_global)
+        for (int b = 0; b < ctx->nblk; ++b) block_set_global_dt(&ctx->b[b], dtMin);
+    *dt_min = dtMin;
+    return 0;
+}
+int orc_dudt_stage(orc_ctx *ctx, int i_stage) {
+    for (int b = 0; b < ctx->nblk; ++b) block_dudt(&ctx->b[b], &ctx->s, i_stage);
+    return 0;
+}
+int orc_update_stage(orc_ctx *ctx, int i_stage, int64_t *bad_cell) {
+    for (int b = 0; b < ctx->nblk; ++b) {
+        int bj = 0;
+        int e = block_update(&ctx->b[b], &ctx->s, i_stage, &bj);
+        if (e) {
+            if (bad_cell) {
+                int bi = e;
+                *bad_cell = ((int64_t)b << 32) | ((int64_t)bj << 16) | bi;
+            }
+            return e;
+        }
+    }
+    return 0;
+}
+/* L1/L2/Max_Norm_Residual :1017-1104 */
+int orc_residual_norms(orc_ctx *ctx, int var, double out[3]) {
+    double l1 = ZERO, l2 = ZERO, mx = ZERO;
+    for (int b = 0; b < ctx->nblk; ++b) {
+        oblk *B = &ctx->b[b];
+        double bl1 = ZERO, bl2 = ZERO, bmx = ZERO;
+        for (int j = B->JCl; j <= B->JCu; ++j)
+            for (int i = B->ICl; i <= B->ICu; ++i) {
+                double v = ((double *)&B->dUdt[0][CI(B, i, j)])[var - 1];
+                bl1 += fabs(v);
+                bl2 += sqr(v);
+                bmx = dmax(bmx, fabs(v));
+            }
+        l1 += bl1;
+        l2 += sqr(sqrt(bl2));
+        mx = dmax(mx, bmx);
+    }
+    out[0] = l1;
+    out[1] = sqrt(l2);
+    out[2] = mx;
+    return 0;
+}
+int orc_freeze_limiters(orc_ctx *ctx, int freeze) {
+    for (int b = 0; b < ctx->nblk; ++b) ctx->b[b].freeze_limiter = freeze;
+    return 0;
+}
+
+/* ---- the explicit time loop, Euler2DQuadSolvers.cc:549-898 --------------
+   Steady (Time_Accurate = 0) loop over nsteps steps.  Blocks are split over
+   nthreads threads, one contiguous block range per thread, mirroring the
+   reference's one-MPI-rank-per-core block decomposition; the reductions
+   (dt minimum, norms, error OR) are combined after a barrier. */
+typedef struct {
+    orc_ctx *ctx;
+    int nthreads, tid, nsteps;
+    double tol;
+    pthread_barrier_t *bar;
+    double *dtmins, *l1s, *l2s, *mxs;
+    int *errs;
+    double *norms, *dt_out;
+    int *steps_done;
+    int64_t *bad_cell;
+} worker_t;
+
+static void *worker(void *arg) {
+    worker_t *w = (worker_t *)arg;
+    orc_ctx *ctx = w->ctx;
+    int nb = ctx->nblk, T = w->nthreads;
+    int b0 = (int)((int64_t)nb * w->tid / T), b1 = (int)((int64_t)nb * (w->tid + 1) / T);
+    const cffc_scheme_desc *s = &ctx->s;
+    int n;
+    for (n = 0; n <= w->nsteps; ++n) {
+        /* CFL + CFFC_Minimum_MPI + Set_Global_TimeStep :657-684 */
+        double dtm = MILLION;
+        for (int b = b0; b < b1; ++b) dtm = dmin(dtm, block_cfl(&ctx->b[b]));
+        w->dtmins[w->tid] = dtm;
+        pthread_barrier_wait(w->bar);
+        double dTime = MILLION;
+        for (int t = 0; t < T; ++t) dTime = dmin(dTime, w->dtmins[t]);
+        if (s->local_time_stepping == CFFC_GLOBAL_TIME_STEPPING)
+            for (int b = b0; b < b1; ++b) block_set_global_dt(&ctx->b[b], dTime);
+        /* norms of the previous step's dUdt[0] :687-696 */
+        if (n > 0) {
+            double l1 = 0, l2 = 0, mx = 0;
+            for (int b = b0; b < b1; ++b) {
+                oblk *B = &ctx->b[b];
+                double bl1 = 0, bl2 = 0, bmx = 0;
+                for (int j = B->JCl; j <= B->JCu; ++j)
+                    for (int i = B->ICl; i <= B->ICu; ++i) {
+                        double v = B->dUdt[0][CI(B, i, j)].d;
+                        bl1 += fabs(v); bl2 += sqr(v); bmx = dmax(bmx, fabs(v));
+                    }
+                l1 += bl1; l2 += sqr(sqrt(bl2)); mx = dmax(mx, bmx);
+            }
+            w->l1s[w->tid] = l1; w->l2s[w->tid] = l2; w->mxs[w->tid] = mx;
+        }
+        pthread_barrier_wait(w->bar);
+        if (n > 0) {
+            double l1 = 0, l2 = 0, mx = 0;
+            for (int t = 0; t < T; ++t) { l1 += w->l1s[t]; l2 += w->l2s[t]; mx = dmax(mx, w->mxs[t]); }
+            l2 = sqrt(l2);
+            if (w->tid == 0 && w->norms) {
+                w->norms[3 * (n - 1) + 0] = l1; w->norms[3 * (n - 1) + 1] = l2; w->norms[3 * (n - 1) + 2] = mx;
+            }
+            if (w->tol > 0 && l2 <= w->tol) break;
+        }
+        if (n == w->nsteps) break;
+        if (w->tid == 0 && w->dt_out) w->dt_out[n] = dTime;
+        for (int st = 1; st <= s->n_stage; ++st) {
+            for (int b = b0; b < b1; ++b) exchange_block(ctx->b, b);
+            pthread_barrier_wait(w->bar);
+            for (int b = b0; b < b1; ++b) block_bcs(&ctx->b[b]);
+            for (int b = b0; b < b1; ++b) block_dudt(&ctx->b[b], s, st);
+            int err = 0;
+            for (int b = b0; b < b1 && !err; ++b) {
+                int bj = 0;
+                err = block_update(&ctx->b[b], s, st, &bj);
+                if (err && w->bad_cell) *w->bad_cell = ((int64_t)b << 32) | ((int64_t)bj << 16) | err;
+            }
+            w->errs[w->tid] = err;
+            pthread_barrier_wait(w->bar);
+            int any = 0;
+            for (int t = 0; t < T; ++t) if (w->errs[t]) { any = w->errs[t]; break; }
+            if (any) { if (w->tid == 0) *w->steps_done = -any; return NULL; }
+        }
+    }
+    if (w->tid == 0) *w->steps_done = n;
+    return NULL;
+}
+
+/* Returns 0, or the reference's error code (> 0); *steps_done = steps run. */
+int orc_run(orc_ctx *ctx, int nsteps, double tol, int nthreads, double *norms, double *dt_out, int *steps_done) {
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > ctx->nblk) nthreads = ctx->nblk;
+    pthread_barrier_t bar;
+    pthread_barrier_init(&bar, NULL, nthreads);
+    double *dtm = calloc(nthreads, sizeof(double)), *l1 = calloc(nthreads, sizeof(double));
+    double *l2 = calloc(nthreads, sizeof(double)), *mx = calloc(nthreads, sizeof(double));
+    int *errs = calloc(nthreads, sizeof(int));
+    worker_t *ws = calloc(nthreads, sizeof(worker_t));
+    pthread_t *th = calloc(nthreads, sizeof(pthread_t));
+    int done = 0;
+    int64_t bad = 0;
+    for (int t = 0; t < nthreads; ++t) {
+        worker_t x = {ctx, nthreads, t, nsteps, tol, &bar, dtm, l1, l2, mx, errs, norms, dt_out, &done, &bad};
+        ws[t] = x;
+    }
+    for (int t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, worker, &ws[t]);
+    worker(&ws[0]);
+    for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
+    pthread_barrier_destroy(&bar);
+    free(dtm); free(l1); free(l2); free(mx); free(errs); free(ws); free(th);
+    if (done < 0) { if (steps_done) *steps_done = 0; return -done; }
+    if (steps_done) *steps_done = done;
+    return 0;
+}
+
+/* ---- scalar entry points for known-answer tests ------------------------ */
+void orc_flux_roe_n(const double *Wl, const double *Wr, double nx, double ny, double *F) {
+    pst l, r; memcpy(&l, Wl, sizeof l); memcpy(&r, Wr, sizeof r);
+    cst f = flux_roe_n(l, r, nx, ny);
+    memcpy(F, &f, sizeof f);
+}
+void orc_flux_roe_x(const double *Wl, const double *Wr, double *F) {
+    pst l, r; memcpy(&l, Wl, sizeof l); memcpy(&r, Wr, sizeof r);
+    cst f = flux_roe_x(l, r);
+    memcpy(F, &f, sizeof f);
+}
+void orc_roe_average(const double *Wl, const double *Wr, double *Wa) {
+    pst l, r; memcpy(&l, Wl, sizeof l); memcpy(&r, Wr, sizeof r);
+    pst a = roe_average(l, r);
+    memcpy(Wa, &a, sizeof a);
+}
+double orc_limiter_venkatakrishnan(const double *uQuad, double u0, double u0Min, double u0Max) {
+    return limiter_venkatakrishnan(uQuad, u0, u0Min, u0Max, 4);
+}
+double orc_harten_fix_pos(double la, double ll, double lr) { return harten_fix_pos(la, ll, lr); }
+double orc_harten_fix_neg(double la, double ll, double lr) { return harten_fix_neg(la, ll, lr); }
+void orc_reflect(const double *W, double nx, double ny, double *out) {
+    pst w; memcpy(&w, W, sizeof w);
+    pst r = reflect(w, nx, ny);
+    memcpy(out, &r, sizeof r);
+}
+void orc_bc_characteristic_pressure(const double *Wi, const double *Wo, double nx, double ny, double *out) {
+    pst a, b; memcpy(&a, Wi, sizeof a); memcpy(&b, Wo, sizeof b);
+    pst r = bc_characteristic_pressure(a, b, nx, ny);
+    memcpy(out, &r, sizeof r);
+}
+void orc_W_of_U(const double *U, double *W) {
+    cst u; memcpy(&u, U, sizeof u);
+    pst w = W_of_U(u);
+    memcpy(W, &w, sizeof w);
+}
+void orc_U_of_W(const double *W, double *U) {
+    pst w; memcpy(&w, W, sizeof w);
+    cst u = U_of_W(w);
+    memcpy(U, &u, sizeof u);
+}
+void orc_Fx(const double *W, double *F) {
+    pst w; memcpy(&w, W, sizeof w);
+    cst f = W_Fx(w);
+    memcpy(F, &f, sizeof f);
+}
+double orc_runge_kutta(int i_stage, int n_stage) { return runge_kutta(i_stage, n_stage); }
+double orc_multistage_optimal_smoothing(int i_stage, int n_stage, int limiter) {
+    return multistage_optimal_smoothing(i_stage, n_stage, limiter);
+}
+void orc_quad_centroid(const double *x4, const double *y4, double *out3) {
+    /* vertices SW, SE, NE, NW */
+    double xs[4] = {x4[0], x4[1], x4[3], x4[2]}, ys[4] = {y4[0], y4[1], y4[3], y4[2]};
+    oblk B;
+    memset(&B, 0, sizeof B);
+    B.nci = 1;
+    B.xn = xs; B.yn = ys; /* node (i,j) at j*(nci+1)+i: (0,0)=SW (1,0)=SE (0,1)=NW (1,1)=NE */
+    quad_centroid(&B, 0, 0, &out3[0], &out3[1], &out3[2]);
+}

@@ -1,0 +1,192 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+FP64 runs must agree with the oracle within 1e-12 relative per cell (the
+kernels keep the reference's operation order, so bit identity is expected and
+reported); the residual norms are reduced in a different order and are
+compared at 1e-12 relative.  Mixed FP32-flux runs report their L1/Linf error
+against the FP64 oracle and are held to a loose bound.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cffc_amd import abi
+from cffc_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def euler2d():
+    from cffc_amd import euler2d as E
+    E.load()
+    return E
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def nonbitwise(a, b):
+    return int(np.sum(np.ascontiguousarray(a).view(np.uint64) != np.ascontiguousarray(b).view(np.uint64)))
+
+
+def run_both(euler2d, mesh, W0, scheme, nsteps, nthreads=8):
+    gpu = euler2d.Euler2DQuadBlocks(mesh, scheme)
+    gpu.set_W(W0)
+    rc, n, norms, dts = gpu.run_steps(nsteps)
+    orc = oracle.Oracle(mesh, scheme)
+    orc.set_W(W0)
+    rc2, n2, norms2, dts2 = orc.run(nsteps, nthreads=nthreads)
+    return gpu, orc, (rc, n, norms, dts), (rc2, n2, norms2, dts2)
+
+
+def test_riemann_4quadrant_small_fp64(euler2d):
+    m = M.cartesian(4, 4, 8, 8)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    gpu, orc, g, o = run_both(euler2d, m, W0, abi.Scheme(), 20)
+    assert g[0] == 0 and o[0] == 0 and g[1] == o[1] == 20
+    np.testing.assert_array_equal(g[3], o[3])          # global dt: min is order independent
+    Wg, Wo = gpu.get_W(), orc.get_W()
+    print("W rel %.3e, %d non-bitwise" % (relerr(Wg, Wo), nonbitwise(Wg, Wo)))
+    assert relerr(Wg, Wo) <= REL
+    assert relerr(gpu.get_U(), orc.get_U()) <= REL
+    assert relerr(g[2], o[2]) <= REL
+
+
+def test_granular_calls_match_oracle_stage_by_stage(euler2d):
+    """The reference-named entry points, one call at a time."""
+    m = M.cartesian(3, 2, 8, 8, jitter=0.1, seed=3)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    s = abi.Scheme()
+    gpu = euler2d.Euler2DQuadBlocks(m, s)
+    orc = oracle.Oracle(m, s)
+    gpu.set_W(W0)
+    orc.set_W(W0)
+    for n in range(3):
+        dg, do = gpu.CFL(), orc.cfl()
+        assert dg == do
+        for st in range(1, 5):
+            gpu.Send_All_Messages_and_BCs()
+            orc.exchange_and_bcs()
+            gpu.dUdt_Multistage_Explicit(st)
+            orc.dudt_stage(st)
+            k = 0 if st == 4 else st - 1
+            assert relerr(gpu.get_dUdt(k), orc.get_dUdt(k)) <= REL, (n, st)
+            assert gpu.Update_Solution_Multistage_Explicit(st) == 0
+            assert orc.update_stage(st)[0] == 0
+            assert relerr(gpu.get_W(), orc.get_W()) <= REL, (n, st)
+        ng, no = gpu.Residual_Norms(1), orc.residual_norms(1)
+        assert relerr(ng, no) <= REL
+
+
+def test_riemann_4quadrant_C1_full_fp64(euler2d):
+    """8 x 8 blocks of 32^2 cells, 200 RK4 steps, CFL 0.5 (the C1 workload)."""
+    m = M.cartesian(8, 8, 32, 32)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    gpu, orc, g, o = run_both(euler2d, m, W0, abi.Scheme(), 200)
+    assert g[0] == 0 and o[0] == 0
+    Wg, Wo = gpu.get_W(), orc.get_W()
+    print("C1 200 steps: W rel %.3e, %d non-bitwise of %d" % (relerr(Wg, Wo), nonbitwise(Wg, Wo), Wg.size))
+    assert relerr(Wg, Wo) <= REL
+    assert relerr(g[2], o[2]) <= REL
+
+
+@pytest.mark.parametrize("bcs", [
+    dict(W=abi.BC_REFLECTION, E=abi.BC_REFLECTION, S=abi.BC_REFLECTION, N=abi.BC_REFLECTION),
+    dict(W=abi.BC_FIXED, E=abi.BC_CHARACTERISTIC, S=abi.BC_REFLECTION, N=abi.BC_CONSTANT_EXTRAPOLATION),
+])
+def test_boundary_conditions_jittered_mesh(euler2d, bcs):
+    m = M.cartesian(3, 3, 8, 8, x0=0, x1=1.5, y0=0, y1=1.0, bcs=bcs, jitter=0.1, seed=11)
+    W0 = M.riemann_4quadrant(m, split=(0.75, 0.5))
+    m.set_wo_from_initial(W0)
+    gpu, orc, g, o = run_both(euler2d, m, W0, abi.Scheme(), 30)
+    assert g[0] == 0 and o[0] == 0
+    Wg, Wo = gpu.get_W(), orc.get_W()
+    print("bcs %s: rel %.3e, %d non-bitwise" % (bcs, relerr(Wg, Wo), nonbitwise(Wg, Wo)))
+    assert relerr(Wg, Wo) <= REL
+
+
+def test_local_time_stepping_and_other_schemes(euler2d):
+    m = M.cartesian(2, 2, 12, 12, jitter=0.05, seed=5)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    for s in (abi.Scheme(local_time_stepping=abi.SCALAR_LOCAL_TIME_STEPPING),
+              abi.Scheme(i_time_integration=abi.TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING, n_stage=5, cfl_number=0.8),
+              abi.Scheme(n_stage=2, cfl_number=0.4),
+              abi.Scheme(i_time_integration=abi.TIME_STEPPING_EXPLICIT_EULER, n_stage=1, cfl_number=0.3)):
+        gpu, orc, g, o = run_both(euler2d, m, W0, s, 15)
+        assert g[0] == o[0] == 0
+        assert relerr(gpu.get_W(), orc.get_W()) <= REL, s
+        assert relerr(g[2], o[2]) <= REL, s
+
+
+def test_graph_replay_equals_stream_loop(euler2d):
+    m = M.cartesian(4, 4, 16, 16)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    a = euler2d.Euler2DQuadBlocks(m, abi.Scheme())
+    b = euler2d.Euler2DQuadBlocks(m, abi.Scheme())
+    a.set_W(W0)
+    b.set_W(W0)
+    ra = a.run_steps(9)
+    rb = b.run_steps(9, graph=True)
+    np.testing.assert_array_equal(a.get_W(), b.get_W())
+    np.testing.assert_array_equal(ra[2], rb[2])
+    np.testing.assert_array_equal(ra[3], rb[3])
+
+
+def test_tile_shapes_agree(euler2d):
+    m = M.cartesian(2, 3, 32, 16, jitter=0.05, seed=9)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    ref = None
+    for ti, tj in ((16, 16), (32, 8), (8, 32), (16, 8), (8, 8), (32, 4)):
+        g = euler2d.Euler2DQuadBlocks(m, abi.Scheme())
+        g.set_tile(ti, tj)
+        g.set_W(W0)
+        g.run_steps(6)
+        W = g.get_W()
+        if ref is None:
+            ref = W
+        np.testing.assert_array_equal(W, ref)
+
+
+def test_mixed_precision_error_stated(euler2d):
+    m = M.cartesian(4, 4, 16, 16)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    g = euler2d.Euler2DQuadBlocks(m, abi.Scheme(flux_precision=abi.FLUX_FP32))
+    g.set_W(W0)
+    rc, n, norms, dts = g.run_steps(50)
+    orc = oracle.Oracle(m, abi.Scheme())
+    orc.set_W(W0)
+    orc.run(50, nthreads=4)
+    Wg, Wo = g.get_W(), orc.get_W()
+    d = np.abs(Wg - Wo)
+    l1 = d.mean(axis=0)
+    linf = d.max(axis=0)
+    print("mixed vs FP64 oracle after 50 steps: L1 %s Linf %s" % (l1, linf))
+    assert rc == 0
+    assert np.all(linf < 5e-3) and np.all(l1 < 1e-4)
+
+
+def test_negative_density_reports_reference_error(euler2d):
+    m = M.cartesian(2, 2, 8, 8)
+    W0 = M.uniform_state(m, (1.0, 0.0, 0.0, 1.0))
+    W0[37] = (1e-6, 5.0, 0.0, 1e-6)  # a near-vacuum cell next to fast flow
+    W0[38] = (1.0, -5.0, 0.0, 1.0)
+    m.set_wo_from_initial(W0)
+    g = euler2d.Euler2DQuadBlocks(m, abi.Scheme())
+    g.set_W(W0)
+    rc, n, _, _ = g.run_steps(5)
+    orc = oracle.Oracle(m, abi.Scheme())
+    orc.set_W(W0)
+    rc2, n2, _, _ = orc.run(5)
+    assert rc2 > 0, "the oracle should fail on this input"
+    assert rc == rc2
