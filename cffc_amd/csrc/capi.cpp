@@ -71,6 +71,13 @@ struct cffc_ctx {
     int phi_valid = 0;   // phi array holds the last evaluated limiters
     int store_phi = 0;   // keep phi stored every step (freeze may follow)
     bool granular_ready = false;
+    // optional per-kernel timing with event pairs (non-graph runs)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<int> ev_kind;   // 0 = exchange, 1 = stage (one entry per pair)
+    size_t ev_used = 0;
+    double t_ms[2] = {0, 0};
+    long long t_n[2] = {0, 0};
 };
 
 // ---------------------------------------------------------------------------
@@ -81,6 +88,7 @@ static void free_ctx(cffc_ctx *c) {
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
@@ -549,13 +557,43 @@ static int ensure_steps(cffc_ctx *c, int nsteps) {
     return 0;
 }
 
+static hipEvent_t next_event(cffc_ctx *c) {
+    if (c->ev_used == c->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        c->ev_pool.push_back(e);
+    }
+    return c->ev_pool[c->ev_used++];
+}
+static void mark(cffc_ctx *c, int kind, bool begin) {
+    if (!c->timing) return;
+    hipEvent_t e = next_event(c);
+    if (!e) return;
+    (void)hipEventRecord(e, c->stream);
+    if (begin) c->ev_kind.push_back(kind);
+}
+static void collect_timing(cffc_ctx *c) {
+    if (!c->timing) return;
+    for (size_t q = 0; q + 1 < c->ev_used && q / 2 < c->ev_kind.size(); q += 2) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, c->ev_pool[q], c->ev_pool[q + 1]) == hipSuccess) {
+            c->t_ms[c->ev_kind[q / 2]] += ms;
+            c->t_n[c->ev_kind[q / 2]] += 1;
+        }
+    }
+    c->ev_used = 0;
+    c->ev_kind.clear();
+}
+
 // enqueue one step (all stages) with step parity par
 static int enqueue_step(cffc_ctx *c, int par, double tol) {
     const int N = c->s.n_stage;
     for (int st = 1; st <= N; ++st) {
         ExchangeParams ex = exchange_params(c, st == 1, st == 1 ? c->scal + (1 - par) : nullptr);
         ex.stop_flag = c->stop_flag;
+        mark(c, 0, true);
         HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
+        mark(c, 0, false);
         StageParams p = base_stage_params(c);
         p.Win = c->W[c->wcur];
         p.Wout = c->W[1 - c->wcur];
@@ -572,7 +610,9 @@ static int enqueue_step(cffc_ctx *c, int par, double tol) {
         if (c->freeze) p.phi_mode = PHI_LOAD;
         else if (c->store_phi && st == N) p.phi_mode = PHI_COMPUTE_STORE;
         else p.phi_mode = PHI_COMPUTE;
+        mark(c, 1, true);
         HIPCHK(launch_stage(c, p, 1));
+        mark(c, 1, false);
         c->wcur = 1 - c->wcur;
     }
     return 0;
@@ -628,11 +668,16 @@ extern "C" int cffc_run_steps(cffc_ctx *c, int nsteps, double tol, double *norms
         }
         // poll for convergence / failure now and then without draining the queue
         if ((n & 63) == 63) {
+            if (c->timing) {
+                (void)hipStreamSynchronize(c->stream);
+                collect_timing(c);
+            }
             if (hflag[0]) break;
             (void)hipMemcpyAsync(hflag, c->stop_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
         }
     }
     (void)hipStreamSynchronize(c->stream);
+    collect_timing(c);
     (void)hipHostFree(hflag);
     return end_run(c, wstart, nsteps, norms, dt_out, steps_done);
 }
@@ -646,10 +691,13 @@ extern "C" int cffc_run_steps_graph(cffc_ctx *c, int nsteps, double *norms, doub
         hipGraph_t g = nullptr;
         hipGraphExec_t ge = nullptr;
         const int w0 = c->wcur;
+            const bool tsave = c->timing;
+        c->timing = false;  // no event pairs inside a captured graph
         HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         int e = enqueue_step(c, 0, 0.0);
         if (!e) e = enqueue_step(c, 1, 0.0);
         hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        c->timing = tsave;
         if (e) return e;
         if (ce != hipSuccess) return fail(CFFC_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
         HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
@@ -671,5 +719,24 @@ extern "C" int cffc_run_steps_graph(cffc_ctx *c, int nsteps, double *norms, doub
 extern "C" int cffc_set_store_limiters(cffc_ctx *c, int on) {
     if (!c) return fail(CFFC_E_ARG, "null context");
     c->store_phi = on ? 1 : 0;
+    return 0;
+}
+
+extern "C" int cffc_set_kernel_timing(cffc_ctx *c, int on) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    c->timing = on != 0;
+    c->t_ms[0] = c->t_ms[1] = 0;
+    c->t_n[0] = c->t_n[1] = 0;
+    c->ev_used = 0;
+    c->ev_kind.clear();
+    return 0;
+}
+
+extern "C" int cffc_kernel_times(cffc_ctx *c, double out[4]) {
+    if (!c || !out) return fail(CFFC_E_ARG, "null argument");
+    out[0] = c->t_ms[1];
+    out[1] = (double)c->t_n[1];
+    out[2] = c->t_ms[0];
+    out[3] = (double)c->t_n[0];
     return 0;
 }

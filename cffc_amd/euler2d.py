@@ -63,6 +63,8 @@ def load():
     L.cffc_stream.restype = vp
     L.cffc_set_tile.argtypes = [vp, i32, i32]
     L.cffc_get_tile.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
+    L.cffc_set_kernel_timing.argtypes = [vp, i32]
+    L.cffc_kernel_times.argtypes = [vp, pd]
     L.cffc_last_error.argtypes = []
     L.cffc_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -200,6 +202,16 @@ class Euler2DQuadBlocks:
         a, b = ctypes.c_int(), ctypes.c_int()
         load().cffc_get_tile(self.h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def kernel_timing(self, on=True):
+        _check(load().cffc_set_kernel_timing(self.h, 1 if on else 0), "cffc_set_kernel_timing")
+
+    def kernel_times(self):
+        """{stage_ms, stage_launches, exchange_ms, exchange_launches} since
+        kernel_timing(True)."""
+        out = np.zeros(4)
+        _check(load().cffc_kernel_times(self.h, _p(out)), "cffc_kernel_times")
+        return dict(stage_ms=out[0], stage_launches=int(out[1]), exchange_ms=out[2], exchange_launches=int(out[3]))
 
     def step_reference_order(self, nsteps):
         """The reference's loop (Euler2DQuadSolvers.cc:549-898) driven call by

@@ -209,6 +209,14 @@ void *cffc_stream(cffc_ctx *ctx);
 int  cffc_set_tile(cffc_ctx *ctx, int tile_i, int tile_j);
 int  cffc_get_tile(const cffc_ctx *ctx, int *tile_i, int *tile_j);
 
+/* Kernel timing for measurement: with timing on, cffc_run_steps brackets
+   every exchange/BC kernel and every fused stage kernel with a pair of HIP
+   events on the context stream; cffc_kernel_times returns {total stage-kernel
+   ms, stage launches, total exchange-kernel ms, exchange launches} accumulated
+   since timing was switched on.  (Not applied inside cffc_run_steps_graph.) */
+int  cffc_set_kernel_timing(cffc_ctx *ctx, int on);
+int  cffc_kernel_times(cffc_ctx *ctx, double out[4]);
+
 /* Last error message of this thread (static storage). */
 const char *cffc_last_error(void);
 

