@@ -113,6 +113,19 @@ def load_traffic(name):
     return None
 
 
+def copy_probe(nbytes):
+    """A plain device copy of a known size (torch's copy kernel), run under
+    rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to calibrate those counters."""
+    import torch
+    n = nbytes // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": n * 8, "copy_probe_bytes_written": n * 8, "launches": 3}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,7 +137,12 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain copy kernel of BYTES bytes (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole mesh on a single MI355X "
               "(multi-GPU block sharding is a design note only); use --gpus 1", file=sys.stderr)

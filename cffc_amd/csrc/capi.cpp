@@ -232,7 +232,9 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
         m.ioff = ioff;
         m.noff = noff;
         m.bcoff = bcoff;
-        m.pad = 0;
+        m.has_bc = 0;
+        for (int q = 0; q < 2 * (m.ni + 4) + 2 * (m.nj + 4); ++q)
+            if (md->bc[bcoff + q] != CFFC_BC_NONE) m.has_bc = 1;
         coff += (int64_t)m.nci * m.ncj;
         ioff += (int64_t)m.ni * m.nj;
         noff += (int64_t)(m.nci + 1) * (m.ncj + 1);

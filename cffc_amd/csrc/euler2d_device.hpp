@@ -30,7 +30,7 @@ struct BlockMeta {
     int64_t ioff;    // first interior cell of the block in the host state arrays
     int64_t noff;    // first node of the block in the node arrays
     int32_t bcoff;   // first entry of the block in the bc / wo arrays
-    int32_t pad;
+    int32_t has_bc;  // nonzero if any BCtypeN/S/E/W entry is not BC_NONE
 };
 
 struct TileDesc {

@@ -463,8 +463,8 @@ hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, u
 hipError_t cffc_launch_stage_fp64(const StageParams *p, int ti, int tj, int fused, hipStream_t st) {
 #define L(TI, TJ)                                                                                          \
     if (ti == TI && tj == TJ) {                                                                            \
-        if (fused) hipLaunchKernelGGL((stage_kernel<double, TI, TJ, true>), dim3(p->ntiles), dim3(TI * TJ), 0, st, *p); \
-        else hipLaunchKernelGGL((stage_kernel<double, TI, TJ, false>), dim3(p->ntiles), dim3(TI * TJ), 0, st, *p); \
+        if (fused) hipLaunchKernelGGL((stage_kernel<double, TI, TJ, true>), dim3(p->ntiles), dim3((TileThreads<TI, TJ>::value)), 0, st, *p); \
+        else hipLaunchKernelGGL((stage_kernel<double, TI, TJ, false>), dim3(p->ntiles), dim3((TileThreads<TI, TJ>::value)), 0, st, *p); \
         return hipGetLastError();                                                                          \
     }
     CFFC_TILE_LIST(L)

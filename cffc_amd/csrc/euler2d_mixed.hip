@@ -12,8 +12,8 @@ using namespace cffc;
 extern "C" hipError_t cffc_launch_stage_mixed(const StageParams *p, int ti, int tj, int fused, hipStream_t st) {
 #define L(TI, TJ)                                                                                          \
     if (ti == TI && tj == TJ) {                                                                            \
-        if (fused) hipLaunchKernelGGL((stage_kernel<float, TI, TJ, true>), dim3(p->ntiles), dim3(TI * TJ), 0, st, *p); \
-        else hipLaunchKernelGGL((stage_kernel<float, TI, TJ, false>), dim3(p->ntiles), dim3(TI * TJ), 0, st, *p); \
+        if (fused) hipLaunchKernelGGL((stage_kernel<float, TI, TJ, true>), dim3(p->ntiles), dim3((TileThreads<TI, TJ>::value)), 0, st, *p); \
+        else hipLaunchKernelGGL((stage_kernel<float, TI, TJ, false>), dim3(p->ntiles), dim3((TileThreads<TI, TJ>::value)), 0, st, *p); \
         return hipGetLastError();                                                                          \
     }
     CFFC_TILE_LIST(L)

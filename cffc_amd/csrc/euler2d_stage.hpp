@@ -109,9 +109,124 @@ __device__ __forceinline__ PS<FT> face_state(const PS<FT> &W, const PS<FT> &px, 
     return r;
 }
 
+// Least-squares gradient and Venkatakrishnan limiter of one cell
+// (Linear_Reconstruction_LeastSquares, SingleBlock.cc:3678-4087).  FULL: the
+// 8-point stencil, straight-line; otherwise the reduced stencil in mask.
+// The window arrays are indexed [row][col] with the cell at (jl, il).
+template <typename FT, bool FULL, int LJ, int LI>
+__device__ __forceinline__ void reconstruct(const StageParams &p, int64_t c, int64_t ncp, int nci, int il, int jl,
+                                            unsigned mask, const FT (&sW)[4][LJ][LI], const double (&sX)[LJ][LI],
+                                            const double (&sY)[LJ][LI], PS<FT> &gx, PS<FT> &gy, PS<FT> &ph) {
+    const PS<FT> W0 = {sW[0][jl][il], sW[1][jl][il], sW[2][jl][il], sW[3][jl][il]};
+    gx = {FT(0), FT(0), FT(0), FT(0)};
+    gy = gx;
+    ph = gx;
+    if (!FULL && mask == 0u) return;
+    const double *G = p.geom;
+    PS<FT> Dx = {FT(0), FT(0), FT(0), FT(0)}, Dy = Dx;
+    FT DxDx = FT(0), DxDy = FT(0), DyDy = FT(0);
+    const double xc0 = sX[jl][il], yc0 = sY[jl][il];
+    int n_pts = 0;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const int kk = s < 4 ? s : s + 1;  // skip the centre of the 3x3 window
+        const int a = il + (kk % 3) - 1, b = jl + (kk / 3) - 1;
+        const FT dX = (FT)(sX[b][a] - xc0), dY = (FT)(sY[b][a] - yc0);
+        const PS<FT> DU = {sW[0][b][a] - W0.d, sW[1][b][a] - W0.vx, sW[2][b][a] - W0.vy, sW[3][b][a] - W0.p};
+        if (FULL || (mask & (1u << s))) {
+            Dx.d += DU.d * dX; Dx.vx += DU.vx * dX; Dx.vy += DU.vy * dX; Dx.p += DU.p * dX;
+            Dy.d += DU.d * dY; Dy.vx += DU.vx * dY; Dy.vy += DU.vy * dY; Dy.p += DU.p * dY;
+            DxDx += dX * dX; DxDy += dX * dY; DyDy += dY * dY;
+            ++n_pts;
+        }
+    }
+    // averages over the stencil: x/8 and x*0.125 round identically, so the
+    // full stencil multiplies; reduced stencils divide (reference: /n_pts)
+    if (FULL || Fast<FT>::on) {
+        const FT rn = FULL ? FT(0.125) : FT(1) / (FT)n_pts;
+        Dx.d *= rn; Dx.vx *= rn; Dx.vy *= rn; Dx.p *= rn;
+        Dy.d *= rn; Dy.vx *= rn; Dy.vy *= rn; Dy.p *= rn;
+        DxDx *= rn; DxDy *= rn; DyDy *= rn;
+    } else {
+        const FT np = (FT)n_pts;
+        Dx.d /= np; Dx.vx /= np; Dx.vy /= np; Dx.p /= np;
+        Dy.d /= np; Dy.vx /= np; Dy.vy /= np; Dy.p /= np;
+        DxDx /= np; DxDy /= np; DyDy /= np;
+    }
+    const FT det = DxDx * DyDy - DxDy * DxDy;
+    if constexpr (Fast<FT>::on) {
+        const FT rd = FT(1) / det;
+        gx.d = (Dx.d * DyDy - Dy.d * DxDy) * rd; gx.vx = (Dx.vx * DyDy - Dy.vx * DxDy) * rd;
+        gx.vy = (Dx.vy * DyDy - Dy.vy * DxDy) * rd; gx.p = (Dx.p * DyDy - Dy.p * DxDy) * rd;
+        gy.d = (Dy.d * DxDx - Dx.d * DxDy) * rd; gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) * rd;
+        gy.vy = (Dy.vy * DxDx - Dx.vy * DxDy) * rd; gy.p = (Dy.p * DxDx - Dx.p * DxDy) * rd;
+    } else {
+        gx.d = (Dx.d * DyDy - Dy.d * DxDy) / det; gx.vx = (Dx.vx * DyDy - Dy.vx * DxDy) / det;
+        gx.vy = (Dx.vy * DyDy - Dy.vy * DxDy) / det; gx.p = (Dx.p * DyDy - Dy.p * DxDy) / det;
+        gy.d = (Dy.d * DxDx - Dx.d * DxDy) / det; gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) / det;
+        gy.vy = (Dy.vy * DxDx - Dx.vy * DxDy) / det; gy.p = (Dy.p * DxDx - Dx.p * DxDy) / det;
+    }
+    if (p.phi_mode == PHI_LOAD) {
+        ph.d = (FT)p.phi[c]; ph.vx = (FT)p.phi[ncp + c]; ph.vy = (FT)p.phi[2 * ncp + c]; ph.p = (FT)p.phi[3 * ncp + c];
+        return;
+    }
+    if (p.limiter == -1) { ph.d = ph.vx = ph.vy = ph.p = FT(1); return; }
+    if (p.limiter == 0) { ph.d = ph.vx = ph.vy = ph.p = FT(0); return; }
+    // face-midpoint offsets: E, W (= E of i-1), N, S (= N of j-1)
+    const FT dxE = (FT)(G[GEO_XFE * ncp + c] - xc0), dyE = (FT)(G[GEO_YFE * ncp + c] - yc0);
+    const FT dxW = (FT)(G[GEO_XFE * ncp + c - 1] - xc0), dyW = (FT)(G[GEO_YFE * ncp + c - 1] - yc0);
+    const FT dxN = (FT)(G[GEO_XFN * ncp + c] - xc0), dyN = (FT)(G[GEO_YFN * ncp + c] - yc0);
+    const FT dxS = (FT)(G[GEO_XFN * ncp + c - nci] - xc0), dyS = (FT)(G[GEO_YFN * ncp + c - nci] - yc0);
+#pragma unroll
+    for (int n = 0; n < 4; ++n) {
+        const FT u0 = n == 0 ? W0.d : n == 1 ? W0.vx : n == 2 ? W0.vy : W0.p;
+        FT umin = u0, umax = u0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+            const int kk = s < 4 ? s : s + 1;
+            const FT v = sW[n][jl + (kk / 3) - 1][il + (kk % 3) - 1];
+            if (FULL || (mask & (1u << s))) {
+                if constexpr (Fast<FT>::on) { umin = fminf(umin, v); umax = fmaxf(umax, v); }
+                else { umin = dmin(umin, v); umax = dmax(umax, v); }
+            }
+        }
+        const FT gxn = n == 0 ? gx.d : n == 1 ? gx.vx : n == 2 ? gx.vy : gx.p;
+        const FT gyn = n == 0 ? gy.d : n == 1 ? gy.vx : n == 2 ? gy.vy : gy.p;
+        FT phi = FT(1);
+        phi = venkat_fold(phi, u0 + gxn * dxE + gyn * dyE, u0, umin, umax);
+        phi = venkat_fold(phi, u0 + gxn * dxW + gyn * dyW, u0, umin, umax);
+        phi = venkat_fold(phi, u0 + gxn * dxN + gyn * dyN, u0, umin, umax);
+        phi = venkat_fold(phi, u0 + gxn * dxS + gyn * dyS, u0, umin, umax);
+        if (n == 0) ph.d = phi; else if (n == 1) ph.vx = phi; else if (n == 2) ph.vy = phi; else ph.p = phi;
+    }
+}
+
+// The update's rejection test U.d <= 0 || U.E <= 0 || U.e() <= 0
+// (Euler2DQuadSingleBlock.cc:5367-5371).  e() costs two divisions; when
+// E*d exceeds 0.5*|dv|^2 by a relative margin of 1e-6 (orders of magnitude
+// above rounding) e() is certainly positive and is not evaluated, so the
+// outcome is the reference's exactly.
+template <typename FT> __device__ __forceinline__ bool positivity_fails(const CS<double> &U) {
+    if (U.d <= 0.0 || U.E <= 0.0) return true;
+    const double k2 = 0.5 * (U.mx * U.mx + U.my * U.my);
+    if (U.E * U.d > k2 * (1.0 + 1e-6)) return false;
+    if constexpr (Fast<FT>::on) return U.E * U.d <= k2;
+    else return U_e(U) <= 0.0;
+}
+
+// Threads per workgroup for a TI x TJ tile: one per tile cell plus one per
+// halo ("ring") cell whose reconstruction the tile's boundary faces need,
+// rounded up to whole wavefronts.  For 16 x 16: 256 + 64 = 320 (5 waves),
+// so every thread reconstructs exactly one cell.
+template <int TI, int TJ> struct TileThreads {
+    static constexpr int tile = TI * TJ, ring = 2 * (TI + TJ);
+    static constexpr int value = ((tile + ring + 63) / 64) * 64;
+};
+
 template <typename FT, int TI, int TJ, bool FUSED>
-__global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
-    constexpr int NT = TI * TJ;
+__global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(StageParams p) {
+    constexpr int NT = TileThreads<TI, TJ>::value;
+    constexpr int NTILE = TI * TJ;
     constexpr int LI = TI + 4, LJ = TJ + 4;
     constexpr int RI = TI + 2, RJ = TJ + 2;
     __shared__ FT sW[4][LJ][LI];
@@ -130,6 +245,18 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
     const int ICu = m.ni + 1, JCu = m.nj + 1;
     const int pi0 = t.i0, pj0 = t.j0;   // padded index of the LDS window origin
     const int32_t *bcW = p.bc + m.bcoff, *bcE = bcW + m.ncj, *bcS = bcE + m.ncj, *bcN = bcS + m.nci;
+    // role of this thread: a tile cell (tc, tr), or a ring cell of side
+    // 0 = W (column -1), 1 = E (column TI), 2 = S (row -1), 3 = N (row TJ)
+    const bool is_tile = tid < NTILE;
+    const int tc = tid % TI, tr = tid / TI;
+    int ring_side = -1, ring_k = 0;
+    if (!is_tile) {
+        int r = tid - NTILE;
+        if (r < TJ) { ring_side = 0; ring_k = r; }
+        else if ((r -= TJ) < TJ) { ring_side = 1; ring_k = r; }
+        else if ((r -= TJ) < TI) { ring_side = 2; ring_k = r; }
+        else if ((r -= TI) < TI) { ring_side = 3; ring_k = r; }
+    }
 
     // ---- phase 1: W and centroids of the window ---------------------------
     for (int q = tid; q < LI * LJ; q += NT) {
@@ -146,119 +273,61 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
     __syncthreads();
 
     // ---- phase 2: least-squares reconstruction + limiter -------------------
-    for (int q = tid; q < RI * RJ; q += NT) {
-        const int ir = q % RI, jr = q / RI;
+    // One cell per thread: tile threads take their own cell, ring threads the
+    // halo cell next to the tile.  Cells of blocks without physical
+    // boundaries, and cells away from the ring next to a physical boundary,
+    // use the full 8-point stencil; only the ring cells of boundary blocks
+    // read the BC arrays (reduced stencils).
+    {
+        int ir = -1, jr = -1;
+        if (is_tile) { ir = tc + 1; jr = tr + 1; }
+        else if (ring_side == 0) { ir = 0; jr = ring_k + 1; }
+        else if (ring_side == 1) { ir = TI + 1; jr = ring_k + 1; }
+        else if (ring_side == 2) { ir = ring_k + 1; jr = 0; }
+        else if (ring_side == 3) { ir = ring_k + 1; jr = TJ + 1; }
         const int il = ir + 1, jl = jr + 1;
         const int ip = pi0 + il, jp = pj0 + jl;
-        if (ip > m.nci - 2 || jp > m.ncj - 2) continue;
-        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
-        const unsigned mask = stencil_mask(ip, jp, ICu, JCu, bcW[jp], bcE[jp], bcS[ip], bcN[ip]);
-        const PS<FT> W0 = {sW[0][jl][il], sW[1][jl][il], sW[2][jl][il], sW[3][jl][il]};
-        PS<FT> gx = {FT(0), FT(0), FT(0), FT(0)}, gy = gx, ph = gx;
-        if (mask) {
-            PS<FT> Dx = {FT(0), FT(0), FT(0), FT(0)}, Dy = Dx;
-            FT DxDx = FT(0), DxDy = FT(0), DyDy = FT(0);
-            const double xc0 = sX[jl][il], yc0 = sY[jl][il];
-            int n_pts = 0;
-#pragma unroll
-            for (int s = 0; s < 8; ++s) {
-                if (!(mask & (1u << s))) continue;
-                const int kk = s < 4 ? s : s + 1;  // skip the centre (1,1)
-                const int a = il + (kk % 3) - 1, b = jl + (kk / 3) - 1;
-                const FT dX = (FT)(sX[b][a] - xc0), dY = (FT)(sY[b][a] - yc0);
-                const PS<FT> DU = {sW[0][b][a] - W0.d, sW[1][b][a] - W0.vx, sW[2][b][a] - W0.vy,
-                                   sW[3][b][a] - W0.p};
-                Dx.d += DU.d * dX; Dx.vx += DU.vx * dX; Dx.vy += DU.vy * dX; Dx.p += DU.p * dX;
-                Dy.d += DU.d * dY; Dy.vx += DU.vx * dY; Dy.vy += DU.vy * dY; Dy.p += DU.p * dY;
-                DxDx += dX * dX; DxDy += dX * dY; DyDy += dY * dY;
-                ++n_pts;
-            }
-            const FT np = (FT)n_pts;
-            Dx.d /= np; Dx.vx /= np; Dx.vy /= np; Dx.p /= np;
-            Dy.d /= np; Dy.vx /= np; Dy.vy /= np; Dy.p /= np;
-            DxDx /= np; DxDy /= np; DyDy /= np;
-            const FT det = DxDx * DyDy - DxDy * DxDy;
-            gx.d = (Dx.d * DyDy - Dy.d * DxDy) / det;
-            gx.vx = (Dx.vx * DyDy - Dy.vx * DxDy) / det;
-            gx.vy = (Dx.vy * DyDy - Dy.vy * DxDy) / det;
-            gx.p = (Dx.p * DyDy - Dy.p * DxDy) / det;
-            gy.d = (Dy.d * DxDx - Dx.d * DxDy) / det;
-            gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) / det;
-            gy.vy = (Dy.vy * DxDx - Dx.vy * DxDy) / det;
-            gy.p = (Dy.p * DxDx - Dx.p * DxDy) / det;
-            if (p.phi_mode == PHI_LOAD) {
-                ph.d = (FT)p.phi[c]; ph.vx = (FT)p.phi[ncp + c];
-                ph.vy = (FT)p.phi[2 * ncp + c]; ph.p = (FT)p.phi[3 * ncp + c];
-            } else if (p.limiter == -1) {
-                ph.d = ph.vx = ph.vy = ph.p = FT(1);
-            } else if (p.limiter == 0) {
-                ph.d = ph.vx = ph.vy = ph.p = FT(0);
-            } else {
-                // face-midpoint offsets: E, W (= E of i-1), N, S (= N of j-1)
-                const FT dxE = (FT)(G[GEO_XFE * ncp + c] - xc0), dyE = (FT)(G[GEO_YFE * ncp + c] - yc0);
-                const FT dxW = (FT)(G[GEO_XFE * ncp + c - 1] - xc0), dyW = (FT)(G[GEO_YFE * ncp + c - 1] - yc0);
-                const FT dxN = (FT)(G[GEO_XFN * ncp + c] - xc0), dyN = (FT)(G[GEO_YFN * ncp + c] - yc0);
-                const FT dxS = (FT)(G[GEO_XFN * ncp + c - m.nci] - xc0),
-                         dyS = (FT)(G[GEO_YFN * ncp + c - m.nci] - yc0);
-#pragma unroll
-                for (int n = 0; n < 4; ++n) {
-                    const FT u0 = n == 0 ? W0.d : n == 1 ? W0.vx : n == 2 ? W0.vy : W0.p;
-                    FT umin = u0, umax = u0;
-#pragma unroll
-                    for (int s = 0; s < 8; ++s) {
-                        if (!(mask & (1u << s))) continue;
-                        const int kk = s < 4 ? s : s + 1;
-                        const FT v = sW[n][jl + (kk / 3) - 1][il + (kk % 3) - 1];
-                        umin = dmin(umin, v);
-                        umax = dmax(umax, v);
-                    }
-                    const FT gxn = n == 0 ? gx.d : n == 1 ? gx.vx : n == 2 ? gx.vy : gx.p;
-                    const FT gyn = n == 0 ? gy.d : n == 1 ? gy.vx : n == 2 ? gy.vy : gy.p;
-                    FT phi = FT(1);
-                    phi = venkat_fold(phi, u0 + gxn * dxE + gyn * dyE, u0, umin, umax);
-                    phi = venkat_fold(phi, u0 + gxn * dxW + gyn * dyW, u0, umin, umax);
-                    phi = venkat_fold(phi, u0 + gxn * dxN + gyn * dyN, u0, umin, umax);
-                    phi = venkat_fold(phi, u0 + gxn * dxS + gyn * dyS, u0, umin, umax);
-                    if (n == 0) ph.d = phi; else if (n == 1) ph.vx = phi; else if (n == 2) ph.vy = phi; else ph.p = phi;
-                }
-            }
+        if (ir >= 0 && ip <= m.nci - 2 && jp <= m.ncj - 2) {
+            const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+            unsigned mask = 0xFFu;
+            if (m.has_bc && (ip <= 1 || ip >= ICu + 1 || jp <= 1 || jp >= JCu + 1))
+                mask = stencil_mask(ip, jp, ICu, JCu, bcW[jp], bcE[jp], bcS[ip], bcN[ip]);
+            PS<FT> gx, gy, ph;
+            if (mask == 0xFFu) reconstruct<FT, true>(p, c, ncp, m.nci, il, jl, 0xFFu, sW, sX, sY, gx, gy, ph);
+            else reconstruct<FT, false>(p, c, ncp, m.nci, il, jl, mask, sW, sX, sY, gx, gy, ph);
             if (p.phi_mode == PHI_COMPUTE_STORE) {
+                // each stored cell is owned by exactly one tile: its own
+                // cells, plus the block's outer ring on the block edge
                 const bool own_i = (ip >= pi0 + 2 && ip < pi0 + 2 + TI) || ip == 1 || ip == m.nci - 2;
                 const bool own_j = (jp >= pj0 + 2 && jp < pj0 + 2 + TJ) || jp == 1 || jp == m.ncj - 2;
                 if (own_i && own_j) {
                     p.phi[c] = ph.d; p.phi[ncp + c] = ph.vx; p.phi[2 * ncp + c] = ph.vy; p.phi[3 * ncp + c] = ph.p;
                 }
             }
-        } else if (p.phi_mode == PHI_COMPUTE_STORE) {
-            const bool own_i = (ip >= pi0 + 2 && ip < pi0 + 2 + TI) || ip == 1 || ip == m.nci - 2;
-            const bool own_j = (jp >= pj0 + 2 && jp < pj0 + 2 + TJ) || jp == 1 || jp == m.ncj - 2;
-            if (own_i && own_j) { p.phi[c] = 0; p.phi[ncp + c] = 0; p.phi[2 * ncp + c] = 0; p.phi[3 * ncp + c] = 0; }
+            // phi^dWdx, phi^dWdy (zero stencil: both products are 0*0)
+            sPx[0][jr][ir] = ph.d * gx.d; sPx[1][jr][ir] = ph.vx * gx.vx;
+            sPx[2][jr][ir] = ph.vy * gx.vy; sPx[3][jr][ir] = ph.p * gx.p;
+            sPy[0][jr][ir] = ph.d * gy.d; sPy[1][jr][ir] = ph.vx * gy.vx;
+            sPy[2][jr][ir] = ph.vy * gy.vy; sPy[3][jr][ir] = ph.p * gy.p;
         }
-        // phi^dWdx, phi^dWdy (zero stencil: both products are 0*0)
-        sPx[0][jr][ir] = ph.d * gx.d; sPx[1][jr][ir] = ph.vx * gx.vx;
-        sPx[2][jr][ir] = ph.vy * gx.vy; sPx[3][jr][ir] = ph.p * gx.p;
-        sPy[0][jr][ir] = ph.d * gy.d; sPy[1][jr][ir] = ph.vx * gy.vx;
-        sPy[2][jr][ir] = ph.vy * gy.vy; sPy[3][jr][ir] = ph.p * gy.p;
     }
     __syncthreads();
 
     // ---- phase 3: Roe fluxes ------------------------------------------------
-    constexpr int NFI = (TI + 1) * TJ, NFJ = TI * (TJ + 1);
-    for (int q = tid; q < NFI + NFJ; q += NT) {
-        const bool idir = q < NFI;
-        int ipL, jpL, rl_i, rl_j, rr_i, rr_j;  // left cell (padded) and recon coords
+    // Tile thread (tc,tr): its E face (i-face tc+1) and N face (j-face tr+1);
+    // W-ring thread k: i-face 0 of row k; S-ring thread k: j-face 0 of
+    // column k.  E/N-ring threads have no face.
+    auto face = [&](bool idir, int f, int g) {
+        // idir: i-face f in row g; else j-face f in column g.  L/R recon coords:
+        int ipL, jpL, rl_i, rl_j, rr_i, rr_j;
         if (idir) {
-            const int fi = q % (TI + 1), tr = q / (TI + 1);
-            ipL = pi0 + 1 + fi; jpL = pj0 + 2 + tr;
-            if (jpL > JCu || ipL > ICu) continue;
-            rl_i = fi; rl_j = tr + 1; rr_i = fi + 1; rr_j = tr + 1;
+            ipL = pi0 + 1 + f; jpL = pj0 + 2 + g;
+            rl_i = f; rl_j = g + 1; rr_i = f + 1; rr_j = g + 1;
         } else {
-            const int qq = q - NFI;
-            const int tc = qq % TI, fj = qq / TI;
-            ipL = pi0 + 2 + tc; jpL = pj0 + 1 + fj;
-            if (ipL > ICu || jpL > JCu) continue;
-            rl_i = tc + 1; rl_j = fj; rr_i = tc + 1; rr_j = fj + 1;
+            ipL = pi0 + 2 + g; jpL = pj0 + 1 + f;
+            rl_i = g + 1; rl_j = f; rr_i = g + 1; rr_j = f + 1;
         }
+        if (jpL > JCu || ipL > ICu) return;
         const int64_t cL = m.coff + (int64_t)jpL * m.nci + ipL;
         const double nxd = idir ? G[GEO_NEX * ncp + cL] : G[GEO_NNX * ncp + cL];
         const double nyd = idir ? G[GEO_NEY * ncp + cL] : G[GEO_NNY * ncp + cL];
@@ -273,14 +342,18 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
             W = face_state(W, px, py, (FT)(xf - sX[jl][il]), (FT)(yf - sY[jl][il]));
         };
         PS<FT> Wl, Wr;
-        int bcL, bcR, slotL, slotR;
+        int bcL = 0, bcR = 0, slotL, slotR;
         if (idir) {
-            bcL = (ipL == 1) ? bcW[jpL] : 0;
-            bcR = (ipL == ICu) ? bcE[jpL] : 0;
+            if (m.has_bc) {
+                bcL = (ipL == 1) ? bcW[jpL] : 0;
+                bcR = (ipL == ICu) ? bcE[jpL] : 0;
+            }
             slotL = jpL; slotR = m.ncj + jpL;
         } else {
-            bcL = (jpL == 1) ? bcS[ipL] : 0;
-            bcR = (jpL == JCu) ? bcN[ipL] : 0;
+            if (m.has_bc) {
+                bcL = (jpL == 1) ? bcS[ipL] : 0;
+                bcR = (jpL == JCu) ? bcN[ipL] : 0;
+            }
             slotL = 2 * m.ncj + ipL; slotR = 2 * m.ncj + m.nci + ipL;
         }
         if (face_bc(bcL)) {
@@ -309,20 +382,24 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
         }
         const CS<FT> F = flux_roe_n(Wl, Wr, nx, ny);
         if (idir) {
-            const int fi = q % (TI + 1), tr = q / (TI + 1);
-            sFi[0][tr][fi] = F.d; sFi[1][tr][fi] = F.mx; sFi[2][tr][fi] = F.my; sFi[3][tr][fi] = F.E;
+            sFi[0][g][f] = F.d; sFi[1][g][f] = F.mx; sFi[2][g][f] = F.my; sFi[3][g][f] = F.E;
         } else {
-            const int qq = q - NFI;
-            const int tc = qq % TI, fj = qq / TI;
-            sFj[0][fj][tc] = F.d; sFj[1][fj][tc] = F.mx; sFj[2][fj][tc] = F.my; sFj[3][fj][tc] = F.E;
+            sFj[0][f][g] = F.d; sFj[1][f][g] = F.mx; sFj[2][f][g] = F.my; sFj[3][f][g] = F.E;
         }
+    };
+    if (is_tile) {
+        face(true, tc + 1, tr);
+        face(false, tr + 1, tc);
+    } else if (ring_side == 0) {
+        face(true, 0, ring_k);
+    } else if (ring_side == 2) {
+        face(false, 0, ring_k);
     }
     __syncthreads();
 
     // ---- phase 4: accumulate (+W -E +S -N) and update -------------------------
-    const int tc = tid % TI, tr = tid / TI;
     const int ip = pi0 + 2 + tc, jp = pj0 + 2 + tr;
-    const bool valid = (ip <= ICu) && (jp <= JCu);
+    const bool valid = is_tile && (ip <= ICu) && (jp <= JCu);
     double l1 = 0.0, l2 = 0.0, mx = 0.0, dtn = MILLION;
     if (valid) {
         const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
@@ -353,16 +430,29 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
                 for (int v = 0; v < 4; ++v) dU[v] = 0.0;
             }
         }
+        if constexpr (Fast<FT>::on) {
+            // mixed path: net face flux in FP32, one FP64 scaling per cell
+            const double s = cdt / A;
+            const FT flW = (FT)lW, flE = (FT)lE, flS = (FT)lS, flN = (FT)lN;
 #pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            const double FW = (double)sFi[v][tr][tc], FE = (double)sFi[v][tr][tc + 1];
-            const double FS = (double)sFj[v][tr][tc], FN = (double)sFj[v][tr + 1][tc];
-            double d = dU[v];
-            d += cdt * FW * lW / A;
-            d -= cdt * FE * lE / A;
-            d += cdt * FS * lS / A;
-            d -= cdt * FN * lN / A;
-            dU[v] = d;
+            for (int v = 0; v < 4; ++v) {
+                const FT net = (sFi[v][tr][tc] * flW - sFi[v][tr][tc + 1] * flE) +
+                               (sFj[v][tr][tc] * flS - sFj[v][tr + 1][tc] * flN);
+                dU[v] += s * (double)net;
+            }
+        } else {
+            // ((CFL*dt)*F*l)/A per face, +W -E +S -N (Euler2DQuad.cc:1502-1509)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const double FW = (double)sFi[v][tr][tc], FE = (double)sFi[v][tr][tc + 1];
+                const double FS = (double)sFj[v][tr][tc], FN = (double)sFj[v][tr + 1][tc];
+                double d = dU[v];
+                d += cdt * FW * lW / A;
+                d -= cdt * FE * lE / A;
+                d += cdt * FS * lS / A;
+                d -= cdt * FN * lN / A;
+                dU[v] = d;
+            }
         }
         if (!FUSED) {
 #pragma unroll
@@ -373,7 +463,7 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
             const double U0[4] = {p.U[c], p.U[ncp + c], p.U[2 * ncp + c], p.U[3 * ncp + c]};
             Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
             Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
-            bool bad = (Un.d <= 0.0 || Un.E <= 0.0 || U_e(Un) <= 0.0);
+            bool bad = positivity_fails<FT>(Un);
             if (bad && p.lts) {
                 // scalar local time stepping: halve dt and dUdt up to 10 times
                 double f = 1.0, dtl = dtc;
@@ -384,10 +474,10 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
                     for (int v = 0; v < 4; ++v) dU[v] = f * dU[v];
                     Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
                     Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
-                    if (Un.d > 0.0 && Un.E > 0.0 && U_e(Un) > 0.0) break;
+                    if (!positivity_fails<FT>(Un)) break;
                 }
                 p.dt[c] = dtl;
-                bad = (Un.d <= 0.0 || Un.E <= 0.0 || U_e(Un) <= 0.0);
+                bad = positivity_fails<FT>(Un);
             }
             if (bad) {
                 const unsigned long long key =
@@ -396,7 +486,14 @@ __global__ __launch_bounds__(TI *TJ) void stage_kernel(StageParams p) {
                 // the reference returns from the failing stage; later kernels skip
                 if (p.stop_flag) atomicExch(p.stop_flag, 1);
             }
-            const PS<double> Wn = W_of_U(Un);
+            PS<double> Wn;
+            if constexpr (Fast<FT>::on) {
+                const double r = 1.0 / Un.d;
+                Wn.d = Un.d; Wn.vx = Un.mx * r; Wn.vy = Un.my * r;
+                Wn.p = gm1<double>() * (Un.E - 0.5 * (Un.mx * Wn.vx + Un.my * Wn.vy));
+            } else {
+                Wn = W_of_U(Un);
+            }
             p.Wout[c] = Wn.d; p.Wout[ncp + c] = Wn.vx; p.Wout[2 * ncp + c] = Wn.vy; p.Wout[3 * ncp + c] = Wn.p;
             if (p.mode == MODE_ZERO_ACC2) {
 #pragma unroll

@@ -40,6 +40,20 @@ template <typename T> CFFC_HD T tsqrt(T x) { return sqrt(x); }
 template <> CFFC_HD float tsqrt<float>(float x) { return sqrtf(x); }
 template <> CFFC_HD float tabs<float>(float x) { return fabsf(x); }
 
+// Arithmetic policy.  double: exact IEEE division and square root, in the
+// reference's order (bitwise path).  float (mixed path, no bitwise
+// requirement): hardware reciprocal / square-root approximations.
+template <typename T> struct Fast { static constexpr bool on = false; };
+template <> struct Fast<float> { static constexpr bool on = true; };
+template <typename T> CFFC_HD T qdiv(T a, T b) {
+    if constexpr (Fast<T>::on) return a * __builtin_amdgcn_rcpf(b);
+    else return a / b;
+}
+template <typename T> CFFC_HD T qsqrt(T a) {
+    if constexpr (Fast<T>::on) return __builtin_amdgcn_sqrtf(a);
+    else return sqrt(a);
+}
+
 template <typename T> struct PS { T d, vx, vy, p; };   // Euler2D_pState
 template <typename T> struct CS { T d, mx, my, E; };   // Euler2D_cState
 
@@ -64,13 +78,13 @@ template <typename T> CFFC_HD CS<T> U_of_W(const PS<T> &W) {
     return U;
 }
 template <typename T> CFFC_HD T W_h(const PS<T> &W) {
-    return gam<T>() * W.p / (gm1<T>() * W.d) + Consts<T>::HALF * (W.vx * W.vx + W.vy * W.vy);
+    return qdiv(gam<T>() * W.p, gm1<T>() * W.d) + Consts<T>::HALF * (W.vx * W.vx + W.vy * W.vy);
 }
 template <typename T> CFFC_HD T W_H(const PS<T> &W) {
     return gam<T>() * gm1i<T>() * W.p + Consts<T>::HALF * W.d * (W.vx * W.vx + W.vy * W.vy);
 }
-template <typename T> CFFC_HD T W_a(const PS<T> &W) { return tsqrt(gam<T>() * W.p / W.d); }
-template <typename T> CFFC_HD T W_a2(const PS<T> &W) { return gam<T>() * W.p / W.d; }
+template <typename T> CFFC_HD T W_a(const PS<T> &W) { return qsqrt(qdiv(gam<T>() * W.p, W.d)); }
+template <typename T> CFFC_HD T W_a2(const PS<T> &W) { return qdiv(gam<T>() * W.p, W.d); }
 template <typename T> CFFC_HD T U_e(const CS<T> &U) {
     return U.E / U.d - Consts<T>::HALF * (U.mx * U.mx + U.my * U.my) / sqr(U.d);
 }
@@ -83,95 +97,109 @@ template <typename T> CFFC_HD CS<T> W_Fx(const PS<T> &W) {
     return F;
 }
 
-// HartenFixPos / HartenFixNeg, CFD.cc:605-643
-template <typename T> CFFC_HD T harten_pos(T la, T ll, T lr) {
+// HartenFixPos / HartenFixNeg, CFD.cc:605-643.  Both share the corrected
+// half-width delta; harten_delta returns it and the callers form
+// HALF*(la+|la|+delta) (positive part) or HALF*(la-|la|-delta) (negative).
+template <typename T> CFFC_HD T harten_delta(T la, T ll, T lr) {
     typedef Consts<T> K;
     T delta = dmax(K::ZERO, K::TWO * (lr - ll));
-    if (tabs(la) > delta || delta < K::TOLER) delta = K::ZERO;
-    else delta = K::HALF * (sqr(la) / delta + delta) - tabs(la);
-    return K::HALF * (la + tabs(la) + delta);
+    if constexpr (Fast<T>::on) {
+        const T fixd = K::HALF * (qdiv(sqr(la), delta) + delta) - tabs(la);
+        return (tabs(la) > delta || delta < K::TOLER) ? K::ZERO : fixd;
+    } else {
+        if (tabs(la) > delta || delta < K::TOLER) return K::ZERO;
+        return K::HALF * (sqr(la) / delta + delta) - tabs(la);
+    }
+}
+template <typename T> CFFC_HD T harten_pos(T la, T ll, T lr) {
+    return Consts<T>::HALF * (la + tabs(la) + harten_delta(la, ll, lr));
 }
 template <typename T> CFFC_HD T harten_neg(T la, T ll, T lr) {
-    typedef Consts<T> K;
-    T delta = dmax(K::ZERO, K::TWO * (lr - ll));
-    if (tabs(la) > delta || delta < K::TOLER) delta = K::ZERO;
-    else delta = K::HALF * (sqr(la) / delta + delta) - tabs(la);
-    return K::HALF * (la - tabs(la) - delta);
+    return Consts<T>::HALF * (la - tabs(la) - harten_delta(la, ll, lr));
 }
 
 // RoeAverage, Euler2DState.cc:654-683
 template <typename T> CFFC_HD PS<T> roe_average(const PS<T> &Wl, const PS<T> &Wr) {
     typedef Consts<T> K;
     T hl = W_h(Wl), hr = W_h(Wr);
-    T sdl = tsqrt(Wl.d), sdr = tsqrt(Wr.d);
+    T sdl = qsqrt(Wl.d), sdr = qsqrt(Wr.d);
     PS<T> Wa;
     Wa.d = sdl * sdr;
-    Wa.vx = (sdl * Wl.vx + sdr * Wr.vx) / (sdl + sdr);
-    Wa.vy = (sdl * Wl.vy + sdr * Wr.vy) / (sdl + sdr);
-    T ha = (sdl * hl + sdr * hr) / (sdl + sdr);
-    T aa2 = gm1<T>() * (ha - K::HALF * (sqr(Wa.vx) + sqr(Wa.vy)));
-    Wa.p = Wa.d * aa2 / gam<T>();
+    if constexpr (Fast<T>::on) {
+        const T r = __builtin_amdgcn_rcpf(sdl + sdr);
+        Wa.vx = (sdl * Wl.vx + sdr * Wr.vx) * r;
+        Wa.vy = (sdl * Wl.vy + sdr * Wr.vy) * r;
+        const T ha = (sdl * hl + sdr * hr) * r;
+        const T aa2 = gm1<T>() * (ha - K::HALF * (sqr(Wa.vx) + sqr(Wa.vy)));
+        Wa.p = Wa.d * aa2 * T(1.0 / 1.4);
+    } else {
+        Wa.vx = (sdl * Wl.vx + sdr * Wr.vx) / (sdl + sdr);
+        Wa.vy = (sdl * Wl.vy + sdr * Wr.vy) / (sdl + sdr);
+        T ha = (sdl * hl + sdr * hr) / (sdl + sdr);
+        T aa2 = gm1<T>() * (ha - K::HALF * (sqr(Wa.vx) + sqr(Wa.vy)));
+        Wa.p = Wa.d * aa2 / gam<T>();
+    }
     return Wa;
 }
 
 // FluxRoe_x, Euler2DState.cc:2311-2353.  The wave loop is unrolled; the
 // left/right eigenvectors are lp_x/rc_x (Euler2DState.h:2702-2720,
 // 2968-2986) and the wave strengths are the pState inner product
-// (Euler2DState.h:1222-1224), all in the reference's order.
+// (Euler2DState.h:1222-1224), all in the reference's order.  The upwind side
+// is a select rather than a branch: F = Fx(W_upwind) +/- sum over the waves
+// whose split eigenvalue is nonzero (HartenFixNeg for Wa.u >= 0, else
+// HartenFixPos, whose results are <= 0 / >= 0 respectively).
 template <typename T> CFFC_HD CS<T> flux_roe_x(const PS<T> &Wl, const PS<T> &Wr) {
     typedef Consts<T> K;
-    PS<T> Wa = roe_average(Wl, Wr);
-    PS<T> dW = {Wr.d - Wl.d, Wr.vx - Wl.vx, Wr.vy - Wl.vy, Wr.p - Wl.p};
-    T cl = W_a(Wl), cr = W_a(Wr), ca = W_a(Wa);
-    // lambdas (vx-c, vx, vx, vx+c) of the left, right and average states
-    T la1 = Wa.vx - ca, la4 = Wa.vx + ca;
-    T ll1 = Wl.vx - cl, ll4 = Wl.vx + cl;
-    T lr1 = Wr.vx - cr, lr4 = Wr.vx + cr;
+    const PS<T> Wa = roe_average(Wl, Wr);
+    const PS<T> dW = {Wr.d - Wl.d, Wr.vx - Wl.vx, Wr.vy - Wl.vy, Wr.p - Wl.p};
+    // a() = sqrt(g*p/d) and a2() = g*p/d share the quotient; h() is needed by
+    // rc_x for waves 1 and 4, computed once
+    const T a2a = W_a2(Wa);
+    const T cl = W_a(Wl), cr = W_a(Wr), ca = qsqrt(a2a);
+    const T ha = W_h(Wa);
+    const T la1 = Wa.vx - ca, la4 = Wa.vx + ca;
     const bool left = (Wa.vx >= K::ZERO);
-    CS<T> F = left ? W_Fx(Wl) : W_Fx(Wr);
-    T w1, w2, w4;
-    if (left) {
-        w1 = harten_neg(la1, ll1, lr1);
-        w2 = K::HALF * (Wa.vx - tabs(Wa.vx));
-        w4 = harten_neg(la4, ll4, lr4);
-    } else {
-        w1 = harten_pos(la1, ll1, lr1);
-        w2 = K::HALF * (Wa.vx + tabs(Wa.vx));
-        w4 = harten_pos(la4, ll4, lr4);
-    }
-    const T sg = left ? T(1) : T(-1);
+    const T d1 = harten_delta(la1, Wl.vx - cl, Wr.vx - cr);
+    const T d4 = harten_delta(la4, Wl.vx + cl, Wr.vx + cr);
+    const T w1 = left ? K::HALF * (la1 - tabs(la1) - d1) : K::HALF * (la1 + tabs(la1) + d1);
+    const T w2 = left ? K::HALF * (Wa.vx - tabs(Wa.vx)) : K::HALF * (Wa.vx + tabs(Wa.vx));
+    const T w4 = left ? K::HALF * (la4 - tabs(la4) - d4) : K::HALF * (la4 + tabs(la4) + d4);
+    const PS<T> Wu = left ? Wl : Wr;
+    CS<T> F = W_Fx(Wu);
+    const T sg = left ? T(1) : T(-1);   // Flux += (left) / Flux -= (right): exact sign flip
     // wave 1: lp = (0, -HALF*d/c, 0, HALF/c^2), rc = (1, u-c, v, h-u*c)
-    if (left ? (w1 < K::ZERO) : (w1 > K::ZERO)) {
-        T lp2 = -K::HALF * Wa.d / ca, lp4 = K::HALF / sqr(ca);
-        T a = w1 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p);
-        T h = W_h(Wa);
-        F.d += sg * (a * K::ONE);   // sg = +/-1 flips exactly (Flux += / Flux -=)
-        F.mx += sg * (a * (Wa.vx - ca));
-        F.my += sg * (a * Wa.vy);
-        F.E += sg * (a * (h - Wa.vx * ca));
+    if (Fast<T>::on || w1 != K::ZERO) {
+        const T lp2 = qdiv(-K::HALF * Wa.d, ca), lp4 = qdiv(K::HALF, sqr(ca));
+        const T a = sg * (w1 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p));
+        F.d += a * K::ONE;
+        F.mx += a * (Wa.vx - ca);
+        F.my += a * Wa.vy;
+        F.E += a * (ha - Wa.vx * ca);
     }
-    // waves 2 and 3 share the eigenvalue u
-    if (left ? (w2 < K::ZERO) : (w2 > K::ZERO)) {
-        T lp4 = -K::ONE / W_a2(Wa);
-        T a = w2 * (K::ONE * dW.d + (K::ZERO * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p);
-        F.d += sg * (a * K::ONE);
-        F.mx += sg * (a * Wa.vx);
-        F.my += sg * (a * Wa.vy);
-        F.E += sg * (a * (K::HALF * (Wa.vx * Wa.vx + Wa.vy * Wa.vy)));
-        T b = w2 * (K::ZERO * dW.d + (K::ZERO * dW.vx + K::ONE * dW.vy) + K::ZERO * dW.p);
-        F.d += sg * (b * K::ZERO);
-        F.mx += sg * (b * K::ZERO);
-        F.my += sg * (b * Wa.d);
-        F.E += sg * (b * (Wa.d * Wa.vy));
+    // waves 2 and 3 share the eigenvalue u; lp2 = (1,0,0,-1/a2), rc2 =
+    // (1,u,v,|v|^2/2); lp3 = (0,0,1,0), rc3 = (0,0,d,d*v)
+    if (Fast<T>::on || w2 != K::ZERO) {
+        const T lp4 = qdiv(-K::ONE, a2a);
+        const T a = sg * (w2 * (K::ONE * dW.d + (K::ZERO * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p));
+        F.d += a * K::ONE;
+        F.mx += a * Wa.vx;
+        F.my += a * Wa.vy;
+        F.E += a * (K::HALF * (Wa.vx * Wa.vx + Wa.vy * Wa.vy));
+        const T b = sg * (w2 * (K::ZERO * dW.d + (K::ZERO * dW.vx + K::ONE * dW.vy) + K::ZERO * dW.p));
+        F.d += b * K::ZERO;
+        F.mx += b * K::ZERO;
+        F.my += b * Wa.d;
+        F.E += b * (Wa.d * Wa.vy);
     }
-    if (left ? (w4 < K::ZERO) : (w4 > K::ZERO)) {
-        T lp2 = K::HALF * Wa.d / ca, lp4 = K::HALF / sqr(ca);
-        T a = w4 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p);
-        T h = W_h(Wa);
-        F.d += sg * (a * K::ONE);
-        F.mx += sg * (a * (Wa.vx + ca));
-        F.my += sg * (a * Wa.vy);
-        F.E += sg * (a * (h + Wa.vx * ca));
+    // wave 4: lp = (0, HALF*d/c, 0, HALF/c^2), rc = (1, u+c, v, h+u*c)
+    if (Fast<T>::on || w4 != K::ZERO) {
+        const T lp2 = qdiv(K::HALF * Wa.d, ca), lp4 = qdiv(K::HALF, sqr(ca));
+        const T a = sg * (w4 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p));
+        F.d += a * K::ONE;
+        F.mx += a * (Wa.vx + ca);
+        F.my += a * Wa.vy;
+        F.E += a * (ha + Wa.vx * ca);
     }
     return F;
 }
@@ -223,19 +251,26 @@ template <typename T> CFFC_HD PS<T> bc_characteristic_pressure(const PS<T> &Wi, 
 }
 
 // Limiter_Venkatakrishnan for one quadrature value, CFD.cc:728-752.  The
-// caller folds phi = min(phi, f) over the four face midpoints.  When
-// y >= 2.5 (and not huge) the rational term is provably >= 1 in floating
-// point and cannot lower phi <= 1, so its division is skipped; the result is
-// unchanged bit for bit.
+// caller folds phi = min(phi, f) over the four face midpoints.  For y >= 2
+// the rational term f(y) = (y^2+2y)/(y^2+y+2) is >= 1 and cannot lower
+// phi <= 1.  When a multiply-only test proves y >= 2.001 (a margin far above
+// rounding) and y is nowhere near overflow, both divisions are skipped: the
+// result is bit-identical to evaluating the reference's expression.
 template <typename T> CFFC_HD T venkat_fold(T phi, T uq, T u0, T u0Min, T u0Max) {
     typedef Consts<T> K;
-    T du = uq - u0;
-    T y;
-    if (du > K::TOLER) y = (u0Max - u0) / du;
-    else if (du < -K::TOLER) y = (u0Min - u0) / du;
-    else return dmin(phi, K::ONE);
-    if (y >= T(2.5) && y <= T(1.0e6)) return phi;
-    return dmin(phi, (sqr(y) + K::TWO * y) / (sqr(y) + y + K::TWO));
+    const T du = uq - u0;
+    T num;
+    if (du > K::TOLER) {
+        num = u0Max - u0;  // y = num/du, both >= 0
+        if (num >= T(2.002) * du && num <= T(1.0e12) * du) return phi;
+    } else if (du < -K::TOLER) {
+        num = u0Min - u0;  // y = num/du, both <= 0
+        if (num <= T(2.002) * du && num >= T(1.0e12) * du) return phi;
+    } else {
+        return dmin(phi, K::ONE);
+    }
+    const T y = qdiv(num, du);
+    return dmin(phi, qdiv(sqr(y) + K::TWO * y, sqr(y) + y + K::TWO));
 }
 
 }  // namespace cffc
