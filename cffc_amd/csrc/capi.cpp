@@ -365,6 +365,7 @@ static int set_state(cffc_ctx *c, const double *host, int64_t n, int kind) {
     HIPCHK(cffc_launch_scatter(c->meta, c->nblk, c->max_int, c->staging, c->W[c->wcur], c->ncp, kind, c->U, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     c->phi_valid = 0;
+    c->freeze = 0;  // a new state starts with limiters evaluated again
     return 0;
 }
 extern "C" int cffc_set_W(cffc_ctx *c, const double *W, int64_t n) { return set_state(c, W, n, 0); }
