@@ -181,6 +181,24 @@ def bump_channel(h, nbi=12, nbj=4, ni=16, nj=16, length=3.0, height=1.0, x_bump=
     return m
 
 
+def concat(meshes):
+    """One multi-block mesh holding several disjoint meshes (cases): block
+    indices of each part are offset so that neighbour tables stay inside the
+    part.  Returns the mesh and the first block of every part."""
+    starts, nb = [], 0
+    nbrs = []
+    for m in meshes:
+        starts.append(nb)
+        t = np.array(m.nbr, dtype=np.int64)
+        nbrs.append(np.where(t >= 0, t + nb, -1).astype(np.int32))
+        nb += m.nblk
+    out = Mesh(nblk=nb, ni=np.concatenate([m.ni for m in meshes]), nj=np.concatenate([m.nj for m in meshes]),
+               xnode=np.concatenate([m.xnode for m in meshes]), ynode=np.concatenate([m.ynode for m in meshes]),
+               nbr=np.concatenate(nbrs), bc=np.concatenate([m.bc for m in meshes]),
+               wo=np.concatenate([m.wo for m in meshes]), info={"kind": "concat", "parts": len(meshes)})
+    return out, starts
+
+
 def freestream(mach, rho=1.4, p=1.0, gamma=1.4):
     """(rho, u, v, p) of a uniform stream at Mach number mach along +x."""
     a = np.sqrt(gamma * p / rho)
