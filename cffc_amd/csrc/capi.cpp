@@ -67,6 +67,12 @@ struct cffc_ctx {
     double *norm_out = nullptr, *dt_out = nullptr;
     int max_steps = 0;
     double *staging = nullptr;
+    // parts: independent cases of a batched steady run (contiguous blocks)
+    int npart = 0;
+    std::vector<int32_t> part_of_block;
+    int32_t *part_tile0 = nullptr, *part_state = nullptr, *part_steps = nullptr;
+    double *part_l2 = nullptr, *part_last = nullptr;
+    int part_l2_steps = 0;
     int freeze = 0;      // limiters frozen (PHI_LOAD)
     int phi_valid = 0;   // phi array holds the last evaluated limiters
     int store_phi = 0;   // keep phi stored every step (freeze may follow)
@@ -85,7 +91,8 @@ static void free_ctx(cffc_ctx *c) {
     if (!c) return;
     void *ptrs[] = {c->meta, c->nbr, c->bc, c->wo, c->geom, c->W[0], c->W[1], c->U, c->Uo, c->dUdt[0],
                     c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
-                    c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging};
+                    c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
+                    c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -98,21 +105,33 @@ template <typename T> static int dalloc(T **p, size_t n) {
     return 0;
 }
 
+// Tiles in block order (so the tiles of a part are contiguous), each tagged
+// with its block's part; part_tile0 gets the first tile of every part.
 static int build_tiles(cffc_ctx *c, int ti, int tj) {
     std::vector<TileDesc> t;
+    std::vector<int32_t> tile0;
     for (int b = 0; b < c->nblk; ++b) {
         const BlockMeta &m = c->meta_h[b];
+        const int part = c->npart ? c->part_of_block[b] : 0;
+        if (c->npart && (int)tile0.size() == part) tile0.push_back((int32_t)t.size());
         for (int j0 = 0; j0 < m.nj; j0 += tj)
-            for (int i0 = 0; i0 < m.ni; i0 += ti) t.push_back(TileDesc{b, i0, j0, 0});
+            for (int i0 = 0; i0 < m.ni; i0 += ti) t.push_back(TileDesc{b, i0, j0, part});
     }
     if (c->tiles) (void)hipFree(c->tiles);
     if (c->norm_part) (void)hipFree(c->norm_part);
+    if (c->part_tile0) (void)hipFree(c->part_tile0);
     c->tiles = nullptr;
     c->norm_part = nullptr;
+    c->part_tile0 = nullptr;
     c->ntiles = (int)t.size();
     if (int e = dalloc(&c->tiles, t.size())) return e;
-    if (int e = dalloc(&c->norm_part, 3 * t.size())) return e;
+    if (int e = dalloc(&c->norm_part, 4 * t.size())) return e;
     HIPCHK(hipMemcpy(c->tiles, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
+    if (c->npart) {
+        tile0.push_back((int32_t)t.size());
+        if (int e = dalloc(&c->part_tile0, tile0.size())) return e;
+        HIPCHK(hipMemcpy(c->part_tile0, tile0.data(), tile0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    }
     c->ti = ti;
     c->tj = tj;
     return 0;
@@ -468,8 +487,6 @@ static StageParams base_stage_params(cffc_ctx *c) {
     p.lts = c->s.local_time_stepping == CFFC_SCALAR_LOCAL_TIME_STEPPING;
     p.err = c->scal + 2;
     p.norm_part = c->norm_part;
-    p.counter = c->counter;
-    p.step_ctr = c->step_ctr;
     p.stop_flag = nullptr;
     p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE;
     return p;
@@ -589,10 +606,12 @@ static void collect_timing(cffc_ctx *c) {
 }
 
 // enqueue one step (all stages) with step parity par
-static int enqueue_step(cffc_ctx *c, int par, double tol) {
+// One fused time step.  parts: run the batched steady form (per-part
+// freeze/convergence with freeze_level and tol per part).
+static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, double freeze_level = 0.0) {
     const int N = c->s.n_stage;
     for (int st = 1; st <= N; ++st) {
-        ExchangeParams ex = exchange_params(c, st == 1, st == 1 ? c->scal + (1 - par) : nullptr);
+        ExchangeParams ex = exchange_params(c, st == 1, nullptr);
         ex.stop_flag = c->stop_flag;
         mark(c, 0, true);
         HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
@@ -605,11 +624,11 @@ static int enqueue_step(cffc_ctx *c, int par, double tol) {
         p.omega = stage_omega(c->s, st);
         p.last_stage = (st == N);
         p.dtmin_in = c->scal + par;
-        p.dtmin_out = c->scal + (1 - par);
-        p.norm_out = c->norm_out;
-        p.dt_out = c->dt_out;
         p.stop_flag = c->stop_flag;
-        p.tol = tol;
+        if (parts) {
+            p.part_state = c->part_state;
+            p.done_copy = N & 1;
+        }
         if (c->freeze) p.phi_mode = PHI_LOAD;
         else if (c->store_phi && st == N) p.phi_mode = PHI_COMPUTE_STORE;
         else p.phi_mode = PHI_COMPUTE;
@@ -618,6 +637,27 @@ static int enqueue_step(cffc_ctx *c, int par, double tol) {
         mark(c, 1, false);
         c->wcur = 1 - c->wcur;
     }
+    FinishParams f;
+    memset(&f, 0, sizeof f);
+    f.norm_part = c->norm_part;
+    f.ntiles = c->ntiles;
+    f.dtmin_in = c->scal + par;
+    f.dtmin_out = c->scal + (1 - par);
+    f.norm_out = c->norm_out;
+    f.dt_out = c->dt_out;
+    f.step_ctr = c->step_ctr;
+    f.stop_flag = c->stop_flag;
+    f.tol = tol;
+    if (parts) {
+        f.npart = c->npart;
+        f.part_tile0 = c->part_tile0;
+        f.part_state = c->part_state;
+        f.part_steps = c->part_steps;
+        f.part_l2 = c->part_l2;
+        f.part_last = c->part_last;
+        f.freeze_level = freeze_level;
+    }
+    HIPCHK(cffc_launch_finish(&f, c->stream));
     return 0;
 }
 
@@ -716,6 +756,110 @@ extern "C" int cffc_run_steps_graph(cffc_ctx *c, int nsteps, double *norms, doub
     int done = 0;
     int rc = end_run(c, wstart, nsteps, norms, dt_out, &done);
     return rc;
+}
+
+// ---- batched steady runs ----------------------------------------------------
+extern "C" int cffc_set_parts(cffc_ctx *c, int npart, const int32_t *first_block) {
+    if (!c || npart < 1 || npart > c->nblk || !first_block) return fail(CFFC_E_ARG, "bad argument");
+    if (first_block[0] != 0 || first_block[npart] != c->nblk) return fail(CFFC_E_ARG, "parts must cover all blocks");
+    std::vector<int32_t> pob(c->nblk);
+    for (int q = 0; q < npart; ++q) {
+        if (first_block[q + 1] <= first_block[q]) return fail(CFFC_E_ARG, "part %d is empty", q);
+        for (int b = first_block[q]; b < first_block[q + 1]; ++b) pob[b] = q;
+    }
+    // parts must not exchange ghost cells with each other
+    std::vector<int32_t> nbr_h(8 * (size_t)c->nblk);
+    HIPCHK(hipMemcpy(nbr_h.data(), c->nbr, nbr_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int b = 0; b < c->nblk; ++b)
+        for (int k = 0; k < 8; ++k) {
+            const int nb = nbr_h[8 * (size_t)b + k];
+            if (nb >= 0 && pob[nb] != pob[b]) return fail(CFFC_E_TOPOLOGY, "block %d neighbours block %d of another part", b, nb);
+        }
+    for (int32_t **q : {&c->part_state, &c->part_steps})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (c->part_last) { (void)hipFree(c->part_last); c->part_last = nullptr; }
+    c->npart = npart;
+    c->part_of_block = pob;
+    if (int e = dalloc(&c->part_state, npart)) return e;
+    if (int e = dalloc(&c->part_steps, npart)) return e;
+    if (int e = dalloc(&c->part_last, npart)) return e;
+    return build_tiles(c, c->ti, c->tj);
+}
+
+extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double freeze_level, int keep_history,
+                               int32_t *part_steps, double *part_l2, int *steps_done) {
+    if (!c || max_steps < 1 || !(tol > 0.0)) return fail(CFFC_E_ARG, "bad argument");
+    if (c->npart == 0) {
+        const int32_t fb[2] = {0, c->nblk};
+        if (int e = cffc_set_parts(c, 1, fb)) return e;
+    }
+    if (int e = ensure_steps(c, max_steps)) return e;
+    if (int e = ensure_phi(c)) return e;
+    if (c->part_l2) { (void)hipFree(c->part_l2); c->part_l2 = nullptr; }
+    c->part_l2_steps = 0;
+    if (keep_history) {
+        if (int e = dalloc(&c->part_l2, (size_t)max_steps * c->npart)) return e;
+        c->part_l2_steps = max_steps;
+    }
+    HIPCHK(hipMemsetAsync(c->part_state, 0, c->npart * sizeof(int32_t), c->stream));       // PART_RUNNING
+    HIPCHK(hipMemsetAsync(c->part_steps, 0xFF, c->npart * sizeof(int32_t), c->stream));    // -1
+    HIPCHK(hipMemsetAsync(c->part_last, 0, c->npart * sizeof(double), c->stream));
+    c->freeze = 0;
+    if (int e = begin_run(c, max_steps)) return e;
+    const int wstart = c->wcur;
+    const int N = c->s.n_stage;
+    // two steps (dt slot parity 0, 1) captured once and replayed
+    hipGraph_t g = nullptr;
+    hipGraphExec_t ge = nullptr;
+    const bool tsave = c->timing;
+    c->timing = false;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int e = enqueue_step(c, 0, tol, true, freeze_level);
+    if (!e) e = enqueue_step(c, 1, tol, true, freeze_level);
+    hipError_t ce = hipStreamEndCapture(c->stream, &g);
+    c->timing = tsave;
+    c->wcur = wstart;  // a pair of steps flips the W buffers 2*N times
+    if (e) { if (g) (void)hipGraphDestroy(g); return e; }
+    if (ce != hipSuccess) return fail(CFFC_E_HIP, "graph capture failed: %s", hipGetErrorString(ce));
+    if (hipGraphInstantiate(&ge, g, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        return fail(CFFC_E_HIP, "graph instantiate failed");
+    }
+    int32_t *hflag = nullptr;
+    HIPCHK(hipHostMalloc((void **)&hflag, sizeof(int32_t), hipHostMallocDefault));
+    hflag[0] = 0;
+    const int pairs = max_steps / 2;
+    int rc = 0;
+    for (int q = 0; q < pairs; ++q) {
+        if (hipGraphLaunch(ge, c->stream) != hipSuccess) { rc = fail(CFFC_E_HIP, "graph launch failed"); break; }
+        // poll the stop flag every 16 pairs without draining the queue
+        if ((q & 15) == 15) {
+            if (hflag[0]) break;
+            (void)hipMemcpyAsync(hflag, c->stop_flag, sizeof(int32_t), hipMemcpyDeviceToHost, c->stream);
+        }
+    }
+    if (!rc && (max_steps & 1)) rc = enqueue_step(c, 0, tol, true, freeze_level);
+    // the single step flipped wcur N times; end_run recomputes it from the count
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipHostFree(hflag);
+    (void)hipGraphExecDestroy(ge);
+    (void)hipGraphDestroy(g);
+    if (rc) return rc;
+    c->wcur = wstart;
+    rc = end_run(c, wstart, max_steps, nullptr, nullptr, steps_done);
+    if (part_steps) HIPCHK(hipMemcpy(part_steps, c->part_steps, c->npart * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (part_l2) HIPCHK(hipMemcpy(part_l2, c->part_last, c->npart * sizeof(double), hipMemcpyDeviceToHost));
+    (void)N;
+    return rc;
+}
+
+extern "C" int cffc_num_parts(const cffc_ctx *c) { return c ? c->npart : CFFC_E_ARG; }
+
+extern "C" int cffc_get_part_history(cffc_ctx *c, int nsteps, double *out) {
+    if (!c || !out || nsteps < 0) return fail(CFFC_E_ARG, "bad argument");
+    if (!c->part_l2 || nsteps > c->part_l2_steps) return fail(CFFC_E_ARG, "no history of %d steps kept", nsteps);
+    if (nsteps) HIPCHK(hipMemcpy(out, c->part_l2, (size_t)nsteps * c->npart * sizeof(double), hipMemcpyDeviceToHost));
+    return 0;
 }
 
 // keep the last stage's limiters stored so that a freeze can follow a run

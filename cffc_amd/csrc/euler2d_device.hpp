@@ -34,8 +34,11 @@ struct BlockMeta {
 };
 
 struct TileDesc {
-    int32_t b, i0, j0, pad;  // block, first interior cell of the tile
+    int32_t b, i0, j0, part;  // block, first interior cell of the tile, part (case) of the block
 };
+
+// per-part state of a batched steady run (one part = one independent case)
+enum PartState { PART_RUNNING = 0, PART_FROZEN = 1, PART_DONE = 2 };
 
 // stage modes of the residual accumulator (dUdt[0] in the fused path)
 enum StageMode {

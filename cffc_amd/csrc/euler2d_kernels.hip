@@ -389,6 +389,87 @@ __global__ __launch_bounds__(256) void norm_final_kernel(const double *part, int
     }
 }
 
+// One workgroup closes a fused time step.  Partials are summed in tile order
+// with a fixed thread assignment, so the norms are reproducible run to run.
+// Skipped when a stage of this step (or an earlier step) raised the stop
+// flag: a failing step is not counted, as in the reference's loop.
+// Without parts the 16 waves stride over all tiles; with parts each wave
+// reduces whole parts (lanes stride over the part's tiles), so a case gives
+// the same norms whether it runs alone or in a batch.
+constexpr int FINISH_NT = 1024;
+__global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) {
+    constexpr int NW = FINISH_NT / 64;
+    __shared__ double sm[NW][4];
+    __shared__ int sdone[NW];
+    if (*p.stop_flag) return;
+    const int n = p.ntiles, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int s = *p.step_ctr;
+    double a = 0.0, b = 0.0, mx = 0.0, dmn = MILLION;
+    int ndone = 0;
+    if (p.npart == 0) {
+        for (int q = threadIdx.x; q < n; q += FINISH_NT) {
+            a += p.norm_part[q];
+            b += p.norm_part[n + q];
+            mx = dmax(mx, p.norm_part[2 * n + q]);
+            dmn = dmin(dmn, p.norm_part[3 * n + q]);
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            a += __shfl_xor(a, off); b += __shfl_xor(b, off);
+            mx = dmax(mx, __shfl_xor(mx, off)); dmn = dmin(dmn, __shfl_xor(dmn, off));
+        }
+    } else {
+        for (int pt = w; pt < p.npart; pt += NW) {
+            double pa = 0.0, pb = 0.0, pm = 0.0, pd = MILLION;
+            for (int q = p.part_tile0[pt] + lane; q < p.part_tile0[pt + 1]; q += 64) {
+                pa += p.norm_part[q];
+                pb += p.norm_part[n + q];
+                pm = dmax(pm, p.norm_part[2 * n + q]);
+                pd = dmin(pd, p.norm_part[3 * n + q]);
+            }
+            for (int off = 32; off > 0; off >>= 1) {
+                pa += __shfl_xor(pa, off); pb += __shfl_xor(pb, off);
+                pm = dmax(pm, __shfl_xor(pm, off)); pd = dmin(pd, __shfl_xor(pd, off));
+            }
+            a += pa; b += pb; mx = dmax(mx, pm); dmn = dmin(dmn, pd);
+            if (lane == 0) {
+                const double l2p = sqrt(pb);
+                int st = p.part_state[pt];
+                if (st != PART_DONE) {
+                    if (p.part_l2) p.part_l2[(int64_t)s * p.npart + pt] = l2p;
+                    p.part_last[pt] = l2p;
+                    if (p.tol > 0.0 && l2p <= p.tol) {
+                        st = PART_DONE;
+                        p.part_steps[pt] = s + 1;
+                    } else if (st == PART_RUNNING && p.freeze_level > 0.0 && l2p <= p.freeze_level) {
+                        st = PART_FROZEN;
+                    }
+                    p.part_state[pt] = st;
+                } else if (p.part_l2) {
+                    p.part_l2[(int64_t)s * p.npart + pt] = 0.0;
+                }
+                ndone += st == PART_DONE;
+            }
+        }
+    }
+    if (lane == 0) { sm[w][0] = a; sm[w][1] = b; sm[w][2] = mx; sm[w][3] = dmn; sdone[w] = ndone; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double A = sm[0][0], B = sm[0][1], C = sm[0][2], D = sm[0][3];
+        int nd = sdone[0];
+        for (int k = 1; k < NW; ++k) {
+            A += sm[k][0]; B += sm[k][1]; C = dmax(C, sm[k][2]); D = dmin(D, sm[k][3]); nd += sdone[k];
+        }
+        const double l2n = sqrt(B);
+        p.norm_out[3 * s + 0] = A;
+        p.norm_out[3 * s + 1] = l2n;
+        p.norm_out[3 * s + 2] = C;
+        p.dt_out[s] = bitsd(*p.dtmin_in);
+        *p.dtmin_out = dbits(D);
+        *p.step_ctr = s + 1;
+        if (p.npart == 0 ? (p.tol > 0.0 && l2n <= p.tol) : nd == p.npart) *p.stop_flag = 1;
+    }
+}
+
 // small device-side scalar writes (graph-safe, no memset nodes)
 __global__ void set_u64_kernel(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n) {
     if (threadIdx.x == 0) { dst[0] = v0; if (n > 1) dst[1] = v1; }
@@ -457,6 +538,11 @@ hipError_t cffc_launch_norms(const BlockMeta *meta, const TileDesc *tiles, int n
 hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n,
                                hipStream_t st) {
     hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(64), 0, st, dst, v0, v1, n);
+    return hipGetLastError();
+}
+
+hipError_t cffc_launch_finish(const FinishParams *p, hipStream_t st) {
+    hipLaunchKernelGGL(step_finish_kernel, dim3(1), dim3(FINISH_NT), 0, st, *p);
     return hipGetLastError();
 }
 

@@ -23,6 +23,30 @@ struct ExchangeParams {
     unsigned long long *reset_slot;  // set to MILLION (next-step dt slot), may be null
 };
 
+// end of a fused time step: reduce the last stage's per-tile partials,
+// record the step's norms and dt, set the next step's global dt, and raise
+// the stop flag on convergence.  With parts (batched steady runs) every part
+// is reduced on its own and moves RUNNING -> FROZEN -> DONE.
+struct FinishParams {
+    const double *norm_part;                // [4][ntiles]
+    int32_t ntiles;
+    const unsigned long long *dtmin_in;     // dt of the step that just ended (bits)
+    unsigned long long *dtmin_out;          // next step's dt (bits)
+    double *norm_out;                       // [max_steps][3]
+    double *dt_out;                         // [max_steps]
+    int32_t *step_ctr;
+    int32_t *stop_flag;
+    double tol;                             // L2 threshold (<= 0: none)
+    // parts (npart == 0: none)
+    int32_t npart;
+    const int32_t *part_tile0;              // [npart + 1] first tile of each part
+    int32_t *part_state;                    // [npart] PartState
+    int32_t *part_steps;                    // [npart] steps to convergence
+    double *part_l2;                        // [max_steps][npart] or null
+    double *part_last;                      // [npart] L2 of the part's last step
+    double freeze_level;                    // part L2 at which limiters freeze (<= 0: never)
+};
+
 }  // namespace cffc
 
 // tile shapes the stage kernel is instantiated for (TI x TJ interior cells)
@@ -50,6 +74,7 @@ hipError_t cffc_launch_norms(const cffc::BlockMeta *meta, const cffc::TileDesc *
                              const double *dU, int64_t ncp, int var, double *part, double *out, hipStream_t st);
 hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n,
                                hipStream_t st);
+hipError_t cffc_launch_finish(const cffc::FinishParams *p, hipStream_t st);
 hipError_t cffc_launch_stage_fp64(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 hipError_t cffc_launch_stage_mixed(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 }

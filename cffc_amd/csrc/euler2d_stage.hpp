@@ -38,7 +38,6 @@ struct StageParams {
     const double *S1, *S2;  // dUdt[1], dUdt[2] (granular RK4 stage 4)
     double *dt;             // per-cell time step (padded)
     const unsigned long long *dtmin_in;  // global dt (bit pattern) of this step
-    unsigned long long *dtmin_out;       // next step's global dt (atomicMin)
     double *phi;            // stored limiter values (freeze), 4 x ncp
     double cfl, omega;
     int32_t mode;           // StageMode (fused) / granular init mode
@@ -47,13 +46,10 @@ struct StageParams {
     int32_t lts;            // scalar local time stepping
     int32_t last_stage;     // fused: produce norms and the next time step
     unsigned long long *err;   // first failing cell key (atomicMin)
-    double *norm_part;      // [ntiles][3]
-    unsigned int *counter;  // last-workgroup counter
-    double *norm_out;       // [max_steps][3]
-    double *dt_out;         // [max_steps]
-    int32_t *step_ctr;      // step index for norm_out/dt_out
+    double *norm_part;      // [4][ntiles]: L1, L2^2, max of the last stage's dUdt (rho), next dt
     int32_t *stop_flag;     // nonzero: converged or failed, skip work
-    double tol;             // convergence threshold on the L2 norm (<= 0: none)
+    const int32_t *part_state;  // per-part PartState (batched steady runs), or null
+    int32_t done_copy;      // parts that are done copy W forward (odd stage count)
 };
 
 // granular init modes (dUdt_Multistage_Explicit called on its own)
@@ -114,8 +110,8 @@ __device__ __forceinline__ PS<FT> face_state(const PS<FT> &W, const PS<FT> &px, 
 // 8-point stencil, straight-line; otherwise the reduced stencil in mask.
 // The window arrays are indexed [row][col] with the cell at (jl, il).
 template <typename FT, bool FULL, int LJ, int LI>
-__device__ __forceinline__ void reconstruct(const StageParams &p, int64_t c, int64_t ncp, int nci, int il, int jl,
-                                            unsigned mask, const FT (&sW)[4][LJ][LI], const double (&sX)[LJ][LI],
+__device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int64_t c, int64_t ncp, int nci, int il,
+                                            int jl, unsigned mask, const FT (&sW)[4][LJ][LI], const double (&sX)[LJ][LI],
                                             const double (&sY)[LJ][LI], PS<FT> &gx, PS<FT> &gy, PS<FT> &ph) {
     const PS<FT> W0 = {sW[0][jl][il], sW[1][jl][il], sW[2][jl][il], sW[3][jl][il]};
     gx = {FT(0), FT(0), FT(0), FT(0)};
@@ -166,7 +162,7 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int64_t c, int
         gy.d = (Dy.d * DxDx - Dx.d * DxDy) / det; gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) / det;
         gy.vy = (Dy.vy * DxDx - Dx.vy * DxDy) / det; gy.p = (Dy.p * DxDx - Dx.p * DxDy) / det;
     }
-    if (p.phi_mode == PHI_LOAD) {
+    if (phim == PHI_LOAD) {
         ph.d = (FT)p.phi[c]; ph.vx = (FT)p.phi[ncp + c]; ph.vy = (FT)p.phi[2 * ncp + c]; ph.p = (FT)p.phi[3 * ncp + c];
         return;
     }
@@ -234,13 +230,38 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     __shared__ FT sPx[4][RJ][RI], sPy[4][RJ][RI];
     __shared__ FT sFi[4][TJ][TI + 1], sFj[4][TJ + 1][TI];
     __shared__ double sRed[NT / 64][4];
-    __shared__ int sLast;
 
     if (p.stop_flag && *p.stop_flag) return;
     const TileDesc t = p.tiles[blockIdx.x];
     const BlockMeta m = p.meta[t.b];
     const int tid = threadIdx.x;
     const int64_t ncp = p.ncp;
+    int phim = p.phi_mode;
+    if (FUSED && p.part_state) {
+        // batched steady run: a converged part is left alone, a frozen part
+        // loads its limiters, a running part stores them at its last stage
+        const int ps = p.part_state[t.part];
+        if (ps == PART_DONE) {
+            if (p.done_copy) {
+                for (int q = tid; q < TI * TJ; q += NT) {
+                    const int ip = t.i0 + 2 + q % TI, jp = t.j0 + 2 + q / TI;
+                    if (ip <= m.ni + 1 && jp <= m.nj + 1) {
+                        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) p.Wout[v * ncp + c] = p.Win[v * ncp + c];
+                    }
+                }
+            }
+            if (p.last_stage && tid == 0) {
+                p.norm_part[blockIdx.x] = 0.0;
+                p.norm_part[p.ntiles + blockIdx.x] = 0.0;
+                p.norm_part[2 * p.ntiles + blockIdx.x] = 0.0;
+                p.norm_part[3 * p.ntiles + blockIdx.x] = MILLION;
+            }
+            return;
+        }
+        phim = ps == PART_FROZEN ? PHI_LOAD : p.last_stage ? PHI_COMPUTE_STORE : PHI_COMPUTE;
+    }
     const double *G = p.geom;
     const int ICu = m.ni + 1, JCu = m.nj + 1;
     const int pi0 = t.i0, pj0 = t.j0;   // padded index of the LDS window origin
@@ -293,9 +314,9 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             if (m.has_bc && (ip <= 1 || ip >= ICu + 1 || jp <= 1 || jp >= JCu + 1))
                 mask = stencil_mask(ip, jp, ICu, JCu, bcW[jp], bcE[jp], bcS[ip], bcN[ip]);
             PS<FT> gx, gy, ph;
-            if (mask == 0xFFu) reconstruct<FT, true>(p, c, ncp, m.nci, il, jl, 0xFFu, sW, sX, sY, gx, gy, ph);
-            else reconstruct<FT, false>(p, c, ncp, m.nci, il, jl, mask, sW, sX, sY, gx, gy, ph);
-            if (p.phi_mode == PHI_COMPUTE_STORE) {
+            if (mask == 0xFFu) reconstruct<FT, true>(p, phim, c, ncp, m.nci, il, jl, 0xFFu, sW, sX, sY, gx, gy, ph);
+            else reconstruct<FT, false>(p, phim, c, ncp, m.nci, il, jl, mask, sW, sX, sY, gx, gy, ph);
+            if (phim == PHI_COMPUTE_STORE) {
                 // each stored cell is owned by exactly one tile: its own
                 // cells, plus the block's outer ring on the block edge
                 const bool own_i = (ip >= pi0 + 2 && ip < pi0 + 2 + TI) || ip == 1 || ip == m.nci - 2;
@@ -538,44 +559,13 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
         for (int w = 1; w < NT / 64; ++w) {
             a += sRed[w][0]; b += sRed[w][1]; cmx = dmax(cmx, sRed[w][2]); dmn = dmin(dmn, sRed[w][3]);
         }
-        p.norm_part[3 * blockIdx.x + 0] = a;
-        p.norm_part[3 * blockIdx.x + 1] = b;
-        p.norm_part[3 * blockIdx.x + 2] = cmx;
-        atomicMin(p.dtmin_out, dbits(dmn));  // positive doubles order like their bits
-        __threadfence();
-        const unsigned prev = atomicAdd(p.counter, 1u);
-        sLast = (prev == (unsigned)(p.ntiles - 1));
-    }
-    __syncthreads();
-    if (!sLast) return;
-    // the last workgroup to finish reduces all partials in tile order
-    __threadfence();
-    double a = 0.0, b = 0.0, cmx = 0.0;
-    for (int q = tid; q < p.ntiles; q += NT) {
-        a += p.norm_part[3 * q + 0];
-        b += p.norm_part[3 * q + 1];
-        cmx = dmax(cmx, p.norm_part[3 * q + 2]);
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        a += __shfl_xor(a, off);
-        b += __shfl_xor(b, off);
-        cmx = dmax(cmx, __shfl_xor(cmx, off));
-    }
-    __syncthreads();
-    if ((tid & 63) == 0) { sRed[wv][0] = a; sRed[wv][1] = b; sRed[wv][2] = cmx; }
-    __syncthreads();
-    if (tid == 0) {
-        double A0 = sRed[0][0], B0 = sRed[0][1], C0 = sRed[0][2];
-        for (int w = 1; w < NT / 64; ++w) { A0 += sRed[w][0]; B0 += sRed[w][1]; C0 = dmax(C0, sRed[w][2]); }
-        const int s = *p.step_ctr;
-        const double l2n = sqrt(B0);
-        p.norm_out[3 * s + 0] = A0;
-        p.norm_out[3 * s + 1] = l2n;
-        p.norm_out[3 * s + 2] = C0;
-        p.dt_out[s] = bitsd(*p.dtmin_in);
-        *p.step_ctr = s + 1;
-        *p.counter = 0u;
-        if ((p.tol > 0.0 && l2n <= p.tol) || *p.err != ~0ull) *p.stop_flag = 1;
+        // per-tile partials (SoA), reduced in tile order by step_finish_kernel;
+        // the kernel boundary orders these stores before that kernel's loads
+        // (no device-scope fence or atomic per workgroup)
+        p.norm_part[blockIdx.x] = a;
+        p.norm_part[p.ntiles + blockIdx.x] = b;
+        p.norm_part[2 * p.ntiles + blockIdx.x] = cmx;
+        p.norm_part[3 * p.ntiles + blockIdx.x] = dmn;
     }
 }
 

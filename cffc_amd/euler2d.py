@@ -65,6 +65,11 @@ def load():
     L.cffc_get_tile.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.cffc_set_kernel_timing.argtypes = [vp, i32]
     L.cffc_kernel_times.argtypes = [vp, pd]
+    pi32 = ctypes.POINTER(ctypes.c_int32)
+    L.cffc_set_parts.argtypes = [vp, i32, pi32]
+    L.cffc_num_parts.argtypes = [vp]
+    L.cffc_run_steady.argtypes = [vp, i32, d, d, i32, pi32, pd, ctypes.POINTER(ctypes.c_int)]
+    L.cffc_get_part_history.argtypes = [vp, i32, pd]
     L.cffc_last_error.argtypes = []
     L.cffc_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -188,6 +193,35 @@ class Euler2DQuadBlocks:
                         "cffc_run_steps")
             n = done.value
         return rc, n, norms[:n], dts[:n]
+
+    # ---- batched steady runs ------------------------------------------------------
+    def set_parts(self, first_block):
+        """Independent cases: part q owns blocks first_block[q]..first_block[q+1]-1."""
+        fb = np.ascontiguousarray(first_block, dtype=np.int32)
+        _check(load().cffc_set_parts(self.h, len(fb) - 1, fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+               "cffc_set_parts")
+        self.npart = len(fb) - 1
+
+    def run_steady(self, max_steps, tol, freeze_level=0.0, history=False):
+        """Every part to its own L2 <= tol (limiters frozen at freeze_level).
+        Returns (rc, steps_done, part_steps[npart], part_l2[npart], history
+        [steps_done, npart] or None)."""
+        L = load()
+        npart = max(L.cffc_num_parts(self.h), 1)
+        steps = np.zeros(npart, dtype=np.int32)
+        l2 = np.zeros(npart)
+        done = ctypes.c_int(0)
+        rc = _check(L.cffc_run_steady(self.h, max_steps, tol, freeze_level, 1 if history else 0,
+                                      steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _p(l2),
+                                      ctypes.byref(done)), "cffc_run_steady")
+        npart = L.cffc_num_parts(self.h)
+        steps, l2 = steps[:npart], l2[:npart]
+        hist = None
+        if history:
+            hist = np.zeros((max(done.value, 1), npart))
+            _check(L.cffc_get_part_history(self.h, done.value, _p(hist)), "cffc_get_part_history")
+            hist = hist[:done.value]
+        return rc, done.value, steps, l2, hist
 
     def synchronize(self):
         _check(load().cffc_synchronize(self.h), "cffc_synchronize")

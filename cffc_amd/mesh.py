@@ -154,7 +154,8 @@ def cartesian(nbi, nbj, ni, nj, x0=0.0, x1=1.0, y0=0.0, y1=1.0, bcs=None, jitter
     return m
 
 
-def bump_channel(h, nbi=12, nbj=4, ni=16, nj=16, length=3.0, height=1.0, x_bump=(1.0, 2.0)):
+def bump_channel(h, nbi=12, nbj=4, ni=16, nj=16, length=3.0, height=1.0, x_bump=(1.0, 2.0),
+                 inflow=abi.BC_FIXED, outflow=abi.BC_CONSTANT_EXTRAPOLATION):
     """Channel [0,length]x[0,height] with a circular-arc bump of height h on
     the lower wall between x_bump[0] and x_bump[1] (chord 1 by default), the
     arc of radius R = (c^2/4 + h^2)/(2h) as in the reference's bump grid
@@ -162,7 +163,7 @@ def bump_channel(h, nbi=12, nbj=4, ni=16, nj=16, length=3.0, height=1.0, x_bump=
     stretched linearly between the wall and the top in y.  Boundary
     conditions follow the reference's bump channel (:5556-5565): reflection
     on the lower and upper walls, fixed inflow (west), constant
-    extrapolation outflow (east)."""
+    extrapolation outflow (east); inflow and outflow select others."""
     NI, NJ = nbi * ni, nbj * nj
     ig = np.arange(-NG, NI + NG + 1, dtype=np.float64)
     jg = np.arange(-NG, NJ + NG + 1, dtype=np.float64)
@@ -175,7 +176,7 @@ def bump_channel(h, nbi=12, nbj=4, ni=16, nj=16, length=3.0, height=1.0, x_bump=
     yb[inside] = np.sqrt(R * R - (xl[inside] - xm) ** 2) - (R - h)
     X = np.tile(xl, (jg.size, 1))
     Y = yb[None, :] + (height - yb[None, :]) * (jg[:, None] / NJ)
-    bcs = dict(W=abi.BC_FIXED, E=abi.BC_CONSTANT_EXTRAPOLATION, S=abi.BC_REFLECTION, N=abi.BC_REFLECTION)
+    bcs = dict(W=inflow, E=outflow, S=abi.BC_REFLECTION, N=abi.BC_REFLECTION)
     m = _structured_blocks(X, Y, nbi, nbj, ni, nj, bcs)
     m.info.update(kind="bump", h=h, R=R, length=length, height=height, NI=NI, NJ=NJ)
     return m

@@ -2,14 +2,13 @@
 steady bump channel (192 x 64 cells in 12 x 4 blocks of 16^2, h ~ U[0.04,
 0.10], M ~ U[0.3, 0.6]).
 
-* Data: many cases are solved at once as disjoint parts of one multi-block
-  mesh (one launch per stage covers every case), from the freestream to an
-  L2 density residual <= tol over all parts together, which bounds every
-  part's residual by tol.
+* Data: the cases are solved together as disjoint parts of one multi-block
+  mesh (one launch per stage covers every case), each from the freestream
+  to its own L2 density residual <= tol (cffc_run_steady).
 * Training: cffc_amd.unet.UNet on the GPU.
-* Evaluation: every held-out case is solved twice on its own, from the
-  freestream and from the network's prediction, to the same tolerance; the
-  steps and wall time to reach it are compared.
+* Evaluation: the held-out cases are solved twice to the same tolerance, from
+  the freestream and from the network's prediction, both as a batch; the
+  steps every case needs and the wall time of each batch are compared.
 """
 import time
 
@@ -19,30 +18,31 @@ from . import abi, euler2d, steady, unet
 from . import mesh as M
 
 
-def solve_batch(cases, tol=1e-10, max_steps=200000, freeze_level=1e-5, chunk=64, log=None):
-    """Steady states (state arrays) of the (h, M) cases, solved chunk at a time."""
-    states = []
+def solve_cases(cases, tol=1e-10, max_steps=200000, freeze_level=steady.STEADY_FREEZE_LEVEL, chunk=512,
+                initial=None, log=None):
+    """Steady states of the (h, M) cases (batches of up to chunk cases).
+    initial: optional list of initial state arrays (default freestream).
+    Returns (states, steps per case, seconds per batch)."""
+    states, steps, secs = [], [], []
     for s in range(0, len(cases), chunk):
         part = cases[s:s + chunk]
-        meshes, W0s = zip(*[steady.bump_case(h, mach) for h, mach in part])
-        m, starts = M.concat(list(meshes))
-        W0 = np.concatenate(W0s)
-        g = euler2d.Euler2DQuadBlocks(m, steady.bump_scheme())
-        g.set_W(W0)
-        t0 = time.perf_counter()
-        r = steady.solve_steady(g, tol=tol, max_steps=max_steps,
-                                freeze_level=freeze_level)
-        W = g.get_W()
-        g.close()
+        batch = []
+        for q, (h, mach) in enumerate(part):
+            m, W0 = steady.bump_case(h, mach)
+            if initial is not None:
+                W0 = initial[s + q]
+            batch.append((m, W0))
+        r = steady.solve_batch(batch, tol=tol, max_steps=max_steps, freeze_level=freeze_level)
         if log:
-            log("cases %d-%d: rc=%d steps=%d l2=%.3e %.1fs" % (s, s + len(part) - 1, r.rc, r.steps, r.l2,
-                                                                  time.perf_counter() - t0))
-        if r.rc != 0 or r.l2 > tol:
-            raise RuntimeError("steady solve of cases %d.. did not converge (rc=%d, l2=%.3e)" % (s, r.rc, r.l2))
-        n = meshes[0].ncell
-        for q in range(len(part)):
-            states.append(W[q * n:(q + 1) * n].copy())
-    return states
+            log("cases %d-%d: rc=%d steps_done=%d max part steps=%d %.1fs" %
+                (s, s + len(part) - 1, r.rc, r.steps_done, int(r.part_steps.max()), r.seconds))
+        if r.rc != 0 or np.any(r.part_steps < 0):
+            raise RuntimeError("steady solve of cases %d.. did not converge (rc=%d, %d parts left)" %
+                               (s, r.rc, int(np.sum(r.part_steps < 0))))
+        states += r.states
+        steps += r.part_steps.tolist()
+        secs.append(r.seconds)
+    return states, np.array(steps), secs
 
 
 def dataset(cases, states, norm):
@@ -60,28 +60,24 @@ def fit_scale(cases, states):
     return np.sqrt(np.mean(d * d, axis=0)) + 1e-6
 
 
-def evaluate(net, norm, cases, tol=1e-10, max_steps=200000, freeze_level=1e-5, log=None):
-    """Per held-out case: (steps, seconds) cold and warm; seconds of the warm
-    solve include the network inference."""
-    rows = []
-    for q, (h, mach) in enumerate(cases):
-        m, W0 = steady.bump_case(h, mach)
-        g = euler2d.Euler2DQuadBlocks(m, steady.bump_scheme())
-        g.set_W(W0)
-        cold = steady.solve_steady(g, tol=tol, max_steps=max_steps, freeze_level=freeze_level)
-        t0 = time.perf_counter()
-        Xq = unet.features(m, h, mach)[None]
-        Wimg = norm.decode(net.predict(Xq)[0], mach)
-        Wp = unet.image_to_state(m, Wimg)
-        g.set_W(Wp)
-        t_inf = time.perf_counter() - t0
-        warm = steady.solve_steady(g, tol=tol, max_steps=max_steps, freeze_level=freeze_level)
-        g.close()
-        rows.append(dict(h=h, mach=mach, cold_steps=cold.steps, cold_s=cold.seconds, cold_rc=cold.rc,
-                         cold_l2=cold.l2, warm_steps=warm.steps, warm_s=warm.seconds + t_inf, warm_rc=warm.rc,
-                         warm_l2=warm.l2, inference_s=t_inf))
-        if log:
-            r = rows[-1]
-            log("held-out %d h=%.3f M=%.3f: cold %d steps %.2fs, warm %d steps %.2fs" %
-                (q, h, mach, r["cold_steps"], r["cold_s"], r["warm_steps"], r["warm_s"]))
-    return rows
+def predict_states(net, norm, cases):
+    """Network prediction of the steady state of every (h, M) case."""
+    m0 = M.bump_channel(cases[0][0])
+    X = np.stack([unet.features(M.bump_channel(h), h, mach) for h, mach in cases])
+    Y = net.predict(X)
+    return [unet.image_to_state(m0, norm.decode(Y[q], mach)) for q, (h, mach) in enumerate(cases)]
+
+
+def evaluate(net, norm, cases, tol=1e-10, max_steps=200000, freeze_level=steady.STEADY_FREEZE_LEVEL, log=None):
+    """Cold (freestream) and warm (predicted) solves of the held-out cases,
+    each set as one batch.  Returns a dict with the steps of every case and
+    the wall time of each batch (the warm one including inference)."""
+    _, cold_steps, cold_s = solve_cases(cases, tol, max_steps, freeze_level, log=log)
+    t0 = time.perf_counter()
+    init = predict_states(net, norm, cases)
+    t_inf = time.perf_counter() - t0
+    _, warm_steps, warm_s = solve_cases(cases, tol, max_steps, freeze_level, initial=init, log=log)
+    return dict(cases=np.asarray(cases).tolist(), cold_steps=cold_steps.tolist(), warm_steps=warm_steps.tolist(),
+                cold_seconds=float(sum(cold_s)), warm_seconds=float(sum(warm_s)) + t_inf, inference_seconds=t_inf,
+                step_ratio_median=float(np.median(cold_steps / np.maximum(warm_steps, 1))),
+                step_ratio_total=float(cold_steps.sum() / max(warm_steps.sum(), 1)))

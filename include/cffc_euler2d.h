@@ -197,6 +197,34 @@ int  cffc_run_steps(cffc_ctx *ctx, int nsteps, double tol, double *norms,
 int  cffc_run_steps_graph(cffc_ctx *ctx, int nsteps, double *norms,
                           double *dt_out);
 
+/* Batched steady runs.  The reference runs one case per solver invocation
+   (Euler2DQuadSolver, Euler2DQuadSolvers.cc:26, explicit loop :549-898, with
+   Freeze_Limiters at Freeze_Limiter_Residual_Level, :796-803).  Here several
+   independent cases ("parts") may share one context: part q owns blocks
+   first_block[q] .. first_block[q+1]-1 (npart+1 entries, first 0, last
+   nblk), and no block may have a neighbour in another part
+   (CFFC_E_TOPOLOGY otherwise). */
+int  cffc_set_parts(cffc_ctx *ctx, int npart, const int32_t *first_block);
+int  cffc_num_parts(const cffc_ctx *ctx);
+
+/* Run every part to its own convergence in one stream of launches: each
+   step is the fused step of cffc_run_steps; after each step every part's
+   density-residual L2 norm (the reference's, over the part's cells) is
+   checked: at <= freeze_level (> 0) the part's limiters freeze (its
+   Freeze_Limiters), at <= tol the part stops updating and part_steps[q]
+   receives the number of steps it took (-1: not converged within
+   max_steps).  part_l2 (may be NULL) receives each part's last norm.
+   Returns when every part has converged or after max_steps steps
+   (*steps_done).  A part run alone and in a batch takes the same steps and
+   ends in the same state.  With keep_history the per-step norms of every
+   part are kept for cffc_get_part_history ([nsteps][npart]).  Without
+   cffc_set_parts the whole mesh is one part.  A non-positive density in any
+   part stops the batch with the reference's error code.  Parts are
+   independent only with local time stepping (a global dt couples them). */
+int  cffc_run_steady(cffc_ctx *ctx, int max_steps, double tol, double freeze_level, int keep_history,
+                     int32_t *part_steps, double *part_l2, int *steps_done);
+int  cffc_get_part_history(cffc_ctx *ctx, int nsteps, double *out);
+
 /* Wait for all work on the context's stream. */
 int  cffc_synchronize(cffc_ctx *ctx);
 

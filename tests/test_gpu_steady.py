@@ -1,0 +1,85 @@
+"""Batched steady runs (cffc_set_parts / cffc_run_steady) on the GPU.
+
+The device-driven steady loop must take exactly the steps of the reference's
+host-driven order of calls (run to Freeze_Limiter_Residual_Level,
+Freeze_Limiters, run on; Euler2DQuadSolvers.cc:796-803), and a case must
+converge in the same steps to the same state whether it runs alone or as
+one part of a batch.
+"""
+import numpy as np
+import pytest
+
+from cffc_amd import abi, euler2d, steady
+from cffc_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(nbi=6, nbj=2, ni=8, nj=8)   # 48 x 16 cells
+
+
+def _case(h, mach, **kw):
+    return steady.bump_case(h, mach, **SMALL, **kw)
+
+
+def test_device_loop_equals_host_loop():
+    m, W0 = _case(0.08, 0.5)
+    s = steady.bump_scheme()
+    g = euler2d.Euler2DQuadBlocks(m, s)
+    g.set_W(W0)
+    ref = steady.solve_steady_host_loop(g, tol=1e-7, max_steps=20000, freeze_level=1e-3)
+    W_ref = g.get_W()
+    g.close()
+    assert ref.rc == 0 and ref.l2 <= 1e-7 and ref.frozen_at > 0
+    g = euler2d.Euler2DQuadBlocks(m, s)
+    g.set_W(W0)
+    r = steady.solve_steady(g, tol=1e-7, max_steps=20000, freeze_level=1e-3)
+    W = g.get_W()
+    g.close()
+    print("host loop %d steps (frozen at %d), device loop %d steps" % (ref.steps, ref.frozen_at, r.steps))
+    assert r.rc == 0
+    assert r.steps == ref.steps
+    assert r.frozen_at == ref.frozen_at
+    np.testing.assert_array_equal(r.norms[:, 0], ref.norms[:, 1])
+    assert np.array_equal(W, W_ref)
+
+
+@pytest.mark.parametrize("n_stage,ti", [(4, abi.TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING),
+                                        (5, abi.TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING)])
+def test_batch_equals_single(n_stage, ti):
+    """Odd stage counts make converged parts copy their state forward."""
+    s = steady.bump_scheme(time_integration=ti, n_stage=n_stage, cfl=1.0)
+    cases = [(0.05, 0.35), (0.09, 0.55), (0.07, 0.45)]
+    singles = []
+    for h, mach in cases:
+        m, W0 = _case(h, mach)
+        g = euler2d.Euler2DQuadBlocks(m, s)
+        g.set_W(W0)
+        r = steady.solve_steady(g, tol=1e-6, max_steps=20000, freeze_level=1e-3, history=False)
+        singles.append((r.steps, g.get_W()))
+        g.close()
+        assert r.rc == 0 and r.steps > 0
+    b = steady.solve_batch([_case(h, mach) for h, mach in cases], scheme=s, tol=1e-6, max_steps=20000,
+                           freeze_level=1e-3)
+    print("single steps", [x[0] for x in singles], "batch", b.part_steps.tolist())
+    assert b.rc == 0
+    assert b.part_steps.tolist() == [x[0] for x in singles]
+    assert b.steps_done == max(x[0] for x in singles)
+    for (n, W), Wb in zip(singles, b.states):
+        assert np.array_equal(W, Wb)
+
+
+def test_parts_must_be_disjoint():
+    m, W0 = _case(0.06, 0.4)
+    g = euler2d.Euler2DQuadBlocks(m, steady.bump_scheme())
+    with pytest.raises(euler2d.CffcError):
+        g.set_parts([0, 3, m.nblk])   # cuts through a block row: neighbours in two parts
+    g.close()
+
+
+def test_unconverged_parts_report_minus_one():
+    b = steady.solve_batch([_case(0.08, 0.5), _case(0.05, 0.3)], tol=1e-12, max_steps=50, freeze_level=0.0,
+                           history=True)
+    assert b.rc == 0 and b.steps_done == 50
+    assert b.part_steps.tolist() == [-1, -1]
+    assert b.history.shape == (50, 2)
+    np.testing.assert_array_equal(b.history[-1], b.part_l2)
