@@ -73,6 +73,7 @@ struct cffc_ctx {
     int32_t *part_tile0 = nullptr, *part_state = nullptr, *part_steps = nullptr;
     double *part_l2 = nullptr, *part_last = nullptr;
     int part_l2_steps = 0;
+    unsigned long long *step_clock = nullptr;  // [max_steps + 1]: start, then end of every step
     int freeze = 0;      // limiters frozen (PHI_LOAD)
     int phi_valid = 0;   // phi array holds the last evaluated limiters
     int store_phi = 0;   // keep phi stored every step (freeze may follow)
@@ -92,7 +93,7 @@ static void free_ctx(cffc_ctx *c) {
     void *ptrs[] = {c->meta, c->nbr, c->bc, c->wo, c->geom, c->W[0], c->W[1], c->U, c->Uo, c->dUdt[0],
                     c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
-                    c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last};
+                    c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -569,10 +570,13 @@ static int ensure_steps(cffc_ctx *c, int nsteps) {
     if (nsteps <= c->max_steps) return 0;
     if (c->norm_out) (void)hipFree(c->norm_out);
     if (c->dt_out) (void)hipFree(c->dt_out);
+    if (c->step_clock) (void)hipFree(c->step_clock);
     c->norm_out = c->dt_out = nullptr;
+    c->step_clock = nullptr;
     int cap = std::max(nsteps, 1024);
     if (int e = dalloc(&c->norm_out, 3 * (size_t)cap)) return e;
     if (int e = dalloc(&c->dt_out, (size_t)cap)) return e;
+    if (int e = dalloc(&c->step_clock, (size_t)cap + 1)) return e;
     c->max_steps = cap;
     return 0;
 }
@@ -657,6 +661,7 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         f.part_last = c->part_last;
         f.freeze_level = freeze_level;
     }
+    f.step_clock = c->step_clock ? c->step_clock + 1 : nullptr;
     HIPCHK(cffc_launch_finish(&f, c->stream));
     return 0;
 }
@@ -674,6 +679,7 @@ static int begin_run(cffc_ctx *c, int nsteps) {
     HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, MILLION_BITS, 2, c->stream));
     HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
                            c->stream));
+    HIPCHK(cffc_launch_stamp(c->step_clock, c->stream));
     return 0;
 }
 
@@ -854,6 +860,108 @@ extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double fr
 }
 
 extern "C" int cffc_num_parts(const cffc_ctx *c) { return c ? c->npart : CFFC_E_ARG; }
+
+extern "C" int cffc_step_times(cffc_ctx *c, int nsteps, double *seconds) {
+    if (!c || !seconds || nsteps < 0) return fail(CFFC_E_ARG, "bad argument");
+    if (nsteps > c->max_steps) return fail(CFFC_E_ARG, "only %d steps are recorded", c->max_steps);
+    std::vector<unsigned long long> t((size_t)nsteps + 1);
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(t.data(), c->step_clock, t.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    int khz = 0;
+    HIPCHK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device));
+    if (khz <= 0) return fail(CFFC_E_HIP, "device wall clock rate unknown");
+    for (int n = 0; n < nsteps; ++n) seconds[n] = (double)(t[n + 1] - t[0]) / (1e3 * khz);
+    return 0;
+}
+
+// ---- tile autotuner -------------------------------------------------------------
+// Times nsteps fused steps with every instantiated tile shape on the
+// context's own mesh and state, restores the state after every trial and
+// keeps the fastest shape.
+extern "C" int cffc_autotune_tile(cffc_ctx *c, int nsteps, int *tile_i, int *tile_j, double *ms_per_step,
+                                  int *ncand) {
+    if (!c || nsteps < 1) return fail(CFFC_E_ARG, "bad argument");
+    static const int cand[][2] = {
+#define T(A, B) {A, B},
+        CFFC_TILE_LIST(T)
+#undef T
+    };
+    const int nc = (int)(sizeof cand / sizeof cand[0]);
+    const size_t n4 = 4 * (size_t)c->ncp;
+    double *sW = nullptr, *sU = nullptr, *sS = nullptr, *sdt = nullptr, *sphi = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    const int w0 = c->wcur, ti0 = c->ti, tj0 = c->tj;
+    const bool tsave = c->timing;
+    int best = -1;
+    double best_ms = 0.0;
+    auto snap = [&](bool save) -> hipError_t {
+        hipError_t e = hipSuccess;
+        const hipMemcpyKind k = hipMemcpyDeviceToDevice;
+        if (save) {
+            if ((e = hipMemcpyAsync(sW, c->W[w0], n4 * sizeof(double), k, c->stream))) return e;
+            if ((e = hipMemcpyAsync(sU, c->U, n4 * sizeof(double), k, c->stream))) return e;
+            if ((e = hipMemcpyAsync(sS, c->dUdt[0], n4 * sizeof(double), k, c->stream))) return e;
+            if ((e = hipMemcpyAsync(sdt, c->dt, c->ncp * sizeof(double), k, c->stream))) return e;
+            if (sphi) e = hipMemcpyAsync(sphi, c->phi, n4 * sizeof(double), k, c->stream);
+        } else {
+            if ((e = hipMemcpyAsync(c->W[w0], sW, n4 * sizeof(double), k, c->stream))) return e;
+            if ((e = hipMemcpyAsync(c->U, sU, n4 * sizeof(double), k, c->stream))) return e;
+            if ((e = hipMemcpyAsync(c->dUdt[0], sS, n4 * sizeof(double), k, c->stream))) return e;
+            if ((e = hipMemcpyAsync(c->dt, sdt, c->ncp * sizeof(double), k, c->stream))) return e;
+            if (sphi) e = hipMemcpyAsync(c->phi, sphi, n4 * sizeof(double), k, c->stream);
+            c->wcur = w0;
+        }
+        return e;
+    };
+    if ((rc = dalloc(&sW, n4)) || (rc = dalloc(&sU, n4)) || (rc = dalloc(&sS, n4)) || (rc = dalloc(&sdt, c->ncp)) ||
+        (c->phi && (rc = dalloc(&sphi, n4))))
+        goto done;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(CFFC_E_HIP, "hipEventCreate failed");
+        goto done;
+    }
+    c->timing = false;
+    if (snap(true) != hipSuccess) { rc = fail(CFFC_E_HIP, "state snapshot failed"); goto done; }
+    for (int k = 0; k < nc; ++k) {
+        float ms = 0.0f;
+        if ((rc = cffc_set_tile(c, cand[k][0], cand[k][1]))) goto done;
+        if (snap(false) != hipSuccess) { rc = fail(CFFC_E_HIP, "state restore failed"); goto done; }
+        if ((rc = begin_run(c, nsteps + 1))) goto done;
+        if ((rc = enqueue_step(c, 0, 0.0))) goto done;   // untimed first step
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = fail(CFFC_E_HIP, "event record failed"); goto done; }
+        for (int n = 1; n <= nsteps; ++n)
+            if ((rc = enqueue_step(c, n & 1, 0.0))) goto done;
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            rc = fail(CFFC_E_HIP, "trial timing failed");
+            goto done;
+        }
+        const double per = ms / nsteps;
+        if (ms_per_step) ms_per_step[k] = per;
+        if (best < 0 || per < best_ms) { best = k; best_ms = per; }
+    }
+done:
+    c->timing = tsave;
+    if (best >= 0 && !rc) {
+        rc = cffc_set_tile(c, cand[best][0], cand[best][1]);
+        if (!rc && snap(false) != hipSuccess) rc = fail(CFFC_E_HIP, "state restore failed");
+        if (!rc) {
+            if (tile_i) *tile_i = cand[best][0];
+            if (tile_j) *tile_j = cand[best][1];
+        }
+    } else if (sW) {
+        (void)cffc_set_tile(c, ti0, tj0);
+        (void)snap(false);
+    }
+    (void)hipStreamSynchronize(c->stream);
+    if (ncand) *ncand = nc;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    for (double *q : {sW, sU, sS, sdt, sphi})
+        if (q) (void)hipFree(q);
+    return rc;
+}
 
 extern "C" int cffc_get_part_history(cffc_ctx *c, int nsteps, double *out) {
     if (!c || !out || nsteps < 0) return fail(CFFC_E_ARG, "bad argument");

@@ -465,9 +465,15 @@ __global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) 
         p.norm_out[3 * s + 2] = C;
         p.dt_out[s] = bitsd(*p.dtmin_in);
         *p.dtmin_out = dbits(D);
+        if (p.step_clock) p.step_clock[s] = wall_clock64();
         *p.step_ctr = s + 1;
         if (p.npart == 0 ? (p.tol > 0.0 && l2n <= p.tol) : nd == p.npart) *p.stop_flag = 1;
     }
+}
+
+// device wall clock (constant rate, hipDeviceAttributeWallClockRate kHz)
+__global__ void stamp_kernel(unsigned long long *dst) {
+    if (threadIdx.x == 0) *dst = wall_clock64();
 }
 
 // small device-side scalar writes (graph-safe, no memset nodes)
@@ -541,6 +547,10 @@ hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, u
     return hipGetLastError();
 }
 
+hipError_t cffc_launch_stamp(unsigned long long *dst, hipStream_t st) {
+    hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(64), 0, st, dst);
+    return hipGetLastError();
+}
 hipError_t cffc_launch_finish(const FinishParams *p, hipStream_t st) {
     hipLaunchKernelGGL(step_finish_kernel, dim3(1), dim3(FINISH_NT), 0, st, *p);
     return hipGetLastError();

@@ -45,6 +45,7 @@ struct FinishParams {
     double *part_l2;                        // [max_steps][npart] or null
     double *part_last;                      // [npart] L2 of the part's last step
     double freeze_level;                    // part L2 at which limiters freeze (<= 0: never)
+    unsigned long long *step_clock;         // [max_steps] device wall clock at the end of each step, or null
 };
 
 }  // namespace cffc
@@ -75,6 +76,7 @@ hipError_t cffc_launch_norms(const cffc::BlockMeta *meta, const cffc::TileDesc *
 hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n,
                                hipStream_t st);
 hipError_t cffc_launch_finish(const cffc::FinishParams *p, hipStream_t st);
+hipError_t cffc_launch_stamp(unsigned long long *dst, hipStream_t st);
 hipError_t cffc_launch_stage_fp64(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 hipError_t cffc_launch_stage_mixed(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 }

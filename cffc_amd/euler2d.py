@@ -19,6 +19,10 @@ LIB_PATH = os.path.join(HERE, "libcffc_hip.so")
 
 _lib = None
 
+# tile shapes the stage kernel is built for, in the autotuner's order
+# (CFFC_TILE_LIST in csrc/euler2d_kernels.hpp)
+TILE_SHAPES = [(16, 16), (32, 8), (8, 32), (16, 8), (8, 8), (32, 4)]
+
 
 class CffcError(RuntimeError):
     pass
@@ -70,6 +74,9 @@ def load():
     L.cffc_num_parts.argtypes = [vp]
     L.cffc_run_steady.argtypes = [vp, i32, d, d, i32, pi32, pd, ctypes.POINTER(ctypes.c_int)]
     L.cffc_get_part_history.argtypes = [vp, i32, pd]
+    L.cffc_step_times.argtypes = [vp, i32, pd]
+    pint = ctypes.POINTER(ctypes.c_int)
+    L.cffc_autotune_tile.argtypes = [vp, i32, pint, pint, pd, pint]
     L.cffc_last_error.argtypes = []
     L.cffc_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -223,6 +230,12 @@ class Euler2DQuadBlocks:
             hist = hist[:done.value]
         return rc, done.value, steps, l2, hist
 
+    def step_times(self, nsteps):
+        """Seconds from the start of the last run to the end of each step."""
+        out = np.zeros(max(nsteps, 1))
+        _check(load().cffc_step_times(self.h, nsteps, _p(out)), "cffc_step_times")
+        return out[:nsteps]
+
     def synchronize(self):
         _check(load().cffc_synchronize(self.h), "cffc_synchronize")
 
@@ -236,6 +249,15 @@ class Euler2DQuadBlocks:
         a, b = ctypes.c_int(), ctypes.c_int()
         load().cffc_get_tile(self.h, ctypes.byref(a), ctypes.byref(b))
         return a.value, b.value
+
+    def autotune(self, nsteps=4):
+        """Pick the fastest tile shape for this mesh (state is preserved).
+        Returns ((ti, tj), {(ti, tj): ms per step})."""
+        a, b, n = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        ms = np.zeros(16)
+        _check(load().cffc_autotune_tile(self.h, nsteps, ctypes.byref(a), ctypes.byref(b), _p(ms), ctypes.byref(n)),
+               "cffc_autotune_tile")
+        return (a.value, b.value), dict(zip(TILE_SHAPES[:n.value], ms[:n.value].tolist()))
 
     def kernel_timing(self, on=True):
         _check(load().cffc_set_kernel_timing(self.h, 1 if on else 0), "cffc_set_kernel_timing")

@@ -94,6 +94,7 @@ class BatchResult:
     seconds: float
     states: list
     history: np.ndarray
+    part_seconds: np.ndarray   # device time from the start to each part's convergence (nan: not converged)
 
 
 def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=STEADY_FREEZE_LEVEL, history=False):
@@ -114,11 +115,14 @@ def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=ST
         rc, n, steps, l2, hist = g.run_steady(max_steps, tol, freeze_level, history=history)
         dt = time.perf_counter() - t0
         W = g.get_W()
+        clock = g.step_times(n) if n > 0 else np.zeros(0)
+        part_s = np.array([clock[k - 1] if k > 0 else np.nan for k in steps])
     finally:
         g.close()
     offs = np.cumsum([0] + [m.ncell for m in meshes])
     states = [W[offs[q]:offs[q + 1]].copy() for q in range(len(meshes))]
-    return BatchResult(rc=rc, steps_done=n, part_steps=steps, part_l2=l2, seconds=dt, states=states, history=hist)
+    return BatchResult(rc=rc, steps_done=n, part_steps=steps, part_l2=l2, seconds=dt, states=states, history=hist,
+                       part_seconds=part_s)
 
 
 def bump_case(h, mach, outflow=abi.BC_CHARACTERISTIC, **kw):

@@ -201,3 +201,21 @@ def image_to_state(mesh, img):
 
 def state_to_image(mesh, W):
     return np.asarray(W)[image_index(mesh)]
+
+
+def resample_image(img, NJ, NI):
+    """Bilinear resampling of a cell-centred (nj, ni, C) bump image to
+    (NJ, NI): the bump meshes are uniform in x and in the normalised height,
+    so cell (I, J) of any of them sits at the same fractional position."""
+    nj, ni = img.shape[:2]
+    jf = np.clip((np.arange(NJ) + 0.5) * nj / NJ - 0.5, 0, nj - 1)
+    if_ = np.clip((np.arange(NI) + 0.5) * ni / NI - 0.5, 0, ni - 1)
+    j0 = np.minimum(np.floor(jf).astype(int), nj - 2)
+    i0 = np.minimum(np.floor(if_).astype(int), ni - 2)
+    wj = (jf - j0)[:, None, None]
+    wi = (if_ - i0)[None, :, None]
+    a = img[j0][:, i0]
+    b = img[j0][:, i0 + 1]
+    c = img[j0 + 1][:, i0]
+    d = img[j0 + 1][:, i0 + 1]
+    return (1 - wj) * ((1 - wi) * a + wi * b) + wj * ((1 - wi) * c + wi * d)

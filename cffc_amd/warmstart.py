@@ -10,6 +10,8 @@ steady bump channel (192 x 64 cells in 12 x 4 blocks of 16^2, h ~ U[0.04,
   the freestream and from the network's prediction, both as a batch; the
   steps every case needs and the wall time of each batch are compared.
 """
+import argparse
+import json
 import time
 
 import numpy as np
@@ -81,3 +83,101 @@ def evaluate(net, norm, cases, tol=1e-10, max_steps=200000, freeze_level=steady.
                 cold_seconds=float(sum(cold_s)), warm_seconds=float(sum(warm_s)) + t_inf, inference_seconds=t_inf,
                 step_ratio_median=float(np.median(cold_steps / np.maximum(warm_steps, 1))),
                 step_ratio_total=float(cold_steps.sum() / max(warm_steps.sum(), 1)))
+
+
+# ---- persistence -------------------------------------------------------------
+def save_model(path, net, norm, meta=None):
+    """Weights of every layer, the target scale and free-form metadata."""
+    arrs = {"scale": norm.scale, "H": net.H, "W": net.W}
+    for l in range(net.nlayers):
+        w, b = net.get_weights(l)
+        arrs["w%d" % l], arrs["b%d" % l] = w, b
+    if meta is not None:
+        arrs["meta"] = np.frombuffer(json.dumps(meta).encode(), dtype=np.uint8)
+    np.savez_compressed(path, **arrs)
+
+
+def load_model(path, max_batch=16):
+    z = np.load(path)
+    net = unet.UNet(H=int(z["H"]), W=int(z["W"]), max_batch=max_batch)
+    for l in range(net.nlayers):
+        net.set_weights(l, z["w%d" % l], z["b%d" % l])
+    return net, unet.Normaliser(z["scale"])
+
+
+def save_dataset(path, cases, states):
+    """Cases and their steady states as freestream deviations (float16
+    images, unit scale): the network's targets before scaling."""
+    m0 = M.bump_channel(cases[0][0])
+    unit = unet.Normaliser((1.0, 1.0, 1.0, 1.0))
+    Y = np.stack([unit.encode(unet.state_to_image(m0, W), mach) for (h, mach), W in zip(cases, states)])
+    np.savez_compressed(path, cases=np.asarray(cases), dev=Y.astype(np.float16))
+
+
+def load_dataset(path):
+    z = np.load(path)
+    return z["cases"], z["dev"].astype(np.float32)
+
+
+def training_arrays(cases, dev, norm):
+    X = np.stack([unet.features(M.bump_channel(h), h, mach) for h, mach in cases])
+    Y = (dev / norm.scale.astype(np.float32)).astype(np.float32)
+    return X, Y
+
+
+def main(argv=None):
+    """generate: solve seeded training cases; train: fit the U-Net; evaluate:
+    cold vs warm solves of held-out cases (all on the GPU)."""
+    ap = argparse.ArgumentParser(prog="python -m cffc_amd.warmstart")
+    ap.add_argument("cmd", choices=["generate", "train", "evaluate"])
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--max-steps", type=int, default=300000)
+    ap.add_argument("--data", default="bump_train.npz")
+    ap.add_argument("--model", default="unet_bump.npz")
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--out", default=None, help="JSON report")
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    report = {"cmd": a.cmd}
+    if a.cmd == "generate":
+        cases = steady.sample_cases(a.n, a.seed)
+        t0 = time.perf_counter()
+        states, steps, secs = solve_cases(cases, a.tol, a.max_steps, log=log)
+        save_dataset(a.data, cases, states)
+        report.update(n=a.n, seed=a.seed, tol=a.tol, seconds=time.perf_counter() - t0, steps=steps.tolist(),
+                      batch_seconds=secs)
+        log("generated %d cases: steps median %d max %d, %.1f s" % (a.n, int(np.median(steps)), int(steps.max()),
+                                                                    report["seconds"]))
+    elif a.cmd == "train":
+        cases, dev = load_dataset(a.data)
+        norm = unet.Normaliser(np.sqrt(np.mean(dev.reshape(-1, 4).astype(np.float64) ** 2, axis=0)) + 1e-6)
+        X, Y = training_arrays(cases, dev, norm)
+        net = unet.UNet(H=X.shape[1], W=X.shape[2], max_batch=a.batch, seed=a.seed)
+        net.set_dataset(X, Y)
+        t0 = time.perf_counter()
+        losses = net.train(a.steps, a.batch, lr=a.lr, seed=a.seed, log_every=max(a.steps // 20, 1))
+        dt = time.perf_counter() - t0
+        save_model(a.model, net, norm, meta={"cases": len(cases), "steps": a.steps, "batch": a.batch, "lr": a.lr})
+        report.update(train_seconds=dt, loss_first=float(losses[:10].mean()), loss_last=float(losses[-50:].mean()),
+                      scale=norm.scale.tolist())
+        log("trained %d steps in %.1f s: loss %.4e -> %.4e" % (a.steps, dt, report["loss_first"],
+                                                               report["loss_last"]))
+    else:
+        net, norm = load_model(a.model)
+        cases = steady.sample_cases(a.n, a.seed)
+        r = evaluate(net, norm, cases, a.tol, a.max_steps, log=log)
+        report.update(r)
+        log("held-out %d: cold steps median %d, warm %d; step ratio median %.2f total %.2f; batch time cold %.1fs "
+            "warm %.1fs" % (a.n, int(np.median(r["cold_steps"])), int(np.median(r["warm_steps"])),
+                            r["step_ratio_median"], r["step_ratio_total"], r["cold_seconds"], r["warm_seconds"]))
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump(report, f)
+
+
+if __name__ == "__main__":
+    main()

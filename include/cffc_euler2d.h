@@ -225,6 +225,11 @@ int  cffc_run_steady(cffc_ctx *ctx, int max_steps, double tol, double freeze_lev
                      int32_t *part_steps, double *part_l2, int *steps_done);
 int  cffc_get_part_history(cffc_ctx *ctx, int nsteps, double *out);
 
+/* Device wall-clock time from the start of the last cffc_run_steps /
+   cffc_run_steps_graph / cffc_run_steady to the end of each of its first
+   nsteps steps (seconds; for per-case latencies of a batched run). */
+int  cffc_step_times(cffc_ctx *ctx, int nsteps, double *seconds);
+
 /* Wait for all work on the context's stream. */
 int  cffc_synchronize(cffc_ctx *ctx);
 
@@ -236,6 +241,14 @@ void *cffc_stream(cffc_ctx *ctx);
    set here: tile_i x tile_j interior cells per workgroup. */
 int  cffc_set_tile(cffc_ctx *ctx, int tile_i, int tile_j);
 int  cffc_get_tile(const cffc_ctx *ctx, int *tile_i, int *tile_j);
+
+/* Tile autotuner (no reference counterpart: the reference's blocks are
+   swept cell by cell).  Times nsteps fused time steps with every
+   instantiated tile shape on this context's mesh and current state,
+   restores the state after each trial, and keeps the fastest shape
+   (returned in tile_i/tile_j).  ms_per_step (may be NULL, room for 16)
+   receives the time per step of every candidate; *ncand their number. */
+int  cffc_autotune_tile(cffc_ctx *ctx, int nsteps, int *tile_i, int *tile_j, double *ms_per_step, int *ncand);
 
 /* Kernel timing for measurement: with timing on, cffc_run_steps brackets
    every exchange/BC kernel and every fused stage kernel with a pair of HIP

@@ -174,6 +174,28 @@ def test_tile_shapes_agree(euler2d):
         np.testing.assert_array_equal(W, ref)
 
 
+def test_autotune_keeps_state_and_results(euler2d):
+    """The autotuner times every tile shape on the live state and restores
+    it: a run after autotuning equals a run without it, bit for bit."""
+    m = M.cartesian(3, 2, 32, 32, jitter=0.05, seed=4)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    g = euler2d.Euler2DQuadBlocks(m, abi.Scheme())
+    g.set_W(W0)
+    g.run_steps(3)
+    (ti, tj), ms = g.autotune(nsteps=3)
+    print("autotune picked %dx%d; ms/step %s" % (ti, tj, ms))
+    assert (ti, tj) in euler2d.TILE_SHAPES and g.get_tile() == (ti, tj)
+    assert len(ms) == len(euler2d.TILE_SHAPES) and all(v > 0 for v in ms.values())
+    assert min(ms, key=ms.get) == (ti, tj)
+    g.run_steps(5)
+    W = g.get_W()
+    ref = euler2d.Euler2DQuadBlocks(m, abi.Scheme())
+    ref.set_W(W0)
+    ref.run_steps(8)
+    np.testing.assert_array_equal(W, ref.get_W())
+
+
 def test_mixed_precision_error_stated(euler2d):
     m = M.cartesian(4, 4, 16, 16)
     W0 = M.riemann_4quadrant(m)
