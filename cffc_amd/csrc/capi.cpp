@@ -52,6 +52,7 @@ struct cffc_ctx {
     BlockMeta *meta = nullptr;
     int32_t *nbr = nullptr, *bc = nullptr;
     double *wo = nullptr, *geom = nullptr;
+    float *geomf = nullptr;   // FP32 geometry planes (mixed path only)
     double *W[2] = {nullptr, nullptr};
     int wcur = 0;
     double *U = nullptr, *Uo = nullptr, *dUdt[3] = {nullptr, nullptr, nullptr};
@@ -73,6 +74,7 @@ struct cffc_ctx {
     int32_t *part_tile0 = nullptr, *part_state = nullptr, *part_steps = nullptr;
     double *part_l2 = nullptr, *part_last = nullptr;
     int part_l2_steps = 0;
+    int steady_done = 0;   // steps of the current steady run before this segment
     unsigned long long *step_clock = nullptr;  // [max_steps + 1]: start, then end of every step
     int freeze = 0;      // limiters frozen (PHI_LOAD)
     int phi_valid = 0;   // phi array holds the last evaluated limiters
@@ -90,7 +92,7 @@ struct cffc_ctx {
 // ---------------------------------------------------------------------------
 static void free_ctx(cffc_ctx *c) {
     if (!c) return;
-    void *ptrs[] = {c->meta, c->nbr, c->bc, c->wo, c->geom, c->W[0], c->W[1], c->U, c->Uo, c->dUdt[0],
+    void *ptrs[] = {c->meta, c->nbr, c->bc, c->wo, c->geom, c->geomf, c->W[0], c->W[1], c->U, c->Uo, c->dUdt[0],
                     c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
                     c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock};
@@ -248,14 +250,18 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
         }
         m.nci = m.ni + 4;
         m.ncj = m.nj + 4;
-        m.coff = coff;
+        // rows pitch cells apart (a multiple of 16 doubles = 128 bytes) and
+        // the block placed so that interior cell (2, j) starts a line
+        m.pitch = (m.nci + 15) & ~15;
+        m.spare = 0;
+        m.coff = ((coff + 15) & ~(int64_t)15) + 14;
         m.ioff = ioff;
         m.noff = noff;
         m.bcoff = bcoff;
         m.has_bc = 0;
         for (int q = 0; q < 2 * (m.ni + 4) + 2 * (m.nj + 4); ++q)
             if (md->bc[bcoff + q] != CFFC_BC_NONE) m.has_bc = 1;
-        coff += (int64_t)m.nci * m.ncj;
+        coff = m.coff + (int64_t)m.pitch * m.ncj;
         ioff += (int64_t)m.ni * m.nj;
         noff += (int64_t)(m.nci + 1) * (m.ncj + 1);
         bcoff += 2 * m.ncj + 2 * m.nci;
@@ -291,7 +297,7 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
             }
         }
     }
-    c->ncp = coff;
+    c->ncp = (coff + 15) & ~(int64_t)15;   // every SoA plane starts 128-byte aligned
     c->ncell = ioff;
     const int64_t ncp = c->ncp;
     int e = 0;
@@ -337,6 +343,13 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
     if (nd) {
         free_ctx(c);
         return fail(CFFC_E_ARG, "mesh has %u degenerate faces (length <= 1e-9 x cell perimeter)", nd);
+    }
+    if (c->s.flux_precision == CFFC_FLUX_FP32) {
+        if ((e = dalloc(&c->geomf, GF_COUNT * (size_t)ncp))) {
+            free_ctx(c);
+            return e;
+        }
+        HIPCHK(cffc_launch_geometry_f32(c->meta, c->nblk, c->max_cells, c->geom, c->geomf, ncp, c->stream));
     }
     // default tile: the largest supported square-ish tile that the blocks fill
     int ti = 16, tj = 16;
@@ -478,6 +491,7 @@ static StageParams base_stage_params(cffc_ctx *c) {
     p.ntiles = c->ntiles;
     p.ncp = c->ncp;
     p.geom = c->geom;
+    p.geomf = c->geomf;
     p.bc = c->bc;
     p.wo = c->wo;
     p.U = c->U;
@@ -660,6 +674,7 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         f.part_l2 = c->part_l2;
         f.part_last = c->part_last;
         f.freeze_level = freeze_level;
+        f.step_offset = c->steady_done;
     }
     f.step_clock = c->step_clock ? c->step_clock + 1 : nullptr;
     HIPCHK(cffc_launch_finish(&f, c->stream));
@@ -792,9 +807,13 @@ extern "C" int cffc_set_parts(cffc_ctx *c, int npart, const int32_t *first_block
     return build_tiles(c, c->ti, c->tj);
 }
 
-extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double freeze_level, int keep_history,
+extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double freeze_level, int flags,
                                int32_t *part_steps, double *part_l2, int *steps_done) {
     if (!c || max_steps < 1 || !(tol > 0.0)) return fail(CFFC_E_ARG, "bad argument");
+    const bool history = flags & CFFC_STEADY_HISTORY, resume = flags & CFFC_STEADY_RESUME;
+    if (history && resume) return fail(CFFC_E_ARG, "history is kept for runs that are not resumed");
+    if (resume && c->npart == 0) return fail(CFFC_E_ARG, "nothing to resume");
+    if (steps_done) *steps_done = 0;
     if (c->npart == 0) {
         const int32_t fb[2] = {0, c->nblk};
         if (int e = cffc_set_parts(c, 1, fb)) return e;
@@ -803,13 +822,28 @@ extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double fr
     if (int e = ensure_phi(c)) return e;
     if (c->part_l2) { (void)hipFree(c->part_l2); c->part_l2 = nullptr; }
     c->part_l2_steps = 0;
-    if (keep_history) {
+    if (history) {
         if (int e = dalloc(&c->part_l2, (size_t)max_steps * c->npart)) return e;
         c->part_l2_steps = max_steps;
     }
-    HIPCHK(hipMemsetAsync(c->part_state, 0, c->npart * sizeof(int32_t), c->stream));       // PART_RUNNING
-    HIPCHK(hipMemsetAsync(c->part_steps, 0xFF, c->npart * sizeof(int32_t), c->stream));    // -1
-    HIPCHK(hipMemsetAsync(c->part_last, 0, c->npart * sizeof(double), c->stream));
+    if (!resume) {
+        HIPCHK(hipMemsetAsync(c->part_state, 0, c->npart * sizeof(int32_t), c->stream));       // PART_RUNNING
+        HIPCHK(hipMemsetAsync(c->part_steps, 0xFF, c->npart * sizeof(int32_t), c->stream));    // -1
+        HIPCHK(hipMemsetAsync(c->part_last, 0, c->npart * sizeof(double), c->stream));
+        c->steady_done = 0;
+    } else {
+        // a resumed run whose parts have all converged has nothing to do
+        std::vector<int32_t> st(c->npart);
+        HIPCHK(hipMemcpyAsync(st.data(), c->part_state, c->npart * sizeof(int32_t), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        bool all = true;
+        for (int32_t v : st) all = all && v == PART_DONE;
+        if (all) {
+            if (part_steps) HIPCHK(hipMemcpy(part_steps, c->part_steps, c->npart * sizeof(int32_t), hipMemcpyDeviceToHost));
+            if (part_l2) HIPCHK(hipMemcpy(part_l2, c->part_last, c->npart * sizeof(double), hipMemcpyDeviceToHost));
+            return 0;
+        }
+    }
     c->freeze = 0;
     if (int e = begin_run(c, max_steps)) return e;
     const int wstart = c->wcur;
@@ -852,7 +886,10 @@ extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double fr
     (void)hipGraphDestroy(g);
     if (rc) return rc;
     c->wcur = wstart;
-    rc = end_run(c, wstart, max_steps, nullptr, nullptr, steps_done);
+    int done = 0;
+    rc = end_run(c, wstart, max_steps, nullptr, nullptr, &done);
+    c->steady_done += done;
+    if (steps_done) *steps_done = done;
     if (part_steps) HIPCHK(hipMemcpy(part_steps, c->part_steps, c->npart * sizeof(int32_t), hipMemcpyDeviceToHost));
     if (part_l2) HIPCHK(hipMemcpy(part_l2, c->part_last, c->npart * sizeof(double), hipMemcpyDeviceToHost));
     (void)N;

@@ -3,8 +3,11 @@
 //
 // HBM layout (one context):
 //   * every block is stored padded with its two ghost layers,
-//     nci = ni+4 by ncj = nj+4 cells, i fastest; blocks are concatenated, so a
-//     padded cell has one global index c (block offset coff + j*nci + i);
+//     nci = ni+4 by ncj = nj+4 cells, i fastest, rows pitch cells apart
+//     (pitch = nci rounded up to 16) and the block offset chosen so that the
+//     first interior cell of every row starts a 128-byte line; blocks are
+//     concatenated, so a padded cell has one global index
+//     c = coff + j*pitch + i;
 //   * every field is SoA: variable v of field F lives at F + v*ncp, where ncp
 //     is the total number of padded cells (coalesced 8-byte lanes along i);
 //   * W (primitive, FP64) is double-buffered: a stage reads W^(s) with filled
@@ -24,9 +27,19 @@ enum {
     GEO_COUNT
 };
 
+// FP32 geometry planes of the mixed path: coordinates relative to the
+// centroid of the block's first interior cell, normals, face lengths
+enum {
+    GF_XC = 0, GF_YC, GF_XFE, GF_YFE, GF_XFN, GF_YFN,
+    GF_NEX, GF_NEY, GF_NNX, GF_NNY, GF_LE, GF_LN,
+    GF_COUNT
+};
+
 struct BlockMeta {
     int32_t ni, nj, nci, ncj;
-    int64_t coff;    // first padded cell of the block
+    int32_t pitch;   // row pitch of the padded block (cells, a multiple of 16)
+    int32_t spare;
+    int64_t coff;    // padded cell (0, 0) of the block; interior cell (2, j) is 128-byte aligned
     int64_t ioff;    // first interior cell of the block in the host state arrays
     int64_t noff;    // first node of the block in the node arrays
     int32_t bcoff;   // first entry of the block in the bc / wo arrays

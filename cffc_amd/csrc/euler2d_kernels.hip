@@ -33,7 +33,7 @@ __global__ void geometry_kernel(const BlockMeta *meta, const double *xn, const d
     const int nn = m.nci + 1;
     auto X = [&](int a, int b) { return xn[m.noff + (int64_t)b * nn + a]; };
     auto Y = [&](int a, int b) { return yn[m.noff + (int64_t)b * nn + a]; };
-    const int64_t c = m.coff + q;
+    const int64_t c = m.coff + (int64_t)j * m.pitch + i;
     // quadCentroid with SW, SE, NE, NW
     const double X0x = X(i, j), X0y = Y(i, j), X1x = X(i + 1, j), X1y = Y(i + 1, j);
     const double X2x = X(i + 1, j + 1), X2y = Y(i + 1, j + 1), X3x = X(i, j + 1), X3y = Y(i, j + 1);
@@ -83,6 +83,31 @@ __global__ void geometry_kernel(const BlockMeta *meta, const double *xn, const d
     if (!(lW > 0.000000001 * perim) || !(lS > 0.000000001 * perim)) atomicAdd(ndegenerate, 1u);
 }
 
+// FP32 geometry of the mixed path: one thread per padded cell.  Positions are
+// taken relative to the centroid of the block's first interior cell, so the
+// FP32 differences the reconstruction forms keep their precision.
+__global__ void geometry_f32_kernel(const BlockMeta *meta, const double *geom, float *gf, int64_t ncp) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.nci * m.ncj) return;
+    const int i = q % m.nci, j = q / m.nci;
+    const int64_t c = m.coff + (int64_t)j * m.pitch + i;
+    const int64_t c0 = m.coff + 2 * (int64_t)m.pitch + 2;
+    const double x0 = geom[GEO_XC * ncp + c0], y0 = geom[GEO_YC * ncp + c0];
+    gf[GF_XC * ncp + c] = (float)(geom[GEO_XC * ncp + c] - x0);
+    gf[GF_YC * ncp + c] = (float)(geom[GEO_YC * ncp + c] - y0);
+    gf[GF_XFE * ncp + c] = (float)(geom[GEO_XFE * ncp + c] - x0);
+    gf[GF_YFE * ncp + c] = (float)(geom[GEO_YFE * ncp + c] - y0);
+    gf[GF_XFN * ncp + c] = (float)(geom[GEO_XFN * ncp + c] - x0);
+    gf[GF_YFN * ncp + c] = (float)(geom[GEO_YFN * ncp + c] - y0);
+    gf[GF_NEX * ncp + c] = (float)geom[GEO_NEX * ncp + c];
+    gf[GF_NEY * ncp + c] = (float)geom[GEO_NEY * ncp + c];
+    gf[GF_NNX * ncp + c] = (float)geom[GEO_NNX * ncp + c];
+    gf[GF_NNY * ncp + c] = (float)geom[GEO_NNY * ncp + c];
+    gf[GF_LE * ncp + c] = (float)geom[GEO_LE * ncp + c];
+    gf[GF_LN * ncp + c] = (float)geom[GEO_LN * ncp + c];
+}
+
 // ---------------------------------------------------------------------------
 // state scatter / gather between host AoS interior order and device SoA padded
 __global__ void scatter_state_kernel(const BlockMeta *meta, const double *aos, double *dst, int64_t ncp, int kind,
@@ -91,7 +116,7 @@ __global__ void scatter_state_kernel(const BlockMeta *meta, const double *aos, d
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.ni * m.nj) return;
     const int i = q % m.ni + 2, j = q / m.ni + 2;
-    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    const int64_t c = m.coff + (int64_t)j * m.pitch + i;
     const double *a = aos + 4 * (m.ioff + q);
     if (kind == 0) {  // W given: W, and U = U(W)
         PS<double> W = {a[0], a[1], a[2], a[3]};
@@ -112,7 +137,7 @@ __global__ void gather_state_kernel(const BlockMeta *meta, const double *src, do
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.ni * m.nj) return;
     const int i = q % m.ni + 2, j = q / m.ni + 2;
-    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    const int64_t c = m.coff + (int64_t)j * m.pitch + i;
     double *a = aos + (int64_t)nvar * (m.ioff + q);
     for (int v = 0; v < nvar; ++v) a[v] = src[v * ncp + c];
 }
@@ -157,8 +182,8 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
         const int w = i1 - i0 + 1, h = j1 - j0 + 1;
         for (int q = tid; q < w * h; q += NT) {
             const int i = i0 + q % w, j = j0 + q / w;
-            const int64_t cd = m.coff + (int64_t)j * m.nci + i;
-            const int64_t cs = S.coff + (int64_t)(j + dj) * S.nci + (i + di);
+            const int64_t cd = m.coff + (int64_t)j * m.pitch + i;
+            const int64_t cs = S.coff + (int64_t)(j + dj) * S.pitch + (i + di);
             PS<double> Wg;
             if (p.from_U) {
                 const CS<double> U = {p.U[cs], p.U[ncp + cs], p.U[2 * ncp + cs], p.U[3 * ncp + cs]};
@@ -172,7 +197,7 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     __syncthreads();
     const int32_t *bcW = p.bc + m.bcoff, *bcE = bcW + m.ncj, *bcS = bcE + m.ncj, *bcN = bcS + m.nci;
     const double *woW = p.wo + 4 * m.bcoff, *woE = woW + 4 * m.ncj, *woS = woE + 4 * m.ncj, *woN = woS + 4 * m.nci;
-    auto cidx = [&](int i, int j) { return m.coff + (int64_t)j * m.nci + i; };
+    auto cidx = [&](int i, int j) { return m.coff + (int64_t)j * m.pitch + i; };
     auto wo = [&](const double *a, int k) { PS<double> r = {a[4 * k], a[4 * k + 1], a[4 * k + 2], a[4 * k + 3]}; return r; };
     // phase 1: West and East boundaries, j over all rows (corner rows when the
     // South/North BC of the ghost column is BC_NONE), SingleBlock.cc:2362-2676
@@ -263,16 +288,16 @@ __global__ __launch_bounds__(256) void cfl_kernel(const BlockMeta *meta, const T
     for (int q = threadIdx.x; q < ti * tj; q += blockDim.x) {
         const int ip = t.i0 + 2 + q % ti, jp = t.j0 + 2 + q / ti;
         if (ip > m.ni + 1 || jp > m.nj + 1) continue;
-        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+        const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
         const double *G = geom;
         const double A = G[GEO_A * ncp + c];
         const double lW = G[GEO_LE * ncp + c - 1], lE = G[GEO_LE * ncp + c];
-        const double lS = G[GEO_LN * ncp + c - m.nci], lN = G[GEO_LN * ncp + c];
+        const double lS = G[GEO_LN * ncp + c - m.pitch], lN = G[GEO_LN * ncp + c];
         const double d_i = 2.0 * (A / (lE + lW));
         const double d_j = 2.0 * (A / (lN + lS));
         const PS<double> Wc = ldW(W, ncp, c);
         const double nWx = -G[GEO_NEX * ncp + c - 1], nWy = -G[GEO_NEY * ncp + c - 1];
-        const double nSx = -G[GEO_NNX * ncp + c - m.nci], nSy = -G[GEO_NNY * ncp + c - m.nci];
+        const double nSx = -G[GEO_NNX * ncp + c - m.pitch], nSy = -G[GEO_NNY * ncp + c - m.pitch];
         const double v_i = 0.5 * (Wc.vx * (G[GEO_NEX * ncp + c] - nWx) + Wc.vy * (G[GEO_NEY * ncp + c] - nWy));
         const double v_j = 0.5 * (Wc.vx * (G[GEO_NNX * ncp + c] - nSx) + Wc.vy * (G[GEO_NNY * ncp + c] - nSy));
         const double a = W_a(Wc);
@@ -296,7 +321,7 @@ __global__ void set_global_dt_kernel(const BlockMeta *meta, double *dt, const un
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.ni * m.nj) return;
     const int i = q % m.ni + 2, j = q / m.ni + 2;
-    dt[m.coff + (int64_t)j * m.nci + i] = bitsd(*dtmin);
+    dt[m.coff + (int64_t)j * m.pitch + i] = bitsd(*dtmin);
 }
 
 // ---------------------------------------------------------------------------
@@ -307,7 +332,7 @@ __global__ void update_kernel(const BlockMeta *meta, double *W, double *U, const
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.ni * m.nj) return;
     const int i = q % m.ni + 2, j = q / m.ni + 2;
-    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    const int64_t c = m.coff + (int64_t)j * m.pitch + i;
     double d[4] = {dU_k[c], dU_k[ncp + c], dU_k[2 * ncp + c], dU_k[3 * ncp + c]};
     const double U0[4] = {Uo[c], Uo[ncp + c], Uo[2 * ncp + c], Uo[3 * ncp + c]};
     CS<double> Un = {U0[0] + omega * d[0], U0[1] + omega * d[1], U0[2] + omega * d[2], U0[3] + omega * d[3]};
@@ -341,7 +366,7 @@ __global__ void copy_interior_kernel(const BlockMeta *meta, const double *src, d
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= m.ni * m.nj) return;
     const int i = q % m.ni + 2, j = q / m.ni + 2;
-    const int64_t c = m.coff + (int64_t)j * m.nci + i;
+    const int64_t c = m.coff + (int64_t)j * m.pitch + i;
     for (int v = 0; v < 4; ++v) dst[v * ncp + c] = src[v * ncp + c];
 }
 
@@ -356,7 +381,7 @@ __global__ __launch_bounds__(256) void norm_partial_kernel(const BlockMeta *meta
     for (int q = threadIdx.x; q < ti * tj; q += blockDim.x) {
         const int ip = t.i0 + 2 + q % ti, jp = t.j0 + 2 + q / ti;
         if (ip > m.ni + 1 || jp > m.nj + 1) continue;
-        const double v = dU[(int64_t)(var - 1) * ncp + m.coff + (int64_t)jp * m.nci + ip];
+        const double v = dU[(int64_t)(var - 1) * ncp + m.coff + (int64_t)jp * m.pitch + ip];
         a += fabs(v);
         b += v * v;
         mx = dmax(mx, fabs(v));
@@ -439,7 +464,7 @@ __global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) 
                     p.part_last[pt] = l2p;
                     if (p.tol > 0.0 && l2p <= p.tol) {
                         st = PART_DONE;
-                        p.part_steps[pt] = s + 1;
+                        p.part_steps[pt] = p.step_offset + s + 1;
                     } else if (st == PART_RUNNING && p.freeze_level > 0.0 && l2p <= p.freeze_level) {
                         st = PART_FROZEN;
                     }
@@ -493,6 +518,12 @@ hipError_t cffc_launch_geometry(const BlockMeta *meta, int nblk, int max_cells, 
                                 double *geom, int64_t ncp, unsigned int *ndeg, hipStream_t st) {
     dim3 grid((max_cells + 255) / 256, nblk);
     hipLaunchKernelGGL(geometry_kernel, grid, dim3(256), 0, st, meta, xn, yn, geom, ncp, ndeg);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_geometry_f32(const BlockMeta *meta, int nblk, int max_cells, const double *geom, float *gf,
+                                    int64_t ncp, hipStream_t st) {
+    dim3 grid((max_cells + 255) / 256, nblk);
+    hipLaunchKernelGGL(geometry_f32_kernel, grid, dim3(256), 0, st, meta, geom, gf, ncp);
     return hipGetLastError();
 }
 hipError_t cffc_launch_scatter(const BlockMeta *meta, int nblk, int max_int, const double *aos, double *W, int64_t ncp,

@@ -45,6 +45,7 @@ struct FinishParams {
     double *part_l2;                        // [max_steps][npart] or null
     double *part_last;                      // [npart] L2 of the part's last step
     double freeze_level;                    // part L2 at which limiters freeze (<= 0: never)
+    int32_t step_offset;                    // steps of earlier segments of the same steady run
     unsigned long long *step_clock;         // [max_steps] device wall clock at the end of each step, or null
 };
 
@@ -56,6 +57,8 @@ struct FinishParams {
 extern "C" {
 hipError_t cffc_launch_geometry(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *xn,
                                 const double *yn, double *geom, int64_t ncp, unsigned int *ndeg, hipStream_t st);
+hipError_t cffc_launch_geometry_f32(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *geom,
+                                    float *gf, int64_t ncp, hipStream_t st);
 hipError_t cffc_launch_scatter(const cffc::BlockMeta *meta, int nblk, int max_int, const double *aos, double *W,
                                int64_t ncp, int kind, double *U, hipStream_t st);
 hipError_t cffc_launch_gather(const cffc::BlockMeta *meta, int nblk, int max_int, const double *src, double *aos,

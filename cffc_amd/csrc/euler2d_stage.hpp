@@ -29,6 +29,7 @@ struct StageParams {
     int32_t ntiles;
     int64_t ncp;            // padded cells of all blocks
     const double *geom;     // GEO_COUNT x ncp
+    const float *geomf;     // GF_COUNT x ncp (mixed path)
     const int32_t *bc;      // concatenated BCtypeW/E/S/N
     const double *wo;       // reference states of the bc entries (AoS, 4)
     const double *Win;      // W^(s), 4 x ncp, ghosts filled
@@ -51,6 +52,16 @@ struct StageParams {
     const int32_t *part_state;  // per-part PartState (batched steady runs), or null
     int32_t done_copy;      // parts that are done copy W forward (odd stage count)
 };
+
+// Workgroups are dispatched to the 8 XCDs round-robin (workgroup k to XCD
+// k mod 8), each XCD with its own L2.  Tiles are listed block by block, so
+// giving XCD x the x-th contiguous eighth of the list keeps neighbouring
+// tiles, which share halo rows and face geometry, in one L2.  A bijection of
+// [0, n) for any n.
+__device__ __forceinline__ int xcd_tile(int k, int n) {
+    const int q = n >> 3, r = n & 7, x = k & 7, idx = k >> 3;
+    return x * q + (x < r ? x : r) + idx;
+}
 
 // granular init modes (dUdt_Multistage_Explicit called on its own)
 enum GranularInit { GINIT_ZERO = 0, GINIT_KEEP = 1, GINIT_RK4 = 2 };
@@ -105,14 +116,25 @@ __device__ __forceinline__ PS<FT> face_state(const PS<FT> &W, const PS<FT> &px, 
     return r;
 }
 
+// Geometry as the two paths read it: the FP64 path reads the FP64 planes
+// (GEO_*); the mixed path reads FP32 planes (GF_*) with coordinates relative
+// to the block origin, which halves the geometry traffic.
+template <typename FT> struct CoordT { typedef double type; };
+template <> struct CoordT<float> { typedef float type; };
+template <typename FT>
+__device__ __forceinline__ typename CoordT<FT>::type rdgeo(const StageParams &p, int gd, int gf, int64_t i) {
+    if constexpr (Fast<FT>::on) return p.geomf[gf * p.ncp + i];
+    else return p.geom[gd * p.ncp + i];
+}
+
 // Least-squares gradient and Venkatakrishnan limiter of one cell
 // (Linear_Reconstruction_LeastSquares, SingleBlock.cc:3678-4087).  FULL: the
 // 8-point stencil, straight-line; otherwise the reduced stencil in mask.
 // The window arrays are indexed [row][col] with the cell at (jl, il).
-template <typename FT, bool FULL, int LJ, int LI>
-__device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int64_t c, int64_t ncp, int nci, int il,
-                                            int jl, unsigned mask, const FT (&sW)[4][LJ][LI], const double (&sX)[LJ][LI],
-                                            const double (&sY)[LJ][LI], PS<FT> &gx, PS<FT> &gy, PS<FT> &ph) {
+template <typename FT, bool FULL, int LJ, int LI, typename XT>
+__device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int64_t c, int64_t ncp, int pitch, int il,
+                                            int jl, unsigned mask, const FT (&sW)[4][LJ][LI], const XT (&sX)[LJ][LI],
+                                            const XT (&sY)[LJ][LI], PS<FT> &gx, PS<FT> &gy, PS<FT> &ph) {
     const PS<FT> W0 = {sW[0][jl][il], sW[1][jl][il], sW[2][jl][il], sW[3][jl][il]};
     gx = {FT(0), FT(0), FT(0), FT(0)};
     gy = gx;
@@ -121,7 +143,7 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
     const double *G = p.geom;
     PS<FT> Dx = {FT(0), FT(0), FT(0), FT(0)}, Dy = Dx;
     FT DxDx = FT(0), DxDy = FT(0), DyDy = FT(0);
-    const double xc0 = sX[jl][il], yc0 = sY[jl][il];
+    const XT xc0 = sX[jl][il], yc0 = sY[jl][il];
     int n_pts = 0;
 #pragma unroll
     for (int s = 0; s < 8; ++s) {
@@ -169,10 +191,12 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
     if (p.limiter == -1) { ph.d = ph.vx = ph.vy = ph.p = FT(1); return; }
     if (p.limiter == 0) { ph.d = ph.vx = ph.vy = ph.p = FT(0); return; }
     // face-midpoint offsets: E, W (= E of i-1), N, S (= N of j-1)
-    const FT dxE = (FT)(G[GEO_XFE * ncp + c] - xc0), dyE = (FT)(G[GEO_YFE * ncp + c] - yc0);
-    const FT dxW = (FT)(G[GEO_XFE * ncp + c - 1] - xc0), dyW = (FT)(G[GEO_YFE * ncp + c - 1] - yc0);
-    const FT dxN = (FT)(G[GEO_XFN * ncp + c] - xc0), dyN = (FT)(G[GEO_YFN * ncp + c] - yc0);
-    const FT dxS = (FT)(G[GEO_XFN * ncp + c - nci] - xc0), dyS = (FT)(G[GEO_YFN * ncp + c - nci] - yc0);
+    const FT dxE = (FT)(rdgeo<FT>(p, GEO_XFE, GF_XFE, c) - xc0), dyE = (FT)(rdgeo<FT>(p, GEO_YFE, GF_YFE, c) - yc0);
+    const FT dxW = (FT)(rdgeo<FT>(p, GEO_XFE, GF_XFE, c - 1) - xc0);
+    const FT dyW = (FT)(rdgeo<FT>(p, GEO_YFE, GF_YFE, c - 1) - yc0);
+    const FT dxN = (FT)(rdgeo<FT>(p, GEO_XFN, GF_XFN, c) - xc0), dyN = (FT)(rdgeo<FT>(p, GEO_YFN, GF_YFN, c) - yc0);
+    const FT dxS = (FT)(rdgeo<FT>(p, GEO_XFN, GF_XFN, c - pitch) - xc0);
+    const FT dyS = (FT)(rdgeo<FT>(p, GEO_YFN, GF_YFN, c - pitch) - yc0);
 #pragma unroll
     for (int n = 0; n < 4; ++n) {
         const FT u0 = n == 0 ? W0.d : n == 1 ? W0.vx : n == 2 ? W0.vy : W0.p;
@@ -226,13 +250,15 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     constexpr int LI = TI + 4, LJ = TJ + 4;
     constexpr int RI = TI + 2, RJ = TJ + 2;
     __shared__ FT sW[4][LJ][LI];
-    __shared__ double sX[LJ][LI], sY[LJ][LI];
+    typedef typename CoordT<FT>::type XT;
+    __shared__ XT sX[LJ][LI], sY[LJ][LI];
     __shared__ FT sPx[4][RJ][RI], sPy[4][RJ][RI];
     __shared__ FT sFi[4][TJ][TI + 1], sFj[4][TJ + 1][TI];
     __shared__ double sRed[NT / 64][4];
 
     if (p.stop_flag && *p.stop_flag) return;
-    const TileDesc t = p.tiles[blockIdx.x];
+    const int tix = xcd_tile(blockIdx.x, p.ntiles);
+    const TileDesc t = p.tiles[tix];
     const BlockMeta m = p.meta[t.b];
     const int tid = threadIdx.x;
     const int64_t ncp = p.ncp;
@@ -246,17 +272,17 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                 for (int q = tid; q < TI * TJ; q += NT) {
                     const int ip = t.i0 + 2 + q % TI, jp = t.j0 + 2 + q / TI;
                     if (ip <= m.ni + 1 && jp <= m.nj + 1) {
-                        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+                        const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
 #pragma unroll
                         for (int v = 0; v < 4; ++v) p.Wout[v * ncp + c] = p.Win[v * ncp + c];
                     }
                 }
             }
             if (p.last_stage && tid == 0) {
-                p.norm_part[blockIdx.x] = 0.0;
-                p.norm_part[p.ntiles + blockIdx.x] = 0.0;
-                p.norm_part[2 * p.ntiles + blockIdx.x] = 0.0;
-                p.norm_part[3 * p.ntiles + blockIdx.x] = MILLION;
+                p.norm_part[tix] = 0.0;
+                p.norm_part[p.ntiles + tix] = 0.0;
+                p.norm_part[2 * p.ntiles + tix] = 0.0;
+                p.norm_part[3 * p.ntiles + tix] = MILLION;
             }
             return;
         }
@@ -284,11 +310,11 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
         const int il = q % LI, jl = q / LI;
         const int ip = pi0 + il, jp = pj0 + jl;
         if (ip < m.nci && jp < m.ncj) {
-            const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+            const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
 #pragma unroll
             for (int v = 0; v < 4; ++v) sW[v][jl][il] = (FT)p.Win[v * ncp + c];
-            sX[jl][il] = G[GEO_XC * ncp + c];
-            sY[jl][il] = G[GEO_YC * ncp + c];
+            sX[jl][il] = rdgeo<FT>(p, GEO_XC, GF_XC, c);
+            sY[jl][il] = rdgeo<FT>(p, GEO_YC, GF_YC, c);
         }
     }
     __syncthreads();
@@ -309,13 +335,13 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
         const int il = ir + 1, jl = jr + 1;
         const int ip = pi0 + il, jp = pj0 + jl;
         if (ir >= 0 && ip <= m.nci - 2 && jp <= m.ncj - 2) {
-            const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+            const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
             unsigned mask = 0xFFu;
             if (m.has_bc && (ip <= 1 || ip >= ICu + 1 || jp <= 1 || jp >= JCu + 1))
                 mask = stencil_mask(ip, jp, ICu, JCu, bcW[jp], bcE[jp], bcS[ip], bcN[ip]);
             PS<FT> gx, gy, ph;
-            if (mask == 0xFFu) reconstruct<FT, true>(p, phim, c, ncp, m.nci, il, jl, 0xFFu, sW, sX, sY, gx, gy, ph);
-            else reconstruct<FT, false>(p, phim, c, ncp, m.nci, il, jl, mask, sW, sX, sY, gx, gy, ph);
+            if (mask == 0xFFu) reconstruct<FT, true>(p, phim, c, ncp, m.pitch, il, jl, 0xFFu, sW, sX, sY, gx, gy, ph);
+            else reconstruct<FT, false>(p, phim, c, ncp, m.pitch, il, jl, mask, sW, sX, sY, gx, gy, ph);
             if (phim == PHI_COMPUTE_STORE) {
                 // each stored cell is owned by exactly one tile: its own
                 // cells, plus the block's outer ring on the block edge
@@ -349,11 +375,11 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             rl_i = g + 1; rl_j = f; rr_i = g + 1; rr_j = f + 1;
         }
         if (jpL > JCu || ipL > ICu) return;
-        const int64_t cL = m.coff + (int64_t)jpL * m.nci + ipL;
-        const double nxd = idir ? G[GEO_NEX * ncp + cL] : G[GEO_NNX * ncp + cL];
-        const double nyd = idir ? G[GEO_NEY * ncp + cL] : G[GEO_NNY * ncp + cL];
-        const double xf = idir ? G[GEO_XFE * ncp + cL] : G[GEO_XFN * ncp + cL];
-        const double yf = idir ? G[GEO_YFE * ncp + cL] : G[GEO_YFN * ncp + cL];
+        const int64_t cL = m.coff + (int64_t)jpL * m.pitch + ipL;
+        const XT nxd = idir ? rdgeo<FT>(p, GEO_NEX, GF_NEX, cL) : rdgeo<FT>(p, GEO_NNX, GF_NNX, cL);
+        const XT nyd = idir ? rdgeo<FT>(p, GEO_NEY, GF_NEY, cL) : rdgeo<FT>(p, GEO_NNY, GF_NNY, cL);
+        const XT xf = idir ? rdgeo<FT>(p, GEO_XFE, GF_XFE, cL) : rdgeo<FT>(p, GEO_XFN, GF_XFN, cL);
+        const XT yf = idir ? rdgeo<FT>(p, GEO_YFE, GF_YFE, cL) : rdgeo<FT>(p, GEO_YFN, GF_YFN, cL);
         const FT nx = (FT)nxd, ny = (FT)nyd;
         auto load_state = [&](int ri, int rj, PS<FT> &W) {
             const int il = ri + 1, jl = rj + 1;
@@ -423,10 +449,10 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     const bool valid = is_tile && (ip <= ICu) && (jp <= JCu);
     double l1 = 0.0, l2 = 0.0, mx = 0.0, dtn = MILLION;
     if (valid) {
-        const int64_t c = m.coff + (int64_t)jp * m.nci + ip;
+        const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
         const double A = G[GEO_A * ncp + c];
-        const double lW = G[GEO_LE * ncp + c - 1], lE = G[GEO_LE * ncp + c];
-        const double lS = G[GEO_LN * ncp + c - m.nci], lN = G[GEO_LN * ncp + c];
+        const double lW = rdgeo<FT>(p, GEO_LE, GF_LE, c - 1), lE = rdgeo<FT>(p, GEO_LE, GF_LE, c);
+        const double lS = rdgeo<FT>(p, GEO_LN, GF_LN, c - m.pitch), lN = rdgeo<FT>(p, GEO_LN, GF_LN, c);
         const double dtc = p.lts ? p.dt[c] : bitsd(*p.dtmin_in);
         const double cdt = p.cfl * dtc;
         double dU[4];
@@ -528,11 +554,14 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                 l1 = fabs(dU[0]);
                 l2 = dU[0] * dU[0];
                 mx = l1;
-                // CFL of the next step, SingleBlock.cc:3036-3053
-                const double d_i = 2.0 * (A / (lE + lW));
-                const double d_j = 2.0 * (A / (lN + lS));
+                // CFL of the next step, SingleBlock.cc:3036-3053, from the
+                // FP64 geometry in both paths
+                const double gE = G[GEO_LE * ncp + c], gW = G[GEO_LE * ncp + c - 1];
+                const double gN = G[GEO_LN * ncp + c], gS = G[GEO_LN * ncp + c - m.pitch];
+                const double d_i = 2.0 * (A / (gE + gW));
+                const double d_j = 2.0 * (A / (gN + gS));
                 const double nWx = -G[GEO_NEX * ncp + c - 1], nWy = -G[GEO_NEY * ncp + c - 1];
-                const double nSx = -G[GEO_NNX * ncp + c - m.nci], nSy = -G[GEO_NNY * ncp + c - m.nci];
+                const double nSx = -G[GEO_NNX * ncp + c - m.pitch], nSy = -G[GEO_NNY * ncp + c - m.pitch];
                 const double v_i = 0.5 * (Wn.vx * (G[GEO_NEX * ncp + c] - nWx) + Wn.vy * (G[GEO_NEY * ncp + c] - nWy));
                 const double v_j = 0.5 * (Wn.vx * (G[GEO_NNX * ncp + c] - nSx) + Wn.vy * (G[GEO_NNY * ncp + c] - nSy));
                 const double a = W_a(Wn);
@@ -562,10 +591,10 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
         // per-tile partials (SoA), reduced in tile order by step_finish_kernel;
         // the kernel boundary orders these stores before that kernel's loads
         // (no device-scope fence or atomic per workgroup)
-        p.norm_part[blockIdx.x] = a;
-        p.norm_part[p.ntiles + blockIdx.x] = b;
-        p.norm_part[2 * p.ntiles + blockIdx.x] = cmx;
-        p.norm_part[3 * p.ntiles + blockIdx.x] = dmn;
+        p.norm_part[tix] = a;
+        p.norm_part[p.ntiles + tix] = b;
+        p.norm_part[2 * p.ntiles + tix] = cmx;
+        p.norm_part[3 * p.ntiles + tix] = dmn;
     }
 }
 

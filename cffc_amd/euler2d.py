@@ -209,8 +209,9 @@ class Euler2DQuadBlocks:
                "cffc_set_parts")
         self.npart = len(fb) - 1
 
-    def run_steady(self, max_steps, tol, freeze_level=0.0, history=False):
+    def run_steady(self, max_steps, tol, freeze_level=0.0, history=False, resume=False):
         """Every part to its own L2 <= tol (limiters frozen at freeze_level).
+        resume continues the previous run for up to max_steps more steps.
         Returns (rc, steps_done, part_steps[npart], part_l2[npart], history
         [steps_done, npart] or None)."""
         L = load()
@@ -218,7 +219,8 @@ class Euler2DQuadBlocks:
         steps = np.zeros(npart, dtype=np.int32)
         l2 = np.zeros(npart)
         done = ctypes.c_int(0)
-        rc = _check(L.cffc_run_steady(self.h, max_steps, tol, freeze_level, 1 if history else 0,
+        flags = (1 if history else 0) | (2 if resume else 0)
+        rc = _check(L.cffc_run_steady(self.h, max_steps, tol, freeze_level, flags,
                                       steps.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), _p(l2),
                                       ctypes.byref(done)), "cffc_run_steady")
         npart = L.cffc_num_parts(self.h)

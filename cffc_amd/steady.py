@@ -18,11 +18,13 @@ from . import mesh as M
 
 
 # Time marching of the steady bump runs: the reference's bump examples march
-# with multistage optimal smoothing and local time stepping
-# (examples/Euler2D/bump_flow/subsonic/explicit/bump.in); 4 stages at CFL 1.5
-# reached 1e-10 fastest of the settings measured without residual smoothing.
+# with 5-stage multistage optimal smoothing and local time stepping
+# (examples/Euler2D/bump_flow/subsonic/explicit/bump.in).  Without residual
+# smoothing, 5 stages at CFL 1.6 reached 1e-10 in 51k-101k steps on eight
+# sampled cases (4 stages at 1.5: 74k to over 150k); 2.0 (5 stages) and 1.8
+# (4 stages) diverge.  CFL 1.5 keeps a margin below the measured limit.
 STEADY_TIME_INTEGRATION = abi.TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING
-STEADY_N_STAGE = 4
+STEADY_N_STAGE = 5
 STEADY_CFL = 1.5
 STEADY_FREEZE_LEVEL = 1e-5
 
@@ -97,11 +99,14 @@ class BatchResult:
     part_seconds: np.ndarray   # device time from the start to each part's convergence (nan: not converged)
 
 
-def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=STEADY_FREEZE_LEVEL, history=False):
+def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=STEADY_FREEZE_LEVEL, history=False,
+                segment=0, log=None):
     """Solve the cases [(mesh, W0), ...] together: one mesh of disjoint
     parts, one launch per stage for all of them, each part converging (and
-    freezing its limiters) on its own.  Returns a BatchResult with the
-    steps of every case and the final state arrays."""
+    freezing its limiters) on its own.  segment > 0 drives the run in
+    resumed segments of that many steps (same result), logging progress
+    after each.  Returns a BatchResult with the steps of every case and the
+    final state arrays."""
     from . import euler2d
     meshes = [m for m, _ in cases]
     mc, starts = M.concat(meshes)
@@ -112,11 +117,30 @@ def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=ST
         g.set_W(W0)
         g.synchronize()
         t0 = time.perf_counter()
-        rc, n, steps, l2, hist = g.run_steady(max_steps, tol, freeze_level, history=history)
+        if segment <= 0 or history:
+            rc, n, steps, l2, hist = g.run_steady(max_steps, tol, freeze_level, history=history)
+            clock = g.step_times(n) if n > 0 else np.zeros(0)
+            part_s = np.array([clock[k - 1] if k > 0 else np.nan for k in steps])
+        else:
+            n, hist, part_s, rc = 0, None, None, 0
+            while n < max_steps:
+                seg = min(segment, max_steps - n)
+                rc, nd, steps, l2, _ = g.run_steady(seg, tol, freeze_level, resume=n > 0)
+                clock = g.step_times(nd) if nd > 0 else np.zeros(0)
+                base = time.perf_counter() - t0 - (clock[-1] if nd else 0.0)
+                if part_s is None:
+                    part_s = np.full(len(steps), np.nan)
+                for q, k in enumerate(steps):
+                    if k > n and k <= n + nd:
+                        part_s[q] = base + clock[k - n - 1]
+                n += nd
+                if log:
+                    log("  %d steps: %d of %d parts converged, max l2 %.3e, %.1f s" %
+                        (n, int(np.sum(steps > 0)), len(steps), float(np.max(l2)), time.perf_counter() - t0))
+                if rc != 0 or np.all(steps > 0) or nd < seg:
+                    break
         dt = time.perf_counter() - t0
         W = g.get_W()
-        clock = g.step_times(n) if n > 0 else np.zeros(0)
-        part_s = np.array([clock[k - 1] if k > 0 else np.nan for k in steps])
     finally:
         g.close()
     offs = np.cumsum([0] + [m.ncell for m in meshes])

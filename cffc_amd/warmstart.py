@@ -21,7 +21,7 @@ from . import mesh as M
 
 
 def solve_cases(cases, tol=1e-10, max_steps=200000, freeze_level=steady.STEADY_FREEZE_LEVEL, chunk=512,
-                initial=None, log=None):
+                initial=None, log=None, segment=10000):
     """Steady states of the (h, M) cases (batches of up to chunk cases).
     initial: optional list of initial state arrays (default freestream).
     Returns (states, steps per case, seconds per batch)."""
@@ -34,7 +34,8 @@ def solve_cases(cases, tol=1e-10, max_steps=200000, freeze_level=steady.STEADY_F
             if initial is not None:
                 W0 = initial[s + q]
             batch.append((m, W0))
-        r = steady.solve_batch(batch, tol=tol, max_steps=max_steps, freeze_level=freeze_level)
+        r = steady.solve_batch(batch, tol=tol, max_steps=max_steps, freeze_level=freeze_level, segment=segment,
+                               log=log)
         if log:
             log("cases %d-%d: rc=%d steps_done=%d max part steps=%d %.1fs" %
                 (s, s + len(part) - 1, r.rc, r.steps_done, int(r.part_steps.max()), r.seconds))

@@ -216,12 +216,19 @@ int  cffc_num_parts(const cffc_ctx *ctx);
    max_steps).  part_l2 (may be NULL) receives each part's last norm.
    Returns when every part has converged or after max_steps steps
    (*steps_done).  A part run alone and in a batch takes the same steps and
-   ends in the same state.  With keep_history the per-step norms of every
+   ends in the same state.  With CFFC_STEADY_HISTORY the per-step norms of every
    part are kept for cffc_get_part_history ([nsteps][npart]).  Without
    cffc_set_parts the whole mesh is one part.  A non-positive density in any
    part stops the batch with the reference's error code.  Parts are
-   independent only with local time stepping (a global dt couples them). */
-int  cffc_run_steady(cffc_ctx *ctx, int max_steps, double tol, double freeze_level, int keep_history,
+   independent only with local time stepping (a global dt couples them).
+   flags: CFFC_STEADY_HISTORY keeps the per-step norms as above;
+   CFFC_STEADY_RESUME continues the previous run (part states, frozen
+   limiters, step counts) for another max_steps steps, so that a long solve
+   can be driven in segments with the same result as one call (part_steps
+   counts all segments; *steps_done this segment's steps). */
+#define CFFC_STEADY_HISTORY 1
+#define CFFC_STEADY_RESUME  2
+int  cffc_run_steady(cffc_ctx *ctx, int max_steps, double tol, double freeze_level, int flags,
                      int32_t *part_steps, double *part_l2, int *steps_done);
 int  cffc_get_part_history(cffc_ctx *ctx, int nsteps, double *out);
 

@@ -83,3 +83,17 @@ def test_unconverged_parts_report_minus_one():
     assert b.part_steps.tolist() == [-1, -1]
     assert b.history.shape == (50, 2)
     np.testing.assert_array_equal(b.history[-1], b.part_l2)
+
+
+def test_segments_equal_one_run():
+    """A steady run driven in resumed segments takes the same steps to the
+    same state as one call."""
+    cases = [_case(0.06, 0.4), _case(0.09, 0.58)]
+    one = steady.solve_batch(cases, tol=1e-6, max_steps=20000, freeze_level=1e-3)
+    seg = steady.solve_batch(cases, tol=1e-6, max_steps=20000, freeze_level=1e-3, segment=700)
+    print("one call", one.part_steps.tolist(), "segments", seg.part_steps.tolist())
+    assert one.rc == 0 and seg.rc == 0
+    assert seg.part_steps.tolist() == one.part_steps.tolist()
+    assert seg.steps_done == one.steps_done
+    for a, b in zip(one.states, seg.states):
+        assert np.array_equal(a, b)
