@@ -103,6 +103,32 @@ def cpu_baseline(threads):
                        "%d RK4 steps, %.1f s wall on %d threads" % (n, t1 - t0, threads))
 
 
+def steady_leg(ncases, seed=7):
+    """Steady bump cases (192 x 64 cells) from the freestream and from the
+    trained U-Net's prediction, each set solved as one batch to the L2
+    density residual 1e-10 (FP64)."""
+    from cffc_amd import steady, warmstart
+    model = os.path.join(ROOT, "cffc_amd", "data", "unet_bump.npz")
+    if not os.path.exists(model):
+        return {"skipped": "no trained model at cffc_amd/data/unet_bump.npz"}
+    net, norm = warmstart.load_model(model)
+    cases = steady.sample_cases(ncases, seed)
+    cold = steady.solve_batch([steady.bump_case(h, m) for h, m in cases], tol=1e-10, max_steps=400000)
+    t0 = time.perf_counter()
+    init = warmstart.predict_states(net, norm, cases)
+    t_inf = time.perf_counter() - t0
+    warm = steady.solve_batch([(steady.bump_case(h, m)[0], W) for (h, m), W in zip(cases, init)], tol=1e-10,
+                              max_steps=400000)
+    cs, ws = cold.part_steps, warm.part_steps
+    return {"cases": ncases, "seed": seed, "tol": 1e-10, "mesh": "192x64 cells, 12x4 blocks of 16^2",
+            "scheme": "5-stage optimal smoothing, CFL 1.5, local dt, freeze 1e-5, FP64",
+            "cold_steps": cs.tolist(), "warm_steps": ws.tolist(),
+            "cold_batch_s": cold.seconds, "warm_batch_s": warm.seconds + t_inf, "inference_s": t_inf,
+            "step_ratio_total": float(cs.sum() / max(ws.sum(), 1)),
+            "time_ratio_batch": float(cold.seconds / (warm.seconds + t_inf)),
+            "all_converged": bool(np.all(cs > 0) and np.all(ws > 0))}
+
+
 def load_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
@@ -137,6 +163,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--steady-cases", type=int, default=8,
+                    help="held-out steady bump cases solved cold and warm (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES bytes (counter calibration)")
     args = ap.parse_args()
@@ -166,7 +194,8 @@ def main():
     head = res["mixed"] if "mixed" in res else res["fp64"]
     head_prec = "mixed" if "mixed" in res else "fp64"
     achieved = BYTES_PER_CELL_STAGE * ncell / (head["stage_ms"] * 1e-3) / 1e9
-    traffic = load_traffic("stage_kernel_%s_c2_traffic.json" % head_prec)
+    traffic = load_traffic("stage_kernel_%s_riemann4q_%dx%d_blocks_of_%dsq_traffic.json"
+                           % (head_prec, args.blocks, args.blocks, args.cells))
     out = {
         "metric": "cell-stage updates/s",
         "value": head["rate"],
@@ -200,6 +229,11 @@ def main():
         out["mixed_vs_fp64"] = {"steps": args.warmup + args.steps,
                                 "L1": d.mean(axis=0).tolist(), "Linf": d.max(axis=0).tolist(),
                                 "vars": ["rho", "u", "v", "p"]}
+    if args.steady_cases > 0:
+        try:
+            out["steady_warm_vs_cold"] = steady_leg(args.steady_cases)
+        except Exception as e:  # reported, never required for the throughput line
+            out["steady_warm_vs_cold"] = {"error": str(e)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_threads)

@@ -173,7 +173,7 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
     }
     const FT det = DxDx * DyDy - DxDy * DxDy;
     if constexpr (Fast<FT>::on) {
-        const FT rd = FT(1) / det;
+        const FT rd = __builtin_amdgcn_rcpf(det);
         gx.d = (Dx.d * DyDy - Dy.d * DxDy) * rd; gx.vx = (Dx.vx * DyDy - Dy.vx * DxDy) * rd;
         gx.vy = (Dx.vy * DyDy - Dy.vy * DxDy) * rd; gx.p = (Dx.p * DyDy - Dy.p * DxDy) * rd;
         gy.d = (Dy.d * DxDx - Dx.d * DxDy) * rd; gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) * rd;

@@ -259,6 +259,16 @@ template <typename T> CFFC_HD PS<T> bc_characteristic_pressure(const PS<T> &Wi, 
 template <typename T> CFFC_HD T venkat_fold(T phi, T uq, T u0, T u0Min, T u0Max) {
     typedef Consts<T> K;
     const T du = uq - u0;
+    if constexpr (Fast<T>::on) {
+        // mixed path: branch-free.  y >= 0 always (num and du share their
+        // sign); capping y at 1e6 keeps y^2 finite, and f(1e6) rounds to 1,
+        // which cannot lower phi <= 1 (nor can any f(y >= 2)).  Inside the
+        // dead band the quotient is discarded by the select.
+        const T num = du > T(0) ? u0Max - u0 : u0Min - u0;
+        const T y = fminf(num * __builtin_amdgcn_rcpf(du), T(1.0e6));
+        const T f = (y * y + K::TWO * y) * __builtin_amdgcn_rcpf(y * y + y + K::TWO);
+        return fminf(phi, fabsf(du) > K::TOLER ? f : K::ONE);
+    }
     T num;
     if (du > K::TOLER) {
         num = u0Max - u0;  // y = num/du, both >= 0

@@ -71,19 +71,48 @@ def predict_states(net, norm, cases):
     return [unet.image_to_state(m0, norm.decode(Y[q], mach)) for q, (h, mach) in enumerate(cases)]
 
 
-def evaluate(net, norm, cases, tol=1e-10, max_steps=200000, freeze_level=steady.STEADY_FREEZE_LEVEL, log=None):
+def solve_single(h, mach, W0=None, tol=1e-10, max_steps=300000, freeze_level=steady.STEADY_FREEZE_LEVEL):
+    """One case on its own context: (steps, seconds of the solve)."""
+    m, Wf = steady.bump_case(h, mach)
+    g = euler2d.Euler2DQuadBlocks(m, steady.bump_scheme())
+    try:
+        g.set_W(Wf if W0 is None else W0)
+        g.synchronize()
+        r = steady.solve_steady(g, tol=tol, max_steps=max_steps, freeze_level=freeze_level, history=False)
+    finally:
+        g.close()
+    return r.steps, r.seconds
+
+
+def evaluate(net, norm, cases, tol=1e-10, max_steps=200000, freeze_level=steady.STEADY_FREEZE_LEVEL, log=None,
+             singles=0):
     """Cold (freestream) and warm (predicted) solves of the held-out cases,
-    each set as one batch.  Returns a dict with the steps of every case and
-    the wall time of each batch (the warm one including inference)."""
+    each set as one batch; the first `singles` cases are also solved alone,
+    cold and warm, for the time of one solve.  Returns a dict with the steps
+    of every case, the wall time of each batch (the warm one including
+    inference) and the single-case times."""
     _, cold_steps, cold_s = solve_cases(cases, tol, max_steps, freeze_level, log=log)
     t0 = time.perf_counter()
     init = predict_states(net, norm, cases)
     t_inf = time.perf_counter() - t0
     _, warm_steps, warm_s = solve_cases(cases, tol, max_steps, freeze_level, initial=init, log=log)
+    single = []
+    for q in range(min(singles, len(cases))):
+        h, mach = cases[q]
+        cs, ct = solve_single(h, mach, None, tol, max_steps, freeze_level)
+        t1 = time.perf_counter()
+        Wp = predict_states(net, norm, [cases[q]])[0]
+        ti = time.perf_counter() - t1
+        ws, wt = solve_single(h, mach, Wp, tol, max_steps, freeze_level)
+        single.append(dict(h=float(h), mach=float(mach), cold_steps=cs, cold_s=ct, warm_steps=ws, warm_s=wt + ti,
+                           inference_s=ti))
+        if log:
+            log("single case %d: cold %d steps %.2f s, warm %d steps %.2f s (inference %.3f s)" %
+                (q, cs, ct, ws, wt + ti, ti))
     return dict(cases=np.asarray(cases).tolist(), cold_steps=cold_steps.tolist(), warm_steps=warm_steps.tolist(),
                 cold_seconds=float(sum(cold_s)), warm_seconds=float(sum(warm_s)) + t_inf, inference_seconds=t_inf,
                 step_ratio_median=float(np.median(cold_steps / np.maximum(warm_steps, 1))),
-                step_ratio_total=float(cold_steps.sum() / max(warm_steps.sum(), 1)))
+                step_ratio_total=float(cold_steps.sum() / max(warm_steps.sum(), 1)), single=single)
 
 
 # ---- persistence -------------------------------------------------------------
@@ -135,11 +164,12 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--max-steps", type=int, default=300000)
-    ap.add_argument("--data", default="bump_train.npz")
+    ap.add_argument("--data", default="bump_train.npz", help="dataset file (train: comma-separated list)")
     ap.add_argument("--model", default="unet_bump.npz")
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--singles", type=int, default=0, help="evaluate: also solve this many cases alone")
     ap.add_argument("--out", default=None, help="JSON report")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
@@ -154,7 +184,10 @@ def main(argv=None):
         log("generated %d cases: steps median %d max %d, %.1f s" % (a.n, int(np.median(steps)), int(steps.max()),
                                                                     report["seconds"]))
     elif a.cmd == "train":
-        cases, dev = load_dataset(a.data)
+        parts = [load_dataset(f) for f in a.data.split(",")]
+        cases = np.concatenate([c for c, _ in parts])
+        dev = np.concatenate([d for _, d in parts])
+        log("training set: %d cases from %s" % (len(cases), a.data))
         norm = unet.Normaliser(np.sqrt(np.mean(dev.reshape(-1, 4).astype(np.float64) ** 2, axis=0)) + 1e-6)
         X, Y = training_arrays(cases, dev, norm)
         net = unet.UNet(H=X.shape[1], W=X.shape[2], max_batch=a.batch, seed=a.seed)
@@ -170,7 +203,7 @@ def main(argv=None):
     else:
         net, norm = load_model(a.model)
         cases = steady.sample_cases(a.n, a.seed)
-        r = evaluate(net, norm, cases, a.tol, a.max_steps, log=log)
+        r = evaluate(net, norm, cases, a.tol, a.max_steps, log=log, singles=a.singles)
         report.update(r)
         log("held-out %d: cold steps median %d, warm %d; step ratio median %.2f total %.2f; batch time cold %.1fs "
             "warm %.1fs" % (a.n, int(np.median(r["cold_steps"])), int(np.median(r["warm_steps"])),

@@ -163,7 +163,7 @@ def test_tile_shapes_agree(euler2d):
     W0 = M.riemann_4quadrant(m)
     m.set_wo_from_initial(W0)
     ref = None
-    for ti, tj in ((16, 16), (32, 8), (8, 32), (16, 8), (8, 8), (32, 4)):
+    for ti, tj in euler2d.TILE_SHAPES:
         g = euler2d.Euler2DQuadBlocks(m, abi.Scheme())
         g.set_tile(ti, tj)
         g.set_W(W0)
