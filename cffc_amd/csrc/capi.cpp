@@ -76,6 +76,11 @@ struct cffc_ctx {
     int part_l2_steps = 0;
     int steady_done = 0;   // steps of the current steady run before this segment
     unsigned long long *step_clock = nullptr;  // [max_steps + 1]: start, then end of every step
+    // push exchange: the fused stage kernel fills same-level ghosts; the
+    // per-stage exchange launch then only applies BCs on boundary blocks
+    bool push_ok = false;
+    int nbc = 0;
+    int32_t *bc_blocks = nullptr;
     int freeze = 0;      // limiters frozen (PHI_LOAD)
     int phi_valid = 0;   // phi array holds the last evaluated limiters
     int store_phi = 0;   // keep phi stored every step (freeze may follow)
@@ -95,7 +100,8 @@ static void free_ctx(cffc_ctx *c) {
     void *ptrs[] = {c->meta, c->nbr, c->bc, c->wo, c->geom, c->geomf, c->W[0], c->W[1], c->U, c->Uo, c->dUdt[0],
                     c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
-                    c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock};
+                    c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock,
+                    c->bc_blocks};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -297,6 +303,28 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
             }
         }
     }
+    {
+        // the push exchange needs every same-level link to be mutual
+        static const int opp[8] = {1, 0, 3, 2, 7, 6, 5, 4};
+        bool sym = true;
+        std::vector<int32_t> bl;
+        for (int b = 0; b < md->nblk; ++b) {
+            for (int s = 0; s < 8; ++s) {
+                const int nb = md->nbr[8 * b + s];
+                if (nb >= 0 && md->nbr[8 * nb + opp[s]] != b) sym = false;
+            }
+            if (c->meta_h[b].has_bc) bl.push_back(b);
+        }
+        c->push_ok = sym;
+        c->nbc = (int)bl.size();
+        if (!bl.empty()) {
+            if (int ea = dalloc(&c->bc_blocks, bl.size())) {
+                free_ctx(c);
+                return ea;
+            }
+            HIPCHK(hipMemcpy(c->bc_blocks, bl.data(), bl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+    }
     c->ncp = (coff + 15) & ~(int64_t)15;   // every SoA plane starts 128-byte aligned
     c->ncell = ioff;
     const int64_t ncp = c->ncp;
@@ -441,6 +469,8 @@ static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long lon
     p.from_U = from_U;
     p.stop_flag = nullptr;
     p.reset_slot = reset_slot;
+    p.blocks = nullptr;
+    p.bcs_only = 0;
     return p;
 }
 
@@ -503,6 +533,8 @@ static StageParams base_stage_params(cffc_ctx *c) {
     p.err = c->scal + 2;
     p.norm_part = c->norm_part;
     p.stop_flag = nullptr;
+    p.nbr = c->nbr;
+    p.push = 0;
     p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE;
     return p;
 }
@@ -632,7 +664,15 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         ExchangeParams ex = exchange_params(c, st == 1, nullptr);
         ex.stop_flag = c->stop_flag;
         mark(c, 0, true);
-        HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
+        if (c->push_ok) {
+            // same-level ghosts were pushed by the previous stage kernel (or
+            // filled by begin_run); only the physical BCs remain
+            ex.bcs_only = 1;
+            ex.blocks = c->bc_blocks;
+            if (c->nbc) HIPCHK(cffc_launch_exchange(&ex, c->nbc, c->stream));
+        } else {
+            HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
+        }
         mark(c, 0, false);
         StageParams p = base_stage_params(c);
         p.Win = c->W[c->wcur];
@@ -643,6 +683,7 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         p.last_stage = (st == N);
         p.dtmin_in = c->scal + par;
         p.stop_flag = c->stop_flag;
+        p.push = c->push_ok ? 1 : 0;
         if (parts) {
             p.part_state = c->part_state;
             p.done_copy = N & 1;
@@ -694,6 +735,12 @@ static int begin_run(cffc_ctx *c, int nsteps) {
     HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, MILLION_BITS, 2, c->stream));
     HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
                            c->stream));
+    if (c->push_ok) {
+        // full exchange (ghosts = W(U) of the neighbours, then BCs): later
+        // stages get their same-level ghosts pushed by the stage kernel
+        ExchangeParams ex = exchange_params(c, 1, nullptr);
+        HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
+    }
     HIPCHK(cffc_launch_stamp(c->step_clock, c->stream));
     return 0;
 }

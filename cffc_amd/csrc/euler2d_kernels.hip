@@ -155,7 +155,7 @@ __device__ __forceinline__ void stW(double *W, int64_t ncp, int64_t c, const PS<
 __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     if (p.stop_flag && *p.stop_flag) return;
     if (blockIdx.x == 0 && threadIdx.x == 0 && p.reset_slot) *p.reset_slot = 0x412E848000000000ull;  // MILLION
-    const int b = blockIdx.x;
+    const int b = p.blocks ? p.blocks[blockIdx.x] : (int)blockIdx.x;
     const BlockMeta m = p.meta[b];
     const int64_t ncp = p.ncp;
     double *W = p.W;
@@ -164,7 +164,7 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     // phase 0: same-level neighbours, faces (2 layers over the interior span)
     // and corners (2 x 2); U is transferred and W = W(U) recomputed on stage 1
     // (Euler2DQuad.h:1643-1647); later stages copy W, which equals W(U).
-    for (int s = 0; s < 8; ++s) {
+    for (int s = 0; s < (p.bcs_only ? 0 : 8); ++s) {
         const int nb = p.nbr[8 * b + s];
         if (nb < 0) continue;
         const BlockMeta S = p.meta[nb];

@@ -21,6 +21,8 @@ struct ExchangeParams {
     int32_t from_U;
     int32_t *stop_flag;
     unsigned long long *reset_slot;  // set to MILLION (next-step dt slot), may be null
+    const int32_t *blocks;     // blocks to process (workgroup k -> blocks[k]), or null: workgroup = block
+    int32_t bcs_only;          // skip the same-level exchange (ghosts already pushed by the stage kernel)
 };
 
 // end of a fused time step: reduce the last stage's per-tile partials,

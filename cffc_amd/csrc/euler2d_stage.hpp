@@ -51,7 +51,38 @@ struct StageParams {
     int32_t *stop_flag;     // nonzero: converged or failed, skip work
     const int32_t *part_state;  // per-part PartState (batched steady runs), or null
     int32_t done_copy;      // parts that are done copy W forward (odd stage count)
+    const int32_t *nbr;     // 8 same-level neighbours per block (push exchange)
+    int32_t push;           // fused: also write W^(s+1) into the neighbours' ghost cells
 };
+
+// Push form of the same-level exchange: the cell (ip, jp) of block b that a
+// neighbour's ghost cell copies (the exchange kernel's phase 0 read in the
+// other direction) is written there by the tile that computes it, so the
+// next stage finds those ghosts filled without a separate exchange.  Needs
+// a symmetric neighbour table (checked by the host).  Slots N, S, E, W, NE,
+// NW, SE, SW; the neighbour's ghost index follows from its slot facing b.
+__device__ __forceinline__ void push_ghosts(const StageParams &p, const BlockMeta &m, int b, int ip, int jp,
+                                            const PS<double> &Wn) {
+    const int ICu = m.ni + 1, JCu = m.nj + 1;
+    const bool nW = ip <= 3, nE = ip >= ICu - 1, nS = jp <= 3, nN = jp >= JCu - 1;
+    if (!(nW || nE || nS || nN)) return;
+    const int32_t *nb8 = p.nbr + 8 * b;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+        const bool hit = s == 0 ? nN : s == 1 ? nS : s == 2 ? nE : s == 3 ? nW
+                       : s == 4 ? (nN && nE) : s == 5 ? (nN && nW) : s == 6 ? (nS && nE) : (nS && nW);
+        if (!hit) continue;
+        const int nb = nb8[s];
+        if (nb < 0) continue;
+        const BlockMeta S = p.meta[nb];
+        const bool up = s == 0 || s == 4 || s == 5, down = s == 1 || s == 6 || s == 7;
+        const bool east = s == 2 || s == 4 || s == 6, west = s == 3 || s == 5 || s == 7;
+        const int ig = east ? ip - m.ni : west ? ip + S.ni : ip;
+        const int jg = up ? jp - m.nj : down ? jp + S.nj : jp;
+        const int64_t cg = S.coff + (int64_t)jg * S.pitch + ig;
+        p.Wout[cg] = Wn.d; p.Wout[p.ncp + cg] = Wn.vx; p.Wout[2 * p.ncp + cg] = Wn.vy; p.Wout[3 * p.ncp + cg] = Wn.p;
+    }
+}
 
 // Workgroups are dispatched to the 8 XCDs round-robin (workgroup k to XCD
 // k mod 8), each XCD with its own L2.  Tiles are listed block by block, so
@@ -542,6 +573,7 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                 Wn = W_of_U(Un);
             }
             p.Wout[c] = Wn.d; p.Wout[ncp + c] = Wn.vx; p.Wout[2 * ncp + c] = Wn.vy; p.Wout[3 * ncp + c] = Wn.p;
+            if (p.push) push_ghosts(p, m, t.b, ip, jp, Wn);
             if (p.mode == MODE_ZERO_ACC2) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = p.S[v * ncp + c] + 2.0 * dU[v];

@@ -43,21 +43,39 @@ def warm_states(net, norm, cases):
     return out
 
 
-def serve(cases, initial=None, tol=1e-10, max_steps=300000, freeze_level=steady.STEADY_FREEZE_LEVEL):
-    """Solve all cases in one batch.  Returns a dict of the batch figures."""
+def serve(cases, initial=None, tol=1e-10, max_steps=400000, freeze_level=steady.STEADY_FREEZE_LEVEL, mixed=True,
+          switch_level=1e-5, segment=20000, log=None):
+    """Solve all cases in one batch.  mixed: FP32 flux down to switch_level,
+    then FP64 to tol (steady.solve_batch_fp32_then_fp64); else FP64 only.
+    Returns a dict of the batch figures."""
     batch = []
     for q, (h, m, k) in enumerate(cases):
         mesh, W0 = case_mesh(h, m, k)
         batch.append((mesh, W0 if initial is None else initial[q]))
-    r = steady.solve_batch(batch, tol=tol, max_steps=max_steps, freeze_level=freeze_level)
-    lat = r.part_seconds
-    ok = np.isfinite(lat)
-    return dict(rc=r.rc, n=len(cases), converged=int(ok.sum()), steps_done=r.steps_done,
-                wall_seconds=r.seconds, solves_per_s=len(cases) / r.seconds if r.seconds > 0 else None,
+    t0 = time.perf_counter()
+    if mixed:
+        first, r = steady.solve_batch_fp32_then_fp64(batch, tol=tol, switch_level=switch_level, max_steps=max_steps,
+                                                     freeze_level=freeze_level, segment=segment, log=log)
+        if r is None:
+            return dict(rc=first.rc, n=len(cases), error="first phase failed")
+        s1 = np.where(first.part_steps > 0, first.part_steps, first.steps_done)
+        steps = s1 + np.where(r.part_steps > 0, r.part_steps, 0)
+        lat = first.seconds + r.part_seconds
+        phases = dict(fp32_steps=first.part_steps.tolist(), fp32_seconds=first.seconds,
+                      fp64_steps=r.part_steps.tolist(), fp64_seconds=r.seconds)
+    else:
+        r = steady.solve_batch(batch, tol=tol, max_steps=max_steps, freeze_level=freeze_level, segment=segment,
+                               log=log)
+        steps, lat, phases = r.part_steps, r.part_seconds, {}
+    wall = time.perf_counter() - t0
+    ok = np.isfinite(lat) & (r.part_steps > 0)
+    return dict(rc=r.rc, n=len(cases), converged=int(ok.sum()), wall_seconds=wall,
+                solves_per_s=len(cases) / wall if wall > 0 else None,
                 p50_s=float(np.percentile(lat[ok], 50)) if ok.any() else None,
                 p99_s=float(np.percentile(lat[ok], 99)) if ok.any() else None,
-                steps=r.part_steps.tolist(), latency_s=[float(x) for x in lat],
-                cells=int(sum(mm.ncell for mm, _ in batch)))
+                steps=[int(x) for x in steps], latency_s=[float(x) for x in lat],
+                cells=int(sum(mm.ncell for mm, _ in batch)), precision="fp32+fp64" if mixed else "fp64",
+                **phases)
 
 
 def serve_cold_warm(net, norm, cases, **kw):
@@ -68,3 +86,37 @@ def serve_cold_warm(net, norm, cases, **kw):
     warm = serve(cases, initial=init, **kw)
     warm["inference_seconds"] = t_inf
     return cold, warm
+
+
+def main(argv=None):
+    import argparse
+    import json
+    from . import warmstart
+    ap = argparse.ArgumentParser(prog="python -m cffc_amd.serving")
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed", type=int, default=5)
+    ap.add_argument("--model", default=None)
+    ap.add_argument("--fp64", action="store_true", help="FP64 only (default: FP32 flux, FP64 finish)")
+    ap.add_argument("--max-steps", type=int, default=400000)
+    ap.add_argument("--out", default=None)
+    a = ap.parse_args(argv)
+    log = lambda s: print(s, flush=True)
+    cases = serving_cases(a.n, a.seed)
+    log("%d cases, %d cells" % (len(cases), sum(96 * k * 32 * k for _, _, k in cases)))
+    if a.model:
+        net, norm = warmstart.load_model(a.model)
+        cold, warm = serve_cold_warm(net, norm, cases, mixed=not a.fp64, max_steps=a.max_steps, log=log)
+    else:
+        cold, warm = serve(cases, mixed=not a.fp64, max_steps=a.max_steps, log=log), None
+    for tag, r in (("cold", cold), ("warm", warm)):
+        if r:
+            log("%s: %d/%d converged, %.2f solves/s, p50 %.2f s, p99 %.2f s, wall %.1f s" %
+                (tag, r.get("converged", 0), r["n"], r.get("solves_per_s") or 0, r.get("p50_s") or 0,
+                 r.get("p99_s") or 0, r.get("wall_seconds", 0)))
+    if a.out:
+        with open(a.out, "w") as f:
+            json.dump({"cases": cases, "cold": cold, "warm": warm}, f)
+
+
+if __name__ == "__main__":
+    main()

@@ -149,6 +149,26 @@ def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=ST
                        part_seconds=part_s)
 
 
+def solve_batch_fp32_then_fp64(cases, tol=1e-10, switch_level=1e-5, max_steps=200000,
+                               freeze_level=STEADY_FREEZE_LEVEL, segment=0, log=None):
+    """Mixed-precision steady batch: every part first marches with FP32
+    reconstruction/limiter/Roe flux (FP64 state and update) down to
+    switch_level, above the FP32 round-off floor of the residual, then the
+    FP64 path takes the conserved state and finishes to tol (limiters frozen
+    at freeze_level as usual).  Steps and latencies of a part add over the
+    two phases; the second phase starts when the first has ended for all
+    parts."""
+    first = solve_batch(cases, scheme=bump_scheme(flux_precision=abi.FLUX_FP32), tol=switch_level,
+                        max_steps=max_steps, freeze_level=0.0, segment=segment, log=log)
+    if first.rc != 0:
+        return first, None
+    # the FP64 phase restarts from the FP64 state the first phase ended in
+    meshes = [m for m, _ in cases]
+    second = solve_batch([(m, W) for m, W in zip(meshes, first.states)], tol=tol,
+                         max_steps=max_steps, freeze_level=freeze_level, segment=segment, log=log)
+    return first, second
+
+
 def bump_case(h, mach, outflow=abi.BC_CHARACTERISTIC, **kw):
     """Mesh, freestream initial state and boundary states of one bump case.
     The outflow is the reference's characteristic (pressure) condition: with

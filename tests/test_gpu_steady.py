@@ -97,3 +97,19 @@ def test_segments_equal_one_run():
     assert seg.steps_done == one.steps_done
     for a, b in zip(one.states, seg.states):
         assert np.array_equal(a, b)
+
+
+def test_fp32_flux_then_fp64_finish():
+    """FP32 reconstruction/flux down to 1e-5 (above its round-off floor),
+    then the FP64 path to the tolerance: every part converges, and the steady
+    states agree with an FP64-only solve to the level the frozen limiters
+    allow."""
+    cases = [_case(0.05, 0.35), _case(0.09, 0.55)]
+    first, second = steady.solve_batch_fp32_then_fp64(cases, tol=1e-8, switch_level=1e-5, max_steps=40000)
+    assert first.rc == 0 and second is not None and second.rc == 0
+    assert np.all(first.part_steps > 0) and np.all(second.part_steps > 0)
+    ref = steady.solve_batch(cases, tol=1e-8, max_steps=40000)
+    print("fp32 steps", first.part_steps.tolist(), "fp64 finish", second.part_steps.tolist(),
+          "fp64 only", ref.part_steps.tolist())
+    for a, b in zip(second.states, ref.states):
+        assert np.max(np.abs(a - b)) < 1e-3
