@@ -22,8 +22,8 @@ context stream in a separate run of the same workload.
 cpu_baseline: the CPU oracle (a C restatement of the reference algorithm,
 -O2, no FMA), one pthread per core over a contiguous block partition like
 the reference's one-MPI-rank-per-core decomposition, timed on a bounded
-sample of the same workload (a sub-domain of 16 x 16 blocks of 64^2 cells,
-6 steps).
+sample of the same workload (a sub-domain of 32 x 32 blocks of 64^2 cells,
+8 steps).
 
 python bench.py [--gpus 1] [--steps 500] [--warmup 20]
 """
@@ -90,16 +90,16 @@ def cpu_baseline(threads):
     import oracle
     from cffc_amd import abi
     oracle.build()
-    m, W0 = make_case(16, 64)
+    m, W0 = make_case(32, 64)
     o = oracle.Oracle(m, abi.Scheme())
     o.set_W(W0)
-    nsteps = 6
+    nsteps = 8
     t0 = time.perf_counter()
     rc, n, _, _ = o.run(nsteps, nthreads=threads)
     t1 = time.perf_counter()
     rate = m.ncell * 4 * n / (t1 - t0)
     return dict(value=rate, unit="cell-stage updates/s", cores=threads, kind="port",
-                sample="CPU oracle, 16x16 blocks of 64^2 cells (1,048,576 cells, same IC family), "
+                sample="CPU oracle, 32x32 blocks of 64^2 cells (4,194,304 cells, same IC family), "
                        "%d RK4 steps, %.1f s wall on %d threads" % (n, t1 - t0, threads))
 
 

@@ -54,6 +54,7 @@ struct cffc_ctx {
     double *wo = nullptr, *geom = nullptr;
     float *geomf = nullptr;   // FP32 geometry planes (mixed path only)
     double *W[2] = {nullptr, nullptr};
+    float *Wf[2] = {nullptr, nullptr};   // fused mixed path: W stored as FP32
     int wcur = 0;
     double *U = nullptr, *Uo = nullptr, *dUdt[3] = {nullptr, nullptr, nullptr};
     double *dt = nullptr, *phi = nullptr;
@@ -81,6 +82,8 @@ struct cffc_ctx {
     bool push_ok = false;
     int nbc = 0;
     int32_t *bc_blocks = nullptr;
+    double rs_eps = 0.0; // Residual_Smoothing_Epsilon
+    int rs_iters = 0;    // Residual_Smoothing_Gauss_Seidel_Iterations (0: no smoothing)
     int freeze = 0;      // limiters frozen (PHI_LOAD)
     int phi_valid = 0;   // phi array holds the last evaluated limiters
     int store_phi = 0;   // keep phi stored every step (freeze may follow)
@@ -101,7 +104,7 @@ static void free_ctx(cffc_ctx *c) {
                     c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
                     c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock,
-                    c->bc_blocks};
+                    c->bc_blocks, c->Wf[0], c->Wf[1]};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -378,6 +381,11 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
             return e;
         }
         HIPCHK(cffc_launch_geometry_f32(c->meta, c->nblk, c->max_cells, c->geom, c->geomf, ncp, c->stream));
+        for (int k = 0; k < 2; ++k)
+            if ((e = dalloc(&c->Wf[k], 4 * (size_t)ncp))) {
+                free_ctx(c);
+                return e;
+            }
     }
     // default tile: the largest supported square-ish tile that the blocks fill
     int ti = 16, tj = 16;
@@ -471,6 +479,7 @@ static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long lon
     p.reset_slot = reset_slot;
     p.blocks = nullptr;
     p.bcs_only = 0;
+    p.Wf = nullptr;
     return p;
 }
 
@@ -486,6 +495,32 @@ static int ensure_granular(cffc_ctx *c) {
             HIPCHK(hipMemsetAsync(c->dUdt[k], 0, n * sizeof(double), c->stream));
         }
     c->granular_ready = true;
+    return 0;
+}
+
+// Residual_Smoothing scratch is dUdt[k+1]: the reference's 4-stage Runge-Kutta
+// smooths dUdt[2] at stage 3 into a fourth residual vector it does not have
+// (NUMBER_OF_RESIDUAL_VECTORS_EULER2D = 3), so that scheme is refused.
+static bool smoothing_supported(const cffc_scheme_desc &s) {
+    return !(s.i_time_integration == CFFC_TIME_STEPPING_EXPLICIT_RUNGE_KUTTA && s.n_stage == 4);
+}
+
+extern "C" int cffc_set_residual_smoothing(cffc_ctx *c, double epsilon, int iterations) {
+    if (!c || iterations < 0 || !(epsilon >= 0.0)) return fail(CFFC_E_ARG, "bad argument");
+    if (iterations > 0 && !smoothing_supported(c->s))
+        return fail(CFFC_E_ARG, "residual smoothing is not supported with 4-stage Runge-Kutta");
+    c->rs_eps = epsilon;
+    c->rs_iters = iterations;
+    return 0;
+}
+
+extern "C" int cffc_residual_smoothing(cffc_ctx *c, int i_stage) {
+    if (!c || i_stage < 1 || i_stage > c->s.n_stage) return fail(CFFC_E_ARG, "bad argument");
+    if (int e = ensure_granular(c)) return e;
+    if (c->rs_iters == 0) return 0;
+    const int k = k_residual(c->s, i_stage);
+    HIPCHK(cffc_launch_residual_smoothing(c->meta, c->nblk, c->dUdt[k], c->dUdt[k + 1], c->ncp, c->rs_eps,
+                                          c->rs_iters, nullptr, nullptr, c->stream));
     return 0;
 }
 
@@ -663,6 +698,7 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
     for (int st = 1; st <= N; ++st) {
         ExchangeParams ex = exchange_params(c, st == 1, nullptr);
         ex.stop_flag = c->stop_flag;
+        ex.Wf = c->Wf[c->wcur];
         mark(c, 0, true);
         if (c->push_ok) {
             // same-level ghosts were pushed by the previous stage kernel (or
@@ -677,6 +713,8 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         StageParams p = base_stage_params(c);
         p.Win = c->W[c->wcur];
         p.Wout = c->W[1 - c->wcur];
+        p.Winf = c->Wf[c->wcur];
+        p.Woutf = c->Wf[1 - c->wcur];
         p.S = c->dUdt[0];
         p.mode = fused_mode(c->s, st);
         p.omega = stage_omega(c->s, st);
@@ -692,7 +730,19 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         else if (c->store_phi && st == N) p.phi_mode = PHI_COMPUTE_STORE;
         else p.phi_mode = PHI_COMPUTE;
         mark(c, 1, true);
-        HIPCHK(launch_stage(c, p, 1));
+        if (c->rs_iters > 0) {
+            // residual, implicit residual smoothing, update (Euler2DQuadSolvers.cc:855-877)
+            p.defer_update = 1;
+            HIPCHK(launch_stage(c, p, 1));
+            HIPCHK(cffc_launch_residual_smoothing(c->meta, c->nblk, c->dUdt[0], c->dUdt[1], c->ncp, c->rs_eps,
+                                                  c->rs_iters, parts ? c->part_state : nullptr, c->stop_flag,
+                                                  c->stream));
+            p.defer_update = 0;
+            if (c->s.flux_precision == CFFC_FLUX_FP32) HIPCHK(cffc_launch_update_tile_mixed(&p, c->ti, c->tj, c->stream));
+            else HIPCHK(cffc_launch_update_tile_fp64(&p, c->ti, c->tj, c->stream));
+        } else {
+            HIPCHK(launch_stage(c, p, 1));
+        }
         mark(c, 1, false);
         c->wcur = 1 - c->wcur;
     }
@@ -724,6 +774,8 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
 
 static int begin_run(cffc_ctx *c, int nsteps) {
     if (int e = ensure_steps(c, nsteps)) return e;
+    if (c->rs_iters > 0)
+        if (int e = ensure_granular(c)) return e;   // dUdt[1] is the smoothing scratch
     if (c->store_phi || c->freeze)
         if (int e = ensure_phi(c)) return e;
     int32_t zero[2] = {0, 0};
@@ -735,10 +787,15 @@ static int begin_run(cffc_ctx *c, int nsteps) {
     HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, MILLION_BITS, 2, c->stream));
     HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
                            c->stream));
-    if (c->push_ok) {
+    if (c->Wf[0]) {
+        // the fused mixed path keeps W in FP32: derive it from U
+        HIPCHK(cffc_launch_w_from_u(c->meta, c->nblk, c->max_int, c->U, nullptr, c->Wf[c->wcur], c->ncp, c->stream));
+    }
+    if (c->push_ok || c->Wf[0]) {
         // full exchange (ghosts = W(U) of the neighbours, then BCs): later
         // stages get their same-level ghosts pushed by the stage kernel
         ExchangeParams ex = exchange_params(c, 1, nullptr);
+        ex.Wf = c->Wf[c->wcur];
         HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
     }
     HIPCHK(cffc_launch_stamp(c->step_clock, c->stream));
@@ -757,6 +814,11 @@ static int end_run(cffc_ctx *c, int wstart, int nsteps, double *norms, double *d
     if (steps_done) *steps_done = done;
     // kernels after a stop are skipped: the newest W is the one after `done` steps
     c->wcur = (wstart + done * c->s.n_stage) & 1;
+    if (c->Wf[0]) {
+        // FP64 W of the state for get_W and the granular entry points
+        HIPCHK(cffc_launch_w_from_u(c->meta, c->nblk, c->max_int, c->U, c->W[c->wcur], nullptr, c->ncp, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     if (c->store_phi) c->phi_valid = 1;
     if (key != ERR_NONE) {
         g_err = "non-positive density or energy";
@@ -848,6 +910,8 @@ extern "C" int cffc_set_parts(cffc_ctx *c, int npart, const int32_t *first_block
     if (c->part_last) { (void)hipFree(c->part_last); c->part_last = nullptr; }
     c->npart = npart;
     c->part_of_block = pob;
+    for (int b = 0; b < c->nblk; ++b) c->meta_h[b].spare = pob[b];
+    HIPCHK(hipMemcpy(c->meta, c->meta_h.data(), c->nblk * sizeof(BlockMeta), hipMemcpyHostToDevice));
     if (int e = dalloc(&c->part_state, npart)) return e;
     if (int e = dalloc(&c->part_steps, npart)) return e;
     if (int e = dalloc(&c->part_last, npart)) return e;

@@ -144,21 +144,26 @@ __global__ void gather_state_kernel(const BlockMeta *meta, const double *src, do
 
 // ---------------------------------------------------------------------------
 // ghost exchange + physical BCs.  One workgroup per block.
-__device__ __forceinline__ PS<double> ldW(const double *W, int64_t ncp, int64_t c) {
-    PS<double> r = {W[c], W[ncp + c], W[2 * ncp + c], W[3 * ncp + c]};
+// W is stored as ST (FP64, or FP32 for the fused mixed path); the BC states
+// are always formed in FP64.
+template <typename ST> __device__ __forceinline__ PS<double> ldW(const ST *W, int64_t ncp, int64_t c) {
+    PS<double> r = {(double)W[c], (double)W[ncp + c], (double)W[2 * ncp + c], (double)W[3 * ncp + c]};
     return r;
 }
-__device__ __forceinline__ void stW(double *W, int64_t ncp, int64_t c, const PS<double> &r) {
-    W[c] = r.d; W[ncp + c] = r.vx; W[2 * ncp + c] = r.vy; W[3 * ncp + c] = r.p;
+template <typename ST> __device__ __forceinline__ void stW(ST *W, int64_t ncp, int64_t c, const PS<double> &r) {
+    W[c] = (ST)r.d; W[ncp + c] = (ST)r.vx; W[2 * ncp + c] = (ST)r.vy; W[3 * ncp + c] = (ST)r.p;
 }
 
+template <typename ST>
 __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     if (p.stop_flag && *p.stop_flag) return;
     if (blockIdx.x == 0 && threadIdx.x == 0 && p.reset_slot) *p.reset_slot = 0x412E848000000000ull;  // MILLION
     const int b = p.blocks ? p.blocks[blockIdx.x] : (int)blockIdx.x;
     const BlockMeta m = p.meta[b];
     const int64_t ncp = p.ncp;
-    double *W = p.W;
+    ST *W;
+    if constexpr (sizeof(ST) == 4) W = p.Wf;
+    else W = p.W;
     const int ICl = 2, ICu = m.ni + 1, JCl = 2, JCu = m.nj + 1;
     const int tid = threadIdx.x, NT = blockDim.x;
     // phase 0: same-level neighbours, faces (2 layers over the interior span)
@@ -496,6 +501,92 @@ __global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) 
     }
 }
 
+// interior W = W(U) (FP64 arithmetic), stored as ST
+template <typename ST>
+__global__ void w_from_u_kernel(const BlockMeta *meta, const double *U, ST *W, int64_t ncp) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.ni * m.nj) return;
+    const int i = q % m.ni + 2, j = q / m.ni + 2;
+    const int64_t c = m.coff + (int64_t)j * m.pitch + i;
+    const CS<double> Uc = {U[c], U[ncp + c], U[2 * ncp + c], U[3 * ncp + c]};
+    stW(W, ncp, c, W_of_U(Uc));
+}
+
+// Residual_Smoothing, Euler2DQuadSingleBlock.cc:4521-4564, one workgroup per
+// block.  T (the scratch residual dUdt[k+1]) takes S = dUdt[k] on the cells
+// one away from the block edge; Gauss-Seidel sweeps in the reference's
+// lexicographic order (j outer, i inner) run as anti-diagonal wavefronts: a
+// cell's new value needs the W and S neighbours of the same sweep (the
+// previous diagonal) and the E and N neighbours of the previous sweep (the
+// next diagonal), so each diagonal is one parallel step and the results are
+// the sequential ones bit for bit.  Blocks of up to 32 x 32 cells sweep in
+// LDS, larger ones in place.
+constexpr int RS_NT = 256, RS_LDS = 32;
+__global__ __launch_bounds__(RS_NT) void residual_smoothing_kernel(const BlockMeta *meta, double *S, double *T,
+                                                                   int64_t ncp, double eps, int iters,
+                                                                   const int32_t *part_state,
+                                                                   const int32_t *stop_flag) {
+    __shared__ double sT[4][RS_LDS][RS_LDS], sR[4][RS_LDS][RS_LDS];
+    if (stop_flag && *stop_flag) return;
+    const BlockMeta m = meta[blockIdx.x];
+    if (part_state && part_state[m.spare] == PART_DONE) return;
+    const int tid = threadIdx.x;
+    const int I = m.ni - 2, J = m.nj - 2;  // smoothed cells: interior minus its edge ring
+    if (I <= 0 || J <= 0) return;
+    auto cidx = [&](int i, int j) { return m.coff + (int64_t)(j + 2) * m.pitch + (i + 2); };  // interior (i, j)
+    const double den = 1.0 + 4.0 * eps;
+    const bool lds = m.ni <= RS_LDS && m.nj <= RS_LDS;
+    // T = S on the smoothed cells (interior coordinates 1..ni-2)
+    for (int q = tid; q < I * J; q += RS_NT) {
+        const int i = 1 + q % I, j = 1 + q / I;
+        const int64_t c = cidx(i, j);
+        for (int v = 0; v < 4; ++v) T[v * ncp + c] = S[v * ncp + c];
+    }
+    if (lds) {
+        for (int q = tid; q < m.ni * m.nj; q += RS_NT) {
+            const int i = q % m.ni, j = q / m.ni;
+            const int64_t c = cidx(i, j);
+            for (int v = 0; v < 4; ++v) {
+                sR[v][j][i] = S[v * ncp + c];
+                sT[v][j][i] = (i >= 1 && i <= I && j >= 1 && j <= J) ? S[v * ncp + c] : T[v * ncp + c];
+            }
+        }
+    }
+    __syncthreads();
+    for (int n = 0; n < iters; ++n) {
+        for (int d = 0; d <= I + J - 2; ++d) {
+            const int lo = d - (J - 1) > 0 ? d - (J - 1) : 0, hi = d < I - 1 ? d : I - 1;
+            for (int a = lo + tid; a <= hi; a += RS_NT) {
+                const int i = 1 + a, j = 1 + (d - a);
+                if (lds) {
+                    for (int v = 0; v < 4; ++v)
+                        sT[v][j][i] = (sR[v][j][i] + eps * (sT[v][j - 1][i] + sT[v][j][i - 1] + sT[v][j][i + 1] +
+                                                            sT[v][j + 1][i])) / den;
+                } else {
+                    const int64_t c = cidx(i, j), cs = cidx(i, j - 1), cw = cidx(i - 1, j), ce = cidx(i + 1, j),
+                                  cn = cidx(i, j + 1);
+                    for (int v = 0; v < 4; ++v) {
+                        const int64_t o = v * ncp;
+                        T[o + c] = (S[o + c] + eps * (T[o + cs] + T[o + cw] + T[o + ce] + T[o + cn])) / den;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // S = T on the smoothed cells
+    for (int q = tid; q < I * J; q += RS_NT) {
+        const int i = 1 + q % I, j = 1 + q / I;
+        const int64_t c = cidx(i, j);
+        for (int v = 0; v < 4; ++v) {
+            const double x = lds ? sT[v][j][i] : T[v * ncp + c];
+            if (lds) T[v * ncp + c] = x;
+            S[v * ncp + c] = x;
+        }
+    }
+}
+
 // device wall clock (constant rate, hipDeviceAttributeWallClockRate kHz)
 __global__ void stamp_kernel(unsigned long long *dst) {
     if (threadIdx.x == 0) *dst = wall_clock64();
@@ -539,7 +630,8 @@ hipError_t cffc_launch_gather(const BlockMeta *meta, int nblk, int max_int, cons
     return hipGetLastError();
 }
 hipError_t cffc_launch_exchange(const ExchangeParams *p, int nblk, hipStream_t st) {
-    hipLaunchKernelGGL(exchange_bc_kernel, dim3(nblk), dim3(256), 0, st, *p);
+    if (p->Wf) hipLaunchKernelGGL(exchange_bc_kernel<float>, dim3(nblk), dim3(256), 0, st, *p);
+    else hipLaunchKernelGGL(exchange_bc_kernel<double>, dim3(nblk), dim3(256), 0, st, *p);
     return hipGetLastError();
 }
 hipError_t cffc_launch_cfl(const BlockMeta *meta, const TileDesc *tiles, int ntiles, int ti, int tj, const double *W,
@@ -578,6 +670,24 @@ hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, u
     return hipGetLastError();
 }
 
+hipError_t cffc_launch_w_from_u(const BlockMeta *meta, int nblk, int max_int, const double *U, double *W, float *Wf,
+                                int64_t ncp, hipStream_t st) {
+    dim3 grid((max_int + 255) / 256, nblk);
+    if (Wf) hipLaunchKernelGGL(w_from_u_kernel<float>, grid, dim3(256), 0, st, meta, U, Wf, ncp);
+    else hipLaunchKernelGGL(w_from_u_kernel<double>, grid, dim3(256), 0, st, meta, U, W, ncp);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_residual_smoothing(const BlockMeta *meta, int nblk, double *S, double *T, int64_t ncp,
+                                          double eps, int iters, const int32_t *part_state, const int32_t *stop_flag,
+                                          hipStream_t st) {
+    hipLaunchKernelGGL(residual_smoothing_kernel, dim3(nblk), dim3(RS_NT), 0, st, meta, S, T, ncp, eps, iters,
+                       part_state, stop_flag);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_update_tile_fp64(const StageParams *p, int ti, int tj, hipStream_t st) {
+    hipLaunchKernelGGL((update_tile_kernel<double>), dim3(p->ntiles), dim3(UPD_NT), 0, st, *p, ti, tj);
+    return hipGetLastError();
+}
 hipError_t cffc_launch_stamp(unsigned long long *dst, hipStream_t st) {
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(64), 0, st, dst);
     return hipGetLastError();

@@ -16,7 +16,8 @@ struct ExchangeParams {
     const double *wo;
     const double *geom;
     int64_t ncp;
-    double *W;                 // W buffer whose ghosts are filled
+    double *W;                 // W buffer whose ghosts are filled (FP64 storage)
+    float *Wf;                 // or, if not null, the FP32 W buffer of the fused mixed path
     const double *U;           // conserved state (stage 1: ghosts = W(U))
     int32_t from_U;
     int32_t *stop_flag;
@@ -82,6 +83,13 @@ hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, u
                                hipStream_t st);
 hipError_t cffc_launch_finish(const cffc::FinishParams *p, hipStream_t st);
 hipError_t cffc_launch_stamp(unsigned long long *dst, hipStream_t st);
+hipError_t cffc_launch_residual_smoothing(const cffc::BlockMeta *meta, int nblk, double *S, double *T, int64_t ncp,
+                                          double eps, int iters, const int32_t *part_state, const int32_t *stop_flag,
+                                          hipStream_t st);
+hipError_t cffc_launch_update_tile_fp64(const cffc::StageParams *p, int ti, int tj, hipStream_t st);
+hipError_t cffc_launch_update_tile_mixed(const cffc::StageParams *p, int ti, int tj, hipStream_t st);
+hipError_t cffc_launch_w_from_u(const cffc::BlockMeta *meta, int nblk, int max_int, const double *U, double *W,
+                                float *Wf, int64_t ncp, hipStream_t st);
 hipError_t cffc_launch_stage_fp64(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 hipError_t cffc_launch_stage_mixed(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 }

@@ -20,3 +20,8 @@ extern "C" hipError_t cffc_launch_stage_mixed(const StageParams *p, int ti, int 
 #undef L
     return hipErrorInvalidValue;
 }
+
+extern "C" hipError_t cffc_launch_update_tile_mixed(const StageParams *p, int ti, int tj, hipStream_t st) {
+    hipLaunchKernelGGL((update_tile_kernel<float>), dim3(p->ntiles), dim3(UPD_NT), 0, st, *p, ti, tj);
+    return hipGetLastError();
+}

@@ -34,6 +34,8 @@ struct StageParams {
     const double *wo;       // reference states of the bc entries (AoS, 4)
     const double *Win;      // W^(s), 4 x ncp, ghosts filled
     double *Wout;           // W^(s+1) interior (fused) -- may alias Win when !fused
+    const float *Winf;      // fused mixed path: W^(s) and W^(s+1) stored as FP32
+    float *Woutf;
     double *U;              // conserved U^n (fused path)
     double *S;              // residual accumulator dUdt[k]
     const double *S1, *S2;  // dUdt[1], dUdt[2] (granular RK4 stage 4)
@@ -53,6 +55,7 @@ struct StageParams {
     int32_t done_copy;      // parts that are done copy W forward (odd stage count)
     const int32_t *nbr;     // 8 same-level neighbours per block (push exchange)
     int32_t push;           // fused: also write W^(s+1) into the neighbours' ghost cells
+    int32_t defer_update;   // fused: store the residual only; update_tile_kernel follows
 };
 
 // Push form of the same-level exchange: the cell (ip, jp) of block b that a
@@ -61,7 +64,8 @@ struct StageParams {
 // next stage finds those ghosts filled without a separate exchange.  Needs
 // a symmetric neighbour table (checked by the host).  Slots N, S, E, W, NE,
 // NW, SE, SW; the neighbour's ghost index follows from its slot facing b.
-__device__ __forceinline__ void push_ghosts(const StageParams &p, const BlockMeta &m, int b, int ip, int jp,
+template <typename ST>
+__device__ __forceinline__ void push_ghosts(const StageParams &p, ST *Wo, const BlockMeta &m, int b, int ip, int jp,
                                             const PS<double> &Wn) {
     const int ICu = m.ni + 1, JCu = m.nj + 1;
     const bool nW = ip <= 3, nE = ip >= ICu - 1, nS = jp <= 3, nN = jp >= JCu - 1;
@@ -80,7 +84,7 @@ __device__ __forceinline__ void push_ghosts(const StageParams &p, const BlockMet
         const int ig = east ? ip - m.ni : west ? ip + S.ni : ip;
         const int jg = up ? jp - m.nj : down ? jp + S.nj : jp;
         const int64_t cg = S.coff + (int64_t)jg * S.pitch + ig;
-        p.Wout[cg] = Wn.d; p.Wout[p.ncp + cg] = Wn.vx; p.Wout[2 * p.ncp + cg] = Wn.vy; p.Wout[3 * p.ncp + cg] = Wn.p;
+        Wo[cg] = (ST)Wn.d; Wo[p.ncp + cg] = (ST)Wn.vx; Wo[2 * p.ncp + cg] = (ST)Wn.vy; Wo[3 * p.ncp + cg] = (ST)Wn.p;
     }
 }
 
@@ -265,6 +269,89 @@ template <typename FT> __device__ __forceinline__ bool positivity_fails(const CS
     else return U_e(U) <= 0.0;
 }
 
+// The fused update of one cell from its accumulated residual dU
+// (Update_Solution_Multistage_Explicit, Euler2DQuadSingleBlock.cc:5300-5507):
+// U^(s+1) = U^n + omega dU with the positivity test and the scalar
+// local-time-step halving, W^(s+1) (and its pushed ghost copies), the
+// accumulator, and at the last stage U^(n+1), the norm terms and the next
+// step's CFL time step (SingleBlock.cc:3036-3053).
+template <typename FT>
+__device__ __forceinline__ void fused_update(const StageParams &p, const BlockMeta &m, const TileDesc &t, int64_t c,
+                                             int ip, int jp, double A, double dtc, double (&dU)[4], double &l1,
+                                             double &l2, double &mx, double &dtn) {
+    const double *G = p.geom;
+    const double om = p.omega;
+    CS<double> Un;
+    const double U0[4] = {p.U[c], p.U[p.ncp + c], p.U[2 * p.ncp + c], p.U[3 * p.ncp + c]};
+    Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
+    Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
+    bool bad = positivity_fails<FT>(Un);
+    if (bad && p.lts) {
+        // scalar local time stepping: halve dt and dUdt up to 10 times
+        double f = 1.0, dtl = dtc;
+        for (int n = 1; n <= 10; ++n) {
+            f = 0.5 * f;
+            dtl = f * dtl;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) dU[v] = f * dU[v];
+            Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
+            Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
+            if (!positivity_fails<FT>(Un)) break;
+        }
+        p.dt[c] = dtl;
+        bad = positivity_fails<FT>(Un);
+    }
+    if (bad) {
+        const unsigned long long key =
+            ((unsigned long long)t.b << 32) | ((unsigned long long)jp << 16) | (unsigned long long)ip;
+        atomicMin(p.err, key);
+        // the reference returns from the failing stage; later kernels skip
+        if (p.stop_flag) atomicExch(p.stop_flag, 1);
+    }
+    PS<double> Wn;
+    if constexpr (Fast<FT>::on) {
+        const double r = 1.0 / Un.d;
+        Wn.d = Un.d; Wn.vx = Un.mx * r; Wn.vy = Un.my * r;
+        Wn.p = gm1<double>() * (Un.E - 0.5 * (Un.mx * Wn.vx + Un.my * Wn.vy));
+    } else {
+        Wn = W_of_U(Un);
+    }
+    if constexpr (Fast<FT>::on) {
+        p.Woutf[c] = (float)Wn.d; p.Woutf[p.ncp + c] = (float)Wn.vx;
+        p.Woutf[2 * p.ncp + c] = (float)Wn.vy; p.Woutf[3 * p.ncp + c] = (float)Wn.p;
+        if (p.push) push_ghosts(p, p.Woutf, m, t.b, ip, jp, Wn);
+    } else {
+        p.Wout[c] = Wn.d; p.Wout[p.ncp + c] = Wn.vx; p.Wout[2 * p.ncp + c] = Wn.vy; p.Wout[3 * p.ncp + c] = Wn.p;
+        if (p.push) push_ghosts(p, p.Wout, m, t.b, ip, jp, Wn);
+    }
+    if (p.mode == MODE_ZERO_ACC2) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) p.S[v * p.ncp + c] = p.S[v * p.ncp + c] + 2.0 * dU[v];
+    } else {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) p.S[v * p.ncp + c] = dU[v];
+    }
+    if (p.last_stage) {
+        p.U[c] = Un.d; p.U[p.ncp + c] = Un.mx; p.U[2 * p.ncp + c] = Un.my; p.U[3 * p.ncp + c] = Un.E;
+        l1 = fabs(dU[0]);
+        l2 = dU[0] * dU[0];
+        mx = l1;
+        // CFL of the next step, SingleBlock.cc:3036-3053, from the
+        // FP64 geometry in both paths
+        const double gE = G[GEO_LE * p.ncp + c], gW = G[GEO_LE * p.ncp + c - 1];
+        const double gN = G[GEO_LN * p.ncp + c], gS = G[GEO_LN * p.ncp + c - m.pitch];
+        const double d_i = 2.0 * (A / (gE + gW));
+        const double d_j = 2.0 * (A / (gN + gS));
+        const double nWx = -G[GEO_NEX * p.ncp + c - 1], nWy = -G[GEO_NEY * p.ncp + c - 1];
+        const double nSx = -G[GEO_NNX * p.ncp + c - m.pitch], nSy = -G[GEO_NNY * p.ncp + c - m.pitch];
+        const double v_i = 0.5 * (Wn.vx * (G[GEO_NEX * p.ncp + c] - nWx) + Wn.vy * (G[GEO_NEY * p.ncp + c] - nWy));
+        const double v_j = 0.5 * (Wn.vx * (G[GEO_NNX * p.ncp + c] - nSx) + Wn.vy * (G[GEO_NNY * p.ncp + c] - nSy));
+        const double a = W_a(Wn);
+        dtn = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
+        if (p.lts) p.dt[c] = dtn;
+    }
+}
+
 // Threads per workgroup for a TI x TJ tile: one per tile cell plus one per
 // halo ("ring") cell whose reconstruction the tile's boundary faces need,
 // rounded up to whole wavefronts.  For 16 x 16: 256 + 64 = 320 (5 waves),
@@ -305,7 +392,10 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                     if (ip <= m.ni + 1 && jp <= m.nj + 1) {
                         const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
 #pragma unroll
-                        for (int v = 0; v < 4; ++v) p.Wout[v * ncp + c] = p.Win[v * ncp + c];
+                        for (int v = 0; v < 4; ++v) {
+                            if constexpr (Fast<FT>::on) p.Woutf[v * ncp + c] = p.Winf[v * ncp + c];
+                            else p.Wout[v * ncp + c] = p.Win[v * ncp + c];
+                        }
                     }
                 }
             }
@@ -343,7 +433,10 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
         if (ip < m.nci && jp < m.ncj) {
             const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) sW[v][jl][il] = (FT)p.Win[v * ncp + c];
+            for (int v = 0; v < 4; ++v) {
+                if constexpr (FUSED && Fast<FT>::on) sW[v][jl][il] = p.Winf[v * ncp + c];
+                else sW[v][jl][il] = (FT)p.Win[v * ncp + c];
+            }
             sX[jl][il] = rdgeo<FT>(p, GEO_XC, GF_XC, c);
             sY[jl][il] = rdgeo<FT>(p, GEO_YC, GF_YC, c);
         }
@@ -532,77 +625,15 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                 dU[v] = d;
             }
         }
-        if (!FUSED) {
+        if (!FUSED || p.defer_update) {
+            // granular residual, or residual smoothing follows: store dUdt
 #pragma unroll
             for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = dU[v];
         } else {
-            const double om = p.omega;
-            CS<double> Un;
-            const double U0[4] = {p.U[c], p.U[ncp + c], p.U[2 * ncp + c], p.U[3 * ncp + c]};
-            Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
-            Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
-            bool bad = positivity_fails<FT>(Un);
-            if (bad && p.lts) {
-                // scalar local time stepping: halve dt and dUdt up to 10 times
-                double f = 1.0, dtl = dtc;
-                for (int n = 1; n <= 10; ++n) {
-                    f = 0.5 * f;
-                    dtl = f * dtl;
-#pragma unroll
-                    for (int v = 0; v < 4; ++v) dU[v] = f * dU[v];
-                    Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
-                    Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
-                    if (!positivity_fails<FT>(Un)) break;
-                }
-                p.dt[c] = dtl;
-                bad = positivity_fails<FT>(Un);
-            }
-            if (bad) {
-                const unsigned long long key =
-                    ((unsigned long long)t.b << 32) | ((unsigned long long)jp << 16) | (unsigned long long)ip;
-                atomicMin(p.err, key);
-                // the reference returns from the failing stage; later kernels skip
-                if (p.stop_flag) atomicExch(p.stop_flag, 1);
-            }
-            PS<double> Wn;
-            if constexpr (Fast<FT>::on) {
-                const double r = 1.0 / Un.d;
-                Wn.d = Un.d; Wn.vx = Un.mx * r; Wn.vy = Un.my * r;
-                Wn.p = gm1<double>() * (Un.E - 0.5 * (Un.mx * Wn.vx + Un.my * Wn.vy));
-            } else {
-                Wn = W_of_U(Un);
-            }
-            p.Wout[c] = Wn.d; p.Wout[ncp + c] = Wn.vx; p.Wout[2 * ncp + c] = Wn.vy; p.Wout[3 * ncp + c] = Wn.p;
-            if (p.push) push_ghosts(p, m, t.b, ip, jp, Wn);
-            if (p.mode == MODE_ZERO_ACC2) {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = p.S[v * ncp + c] + 2.0 * dU[v];
-            } else {
-#pragma unroll
-                for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = dU[v];
-            }
-            if (p.last_stage) {
-                p.U[c] = Un.d; p.U[ncp + c] = Un.mx; p.U[2 * ncp + c] = Un.my; p.U[3 * ncp + c] = Un.E;
-                l1 = fabs(dU[0]);
-                l2 = dU[0] * dU[0];
-                mx = l1;
-                // CFL of the next step, SingleBlock.cc:3036-3053, from the
-                // FP64 geometry in both paths
-                const double gE = G[GEO_LE * ncp + c], gW = G[GEO_LE * ncp + c - 1];
-                const double gN = G[GEO_LN * ncp + c], gS = G[GEO_LN * ncp + c - m.pitch];
-                const double d_i = 2.0 * (A / (gE + gW));
-                const double d_j = 2.0 * (A / (gN + gS));
-                const double nWx = -G[GEO_NEX * ncp + c - 1], nWy = -G[GEO_NEY * ncp + c - 1];
-                const double nSx = -G[GEO_NNX * ncp + c - m.pitch], nSy = -G[GEO_NNY * ncp + c - m.pitch];
-                const double v_i = 0.5 * (Wn.vx * (G[GEO_NEX * ncp + c] - nWx) + Wn.vy * (G[GEO_NEY * ncp + c] - nWy));
-                const double v_j = 0.5 * (Wn.vx * (G[GEO_NNX * ncp + c] - nSx) + Wn.vy * (G[GEO_NNY * ncp + c] - nSy));
-                const double a = W_a(Wn);
-                dtn = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
-                if (p.lts) p.dt[c] = dtn;
-            }
+            fused_update<FT>(p, m, t, c, ip, jp, A, dtc, dU, l1, l2, mx, dtn);
         }
     }
-    if (!(FUSED) || !p.last_stage) return;
+    if (!(FUSED) || !p.last_stage || p.defer_update) return;
 
     // ---- last stage: workgroup partials of the norms and the next dt --------
     // wavefront reductions with DPP/permute shuffles, then across the 4 waves
@@ -623,6 +654,76 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
         // per-tile partials (SoA), reduced in tile order by step_finish_kernel;
         // the kernel boundary orders these stores before that kernel's loads
         // (no device-scope fence or atomic per workgroup)
+        p.norm_part[tix] = a;
+        p.norm_part[p.ntiles + tix] = b;
+        p.norm_part[2 * p.ntiles + tix] = cmx;
+        p.norm_part[3 * p.ntiles + tix] = dmn;
+    }
+}
+
+// Update of a tile from a stored residual: the second half of a fused stage
+// when implicit residual smoothing runs between the residual and the update
+// (Euler2DQuadSolvers.cc:855-877).  Same tile order, part handling and
+// last-stage partials as the stage kernel.
+constexpr int UPD_NT = 256;
+template <typename FT>
+__global__ __launch_bounds__(UPD_NT) void update_tile_kernel(StageParams p, int TI, int TJ) {
+    __shared__ double sRed[UPD_NT / 64][4];
+    if (p.stop_flag && *p.stop_flag) return;
+    const int tix = xcd_tile(blockIdx.x, p.ntiles);
+    const TileDesc t = p.tiles[tix];
+    const BlockMeta m = p.meta[t.b];
+    const int tid = threadIdx.x;
+    const int64_t ncp = p.ncp;
+    if (p.part_state && p.part_state[t.part] == PART_DONE) {
+        if (p.done_copy) {
+            for (int q = tid; q < TI * TJ; q += UPD_NT) {
+                const int ip = t.i0 + 2 + q % TI, jp = t.j0 + 2 + q / TI;
+                if (ip <= m.ni + 1 && jp <= m.nj + 1) {
+                    const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        if constexpr (Fast<FT>::on) p.Woutf[v * ncp + c] = p.Winf[v * ncp + c];
+                        else p.Wout[v * ncp + c] = p.Win[v * ncp + c];
+                    }
+                }
+            }
+        }
+        if (p.last_stage && tid == 0) {
+            p.norm_part[tix] = 0.0;
+            p.norm_part[p.ntiles + tix] = 0.0;
+            p.norm_part[2 * p.ntiles + tix] = 0.0;
+            p.norm_part[3 * p.ntiles + tix] = MILLION;
+        }
+        return;
+    }
+    double l1 = 0.0, l2 = 0.0, mx = 0.0, dtn = MILLION;
+    for (int q = tid; q < TI * TJ; q += UPD_NT) {
+        const int ip = t.i0 + 2 + q % TI, jp = t.j0 + 2 + q / TI;
+        if (ip > m.ni + 1 || jp > m.nj + 1) continue;
+        const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
+        const double A = p.geom[GEO_A * ncp + c];
+        const double dtc = p.lts ? p.dt[c] : bitsd(*p.dtmin_in);
+        double dU[4] = {p.S[c], p.S[ncp + c], p.S[2 * ncp + c], p.S[3 * ncp + c]};
+        double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = MILLION;
+        fused_update<FT>(p, m, t, c, ip, jp, A, dtc, dU, a1, a2, a3, a4);
+        l1 += a1; l2 += a2; mx = dmax(mx, a3); dtn = dmin(dtn, a4);
+    }
+    if (!p.last_stage) return;
+    for (int off = 32; off > 0; off >>= 1) {
+        l1 += __shfl_xor(l1, off);
+        l2 += __shfl_xor(l2, off);
+        mx = dmax(mx, __shfl_xor(mx, off));
+        dtn = dmin(dtn, __shfl_xor(dtn, off));
+    }
+    const int wv = tid >> 6;
+    if ((tid & 63) == 0) { sRed[wv][0] = l1; sRed[wv][1] = l2; sRed[wv][2] = mx; sRed[wv][3] = dtn; }
+    __syncthreads();
+    if (tid == 0) {
+        double a = sRed[0][0], b = sRed[0][1], cmx = sRed[0][2], dmn = sRed[0][3];
+        for (int w = 1; w < UPD_NT / 64; ++w) {
+            a += sRed[w][0]; b += sRed[w][1]; cmx = dmax(cmx, sRed[w][2]); dmn = dmin(dmn, sRed[w][3]);
+        }
         p.norm_part[tix] = a;
         p.norm_part[p.ntiles + tix] = b;
         p.norm_part[2 * p.ntiles + tix] = cmx;

@@ -75,6 +75,8 @@ def load():
     L.cffc_run_steady.argtypes = [vp, i32, d, d, i32, pi32, pd, ctypes.POINTER(ctypes.c_int)]
     L.cffc_get_part_history.argtypes = [vp, i32, pd]
     L.cffc_step_times.argtypes = [vp, i32, pd]
+    L.cffc_set_residual_smoothing.argtypes = [vp, d, i32]
+    L.cffc_residual_smoothing.argtypes = [vp, i32]
     pint = ctypes.POINTER(ctypes.c_int)
     L.cffc_autotune_tile.argtypes = [vp, i32, pint, pint, pd, pint]
     L.cffc_last_error.argtypes = []
@@ -178,6 +180,13 @@ class Euler2DQuadBlocks:
         out = np.zeros(3)
         _check(load().cffc_residual_norms(self.h, var, _p(out)), "cffc_residual_norms")
         return out
+
+    def set_residual_smoothing(self, epsilon, iterations):
+        """Residual_Smoothing_Epsilon / _Gauss_Seidel_Iterations (0: off)."""
+        _check(load().cffc_set_residual_smoothing(self.h, epsilon, iterations), "cffc_set_residual_smoothing")
+
+    def Residual_Smoothing(self, i_stage):
+        return _check(load().cffc_residual_smoothing(self.h, i_stage), "cffc_residual_smoothing")
 
     def Freeze_Limiters(self, freeze=True):
         return _check(load().cffc_freeze_limiters(self.h, 1 if freeze else 0), "cffc_freeze_limiters")

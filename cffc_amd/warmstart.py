@@ -91,10 +91,15 @@ def evaluate(net, norm, cases, tol=1e-10, max_steps=200000, freeze_level=steady.
     cold and warm, for the time of one solve.  Returns a dict with the steps
     of every case, the wall time of each batch (the warm one including
     inference) and the single-case times."""
-    _, cold_steps, cold_s = solve_cases(cases, tol, max_steps, freeze_level, log=log)
+    cold_states, cold_steps, cold_s = solve_cases(cases, tol, max_steps, freeze_level, log=log)
     t0 = time.perf_counter()
     init = predict_states(net, norm, cases)
     t_inf = time.perf_counter() - t0
+    # prediction error on the held-out cases, in the training loss's units
+    m0 = M.bump_channel(cases[0][0])
+    err = [np.mean((norm.encode(unet.state_to_image(m0, Wp), mach) -
+                    norm.encode(unet.state_to_image(m0, Wc), mach)) ** 2)
+           for (h, mach), Wp, Wc in zip(cases, init, cold_states)]
     _, warm_steps, warm_s = solve_cases(cases, tol, max_steps, freeze_level, initial=init, log=log)
     single = []
     for q in range(min(singles, len(cases))):
@@ -112,7 +117,8 @@ def evaluate(net, norm, cases, tol=1e-10, max_steps=200000, freeze_level=steady.
     return dict(cases=np.asarray(cases).tolist(), cold_steps=cold_steps.tolist(), warm_steps=warm_steps.tolist(),
                 cold_seconds=float(sum(cold_s)), warm_seconds=float(sum(warm_s)) + t_inf, inference_seconds=t_inf,
                 step_ratio_median=float(np.median(cold_steps / np.maximum(warm_steps, 1))),
-                step_ratio_total=float(cold_steps.sum() / max(warm_steps.sum(), 1)), single=single)
+                step_ratio_total=float(cold_steps.sum() / max(warm_steps.sum(), 1)), single=single,
+                heldout_mse=float(np.mean(err)))
 
 
 # ---- persistence -------------------------------------------------------------
@@ -206,8 +212,9 @@ def main(argv=None):
         r = evaluate(net, norm, cases, a.tol, a.max_steps, log=log, singles=a.singles)
         report.update(r)
         log("held-out %d: cold steps median %d, warm %d; step ratio median %.2f total %.2f; batch time cold %.1fs "
-            "warm %.1fs" % (a.n, int(np.median(r["cold_steps"])), int(np.median(r["warm_steps"])),
-                            r["step_ratio_median"], r["step_ratio_total"], r["cold_seconds"], r["warm_seconds"]))
+            "warm %.1fs; prediction MSE %.3e" % (a.n, int(np.median(r["cold_steps"])), int(np.median(r["warm_steps"])),
+                                                 r["step_ratio_median"], r["step_ratio_total"], r["cold_seconds"],
+                                                 r["warm_seconds"], r["heldout_mse"]))
     if a.out:
         with open(a.out, "w") as f:
             json.dump(report, f)

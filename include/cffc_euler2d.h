@@ -197,6 +197,18 @@ int  cffc_run_steps(cffc_ctx *ctx, int nsteps, double tol, double *norms,
 int  cffc_run_steps_graph(cffc_ctx *ctx, int nsteps, double *norms,
                           double *dt_out);
 
+/* Implicit residual smoothing (Residual_Smoothing, Euler2DQuadSingleBlock.cc:
+   4521-4564, applied between dUdt_Multistage_Explicit and the update as in
+   Euler2DQuadSolvers.cc:861-867).  Inputs Residual_Smoothing_Epsilon and
+   Residual_Smoothing_Gauss_Seidel_Iterations; iterations = 0 switches it
+   off (the default).  Gauss-Seidel sweeps in the reference's order, bit for
+   bit.  Refused (CFFC_E_ARG) for 4-stage Runge-Kutta, whose stage 3 would
+   need a fourth residual vector. */
+int  cffc_set_residual_smoothing(cffc_ctx *ctx, double epsilon, int iterations);
+/* == Residual_Smoothing(blocks, list, IP, i_stage) (MultiBlock.cc:1227),
+   the granular form; cffc_run_steps / cffc_run_steady apply it by themselves. */
+int  cffc_residual_smoothing(cffc_ctx *ctx, int i_stage);
+
 /* Batched steady runs.  The reference runs one case per solver invocation
    (Euler2DQuadSolver, Euler2DQuadSolvers.cc:26, explicit loop :549-898, with
    Freeze_Limiters at Freeze_Limiter_Residual_Level, :796-803).  Here several

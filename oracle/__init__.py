@@ -69,6 +69,8 @@ def lib():
         L.orc_update_stage.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
         L.orc_residual_norms.argtypes = [vp, ctypes.c_int, pd]
         L.orc_freeze_limiters.argtypes = [vp, ctypes.c_int]
+        L.orc_set_residual_smoothing.argtypes = [vp, d, ctypes.c_int]
+        L.orc_residual_smoothing.argtypes = [vp, ctypes.c_int]
         L.orc_run.argtypes = [vp, ctypes.c_int, d, ctypes.c_int, pd, pd, ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
@@ -248,6 +250,12 @@ class Oracle:
 
     def freeze_limiters(self, freeze=True):
         lib().orc_freeze_limiters(self.h, 1 if freeze else 0)
+
+    def set_residual_smoothing(self, epsilon, iterations):
+        lib().orc_set_residual_smoothing(self.h, epsilon, iterations)
+
+    def residual_smoothing(self, i_stage):
+        lib().orc_residual_smoothing(self.h, i_stage)
 
     def run(self, nsteps, tol=0.0, nthreads=1):
         """Explicit time-marching loop; returns (rc, steps_done, norms[n,3], dt[n])."""

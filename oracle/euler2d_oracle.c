@@ -39,6 +39,7 @@
 #define ZERO 0.0
 #define ONE 1.0
 #define TWO 2.0
+#define FOUR 4.0
 #define SIX 6.0
 #define HALF 0.50
 #define MILLION 1000000.00
@@ -360,6 +361,8 @@ typedef struct orc_ctx {
     cffc_scheme_desc s;
     int64_t ncell;
     double *xnode, *ynode; /* owned copies */
+    double rs_epsilon;     /* Residual_Smoothing_Epsilon */
+    int rs_iterations;     /* Residual_Smoothing_Gauss_Seidel_Iterations (0: off) */
 } orc_ctx;
 
 #define CI(B, i, j) ((j) * (B)->nci + (i))
@@ -1166,6 +1169,43 @@ int orc_update_stage(orc_ctx *ctx, int i_stage, int64_t *bad_cell) {
     }
     return 0;
 }
+/* Residual_Smoothing, Euler2DQuadSingleBlock.cc:4521-4564: implicit
+   smoothing of dUdt[k] on the cells one away from the block edge, by
+   lexicographic Gauss-Seidel sweeps (j outer, i inner) on dUdt[k+1], whose
+   values on the edge ring are whatever that array holds (zero unless a
+   scheme writes it).  k follows Residual_Smoothing, MultiBlock.cc:1227-1257. */
+static void block_residual_smoothing(oblk *B, int k, double epsilon, int iters) {
+    for (int j = B->JCl + 1; j <= B->JCu - 1; ++j)
+        for (int i = B->ICl + 1; i <= B->ICu - 1; ++i) B->dUdt[k + 1][CI(B, i, j)] = B->dUdt[k][CI(B, i, j)];
+    for (int n = 1; n <= iters; ++n)
+        for (int j = B->JCl + 1; j <= B->JCu - 1; ++j)
+            for (int i = B->ICl + 1; i <= B->ICu - 1; ++i) {
+                const cst r = B->dUdt[k][CI(B, i, j)];
+                const cst a = B->dUdt[k + 1][CI(B, i, j - 1)], b = B->dUdt[k + 1][CI(B, i - 1, j)];
+                const cst c = B->dUdt[k + 1][CI(B, i + 1, j)], d = B->dUdt[k + 1][CI(B, i, j + 1)];
+                const double den = ONE + FOUR * epsilon;
+                cst o;
+                o.d = (r.d + epsilon * (a.d + b.d + c.d + d.d)) / den;
+                o.mx = (r.mx + epsilon * (a.mx + b.mx + c.mx + d.mx)) / den;
+                o.my = (r.my + epsilon * (a.my + b.my + c.my + d.my)) / den;
+                o.E = (r.E + epsilon * (a.E + b.E + c.E + d.E)) / den;
+                B->dUdt[k + 1][CI(B, i, j)] = o;
+            }
+    for (int j = B->JCl + 1; j <= B->JCu - 1; ++j)
+        for (int i = B->ICl + 1; i <= B->ICu - 1; ++i) B->dUdt[k][CI(B, i, j)] = B->dUdt[k + 1][CI(B, i, j)];
+}
+int orc_set_residual_smoothing(orc_ctx *ctx, double epsilon, int iterations) {
+    ctx->rs_epsilon = epsilon;
+    ctx->rs_iterations = iterations;
+    return 0;
+}
+int orc_residual_smoothing(orc_ctx *ctx, int i_stage) {
+    const int k = stage_k_residual(&ctx->s, i_stage);
+    for (int b = 0; b < ctx->nblk; ++b)
+        block_residual_smoothing(&ctx->b[b], k, ctx->rs_epsilon, ctx->rs_iterations);
+    return 0;
+}
+
 /* L1/L2/Max_Norm_Residual :1017-1104 */
 int orc_residual_norms(orc_ctx *ctx, int var, double out[3]) {
     double l1 = ZERO, l2 = ZERO, mx = ZERO;
@@ -1259,6 +1299,10 @@ static void *worker(void *arg) {
             pthread_barrier_wait(w->bar);
             for (int b = b0; b < b1; ++b) block_bcs(&ctx->b[b]);
             for (int b = b0; b < b1; ++b) block_dudt(&ctx->b[b], s, st);
+            /* implicit residual smoothing, Euler2DQuadSolvers.cc:861-867 */
+            if (ctx->rs_iterations > 0)
+                for (int b = b0; b < b1; ++b)
+                    block_residual_smoothing(&ctx->b[b], stage_k_residual(s, st), ctx->rs_epsilon, ctx->rs_iterations);
             int err = 0;
             for (int b = b0; b < b1 && !err; ++b) {
                 int bj = 0;
