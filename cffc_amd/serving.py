@@ -44,7 +44,7 @@ def warm_states(net, norm, cases):
 
 
 def serve(cases, initial=None, tol=1e-10, max_steps=400000, freeze_level=steady.STEADY_FREEZE_LEVEL, mixed=True,
-          switch_level=1e-5, segment=20000, log=None):
+          switch_level=3e-4, segment=20000, log=None):
     """Solve all cases in one batch.  mixed: FP32 flux down to switch_level,
     then FP64 to tol (steady.solve_batch_fp32_then_fp64); else FP64 only.
     Returns a dict of the batch figures."""
