@@ -51,6 +51,18 @@ def make_case(nb, nc, seed=2026):
     return m, W0
 
 
+def autotune(m, W0, precision):
+    """The library's tile autotuner on this workload (a few steps per
+    instantiated tile shape, state restored).  Returns (tile, ms per step)."""
+    from cffc_amd import abi, euler2d
+    s = abi.Scheme(flux_precision=abi.FLUX_FP32 if precision == "mixed" else abi.FLUX_FP64)
+    g = euler2d.Euler2DQuadBlocks(m, s)
+    g.set_W(W0)
+    tile, ms = g.autotune(nsteps=3)
+    g.close()
+    return tile, ms
+
+
 def gpu_run(m, W0, precision, steps, warmup, tile=None):
     from cffc_amd import abi, euler2d
     s = abi.Scheme(flux_precision=abi.FLUX_FP32 if precision == "mixed" else abi.FLUX_FP64)
@@ -163,6 +175,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default 16x8 tile")
+    ap.add_argument("--tile", default=None, metavar="TIxTJ", help="use this tile shape (no autotuning)")
     ap.add_argument("--steady-cases", type=int, default=8,
                     help="held-out steady bump cases solved cold and warm (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -181,12 +195,19 @@ def main():
     res = {}
     precs = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
     for prec in precs:
-        g, rc, dt, norms, W = gpu_run(m, W0, prec, args.steps, args.warmup)
+        tile, tune = (16, 8), {}   # the library's default tile
+        if args.tile:
+            tile = tuple(int(x) for x in args.tile.lower().split("x"))
+        elif not args.no_autotune:
+            tile, tune = autotune(m, W0, prec)
+            print("[%s] autotuned tile %dx%d; ms/step %s" % (prec, tile[0], tile[1],
+                  {"%dx%d" % k: round(v, 3) for k, v in tune.items()}), file=sys.stderr, flush=True)
+        g, rc, dt, norms, W = gpu_run(m, W0, prec, args.steps, args.warmup, tile)
         g.close()
-        kt = kernel_time(m, W0, prec, args.timing_steps)
+        kt = kernel_time(m, W0, prec, args.timing_steps, tile)
         stage_ms = kt["stage_ms"] / max(kt["stage_launches"], 1)
         exch_ms = kt["exchange_ms"] / max(kt["exchange_launches"], 1)
-        res[prec] = dict(rc=rc, seconds=dt, rate=ncell * 4 * args.steps / dt, W=W,
+        res[prec] = dict(rc=rc, seconds=dt, rate=ncell * 4 * args.steps / dt, W=W, tile=tile,
                          stage_ms=stage_ms, exchange_ms=exch_ms, l2_last=float(norms[-1, 1]) if len(norms) else None)
         print("[%s] %.3f s for %d steps: %.3e cell-stage/s; stage kernel %.3f ms, exchange %.3f ms"
               % (prec, dt, args.steps, res[prec]["rate"], stage_ms, exch_ms), file=sys.stderr, flush=True)
@@ -196,6 +217,9 @@ def main():
     achieved = BYTES_PER_CELL_STAGE * ncell / (head["stage_ms"] * 1e-3) / 1e9
     traffic = load_traffic("stage_kernel_%s_riemann4q_%dx%d_blocks_of_%dsq_traffic.json"
                            % (head_prec, args.blocks, args.blocks, args.cells))
+    kname = "stage_kernel<%s,%d,%d,fused>" % ("float" if head_prec == "mixed" else "double", *head["tile"])
+    if traffic and traffic.get("kernel") != kname:
+        traffic = None   # the profiled kernel is another tile shape
     out = {
         "metric": "cell-stage updates/s",
         "value": head["rate"],
@@ -210,13 +234,14 @@ def main():
         "dtype": "fp32-flux/fp64-state" if head_prec == "mixed" else "f64",
         "data": "synthetic: 4-quadrant Riemann IC + seeded 1% density perturbation",
         "config": {"workload": "euler2d_riemann4q_64x64_blocks_of_64x64", "cells": ncell,
+                   "tile": list(head["tile"]),
                    "blocks": args.blocks * args.blocks, "block_cells": [args.cells, args.cells],
                    "scheme": "RK4/Roe/LSQ/Venkatakrishnan/CFL0.5/global-dt",
                    "parallelism": "single-gpu"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": "stage_kernel<%s,16,16,fused>" % ("float" if head_prec == "mixed" else "double"),
+                     "kernel": kname,
                      "kernel_avg_ms": head["stage_ms"]},
     }
     if "fp64" in res:

@@ -55,7 +55,7 @@ struct FinishParams {
 }  // namespace cffc
 
 // tile shapes the stage kernel is instantiated for (TI x TJ interior cells)
-#define CFFC_TILE_LIST(X) X(16, 16) X(32, 8) X(8, 32) X(16, 8) X(8, 8) X(32, 4) X(32, 16)
+#define CFFC_TILE_LIST(X) X(16, 16) X(32, 8) X(8, 32) X(16, 8) X(8, 8) X(32, 4) X(32, 16) X(16, 4) X(8, 16)
 
 extern "C" {
 hipError_t cffc_launch_geometry(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *xn,

@@ -370,8 +370,16 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     __shared__ FT sW[4][LJ][LI];
     typedef typename CoordT<FT>::type XT;
     __shared__ XT sX[LJ][LI], sY[LJ][LI];
-    __shared__ FT sPx[4][RJ][RI], sPy[4][RJ][RI];
-    __shared__ FT sFi[4][TJ][TI + 1], sFj[4][TJ + 1][TI];
+    // phi*gradient (phases 2-3) and face fluxes (phases 3-4) share one LDS
+    // buffer: the fluxes are stored only after every thread has read the
+    // gradients, which keeps the workgroup's LDS small enough for more
+    // resident workgroups per CU
+    constexpr int NP = 2 * 4 * RJ * RI, NF = 4 * TJ * (TI + 1) + 4 * (TJ + 1) * TI;
+    __shared__ FT sPF[NP > NF ? NP : NF];
+    FT(*sPx)[RJ][RI] = reinterpret_cast<FT(*)[RJ][RI]>(sPF);
+    FT(*sPy)[RJ][RI] = reinterpret_cast<FT(*)[RJ][RI]>(sPF + 4 * RJ * RI);
+    FT(*sFi)[TJ][TI + 1] = reinterpret_cast<FT(*)[TJ][TI + 1]>(sPF);
+    FT(*sFj)[TJ + 1][TI] = reinterpret_cast<FT(*)[TJ + 1][TI]>(sPF + 4 * TJ * (TI + 1));
     __shared__ double sRed[NT / 64][4];
 
     if (p.stop_flag && *p.stop_flag) return;
@@ -488,7 +496,7 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     // Tile thread (tc,tr): its E face (i-face tc+1) and N face (j-face tr+1);
     // W-ring thread k: i-face 0 of row k; S-ring thread k: j-face 0 of
     // column k.  E/N-ring threads have no face.
-    auto face = [&](bool idir, int f, int g) {
+    auto face = [&](bool idir, int f, int g, CS<FT> &Fout) -> bool {
         // idir: i-face f in row g; else j-face f in column g.  L/R recon coords:
         int ipL, jpL, rl_i, rl_j, rr_i, rr_j;
         if (idir) {
@@ -498,7 +506,7 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             ipL = pi0 + 2 + g; jpL = pj0 + 1 + f;
             rl_i = g + 1; rl_j = f; rr_i = g + 1; rr_j = f + 1;
         }
-        if (jpL > JCu || ipL > ICu) return;
+        if (jpL > JCu || ipL > ICu) return false;
         const int64_t cL = m.coff + (int64_t)jpL * m.pitch + ipL;
         const XT nxd = idir ? rdgeo<FT>(p, GEO_NEX, GF_NEX, cL) : rdgeo<FT>(p, GEO_NNX, GF_NNX, cL);
         const XT nyd = idir ? rdgeo<FT>(p, GEO_NEY, GF_NEY, cL) : rdgeo<FT>(p, GEO_NNY, GF_NNY, cL);
@@ -551,20 +559,28 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             load_state(rl_i, rl_j, Wl);
             load_state(rr_i, rr_j, Wr);
         }
-        const CS<FT> F = flux_roe_n(Wl, Wr, nx, ny);
-        if (idir) {
-            sFi[0][g][f] = F.d; sFi[1][g][f] = F.mx; sFi[2][g][f] = F.my; sFi[3][g][f] = F.E;
-        } else {
-            sFj[0][f][g] = F.d; sFj[1][f][g] = F.mx; sFj[2][f][g] = F.my; sFj[3][f][g] = F.E;
-        }
+        Fout = flux_roe_n(Wl, Wr, nx, ny);
+        return true;
     };
+    // up to two faces per thread: (idir, f, g) of each
+    CS<FT> F1, F2;
+    bool h1 = false, h2 = false;
     if (is_tile) {
-        face(true, tc + 1, tr);
-        face(false, tr + 1, tc);
+        h1 = face(true, tc + 1, tr, F1);
+        h2 = face(false, tr + 1, tc, F2);
     } else if (ring_side == 0) {
-        face(true, 0, ring_k);
+        h1 = face(true, 0, ring_k, F1);
     } else if (ring_side == 2) {
-        face(false, 0, ring_k);
+        h2 = face(false, 0, ring_k, F2);
+    }
+    __syncthreads();  // all gradient reads done: the flux arrays may now overwrite them
+    if (h1) {
+        const int f = is_tile ? tc + 1 : 0, g = is_tile ? tr : ring_k;
+        sFi[0][g][f] = F1.d; sFi[1][g][f] = F1.mx; sFi[2][g][f] = F1.my; sFi[3][g][f] = F1.E;
+    }
+    if (h2) {
+        const int f = is_tile ? tr + 1 : 0, g = is_tile ? tc : ring_k;
+        sFj[0][f][g] = F2.d; sFj[1][f][g] = F2.mx; sFj[2][f][g] = F2.my; sFj[3][f][g] = F2.E;
     }
     __syncthreads();
 

@@ -21,7 +21,7 @@ _lib = None
 
 # tile shapes the stage kernel is built for, in the autotuner's order
 # (CFFC_TILE_LIST in csrc/euler2d_kernels.hpp)
-TILE_SHAPES = [(16, 16), (32, 8), (8, 32), (16, 8), (8, 8), (32, 4), (32, 16)]
+TILE_SHAPES = [(16, 16), (32, 8), (8, 32), (16, 8), (8, 8), (32, 4), (32, 16), (16, 4), (8, 16)]
 
 
 class CffcError(RuntimeError):
