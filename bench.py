@@ -23,7 +23,7 @@ cpu_baseline: the CPU oracle (a C restatement of the reference algorithm,
 -O2, no FMA), one pthread per core over a contiguous block partition like
 the reference's one-MPI-rank-per-core decomposition, timed on a bounded
 sample of the same workload (a sub-domain of 32 x 32 blocks of 64^2 cells,
-8 steps).
+32 steps, about 10 s on 16 threads).
 
 python bench.py [--gpus 1] [--steps 500] [--warmup 20]
 """
@@ -105,7 +105,7 @@ def cpu_baseline(threads):
     m, W0 = make_case(32, 64)
     o = oracle.Oracle(m, abi.Scheme())
     o.set_W(W0)
-    nsteps = 8
+    nsteps = 32
     t0 = time.perf_counter()
     rc, n, _, _ = o.run(nsteps, nthreads=threads)
     t1 = time.perf_counter()
