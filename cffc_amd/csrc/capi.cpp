@@ -77,6 +77,7 @@ struct cffc_ctx {
     int part_l2_steps = 0;
     int steady_done = 0;   // steps of the current steady run before this segment
     unsigned long long *step_clock = nullptr;  // [max_steps + 1]: start, then end of every step
+    double *chunk_part = nullptr;               // [4][256] first-level step reduction (large meshes)
     // push exchange: the fused stage kernel fills same-level ghosts; the
     // per-stage exchange launch then only applies BCs on boundary blocks
     bool push_ok = false;
@@ -104,7 +105,7 @@ static void free_ctx(cffc_ctx *c) {
                     c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
                     c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock,
-                    c->bc_blocks, c->Wf[0], c->Wf[1]};
+                    c->bc_blocks, c->Wf[0], c->Wf[1], c->chunk_part};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -138,6 +139,12 @@ static int build_tiles(cffc_ctx *c, int ti, int tj) {
     c->ntiles = (int)t.size();
     if (int e = dalloc(&c->tiles, t.size())) return e;
     if (int e = dalloc(&c->norm_part, 4 * t.size())) return e;
+    if (t.size() >= 16384 && !c->chunk_part)
+        if (int e = dalloc(&c->chunk_part, 4 * 256)) return e;
+    if (t.size() < 16384 && c->chunk_part) {
+        (void)hipFree(c->chunk_part);
+        c->chunk_part = nullptr;
+    }
     HIPCHK(hipMemcpy(c->tiles, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
     if (c->npart) {
         tile0.push_back((int32_t)t.size());
@@ -770,7 +777,9 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         f.step_offset = c->steady_done;
     }
     f.step_clock = c->step_clock ? c->step_clock + 1 : nullptr;
-    HIPCHK(cffc_launch_finish(&f, c->stream));
+    // meshes of many tiles reduce their partials in two levels (256 chunks)
+    const int nchunk = (!parts && c->chunk_part) ? 256 : 0;
+    HIPCHK(cffc_launch_finish(&f, c->chunk_part, nchunk, c->stream));
     return 0;
 }
 

@@ -437,11 +437,14 @@ __global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) 
     double a = 0.0, b = 0.0, mx = 0.0, dmn = MILLION;
     int ndone = 0;
     if (p.npart == 0) {
-        for (int q = threadIdx.x; q < n; q += FINISH_NT) {
-            a += p.norm_part[q];
-            b += p.norm_part[n + q];
-            mx = dmax(mx, p.norm_part[2 * n + q]);
-            dmn = dmin(dmn, p.norm_part[3 * n + q]);
+        // partials of the tiles, or of chunks of tiles (chunk_partial_kernel)
+        const double *src = p.nchunk > 0 ? p.chunk_part : p.norm_part;
+        const int ns = p.nchunk > 0 ? p.nchunk : n;
+        for (int q = threadIdx.x; q < ns; q += FINISH_NT) {
+            a += src[q];
+            b += src[ns + q];
+            mx = dmax(mx, src[2 * ns + q]);
+            dmn = dmin(dmn, src[3 * ns + q]);
         }
         for (int off = 32; off > 0; off >>= 1) {
             a += __shfl_xor(a, off); b += __shfl_xor(b, off);
@@ -592,6 +595,35 @@ __global__ void stamp_kernel(unsigned long long *dst) {
     if (threadIdx.x == 0) *dst = wall_clock64();
 }
 
+// First level of the step reduction for large meshes: workgroup k sums the
+// tile partials of chunk k (a fixed contiguous range, so the order is fixed).
+constexpr int CHUNK_NT = 256;
+__global__ __launch_bounds__(CHUNK_NT) void chunk_partial_kernel(const double *part, int n, double *out, int nchunk,
+                                                                 const int32_t *stop_flag) {
+    __shared__ double sm[CHUNK_NT / 64][4];
+    if (*stop_flag) return;
+    const int k = blockIdx.x;
+    const int q0 = (int)((int64_t)n * k / nchunk), q1 = (int)((int64_t)n * (k + 1) / nchunk);
+    double a = 0.0, b = 0.0, mx = 0.0, dmn = MILLION;
+    for (int q = q0 + (int)threadIdx.x; q < q1; q += CHUNK_NT) {
+        a += part[q]; b += part[n + q]; mx = dmax(mx, part[2 * n + q]); dmn = dmin(dmn, part[3 * n + q]);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        a += __shfl_xor(a, off); b += __shfl_xor(b, off);
+        mx = dmax(mx, __shfl_xor(mx, off)); dmn = dmin(dmn, __shfl_xor(dmn, off));
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { sm[w][0] = a; sm[w][1] = b; sm[w][2] = mx; sm[w][3] = dmn; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int j = 1; j < CHUNK_NT / 64; ++j) {
+            sm[0][0] += sm[j][0]; sm[0][1] += sm[j][1]; sm[0][2] = dmax(sm[0][2], sm[j][2]);
+            sm[0][3] = dmin(sm[0][3], sm[j][3]);
+        }
+        out[k] = sm[0][0]; out[nchunk + k] = sm[0][1]; out[2 * nchunk + k] = sm[0][2]; out[3 * nchunk + k] = sm[0][3];
+    }
+}
+
 // small device-side scalar writes (graph-safe, no memset nodes)
 __global__ void set_u64_kernel(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n) {
     if (threadIdx.x == 0) { dst[0] = v0; if (n > 1) dst[1] = v1; }
@@ -692,8 +724,17 @@ hipError_t cffc_launch_stamp(unsigned long long *dst, hipStream_t st) {
     hipLaunchKernelGGL(stamp_kernel, dim3(1), dim3(64), 0, st, dst);
     return hipGetLastError();
 }
-hipError_t cffc_launch_finish(const FinishParams *p, hipStream_t st) {
-    hipLaunchKernelGGL(step_finish_kernel, dim3(1), dim3(FINISH_NT), 0, st, *p);
+hipError_t cffc_launch_finish(const FinishParams *p, double *chunk_part, int nchunk, hipStream_t st) {
+    FinishParams q = *p;
+    q.chunk_part = nullptr;
+    q.nchunk = 0;
+    if (p->npart == 0 && nchunk > 0) {
+        hipLaunchKernelGGL(chunk_partial_kernel, dim3(nchunk), dim3(CHUNK_NT), 0, st, p->norm_part, p->ntiles,
+                           chunk_part, nchunk, p->stop_flag);
+        q.chunk_part = chunk_part;
+        q.nchunk = nchunk;
+    }
+    hipLaunchKernelGGL(step_finish_kernel, dim3(1), dim3(FINISH_NT), 0, st, q);
     return hipGetLastError();
 }
 

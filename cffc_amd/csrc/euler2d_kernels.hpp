@@ -50,6 +50,8 @@ struct FinishParams {
     double freeze_level;                    // part L2 at which limiters freeze (<= 0: never)
     int32_t step_offset;                    // steps of earlier segments of the same steady run
     unsigned long long *step_clock;         // [max_steps] device wall clock at the end of each step, or null
+    const double *chunk_part;               // [4][nchunk] chunk partials of norm_part (nchunk > 0, no parts)
+    int32_t nchunk;
 };
 
 }  // namespace cffc
@@ -81,7 +83,7 @@ hipError_t cffc_launch_norms(const cffc::BlockMeta *meta, const cffc::TileDesc *
                              const double *dU, int64_t ncp, int var, double *part, double *out, hipStream_t st);
 hipError_t cffc_launch_set_u64(unsigned long long *dst, unsigned long long v0, unsigned long long v1, int n,
                                hipStream_t st);
-hipError_t cffc_launch_finish(const cffc::FinishParams *p, hipStream_t st);
+hipError_t cffc_launch_finish(const cffc::FinishParams *p, double *chunk_part, int nchunk, hipStream_t st);
 hipError_t cffc_launch_stamp(unsigned long long *dst, hipStream_t st);
 hipError_t cffc_launch_residual_smoothing(const cffc::BlockMeta *meta, int nblk, double *S, double *T, int64_t ncp,
                                           double eps, int iters, const int32_t *part_state, const int32_t *stop_flag,

@@ -151,6 +151,39 @@ template <typename T> CFFC_HD PS<T> roe_average(const PS<T> &Wl, const PS<T> &Wr
 // HartenFixPos, whose results are <= 0 / >= 0 respectively).
 template <typename T> CFFC_HD CS<T> flux_roe_x(const PS<T> &Wl, const PS<T> &Wr) {
     typedef Consts<T> K;
+    if constexpr (Fast<T>::on) {
+        // mixed path: the same upwind form with the eigenvector algebra
+        // folded (no multiplications by the zero entries of lp/rc, one
+        // reciprocal of the Roe sound speed)
+        const PS<T> Wa = roe_average(Wl, Wr);
+        const T dd = Wr.d - Wl.d, du = Wr.vx - Wl.vx, dv = Wr.vy - Wl.vy, dp = Wr.p - Wl.p;
+        const T a2a = W_a2(Wa);
+        const T cl = W_a(Wl), cr = W_a(Wr), ca = qsqrt(a2a);
+        const T rca = __builtin_amdgcn_rcpf(ca), rca2 = rca * rca;
+        const T ha = W_h(Wa);
+        const T la1 = Wa.vx - ca, la4 = Wa.vx + ca;
+        const bool left = (Wa.vx >= K::ZERO);
+        const T d1 = harten_delta(la1, Wl.vx - cl, Wr.vx - cr);
+        const T d4 = harten_delta(la4, Wl.vx + cl, Wr.vx + cr);
+        const T w1 = left ? K::HALF * (la1 - tabs(la1) - d1) : K::HALF * (la1 + tabs(la1) + d1);
+        const T w2 = left ? K::HALF * (Wa.vx - tabs(Wa.vx)) : K::HALF * (Wa.vx + tabs(Wa.vx));
+        const T w4 = left ? K::HALF * (la4 - tabs(la4) - d4) : K::HALF * (la4 + tabs(la4) + d4);
+        const PS<T> Wu = left ? Wl : Wr;
+        CS<T> F = W_Fx(Wu);
+        const T sg = left ? T(1) : T(-1);
+        const T pa = K::HALF * rca2 * dp, ua = K::HALF * Wa.d * rca * du;
+        const T a1 = sg * w1 * (pa - ua);             // lp1 . dW
+        const T a4 = sg * w4 * (pa + ua);             // lp4 . dW
+        const T a2 = sg * w2 * (dd - rca2 * dp);      // lp2 . dW (1/a^2 = 1/c^2)
+        const T a3 = sg * w2 * dv;                    // lp3 . dW
+        const T uc = Wa.vx * ca;
+        F.d += a1 + a2 + a4;
+        F.mx += a1 * la1 + a2 * Wa.vx + a4 * la4;
+        F.my += (a1 + a2 + a4) * Wa.vy + a3 * Wa.d;
+        F.E += a1 * (ha - uc) + a2 * (K::HALF * (Wa.vx * Wa.vx + Wa.vy * Wa.vy)) + a3 * (Wa.d * Wa.vy) +
+               a4 * (ha + uc);
+        return F;
+    }
     const PS<T> Wa = roe_average(Wl, Wr);
     const PS<T> dW = {Wr.d - Wl.d, Wr.vx - Wl.vx, Wr.vy - Wl.vy, Wr.p - Wl.p};
     // a() = sqrt(g*p/d) and a2() = g*p/d share the quotient; h() is needed by

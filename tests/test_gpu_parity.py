@@ -234,3 +234,18 @@ def test_negative_density_reports_reference_error(euler2d):
     assert rc2 > 0, "the oracle should fail on this input"
     assert rc == rc2
     assert n == n2 == 0
+
+
+def test_two_level_step_reduction_large_mesh(euler2d):
+    """16,384 tiles or more: the step's norms are reduced in 256 chunks
+    first; norms and dt still match the oracle and the state stays bitwise."""
+    m = M.cartesian(64, 32, 32, 32)
+    W0 = M.riemann_4quadrant(m)
+    W0 = M.density_perturbation(m, W0, amplitude=0.01, seed=3)
+    m.set_wo_from_initial(W0)
+    gpu, orc, g, o = run_both(euler2d, m, W0, abi.Scheme(), 2)
+    assert gpu.get_tile() == (16, 8) and m.ncell // 128 >= 16384
+    assert g[0] == o[0] == 0
+    np.testing.assert_array_equal(g[3], o[3])
+    assert relerr(g[2], o[2]) <= REL
+    assert nonbitwise(gpu.get_W(), orc.get_W()) == 0
