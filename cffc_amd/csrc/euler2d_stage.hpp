@@ -195,8 +195,10 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
     }
     // averages over the stencil: x/8 and x*0.125 round identically, so the
     // full stencil multiplies; reduced stencils divide (reference: /n_pts)
-    if (FULL || Fast<FT>::on) {
-        const FT rn = FULL ? FT(0.125) : FT(1) / (FT)n_pts;
+    if constexpr (Fast<FT>::on) {
+        // mixed path: the common 1/n_pts of all sums cancels in the solve
+    } else if (FULL) {
+        const FT rn = FT(0.125);
         Dx.d *= rn; Dx.vx *= rn; Dx.vy *= rn; Dx.p *= rn;
         Dy.d *= rn; Dy.vx *= rn; Dy.vy *= rn; Dy.p *= rn;
         DxDx *= rn; DxDy *= rn; DyDy *= rn;
@@ -248,10 +250,28 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
         const FT gxn = n == 0 ? gx.d : n == 1 ? gx.vx : n == 2 ? gx.vy : gx.p;
         const FT gyn = n == 0 ? gy.d : n == 1 ? gy.vx : n == 2 ? gy.vy : gy.p;
         FT phi = FT(1);
-        phi = venkat_fold(phi, u0 + gxn * dxE + gyn * dyE, u0, umin, umax);
-        phi = venkat_fold(phi, u0 + gxn * dxW + gyn * dyW, u0, umin, umax);
-        phi = venkat_fold(phi, u0 + gxn * dxN + gyn * dyN, u0, umin, umax);
-        phi = venkat_fold(phi, u0 + gxn * dxS + gyn * dyS, u0, umin, umax);
+        if constexpr (Fast<FT>::on) {
+            // mixed path: min(1, f(y)) is non-decreasing in y >= 0 (f < 1
+            // rising on [0, 2), f >= 1 beyond), so the minimum over the four
+            // quadrature points is f at the smallest y, capped at y = 2
+            // (f(2) = 1); dead-band points do not lower it
+            const FT up = umax - u0, dn = umin - u0;
+            FT y = FT(2);
+            const FT dq[4] = {gxn * dxE + gyn * dyE, gxn * dxW + gyn * dyW, gxn * dxN + gyn * dyN,
+                              gxn * dxS + gyn * dyS};
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const FT du = dq[q];
+                const FT yq = (du > FT(0) ? up : dn) * __builtin_amdgcn_rcpf(du);
+                y = fabsf(du) > Consts<FT>::TOLER ? fminf(y, yq) : y;
+            }
+            phi = (y * y + FT(2) * y) * __builtin_amdgcn_rcpf(y * y + y + FT(2));
+        } else {
+            phi = venkat_fold(phi, u0 + gxn * dxE + gyn * dyE, u0, umin, umax);
+            phi = venkat_fold(phi, u0 + gxn * dxW + gyn * dyW, u0, umin, umax);
+            phi = venkat_fold(phi, u0 + gxn * dxN + gyn * dyN, u0, umin, umax);
+            phi = venkat_fold(phi, u0 + gxn * dxS + gyn * dyS, u0, umin, umax);
+        }
         if (n == 0) ph.d = phi; else if (n == 1) ph.vx = phi; else if (n == 2) ph.vy = phi; else ph.p = phi;
     }
 }
