@@ -37,7 +37,7 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 class MeshDesc(ctypes.Structure):
     _fields_ = [("nblk", ctypes.c_int32), ("nghost", ctypes.c_int32),
                 ("ni", _i32p), ("nj", _i32p), ("xnode", _f64p), ("ynode", _f64p),
-                ("nbr", _i32p), ("bc", _i32p), ("wo", _f64p)]
+                ("nbr", _i32p), ("bc", _i32p), ("wo", _f64p), ("level", _i32p), ("nbr2", _i32p)]
 
 
 class SchemeDesc(ctypes.Structure):
@@ -78,9 +78,13 @@ def mesh_desc(mesh, keep):
     nbr = _keep(mesh.nbr, np.int32, keep)
     bc = _keep(mesh.bc, np.int32, keep)
     wo = _keep(mesh.wo, np.float64, keep)
+    level = getattr(mesh, "level", None)
+    nbr2 = getattr(mesh, "nbr2", None)
+    lv = _keep(level, np.int32, keep).ctypes.data_as(_i32p) if level is not None else None
+    n2 = _keep(nbr2, np.int32, keep).ctypes.data_as(_i32p) if nbr2 is not None else None
     return MeshDesc(int(mesh.nblk), 2, ni.ctypes.data_as(_i32p), nj.ctypes.data_as(_i32p),
                     xn.ctypes.data_as(_f64p), yn.ctypes.data_as(_f64p), nbr.ctypes.data_as(_i32p),
-                    bc.ctypes.data_as(_i32p), wo.ctypes.data_as(_f64p))
+                    bc.ctypes.data_as(_i32p), wo.ctypes.data_as(_f64p), lv, n2)
 
 
 def scheme_desc(s):

@@ -33,6 +33,8 @@ class Mesh:
     bc: np.ndarray            # concatenated W[ncj], E[ncj], S[nci], N[nci]
     wo: np.ndarray            # [len(bc), 4]
     info: dict = field(default_factory=dict)
+    level: np.ndarray = None  # [nblk] int32 quadtree level, or None (one level)
+    nbr2: np.ndarray = None   # [nblk, 4] int32 second finer face neighbour, or None
 
     # --- offsets -----------------------------------------------------------
     def node_offsets(self):
@@ -182,6 +184,316 @@ def bump_channel(h, nbi=12, nbj=4, ni=16, nj=16, length=3.0, height=1.0, x_bump=
     return m
 
 
+# ---- quadtree (AMR) block meshes --------------------------------------------------
+# Face slots N, S, E, W as (dx, dy); the two finer blocks of a face in the order
+# of the reference's infoX[0], infoX[1] (west then east, south then north).
+_FACE_DIR = ((0, 1), (0, -1), (1, 0), (-1, 0))
+_CORNER_DIR = ((1, 1), (-1, 1), (1, -1), (-1, -1))   # NE, NW, SE, SW
+
+
+def _children_on_face(level, x, y, f):
+    """The two level+1 squares across face f of square (level, x, y)."""
+    if f == 0:
+        return (level + 1, 2 * x, 2 * y + 2), (level + 1, 2 * x + 1, 2 * y + 2)
+    if f == 1:
+        return (level + 1, 2 * x, 2 * y - 1), (level + 1, 2 * x + 1, 2 * y - 1)
+    if f == 2:
+        return (level + 1, 2 * x + 2, 2 * y), (level + 1, 2 * x + 2, 2 * y + 1)
+    return (level + 1, 2 * x - 1, 2 * y), (level + 1, 2 * x - 1, 2 * y + 1)
+
+
+def _child_at_corner(level, x, y, c):
+    dx, dy = _CORNER_DIR[c]
+    return (level + 1, 2 * x + (2 if dx > 0 else -1), 2 * y + (2 if dy > 0 else -1))
+
+
+def _morton(x, y):
+    z = 0
+    for bit in range(20):
+        z |= ((x >> bit) & 1) << (2 * bit) | ((y >> bit) & 1) << (2 * bit + 1)
+    return z
+
+
+def _quad_centroid(x, y):
+    """quadCentroid (Math.h:311-350) of vertices SW, SE, NE, NW."""
+    a = xs = ys = 0.0
+    for p, q in ((3, 0), (0, 1), (1, 2), (2, 3)):
+        ai = x[p] * y[q] - x[q] * y[p]
+        a += ai
+        xs += (x[q] + x[p]) * ai
+        ys += (y[q] + y[p]) * ai
+    return xs / (3 * a), ys / (3 * a)
+
+
+def quadtree_box(nroot=4, n=8, max_level=2, refine=None, x0=0.0, x1=1.0, y0=0.0, y1=1.0, bcs=None,
+                 jitter=0.1, seed=0):
+    """Square domain [x0,x1]x[y0,y1] split into nroot x nroot root blocks,
+    refined as a quadtree (the reference's AMR, QuadTree.h) up to max_level:
+    a leaf is split into four when refine(level, xlo, xhi, ylo, yhi) is true,
+    then leaves are split until face and corner neighbours differ by at most
+    one level.  Every block has n x n cells; blocks are listed in Morton
+    order.  With jitter > 0, the nodes strictly inside every block move by
+    jitter*h at a uniform random angle (h the block's cell size; the
+    reference's Disturb_Interior_Nodes_Without_Update, HO_Grid2DQuad.cc:
+    12144-12170, moves them by 0.1 x the node-edge distance, which is h on
+    the undisturbed lattice).  Ghost nodes follow the reference's geometry
+    messages: a same-level neighbour's nodes; at a finer neighbour, the fine
+    nodes at stride 2 (Load_Send_Message_Buffers_ResChange_FineToCoarse,
+    AdaptiveBlock2D_MessagePassing.h:1290-1320); at a coarser neighbour, the
+    quarters of the coarse cells, cut at the edge midpoints and the cell
+    centroid (Load_Send_Message_Buffers_ResChange_CoarseToFine :2518-2570);
+    ghost nodes on the domain boundary stay on the lattice."""
+    if bcs is None:
+        bcs = dict(W=abi.BC_REFLECTION, E=abi.BC_REFLECTION, S=abi.BC_REFLECTION, N=abi.BC_REFLECTION)
+    if n % 2 or n < 4:
+        raise ValueError("blocks across a resolution change need an even n >= 4")
+    leaves = {(0, ix, iy) for ix in range(nroot) for iy in range(nroot)}
+    Lx, Ly = x1 - x0, y1 - y0
+
+    def extent(k):
+        lv, ix, iy = k
+        s = 1.0 / (nroot << lv)
+        return x0 + Lx * ix * s, x0 + Lx * (ix + 1) * s, y0 + Ly * iy * s, y0 + Ly * (iy + 1) * s
+
+    def inside(lv, x, y):
+        m = nroot << lv
+        return 0 <= x < m and 0 <= y < m
+
+    def find(lv, x, y):
+        """leaf containing square (lv, x, y) at its level or coarser, else None"""
+        for l in range(lv, -1, -1):
+            k = (l, x >> (lv - l), y >> (lv - l))
+            if k in leaves:
+                return k
+        return None
+
+    def split(k):
+        lv, x, y = k
+        leaves.remove(k)
+        for cy in range(2):
+            for cx in range(2):
+                leaves.add((lv + 1, 2 * x + cx, 2 * y + cy))
+
+    if refine is not None:
+        for lv in range(max_level):
+            for k in sorted(kk for kk in leaves if kk[0] == lv):
+                if refine(lv, *extent(k)):
+                    split(k)
+    changed = True
+    while changed:
+        changed = False
+        for k in sorted(leaves):
+            if k not in leaves:
+                continue
+            lv, x, y = k
+            for dx, dy in _FACE_DIR + _CORNER_DIR:
+                if not inside(lv, x + dx, y + dy):
+                    continue
+                o = find(lv, x + dx, y + dy)
+                if o is not None and o[0] < lv - 1:
+                    split(o)
+                    changed = True
+    order = sorted(leaves, key=lambda k: _morton(k[1] << (max_level + 1 - k[0]), k[2] << (max_level + 1 - k[0])))
+    bid = {k: b for b, k in enumerate(order)}
+    nblk = len(order)
+    level = np.array([k[0] for k in order], np.int32)
+    nbr = np.full((nblk, 8), -1, np.int32)
+    nbr2 = np.full((nblk, 4), -1, np.int32)
+    for b, (lv, x, y) in enumerate(order):
+        for f, (dx, dy) in enumerate(_FACE_DIR):
+            if not inside(lv, x + dx, y + dy):
+                continue
+            o = find(lv, x + dx, y + dy)
+            if o is not None:
+                nbr[b, f] = bid[o]
+            else:
+                c0, c1 = _children_on_face(lv, x, y, f)
+                nbr[b, f], nbr2[b, f] = bid[c0], bid[c1]
+        for c, (dx, dy) in enumerate(_CORNER_DIR):
+            if not inside(lv, x + dx, y + dy):
+                continue
+            o = find(lv, x + dx, y + dy)
+            nbr[b, 4 + c] = bid[o] if o is not None else bid[_child_at_corner(lv, x, y, c)]
+
+    # own nodes: lattice (ghost nodes included), interior nodes disturbed
+    rng = np.random.default_rng(seed)
+    N = n + 2 * NG + 1
+    X = np.empty((nblk, N, N))
+    Y = np.empty((nblk, N, N))
+    for b, k in enumerate(order):
+        xa, xb, ya, yb = extent(k)
+        hx, hy = (xb - xa) / n, (yb - ya) / n
+        idx = np.arange(-NG, n + NG + 1, dtype=np.float64)
+        X[b] = (xa + hx * idx)[None, :]
+        Y[b] = (ya + hy * idx)[:, None]
+        if jitter > 0:
+            th = rng.uniform(0.0, 2 * np.pi, (n - 1, n - 1))
+            d = jitter * min(hx, hy)
+            X[b, NG + 1:NG + n, NG + 1:NG + n] += d * np.cos(th)
+            Y[b, NG + 1:NG + n, NG + 1:NG + n] += d * np.sin(th)
+    X0, Y0 = X.copy(), Y.copy()   # the blocks' own nodes (sources of the ghost geometry)
+
+    def centroid(b, i, j):
+        xs = [X0[b, j, i], X0[b, j, i + 1], X0[b, j + 1, i + 1], X0[b, j + 1, i]]
+        ys = [Y0[b, j, i], Y0[b, j, i + 1], Y0[b, j + 1, i + 1], Y0[b, j + 1, i]]
+        return _quad_centroid(xs, ys)
+
+    def quarter_nodes(b, i, j, qx, qy):
+        """nodes (SW, SE, NW, NE order as (dx, dy) -> xy) of quarter (qx, qy) of cell (i, j)"""
+        P = {}
+        for a in range(3):
+            for c in range(3):
+                if a % 2 == 0 and c % 2 == 0:
+                    P[a, c] = (X0[b, j + c // 2, i + a // 2], Y0[b, j + c // 2, i + a // 2])
+                elif a == 1 and c == 1:
+                    P[a, c] = centroid(b, i, j)
+                elif a == 1:
+                    P[a, c] = (0.5 * (X0[b, j + c // 2, i] + X0[b, j + c // 2, i + 1]),
+                               0.5 * (Y0[b, j + c // 2, i] + Y0[b, j + c // 2, i + 1]))
+                else:
+                    P[a, c] = (0.5 * (X0[b, j, i + a // 2] + X0[b, j + 1, i + a // 2]),
+                               0.5 * (Y0[b, j, i + a // 2] + Y0[b, j + 1, i + a // 2]))
+        return {(dx, dy): P[qx + dx, qy + dy] for dx in (0, 1) for dy in (0, 1)}
+
+    def put(b, i, j, nodes):
+        for (dx, dy), (px, py) in nodes.items():
+            ii, jj = i + dx, j + dy
+            if NG <= ii <= n + NG and NG <= jj <= n + NG:
+                continue      # the block's own nodes are never replaced
+            X[b, jj, ii], Y[b, jj, ii] = px, py
+
+    lo, hi = NG, n + NG - 1
+    rng_face = {0: ((lo, hi), (hi + 1, hi + 2)), 1: ((lo, hi), (lo - 2, lo - 1)),
+                2: ((hi + 1, hi + 2), (lo, hi)), 3: ((lo - 2, lo - 1), (lo, hi))}
+    rng_corner = {0: ((hi + 1, hi + 2), (hi + 1, hi + 2)), 1: ((lo - 2, lo - 1), (hi + 1, hi + 2)),
+                  2: ((hi + 1, hi + 2), (lo - 2, lo - 1)), 3: ((lo - 2, lo - 1), (lo - 2, lo - 1))}
+
+    def cells(r):
+        (ia, ib), (ja, jb) = r
+        return [(i, j) for j in range(ja, jb + 1) for i in range(ia, ib + 1)]
+
+    for stage in ("same", "finer", "coarser"):
+        for b in range(nblk):
+            lv = level[b]
+            covered = set()
+            regions = [(f, rng_face[f], _FACE_DIR[f]) for f in range(4)]
+            regions += [(4 + c, rng_corner[c], _CORNER_DIR[c]) for c in range(4)]
+            for f in range(4):
+                C = nbr[b, f]
+                if C >= 0 and level[C] == lv - 1:
+                    sig = 0 if nbr[C, f ^ 1] == b else 1
+                    covered.add({0: (4, 5), 1: (6, 7), 2: (4, 6), 3: (5, 7)}[f][sig])
+            for slot, r, (dx, dy) in regions:
+                o = nbr[b, slot]
+                if o < 0:
+                    continue
+                rel = level[o] - lv
+                if slot >= 4 and slot in covered:
+                    continue
+                if stage == "same" and rel == 0:
+                    for i, j in cells(r):
+                        si, sj = i - dx * n, j - dy * n
+                        put(b, i, j, {(a, c): (X0[o, sj + c, si + a], Y0[o, sj + c, si + a])
+                                      for a in (0, 1) for c in (0, 1)})
+                elif stage == "finer" and rel == 1:
+                    for i, j in cells(r):
+                        if slot < 4:
+                            along = (i - lo) if dy != 0 else (j - lo)
+                            F = nbr[b, slot] if along < n // 2 else nbr2[b, slot]
+                            kk = along % (n // 2)
+                            g = (j - hi) if dy > 0 else (lo - j) if dy < 0 else (i - hi) if dx > 0 else (lo - i)
+                            if dy != 0:
+                                fi = lo + 2 * kk
+                                fj = lo + 2 * (g - 1) if dy > 0 else hi - 2 * g + 1
+                            else:
+                                fj = lo + 2 * kk
+                                fi = lo + 2 * (g - 1) if dx > 0 else hi - 2 * g + 1
+                        else:
+                            F = o
+                            gi = (i - hi) if dx > 0 else (lo - i)
+                            gj = (j - hi) if dy > 0 else (lo - j)
+                            fi = lo + 2 * (gi - 1) if dx > 0 else hi - 2 * gi + 1
+                            fj = lo + 2 * (gj - 1) if dy > 0 else hi - 2 * gj + 1
+                        put(b, i, j, {(a, c): (X0[F, fj + 2 * c, fi + 2 * a], Y0[F, fj + 2 * c, fi + 2 * a])
+                                      for a in (0, 1) for c in (0, 1)})
+            if stage != "coarser":
+                continue
+            for f in range(4):
+                C = nbr[b, f]
+                if C < 0 or level[C] != lv - 1:
+                    continue
+                sig = 0 if nbr[C, f ^ 1] == b else 1
+                dx, dy = _FACE_DIR[f]
+                span = range(lo, hi + 3) if sig == 0 else range(lo - 2, hi + 1)
+                for t in span:
+                    k = t - lo
+                    kf = k // 2
+                    for g in (1, 2):
+                        if dy != 0:
+                            i, j = t, (hi + g if dy > 0 else lo - g)
+                            ic, jc = lo + sig * (n // 2) + kf, (lo if dy > 0 else hi)
+                            qx, qy = k - 2 * kf, (g - 1 if dy > 0 else 2 - g)
+                        else:
+                            i, j = (hi + g if dx > 0 else lo - g), t
+                            ic, jc = (lo if dx > 0 else hi), lo + sig * (n // 2) + kf
+                            qx, qy = (g - 1 if dx > 0 else 2 - g), k - 2 * kf
+                        put(b, i, j, quarter_nodes(C, ic, jc, qx, qy))
+            for c in range(4):
+                D = nbr[b, 4 + c]
+                if (4 + c) in covered or D < 0 or level[D] != lv - 1:
+                    continue
+                dx, dy = _CORNER_DIR[c]
+                ic, jc = (lo if dx > 0 else hi), (lo if dy > 0 else hi)
+                i0, j0 = (hi + 1 if dx > 0 else lo - 2), (hi + 1 if dy > 0 else lo - 2)
+                for qy in (0, 1):
+                    for qx in (0, 1):
+                        put(b, i0 + qx, j0 + qy, quarter_nodes(D, ic, jc, qx, qy))
+
+    nc = n + 2 * NG
+    bc = []
+    for b, (lv, x, y) in enumerate(order):
+        m = nroot << lv
+        bc.append(np.full(nc, bcs["W"] if x == 0 else abi.BC_NONE, np.int32))
+        bc.append(np.full(nc, bcs["E"] if x == m - 1 else abi.BC_NONE, np.int32))
+        bc.append(np.full(nc, bcs["S"] if y == 0 else abi.BC_NONE, np.int32))
+        bc.append(np.full(nc, bcs["N"] if y == m - 1 else abi.BC_NONE, np.int32))
+    bc = np.concatenate(bc)
+    mesh = Mesh(nblk=nblk, ni=np.full(nblk, n, np.int32), nj=np.full(nblk, n, np.int32),
+                xnode=X.reshape(-1), ynode=Y.reshape(-1), nbr=nbr, bc=bc, wo=np.zeros((bc.size, 4)),
+                info={"kind": "quadtree", "nroot": nroot, "n": n, "max_level": max_level, "jitter": jitter,
+                      "seed": seed, "box": (x0, x1, y0, y1), "keys": order},
+                level=level, nbr2=nbr2)
+    return mesh
+
+
+def circle_refinement(radius=0.3, centre=(0.5, 0.5), band=0.0):
+    """Refinement test of quadtree_box: split blocks whose square comes within
+    band of the circle."""
+    cx, cy = centre
+
+    def refine(lv, xa, xb, ya, yb):
+        dx = max(xa - cx, 0.0, cx - xb)
+        dy = max(ya - cy, 0.0, cy - yb)
+        dmin = np.hypot(dx, dy)
+        dmax = max(np.hypot(xa - cx, ya - cy), np.hypot(xb - cx, ya - cy), np.hypot(xa - cx, yb - cy),
+                   np.hypot(xb - cx, yb - cy))
+        return dmin - band <= radius <= dmax + band
+    return refine
+
+
+def cylindrical_explosion(mesh, radius=0.3, centre=(0.5, 0.5), inside=(8.0, 0.0, 0.0, 10.0),
+                          outside=(1.0, 0.0, 0.0, 1.0)):
+    """Pressure ratio 10 and density ratio 8 across a circle, at rest: the
+    states of the reference's IC_CYLINDRICAL_EXPLOSION
+    (Euler2DQuadSingleBlock.cc:1978-1990), with the radius as a parameter."""
+    c = mesh.cell_centres()
+    r = np.hypot(c[:, 0] - centre[0], c[:, 1] - centre[1])
+    W = np.empty((mesh.ncell, 4))
+    W[r <= radius] = inside
+    W[r > radius] = outside
+    return W
+
+
 def concat(meshes):
     """One multi-block mesh holding several disjoint meshes (cases): block
     indices of each part are offset so that neighbour tables stay inside the
@@ -197,6 +509,14 @@ def concat(meshes):
                xnode=np.concatenate([m.xnode for m in meshes]), ynode=np.concatenate([m.ynode for m in meshes]),
                nbr=np.concatenate(nbrs), bc=np.concatenate([m.bc for m in meshes]),
                wo=np.concatenate([m.wo for m in meshes]), info={"kind": "concat", "parts": len(meshes)})
+    if any(m.level is not None for m in meshes):
+        out.level = np.concatenate([m.level if m.level is not None else np.zeros(m.nblk, np.int32)
+                                    for m in meshes]).astype(np.int32)
+        n2 = []
+        for m, s in zip(meshes, starts):
+            t = np.full((m.nblk, 4), -1, np.int64) if m.nbr2 is None else np.array(m.nbr2, dtype=np.int64)
+            n2.append(np.where(t >= 0, t + s, -1).astype(np.int32))
+        out.nbr2 = np.concatenate(n2)
     return out, starts
 
 

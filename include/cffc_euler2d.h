@@ -91,6 +91,18 @@ enum { CFFC_NBR_N = 0, CFFC_NBR_S, CFFC_NBR_E, CFFC_NBR_W,
  *                 BCtypeN[nci] (ghost entries included).
  *   wo          : the reference state (rho,u,v,p) for each bc entry, same
  *                 order, 4 doubles per entry (used by FIXED/CHARACTERISTIC).
+ *   level       : quadtree refinement level per block (info.level), or NULL
+ *                 for a single-level mesh.
+ *   nbr2        : 4 ints per block (faces N, S, E, W), or NULL: the second
+ *                 neighbour of a face that has two finer neighbours (nX == 2,
+ *                 infoX[1]); nbr then holds the first (infoX[0]: the west one
+ *                 of an N/S face, the south one of an E/W face); else -1.
+ * Across a resolution change (neighbours one level apart, on faces or
+ * corners) the blocks have equal, even ni and nj; the fine side's ghosts are
+ * prolonged from the coarse block, the coarse side's restricted from the fine
+ * blocks, and the coarse face fluxes corrected to the fine ones
+ * (AdaptiveBlock2D_MessagePassing.h:1185-2398, 2399-5679, 6284-7882, 8328).
+ * Blocks next to a resolution change may not carry physical boundaries.
  */
 typedef struct cffc_mesh_desc {
     int32_t        nblk;
@@ -102,6 +114,8 @@ typedef struct cffc_mesh_desc {
     const int32_t *nbr;
     const int32_t *bc;
     const double  *wo;
+    const int32_t *level;
+    const int32_t *nbr2;
 } cffc_mesh_desc;
 
 /* The scheme fields of Euler2D_Input_Parameters read on this path

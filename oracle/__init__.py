@@ -64,6 +64,9 @@ def lib():
         L.orc_get_block_W_padded.argtypes = [vp, ctypes.c_int, pd]
         L.orc_get_block_geometry.argtypes = [vp, ctypes.c_int, pd, pd, pd]
         L.orc_exchange_and_bcs.argtypes = [vp]
+        L.orc_apply_flux_corrections.argtypes = [vp, ctypes.c_int]
+        L.orc_prime_ghosts.argtypes = [vp]
+        L.orc_get_block_fluxes.argtypes = [vp, ctypes.c_int, pd]
         L.orc_cfl.argtypes = [vp, pd, ctypes.c_int]
         L.orc_dudt_stage.argtypes = [vp, ctypes.c_int]
         L.orc_update_stage.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
@@ -229,6 +232,21 @@ class Oracle:
 
     def exchange_and_bcs(self):
         return lib().orc_exchange_and_bcs(self.h)
+
+    def prime_ghosts(self):
+        """Rounds of messages + BCs until every ghost derives from the interior."""
+        return lib().orc_prime_ghosts(self.h)
+
+    def apply_flux_corrections(self, i_stage):
+        return lib().orc_apply_flux_corrections(self.h, i_stage)
+
+    def block_fluxes(self, b):
+        """Saved boundary fluxes of block b: dict N, S ([ni,4]), E, W ([nj,4])."""
+        ni, nj = int(self.mesh.ni[b]), int(self.mesh.nj[b])
+        out = np.zeros(4 * (2 * ni + 2 * nj))
+        lib().orc_get_block_fluxes(self.h, b, _p(out))
+        o = out.reshape(-1, 4)
+        return {"N": o[:ni], "S": o[ni:2 * ni], "E": o[2 * ni:2 * ni + nj], "W": o[2 * ni + nj:]}
 
     def cfl(self, set_global=True):
         d = ctypes.c_double()

@@ -353,6 +353,9 @@ typedef struct {
     double *dt;
     int nbr[8];
     int freeze_limiter;
+    int level;             /* AdaptiveBlock2D_List Block[].info.level */
+    int nbr2[4];           /* second neighbour of a face with two finer neighbours (infoX[1]) */
+    cst *FluxN, *FluxS, *FluxE, *FluxW; /* block-edge fluxes, Euler2DQuad.h:140-205 */
 } oblk;
 
 typedef struct orc_ctx {
@@ -361,6 +364,8 @@ typedef struct orc_ctx {
     cffc_scheme_desc s;
     int64_t ncell;
     double *xnode, *ynode; /* owned copies */
+    int amr;               /* some neighbours differ in level */
+    int max_level;
     double rs_epsilon;     /* Residual_Smoothing_Epsilon */
     int rs_iterations;     /* Residual_Smoothing_Gauss_Seidel_Iterations (0: off) */
 } orc_ctx;
@@ -617,7 +622,7 @@ static void exchange_block(oblk *all, int b) {
     int Ng = B->ng;
     for (int slot = 0; slot < 8; ++slot) {
         int nb = B->nbr[slot];
-        if (nb < 0) continue;
+        if (nb < 0 || all[nb].level != B->level) continue;
         const oblk *S = &all[nb];
         int i0, i1, j0, j1; /* destination ranges in B */
         int di, dj;         /* source index = dest index + (di,dj) */
@@ -797,6 +802,14 @@ static inline void acc_add(cst *d, double cdt, cst F, double l, double A) {
 
 static int is_face_bc(int bc) { return bc == CFFC_BC_REFLECTION || bc == CFFC_BC_CHARACTERISTIC; }
 
+/* sign*Flux*l as the reference writes the saved boundary fluxes: FluxE/N =
+   Flux*lface, FluxW/S = -Flux*lface (the unary minus first, exact) */
+static inline cst flux_times(cst F, double sign, double l) {
+    cst r;
+    r.d = (sign * F.d) * l; r.mx = (sign * F.mx) * l; r.my = (sign * F.my) * l; r.E = (sign * F.E) * l;
+    return r;
+}
+
 /* Euler2D_Quad_Block::dUdt_Multistage_Explicit, Euler2DQuad.cc:1235-1727 */
 static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
     int ICl = B->ICl, ICu = B->ICu, JCl = B->JCl, JCu = B->JCu, Ng = B->ng;
@@ -859,6 +872,9 @@ static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
                 cst F = flux_roe_n(Wl, Wr, nx, ny);
                 acc_sub(&B->dUdt[k][cL], CFL * B->dt[cL], F, lfaceE(B, i, j), B->A[cL]);
                 acc_add(&B->dUdt[k][cR], CFL * B->dt[cR], F, lfaceW(B, i + 1, j), B->A[cR]);
+                /* west and east boundary fluxes :1518-1523 */
+                if (i == ICl - 1) B->FluxW[j - JCl] = flux_times(F, -ONE, lfaceW(B, i + 1, j));
+                else if (i == ICu) B->FluxE[j - JCl] = flux_times(F, ONE, lfaceE(B, i, j));
             }
         }
         if (j > JCl - 1 && j < JCu + 1) {
@@ -894,9 +910,298 @@ static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
             cst F = flux_roe_n(Wl, Wr, nx, ny);
             acc_sub(&B->dUdt[k][cL], CFL * B->dt[cL], F, lfaceN(B, i, j), B->A[cL]);
             acc_add(&B->dUdt[k][cR], CFL * B->dt[cR], F, lfaceS(B, i, j + 1), B->A[cR]);
+            /* south and north boundary fluxes :1709-1715 */
+            if (j == JCl - 1) B->FluxS[i - ICl] = flux_times(F, -ONE, lfaceS(B, i, j + 1));
+            else if (j == JCu) B->FluxN[i - ICl] = flux_times(F, ONE, lfaceN(B, i, j));
         }
         B->dUdt[k][CI(B, i, JCl - 1)] = Z;
         B->dUdt[k][CI(B, i, JCu + 1)] = Z;
+    }
+}
+
+/* ====================================================================== */
+/* Resolution changes (quadtree AMR): Send_All_Messages' fine-to-coarse and */
+/* coarse-to-fine legs (AdaptiveBlock2D_MessagePassing.h:1185, 2399, 6284,  */
+/* 7140, in the order of :7883-8020) and the conservative flux correction   */
+/* (Send_Conservative_Flux_Corrections :8328, Apply_Boundary_Flux_          */
+/* Corrections_Multistage_Explicit Euler2DQuadSingleBlock.cc:4992-5080).    */
+/* Neighbours share orientation and, across a resolution change, the cell   */
+/* counts ni, nj (both even).  A face with two finer neighbours lists them  */
+/* in nbr[f] (the west / south one, infoX[0]) and nbr2[f] (infoX[1]).       */
+/* ====================================================================== */
+
+/* sector of fine block b on the face of its coarser neighbour nbr[f]:
+   0 = west/south half (infoX[0] of the coarse block), 1 = east/north half */
+static int face_sector(const oblk *all, int b, int f) {
+    const oblk *C = &all[all[b].nbr[f]];
+    if (C->nbr[f ^ 1] == b) return 0;
+    if (C->nbr2[f ^ 1] == b) return 1;
+    return -1;
+}
+static inline int floor_half(int k) { return k >= 0 ? k / 2 : -((1 - k) / 2); }
+
+static void set_ghost_U(oblk *B, int i, int j, cst U) {
+    B->U[CI(B, i, j)] = U;
+    B->W[CI(B, i, j)] = W_of_U(U);
+}
+
+/* LoadSendBuffer_F2C, Euler2DQuad.h:1597-1624: area-weighted average of the
+   fine cells (i,j), (i+1,j), (i,j+1), (i+1,j+1); the coarse ghost cell then
+   takes U and W = W(U) (UnloadReceiveBuffer_F2C :1658-1684) */
+static cst restrict4(const oblk *F, int i, int j) {
+    int c00 = CI(F, i, j), c10 = CI(F, i + 1, j), c01 = CI(F, i, j + 1), c11 = CI(F, i + 1, j + 1);
+    double A00 = F->A[c00], A10 = F->A[c10], A01 = F->A[c01], A11 = F->A[c11];
+    double den = A00 + A10 + A01 + A11;
+    cst u00 = F->U[c00], u10 = F->U[c10], u01 = F->U[c01], u11 = F->U[c11], r;
+    r.d = (A00 * u00.d + A10 * u10.d + A01 * u01.d + A11 * u11.d) / den;
+    r.mx = (A00 * u00.mx + A10 * u10.mx + A01 * u01.mx + A11 * u11.mx) / den;
+    r.my = (A00 * u00.my + A10 * u10.my + A01 * u01.my + A11 * u11.my) / den;
+    r.E = (A00 * u00.E + A10 * u10.E + A01 * u01.E + A11 * u11.E) / den;
+    return r;
+}
+
+/* fine-to-coarse leg for coarse block b: its ghost cells facing finer
+   neighbours (faces: two blocks, each half the face; corners: one block)
+   take the restriction of the fine cells they cover */
+static void f2c_block(oblk *all, int b) {
+    oblk *B = &all[b];
+    int ICl = B->ICl, ICu = B->ICu, JCl = B->JCl, JCu = B->JCu;
+    int hi = B->ni / 2, hj = B->nj / 2;
+    for (int f = 0; f < 4; ++f) {
+        int nb = B->nbr[f];
+        if (nb < 0 || all[nb].level != B->level + 1) continue;
+        for (int g = 1; g <= 2; ++g) {
+            if (f == CFFC_NBR_N || f == CFFC_NBR_S) {
+                for (int i = ICl; i <= ICu; ++i) {
+                    int k = i - ICl;
+                    const oblk *F = &all[k < hi ? B->nbr[f] : B->nbr2[f]];
+                    int i_f = F->ICl + 2 * (k % hi);
+                    int j_f = (f == CFFC_NBR_N) ? F->JCl + 2 * (g - 1) : F->JCu - 2 * g + 1;
+                    set_ghost_U(B, i, (f == CFFC_NBR_N) ? JCu + g : JCl - g, restrict4(F, i_f, j_f));
+                }
+            } else {
+                for (int j = JCl; j <= JCu; ++j) {
+                    int k = j - JCl;
+                    const oblk *F = &all[k < hj ? B->nbr[f] : B->nbr2[f]];
+                    int j_f = F->JCl + 2 * (k % hj);
+                    int i_f = (f == CFFC_NBR_E) ? F->ICl + 2 * (g - 1) : F->ICu - 2 * g + 1;
+                    set_ghost_U(B, (f == CFFC_NBR_E) ? ICu + g : ICl - g, j, restrict4(F, i_f, j_f));
+                }
+            }
+        }
+    }
+    for (int s = CFFC_NBR_NE; s <= CFFC_NBR_SW; ++s) {
+        int nb = B->nbr[s];
+        if (nb < 0 || all[nb].level != B->level + 1) continue;
+        const oblk *F = &all[nb];
+        int east = (s == CFFC_NBR_NE || s == CFFC_NBR_SE), north = (s == CFFC_NBR_NE || s == CFFC_NBR_NW);
+        for (int gj = 1; gj <= 2; ++gj)
+            for (int gi = 1; gi <= 2; ++gi) {
+                int i_f = east ? F->ICl + 2 * (gi - 1) : F->ICu - 2 * gi + 1;
+                int j_f = north ? F->JCl + 2 * (gj - 1) : F->JCu - 2 * gj + 1;
+                set_ghost_U(B, east ? ICu + gi : ICl - gi, north ? JCu + gj : JCl - gj, restrict4(F, i_f, j_f));
+            }
+    }
+}
+
+/* quadCentroid (Math.h:311-350) of the vertices (x[q], y[q]), q = SW, SE, NE, NW */
+static void quad_centroid_v(const double *x, const double *y, double *xc, double *yc) {
+    double ai, atmp = 0, xtmp = 0, ytmp = 0;
+    ai = x[3] * y[0] - x[0] * y[3]; atmp += ai; xtmp += (x[0] + x[3]) * ai; ytmp += (y[0] + y[3]) * ai;
+    ai = x[0] * y[1] - x[1] * y[0]; atmp += ai; xtmp += (x[1] + x[0]) * ai; ytmp += (y[1] + y[0]) * ai;
+    ai = x[1] * y[2] - x[2] * y[1]; atmp += ai; xtmp += (x[2] + x[1]) * ai; ytmp += (y[2] + y[1]) * ai;
+    ai = x[2] * y[3] - x[3] * y[2]; atmp += ai; xtmp += (x[3] + x[2]) * ai; ytmp += (y[3] + y[2]) * ai;
+    if (atmp != 0) { *xc = xtmp / (3 * atmp); *yc = ytmp / (3 * atmp); }
+    else { *xc = 0; *yc = 0; }
+}
+
+/* Grid2D_Quad_Block_HO::centroidSW/SE/NW/NE (HO_Grid2DQuad.h:1486-1660):
+   centroid of quarter q (0 SW, 1 SE, 2 NW, 3 NE) of cell (i, j), the
+   quarter spanned by a corner node, the two adjacent edge midpoints and
+   the average of the four nodes */
+static void subcell_centroid(const oblk *C, int i, int j, int q, double *xc, double *yc) {
+    double x00, y00, x10, y10, x01, y01, x11, y11;
+    nodeX(C, i, j, &x00, &y00); nodeX(C, i + 1, j, &x10, &y10);
+    nodeX(C, i, j + 1, &x01, &y01); nodeX(C, i + 1, j + 1, &x11, &y11);
+    double cx = 0.25 * (x00 + x10 + x01 + x11), cy = 0.25 * (y00 + y10 + y01 + y11);
+    double sx = HALF * (x00 + x10), sy = HALF * (y00 + y10);   /* south edge midpoint */
+    double ex = HALF * (x10 + x11), ey = HALF * (y10 + y11);   /* east */
+    double nx = HALF * (x01 + x11), ny = HALF * (y01 + y11);   /* north */
+    double wx = HALF * (x00 + x01), wy = HALF * (y00 + y01);   /* west */
+    double X[4], Y[4];
+    switch (q) {
+    case 0: X[0] = x00; Y[0] = y00; X[1] = sx; Y[1] = sy; X[2] = cx; Y[2] = cy; X[3] = wx; Y[3] = wy; break;
+    case 1: X[0] = sx; Y[0] = sy; X[1] = x10; Y[1] = y10; X[2] = ex; Y[2] = ey; X[3] = cx; Y[3] = cy; break;
+    case 2: X[0] = wx; Y[0] = wy; X[1] = cx; Y[1] = cy; X[2] = nx; Y[2] = ny; X[3] = x01; Y[3] = y01; break;
+    default: X[0] = cx; Y[0] = cy; X[1] = ex; Y[1] = ey; X[2] = x11; Y[2] = y11; X[3] = nx; Y[3] = ny; break;
+    }
+    quad_centroid_v(X, Y, xc, yc);
+}
+
+/* SubcellReconstruction (Euler2DQuad.h:1721-1950) of an interior cell (i, j)
+   with LIMITER_VENKATAKRISHNAN and the 8-point stencil, into (gx, gy, ph)
+   rather than the block's arrays (the main reconstruction of the stage
+   recomputes the same values there); frozen limiters keep the stored phi */
+static void subcell_reconstruct(const oblk *C, int i, int j, pst *gx, pst *gy, pst *ph) {
+    static const int di[8] = {-1, 0, 1, -1, 1, -1, 0, 1}, dj[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+    int c = CI(C, i, j);
+    pst W0 = C->W[c];
+    pst DUDx = {0, 0, 0, 0}, DUDy = {0, 0, 0, 0};
+    double DxDx = ZERO, DxDy = ZERO, DyDy = ZERO;
+    for (int n2 = 0; n2 < 8; ++n2) {
+        int cn = CI(C, i + di[n2], j + dj[n2]);
+        double dXx = C->xc[cn] - C->xc[c], dXy = C->yc[cn] - C->yc[c];
+        pst Wn = C->W[cn];
+        pst DU = {Wn.d - W0.d, Wn.vx - W0.vx, Wn.vy - W0.vy, Wn.p - W0.p};
+        DUDx.d += DU.d * dXx; DUDx.vx += DU.vx * dXx; DUDx.vy += DU.vy * dXx; DUDx.p += DU.p * dXx;
+        DUDy.d += DU.d * dXy; DUDy.vx += DU.vx * dXy; DUDy.vy += DU.vy * dXy; DUDy.p += DU.p * dXy;
+        DxDx += dXx * dXx; DxDy += dXx * dXy; DyDy += dXy * dXy;
+    }
+    double np = 8.0;
+    DUDx.d /= np; DUDx.vx /= np; DUDx.vy /= np; DUDx.p /= np;
+    DUDy.d /= np; DUDy.vx /= np; DUDy.vy /= np; DUDy.p /= np;
+    DxDx /= np; DxDy /= np; DyDy /= np;
+    double det = DxDx * DyDy - DxDy * DxDy;
+    gx->d = (DUDx.d * DyDy - DUDy.d * DxDy) / det;
+    gx->vx = (DUDx.vx * DyDy - DUDy.vx * DxDy) / det;
+    gx->vy = (DUDx.vy * DyDy - DUDy.vy * DxDy) / det;
+    gx->p = (DUDx.p * DyDy - DUDy.p * DxDy) / det;
+    gy->d = (DUDy.d * DxDx - DUDx.d * DxDy) / det;
+    gy->vx = (DUDy.vx * DxDx - DUDx.vx * DxDy) / det;
+    gy->vy = (DUDy.vy * DxDx - DUDx.vy * DxDy) / det;
+    gy->p = (DUDy.p * DxDx - DUDx.p * DxDy) / det;
+    if (C->freeze_limiter) {
+        *ph = C->phi[c];
+        return;
+    }
+    double xE, yE, xW, yW, xN, yN, xS, yS;
+    xfaceE(C, i, j, &xE, &yE); xfaceW(C, i, j, &xW, &yW);
+    xfaceN(C, i, j, &xN, &yN); xfaceS(C, i, j, &xS, &yS);
+    double xc0 = C->xc[c], yc0 = C->yc[c];
+    for (int n = 1; n <= 4; ++n) {
+        double u0 = pget(&W0, n), u0Min = u0, u0Max = u0;
+        for (int n2 = 0; n2 < 8; ++n2) {
+            double v = pget(&C->W[CI(C, i + di[n2], j + dj[n2])], n);
+            u0Min = dmin(u0Min, v);
+            u0Max = dmax(u0Max, v);
+        }
+        double gxn = pget(gx, n), gyn = pget(gy, n), uQuad[4];
+        uQuad[0] = u0 + gxn * (xE - xc0) + gyn * (yE - yc0);
+        uQuad[1] = u0 + gxn * (xW - xc0) + gyn * (yW - yc0);
+        uQuad[2] = u0 + gxn * (xN - xc0) + gyn * (yN - yc0);
+        uQuad[3] = u0 + gxn * (xS - xc0) + gyn * (yS - yc0);
+        *pref(ph, n) = limiter_venkatakrishnan(uQuad, u0, u0Min, u0Max, 4);
+    }
+}
+
+/* LoadSendBuffer_C2F (Euler2DQuad.cc:266-360): the limited linear state of
+   coarse cell (i, j) at the centroid of its quarter q, sent as U; the fine
+   ghost cell takes U and W = W(U) (UnloadReceiveBuffer_C2F Euler2DQuad.h:1691) */
+static cst prolong(const oblk *C, int i, int j, int q, const pst *gx, const pst *gy, const pst *ph) {
+    double xq, yq;
+    subcell_centroid(C, i, j, q, &xq, &yq);
+    int c = CI(C, i, j);
+    double dXx = xq - C->xc[c], dXy = yq - C->yc[c];
+    pst W = C->W[c], r;
+    r.d = W.d + (ph->d * gx->d) * dXx + (ph->d * gy->d) * dXy;
+    r.vx = W.vx + (ph->vx * gx->vx) * dXx + (ph->vx * gy->vx) * dXy;
+    r.vy = W.vy + (ph->vy * gx->vy) * dXx + (ph->vy * gy->vy) * dXy;
+    r.p = W.p + (ph->p * gx->p) * dXx + (ph->p * gy->p) * dXy;
+    return U_of_W(r);
+}
+
+/* coarse-to-fine leg for fine block b.  A face with a coarser neighbour
+   receives the quarters of ni/2 + 1 (nj/2 + 1) coarse cells: its own span
+   plus the corner over the middle of the coarse face
+   (Load_Send_Message_Buffers_ResChange_CoarseToFine, :2399-2470); other
+   corners with a coarser corner neighbour receive the quarters of that
+   block's facing corner cell. */
+static void c2f_block(oblk *all, int b) {
+    oblk *B = &all[b];
+    int ICl = B->ICl, ICu = B->ICu, JCl = B->JCl, JCu = B->JCu;
+    int covered[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    pst gx, gy, ph;
+    for (int f = 0; f < 4; ++f) {
+        int nb = B->nbr[f];
+        if (nb < 0 || all[nb].level != B->level - 1) continue;
+        const oblk *C = &all[nb];
+        int sig = face_sector(all, b, f);
+        if (f == CFFC_NBR_N || f == CFFC_NBR_S) {
+            int i0 = sig == 0 ? ICl : ICl - 2, i1 = sig == 0 ? ICu + 2 : ICu;
+            int jc = (f == CFFC_NBR_N) ? C->JCl : C->JCu;
+            int ic_prev = -1000;
+            for (int i = i0; i <= i1; ++i) {
+                int k = i - ICl, kf = floor_half(k), qx = k - 2 * kf;
+                int ic = C->ICl + sig * (B->ni / 2) + kf;
+                if (ic != ic_prev) { subcell_reconstruct(C, ic, jc, &gx, &gy, &ph); ic_prev = ic; }
+                for (int g = 1; g <= 2; ++g) {
+                    int qy = (f == CFFC_NBR_N) ? g - 1 : 2 - g;
+                    set_ghost_U(B, i, (f == CFFC_NBR_N) ? JCu + g : JCl - g, prolong(C, ic, jc, 2 * qy + qx, &gx, &gy, &ph));
+                }
+            }
+            if (f == CFFC_NBR_N) covered[sig == 0 ? CFFC_NBR_NE : CFFC_NBR_NW] = 1;
+            else covered[sig == 0 ? CFFC_NBR_SE : CFFC_NBR_SW] = 1;
+        } else {
+            int j0 = sig == 0 ? JCl : JCl - 2, j1 = sig == 0 ? JCu + 2 : JCu;
+            int ic = (f == CFFC_NBR_E) ? C->ICl : C->ICu;
+            int jc_prev = -1000;
+            for (int j = j0; j <= j1; ++j) {
+                int k = j - JCl, kf = floor_half(k), qy = k - 2 * kf;
+                int jc = C->JCl + sig * (B->nj / 2) + kf;
+                if (jc != jc_prev) { subcell_reconstruct(C, ic, jc, &gx, &gy, &ph); jc_prev = jc; }
+                for (int g = 1; g <= 2; ++g) {
+                    int qx = (f == CFFC_NBR_E) ? g - 1 : 2 - g;
+                    set_ghost_U(B, (f == CFFC_NBR_E) ? ICu + g : ICl - g, j, prolong(C, ic, jc, 2 * qy + qx, &gx, &gy, &ph));
+                }
+            }
+            if (f == CFFC_NBR_E) covered[sig == 0 ? CFFC_NBR_NE : CFFC_NBR_SE] = 1;
+            else covered[sig == 0 ? CFFC_NBR_NW : CFFC_NBR_SW] = 1;
+        }
+    }
+    for (int s = CFFC_NBR_NE; s <= CFFC_NBR_SW; ++s) {
+        int nb = B->nbr[s];
+        if (covered[s] || nb < 0 || all[nb].level != B->level - 1) continue;
+        const oblk *D = &all[nb];
+        int east = (s == CFFC_NBR_NE || s == CFFC_NBR_SE), north = (s == CFFC_NBR_NE || s == CFFC_NBR_NW);
+        int ic = east ? D->ICl : D->ICu, jc = north ? D->JCl : D->JCu;
+        int ig0 = east ? ICu + 1 : ICl - 2, jg0 = north ? JCu + 1 : JCl - 2;
+        subcell_reconstruct(D, ic, jc, &gx, &gy, &ph);
+        for (int qy = 0; qy < 2; ++qy)
+            for (int qx = 0; qx < 2; ++qx) set_ghost_U(B, ig0 + qx, jg0 + qy, prolong(D, ic, jc, 2 * qy + qx, &gx, &gy, &ph));
+    }
+}
+
+/* Send_Conservative_Flux_Corrections + Apply_Boundary_Flux_Corrections_
+   Multistage_Explicit for coarse block b: on a face with two finer
+   neighbours, the saved coarse flux becomes -(sum of the two fine fluxes
+   it covers) - own (UnloadReceiveBuffer_Flux_F2C Euler2DQuad.h:2120-2170,
+   fine pairs from LoadSendBuffer_Flux_F2C :2064-2112), and
+   dUdt[k] -= (CFL*dt)*Flux/A on the face's cells, faces in the order N, S, E, W */
+static void flux_correction_block(oblk *all, int b, const cffc_scheme_desc *s, int i_stage) {
+    oblk *B = &all[b];
+    int k = stage_k_residual(s, i_stage);
+    double CFL = s->cfl_number;
+    int hi = B->ni / 2, hj = B->nj / 2;
+    for (int f = 0; f < 4; ++f) {
+        int nb = B->nbr[f];
+        if (nb < 0 || all[nb].level != B->level + 1) continue;
+        int n = (f == CFFC_NBR_N || f == CFFC_NBR_S) ? B->ni : B->nj, h = (n == B->ni) ? hi : hj;
+        for (int q = 0; q < n; ++q) {
+            const oblk *F = &all[q < h ? B->nbr[f] : B->nbr2[f]];
+            int qf = 2 * (q % h);
+            const cst *fine = f == CFFC_NBR_N ? F->FluxS : f == CFFC_NBR_S ? F->FluxN : f == CFFC_NBR_E ? F->FluxW : F->FluxE;
+            cst *own = f == CFFC_NBR_N ? B->FluxN : f == CFFC_NBR_S ? B->FluxS : f == CFFC_NBR_E ? B->FluxE : B->FluxW;
+            cst a = fine[qf], c2 = fine[qf + 1], o = own[q], r;
+            r.d = -(a.d + c2.d) - o.d; r.mx = -(a.mx + c2.mx) - o.mx;
+            r.my = -(a.my + c2.my) - o.my; r.E = -(a.E + c2.E) - o.E;
+            own[q] = r;
+            int i = (f == CFFC_NBR_N || f == CFFC_NBR_S) ? B->ICl + q : (f == CFFC_NBR_E ? B->ICu : B->ICl);
+            int j = (f == CFFC_NBR_E || f == CFFC_NBR_W) ? B->JCl + q : (f == CFFC_NBR_N ? B->JCu : B->JCl);
+            int c = CI(B, i, j);
+            double cdt = CFL * B->dt[c], A = B->A[c];
+            cst *d = &B->dUdt[k][c];
+            d->d -= cdt * r.d / A; d->mx -= cdt * r.mx / A; d->my -= cdt * r.my / A; d->E -= cdt * r.E / A;
+        }
     }
 }
 
@@ -980,6 +1285,7 @@ static void blk_free(oblk *B) {
     free(B->W); free(B->dWdx); free(B->dWdy); free(B->phi);
     free(B->U); free(B->Uo); free(B->dUdt[0]); free(B->dUdt[1]); free(B->dUdt[2]);
     free(B->dt);
+    free(B->FluxN); free(B->FluxS); free(B->FluxE); free(B->FluxW);
 }
 
 void orc_destroy(orc_ctx *ctx) {
@@ -1043,13 +1349,23 @@ int orc_create(const cffc_mesh_desc *m, const cffc_scheme_desc *s, orc_ctx **out
             boff += len[f];
         }
         for (int q = 0; q < 8; ++q) B->nbr[q] = m->nbr[8 * b + q];
+        B->level = m->level ? m->level[b] : 0;
+        for (int q = 0; q < 4; ++q) B->nbr2[q] = m->nbr2 ? m->nbr2[4 * b + q] : -1;
+        B->FluxN = (cst *)calloc(B->ni, sizeof(cst)); B->FluxS = (cst *)calloc(B->ni, sizeof(cst));
+        B->FluxE = (cst *)calloc(B->nj, sizeof(cst)); B->FluxW = (cst *)calloc(B->nj, sizeof(cst));
         B->W = (pst *)calloc(nc, sizeof(pst)); B->dWdx = (pst *)calloc(nc, sizeof(pst));
         B->dWdy = (pst *)calloc(nc, sizeof(pst)); B->phi = (pst *)calloc(nc, sizeof(pst));
         B->U = (cst *)calloc(nc, sizeof(cst)); B->Uo = (cst *)calloc(nc, sizeof(cst));
         for (int q = 0; q < 3; ++q) B->dUdt[q] = (cst *)calloc(nc, sizeof(cst));
         B->dt = (double *)calloc(nc, sizeof(double));
         ctx->ncell += (int64_t)B->ni * B->nj;
+        if (B->level > ctx->max_level) ctx->max_level = B->level;
     }
+    for (int b = 0; b < m->nblk; ++b)
+        for (int q = 0; q < 8; ++q) {
+            int nb = ctx->b[b].nbr[q];
+            if (nb >= 0 && ctx->b[nb].level != ctx->b[b].level) ctx->amr = 1;
+        }
     *out = ctx;
     return 0;
 }
@@ -1140,7 +1456,35 @@ typedef struct {
 
 int orc_exchange_and_bcs(orc_ctx *ctx) {
     for (int b = 0; b < ctx->nblk; ++b) exchange_block(ctx->b, b);
+    for (int b = 0; b < ctx->nblk; ++b) f2c_block(ctx->b, b);
+    for (int b = 0; b < ctx->nblk; ++b) c2f_block(ctx->b, b);
     for (int b = 0; b < ctx->nblk; ++b) block_bcs(&ctx->b[b]);
+    return 0;
+}
+/* Ghost cells of a new interior state on a mesh with resolution changes.
+   The coarse-to-fine leg reads coarse ghosts that the physical BCs fill
+   after it, whose corner rules in turn read ghosts from that leg one level
+   coarser; max_level + 2 rounds of messages + BCs leave every ghost a
+   function of the interior state alone (on one level, one round). */
+int orc_prime_ghosts(orc_ctx *ctx) {
+    int rounds = ctx->amr ? ctx->max_level + 2 : 1;
+    for (int r = 0; r < rounds; ++r) orc_exchange_and_bcs(ctx);
+    return rounds;
+}
+/* == Send_Conservative_Flux_Corrections + Apply_Boundary_Flux_Corrections_
+      Multistage_Explicit (Euler2DQuadMultiBlock.cc:1307) */
+int orc_apply_flux_corrections(orc_ctx *ctx, int i_stage) {
+    for (int b = 0; b < ctx->nblk; ++b) flux_correction_block(ctx->b, b, &ctx->s, i_stage);
+    return 0;
+}
+/* the saved boundary fluxes FluxN/S/E/W of one block (4 doubles each, in
+   that order: ni, ni, nj, nj entries) */
+int orc_get_block_fluxes(orc_ctx *ctx, int b, double *out) {
+    oblk *B = &ctx->b[b];
+    memcpy(out, B->FluxN, sizeof(cst) * B->ni); out += 4 * B->ni;
+    memcpy(out, B->FluxS, sizeof(cst) * B->ni); out += 4 * B->ni;
+    memcpy(out, B->FluxE, sizeof(cst) * B->nj); out += 4 * B->nj;
+    memcpy(out, B->FluxW, sizeof(cst) * B->nj);
     return 0;
 }
 int orc_cfl(orc_ctx *ctx, double *dt_min, int set_global) {
@@ -1295,10 +1639,22 @@ static void *worker(void *arg) {
         if (n == w->nsteps) break;
         if (w->tid == 0 && w->dt_out) w->dt_out[n] = dTime;
         for (int st = 1; st <= s->n_stage; ++st) {
-            for (int b = b0; b < b1; ++b) exchange_block(ctx->b, b);
+            /* same-level and fine-to-coarse legs read interior cells only */
+            for (int b = b0; b < b1; ++b) { exchange_block(ctx->b, b); f2c_block(ctx->b, b); }
             pthread_barrier_wait(w->bar);
+            if (ctx->amr) {
+                /* coarse-to-fine leg: reads the coarse blocks' cells and ghosts
+                   (those of the physical BCs from the previous BCs call) */
+                for (int b = b0; b < b1; ++b) c2f_block(ctx->b, b);
+                pthread_barrier_wait(w->bar);
+            }
             for (int b = b0; b < b1; ++b) block_bcs(&ctx->b[b]);
             for (int b = b0; b < b1; ++b) block_dudt(&ctx->b[b], s, st);
+            if (ctx->amr) {
+                /* every block's saved fluxes precede the corrections */
+                pthread_barrier_wait(w->bar);
+                for (int b = b0; b < b1; ++b) flux_correction_block(ctx->b, b, s, st);
+            }
             /* implicit residual smoothing, Euler2DQuadSolvers.cc:861-867 */
             if (ctx->rs_iterations > 0)
                 for (int b = b0; b < b1; ++b)
@@ -1333,6 +1689,7 @@ int orc_run(orc_ctx *ctx, int nsteps, double tol, int nthreads, double *norms, d
     pthread_t *th = calloc(nthreads, sizeof(pthread_t));
     int done = 0;
     int64_t bad = 0;
+    if (ctx->amr) orc_prime_ghosts(ctx);
     for (int t = 0; t < nthreads; ++t) {
         worker_t x = {ctx, nthreads, t, nsteps, tol, &bar, dtm, l1, l2, mx, errs, norms, dt_out, &done, &bad};
         ws[t] = x;
