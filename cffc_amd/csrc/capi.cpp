@@ -83,6 +83,22 @@ struct cffc_ctx {
     bool push_ok = false;
     int nbc = 0;
     int32_t *bc_blocks = nullptr;
+    // resolution changes (quadtree meshes): level-change legs of the exchange
+    // and the conservative flux correction, as operation lists
+    bool amr = false;
+    int max_level = 0;
+    std::vector<uint8_t> fine_faces;   // per block: bit f = face f has two finer neighbours
+    RestrictOp *rops = nullptr;
+    ProlongOp *pops = nullptr;
+    FluxFixOp *fops = nullptr;
+    int nrops = 0, npops = 0, nfops = 0;
+    double *eflux = nullptr;           // saved block-edge fluxes
+    double *Ustage = nullptr;          // fused: U^(s) of cells a coarser neighbour restricts
+    uint8_t *ustore = nullptr;         // per padded cell: that cell is such a source
+    double *Sd = nullptr;              // fused: residual of deferred tiles
+    uint8_t *tile_defer = nullptr;
+    int32_t *defer_list = nullptr;
+    int ndefer = 0;
     double rs_eps = 0.0; // Residual_Smoothing_Epsilon
     int rs_iters = 0;    // Residual_Smoothing_Gauss_Seidel_Iterations (0: no smoothing)
     int freeze = 0;      // limiters frozen (PHI_LOAD)
@@ -105,7 +121,8 @@ static void free_ctx(cffc_ctx *c) {
                     c->dUdt[1], c->dUdt[2], c->dt, c->phi, c->tiles, c->scal, c->norm_part, c->norm_small,
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
                     c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock,
-                    c->bc_blocks, c->Wf[0], c->Wf[1], c->chunk_part};
+                    c->bc_blocks, c->Wf[0], c->Wf[1], c->chunk_part, c->rops, c->pops, c->fops, c->eflux,
+                    c->Ustage, c->ustore, c->Sd, c->tile_defer, c->defer_list};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -146,6 +163,34 @@ static int build_tiles(cffc_ctx *c, int ti, int tj) {
         c->chunk_part = nullptr;
     }
     HIPCHK(hipMemcpy(c->tiles, t.data(), t.size() * sizeof(TileDesc), hipMemcpyHostToDevice));
+    if (c->amr) {
+        // tiles touching a face with finer neighbours update after the flux
+        // correction of that face
+        std::vector<uint8_t> defer(t.size(), 0);
+        std::vector<int32_t> list;
+        for (size_t q = 0; q < t.size(); ++q) {
+            const TileDesc &d = t[q];
+            const BlockMeta &m = c->meta_h[d.b];
+            const int ff = c->fine_faces[d.b];
+            const bool hit = ((ff & 1) && d.j0 + tj >= m.nj) || ((ff & 2) && d.j0 == 0) ||
+                             ((ff & 4) && d.i0 + ti >= m.ni) || ((ff & 8) && d.i0 == 0);
+            if (hit) {
+                defer[q] = 1;
+                list.push_back((int32_t)q);
+            }
+        }
+        if (c->tile_defer) (void)hipFree(c->tile_defer);
+        if (c->defer_list) (void)hipFree(c->defer_list);
+        c->tile_defer = nullptr;
+        c->defer_list = nullptr;
+        c->ndefer = (int)list.size();
+        if (int e = dalloc(&c->tile_defer, t.size())) return e;
+        HIPCHK(hipMemcpy(c->tile_defer, defer.data(), t.size(), hipMemcpyHostToDevice));
+        if (!list.empty()) {
+            if (int e = dalloc(&c->defer_list, list.size())) return e;
+            HIPCHK(hipMemcpy(c->defer_list, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+        }
+    }
     if (c->npart) {
         tile0.push_back((int32_t)t.size());
         if (int e = dalloc(&c->part_tile0, tile0.size())) return e;
@@ -234,6 +279,334 @@ static int fused_mode(const cffc_scheme_desc &s, int i) {
     return MODE_ZERO;
 }
 
+
+// ---- resolution changes -------------------------------------------------------
+// quadCentroid (Math.h:311-350) of SW, SE, NE, NW and the centroid of quarter
+// q (0 SW, 1 SE, 2 NW, 3 NE) of a cell (Grid2D_Quad_Block_HO::centroidSW/SE/
+// NW/NE, HO_Grid2DQuad.h:1486-1660), in the reference's operation order
+#pragma clang fp contract(off)
+static void host_quad_centroid(const double *x, const double *y, double *xc, double *yc) {
+    double ai, atmp = 0, xtmp = 0, ytmp = 0;
+    ai = x[3] * y[0] - x[0] * y[3]; atmp += ai; xtmp += (x[0] + x[3]) * ai; ytmp += (y[0] + y[3]) * ai;
+    ai = x[0] * y[1] - x[1] * y[0]; atmp += ai; xtmp += (x[1] + x[0]) * ai; ytmp += (y[1] + y[0]) * ai;
+    ai = x[1] * y[2] - x[2] * y[1]; atmp += ai; xtmp += (x[2] + x[1]) * ai; ytmp += (y[2] + y[1]) * ai;
+    ai = x[2] * y[3] - x[3] * y[2]; atmp += ai; xtmp += (x[3] + x[2]) * ai; ytmp += (y[3] + y[2]) * ai;
+    if (atmp != 0) { *xc = xtmp / (3 * atmp); *yc = ytmp / (3 * atmp); }
+    else { *xc = 0; *yc = 0; }
+}
+static void host_quarter_centroid(const cffc_mesh_desc *md, const BlockMeta &m, int i, int j, int q, double *xc,
+                                  double *yc) {
+    const int nn = m.nci + 1;
+    auto X = [&](int a, int b) { return md->xnode[m.noff + (int64_t)b * nn + a]; };
+    auto Y = [&](int a, int b) { return md->ynode[m.noff + (int64_t)b * nn + a]; };
+    const double x00 = X(i, j), y00 = Y(i, j), x10 = X(i + 1, j), y10 = Y(i + 1, j);
+    const double x01 = X(i, j + 1), y01 = Y(i, j + 1), x11 = X(i + 1, j + 1), y11 = Y(i + 1, j + 1);
+    const double cx = 0.25 * (x00 + x10 + x01 + x11), cy = 0.25 * (y00 + y10 + y01 + y11);
+    const double sx = 0.5 * (x00 + x10), sy = 0.5 * (y00 + y10), ex = 0.5 * (x10 + x11), ey = 0.5 * (y10 + y11);
+    const double nx = 0.5 * (x01 + x11), ny = 0.5 * (y01 + y11), wx = 0.5 * (x00 + x01), wy = 0.5 * (y00 + y01);
+    double Xq[4], Yq[4];
+    switch (q) {
+    case 0: Xq[0] = x00; Yq[0] = y00; Xq[1] = sx; Yq[1] = sy; Xq[2] = cx; Yq[2] = cy; Xq[3] = wx; Yq[3] = wy; break;
+    case 1: Xq[0] = sx; Yq[0] = sy; Xq[1] = x10; Yq[1] = y10; Xq[2] = ex; Yq[2] = ey; Xq[3] = cx; Yq[3] = cy; break;
+    case 2: Xq[0] = wx; Yq[0] = wy; Xq[1] = cx; Yq[1] = cy; Xq[2] = nx; Yq[2] = ny; Xq[3] = x01; Yq[3] = y01; break;
+    default: Xq[0] = cx; Yq[0] = cy; Xq[1] = ex; Yq[1] = ey; Xq[2] = x11; Yq[2] = y11; Xq[3] = nx; Yq[3] = ny; break;
+    }
+    host_quad_centroid(Xq, Yq, xc, yc);
+}
+#pragma clang fp contract(on)
+
+static int floor_half(int k) { return k >= 0 ? k / 2 : -((1 - k) / 2); }
+
+// Levels, second face neighbours and the same-level table of the push /
+// pull exchange; for a mesh with resolution changes, the restriction,
+// prolongation and flux-correction lists, the per-block AMR flags and the
+// edge-flux offsets.  Mirrors the reference's message rules
+// (AdaptiveBlock2D_MessagePassing.h:1185-2398, 2399-5679, 8328).
+static int setup_levels(cffc_ctx *c, const cffc_mesh_desc *md, std::vector<int32_t> &same, std::vector<RestrictOp> &rops,
+                        std::vector<ProlongOp> &pops, std::vector<FluxFixOp> &fops, std::vector<uint8_t> &ustore,
+                        int64_t &nentries) {
+    const int nb = md->nblk;
+    std::vector<int32_t> lev(nb, 0), n2(4 * (size_t)nb, -1);
+    if (md->level)
+        for (int b = 0; b < nb; ++b) {
+            lev[b] = md->level[b];
+            if (lev[b] < 0 || lev[b] > 30) return fail(CFFC_E_ARG, "block %d: level %d out of range", b, lev[b]);
+        }
+    if (md->nbr2)
+        for (int q = 0; q < 4 * nb; ++q) {
+            n2[q] = md->nbr2[q];
+            if (n2[q] < -1 || n2[q] >= nb) return fail(CFFC_E_TOPOLOGY, "nbr2 entry %d out of range", q);
+        }
+    same.assign(md->nbr, md->nbr + 8 * (size_t)nb);
+    c->amr = false;
+    c->max_level = 0;
+    c->fine_faces.assign(nb, 0);
+    for (int b = 0; b < nb; ++b) {
+        c->max_level = std::max(c->max_level, lev[b]);
+        for (int s = 0; s < 8; ++s) {
+            const int o = md->nbr[8 * b + s];
+            if (o < 0) continue;
+            const int d = lev[o] - lev[b];
+            if (d == 0) {
+                if (s < 4 && n2[4 * b + s] >= 0)
+                    return fail(CFFC_E_TOPOLOGY, "block %d face %d: second neighbour on a same-level face", b, s);
+                continue;
+            }
+            if (d != 1 && d != -1) return fail(CFFC_E_TOPOLOGY, "blocks %d and %d differ by %d levels", b, o, d);
+            const BlockMeta &m = c->meta_h[b], &n = c->meta_h[o];
+            if (m.ni != n.ni || m.nj != n.nj || (m.ni & 1) || (m.nj & 1) || m.ni < 4 || m.nj < 4)
+                return fail(CFFC_E_TOPOLOGY, "blocks %d and %d: a resolution change needs equal even ni, nj >= 4", b, o);
+            same[8 * b + s] = -1;
+            c->amr = true;
+        }
+    }
+    if (!c->amr) return 0;
+    if (c->s.n_stage < 1) return fail(CFFC_E_ARG, "bad scheme");
+    static const int oppf[4] = {1, 0, 3, 2};
+    auto cell = [&](int b, int i, int j) { const BlockMeta &m = c->meta_h[b]; return m.coff + (int64_t)j * m.pitch + i; };
+    // sector of fine block b on its coarser face neighbour nbr[f]
+    auto sector = [&](int b, int f) -> int {
+        const int C = md->nbr[8 * b + f];
+        if (md->nbr[8 * C + oppf[f]] == b) return 0;
+        if (n2[4 * C + oppf[f]] == b) return 1;
+        return -1;
+    };
+    std::vector<int32_t> amr(nb, 0);
+    for (int b = 0; b < nb; ++b)
+        for (int f = 0; f < 4; ++f) {
+            const int o = md->nbr[8 * b + f];
+            if (o < 0 || lev[o] == lev[b]) continue;
+            if (lev[o] == lev[b] + 1) {
+                const int o2 = n2[4 * b + f];
+                if (o2 < 0 || lev[o2] != lev[b] + 1 || md->nbr[8 * o + oppf[f]] != b || md->nbr[8 * o2 + oppf[f]] != b)
+                    return fail(CFFC_E_TOPOLOGY, "block %d face %d: finer neighbours %d, %d do not both face it", b, f, o, o2);
+                c->fine_faces[b] |= (uint8_t)(1 << f);
+                amr[b] |= AMR_FLUX_N << f;
+                amr[o] |= AMR_FLUX_N << oppf[f];
+                amr[o2] |= AMR_FLUX_N << oppf[f];
+            } else if (sector(b, f) < 0) {
+                return fail(CFFC_E_TOPOLOGY, "block %d face %d: coarser neighbour %d does not list it", b, f, o);
+            }
+        }
+    // edge-flux entries: N [ni], S [ni], E [nj], W [nj] per flagged block
+    nentries = 0;
+    for (int b = 0; b < nb; ++b) {
+        BlockMeta &m = c->meta_h[b];
+        m.amr = amr[b];
+        m.eoff = 0;
+        if (amr[b] & AMR_FLUX_ANY) {
+            m.eoff = nentries;
+            nentries += 2 * (int64_t)m.ni + 2 * (int64_t)m.nj;
+        }
+    }
+    ustore.assign((size_t)c->ncp, 0);
+    auto src4 = [&](int F, int i, int j) {
+        const BlockMeta &m = c->meta_h[F];
+        const int64_t s0 = cell(F, i, j);
+        ustore[s0] = ustore[s0 + 1] = ustore[s0 + m.pitch] = ustore[s0 + m.pitch + 1] = 1;
+        c->meta_h[F].amr |= AMR_USTORE;
+        return s0;
+    };
+    for (int b = 0; b < nb; ++b) {
+        const BlockMeta m = c->meta_h[b];
+        const int ICl = 2, ICu = m.ni + 1, JCl = 2, JCu = m.nj + 1, hi = m.ni / 2, hj = m.nj / 2;
+        // restriction into b's ghosts from finer faces and corners
+        for (int f = 0; f < 4; ++f) {
+            const int o = md->nbr[8 * b + f];
+            if (o < 0 || lev[o] != lev[b] + 1) continue;
+            for (int g = 1; g <= 2; ++g) {
+                if (f < 2) {
+                    for (int i = ICl; i <= ICu; ++i) {
+                        const int k = i - ICl, F = k < hi ? o : n2[4 * b + f];
+                        const int i_f = 2 + 2 * (k % hi);
+                        const int j_f = f == 0 ? 2 + 2 * (g - 1) : c->meta_h[F].nj + 2 - 2 * g;
+                        rops.push_back(RestrictOp{cell(b, i, f == 0 ? JCu + g : JCl - g), src4(F, i_f, j_f),
+                                                  c->meta_h[F].pitch, 0});
+                    }
+                } else {
+                    for (int j = JCl; j <= JCu; ++j) {
+                        const int k = j - JCl, F = k < hj ? o : n2[4 * b + f];
+                        const int j_f = 2 + 2 * (k % hj);
+                        const int i_f = f == 2 ? 2 + 2 * (g - 1) : c->meta_h[F].ni + 2 - 2 * g;
+                        rops.push_back(RestrictOp{cell(b, f == 2 ? ICu + g : ICl - g, j), src4(F, i_f, j_f),
+                                                  c->meta_h[F].pitch, 0});
+                    }
+                }
+            }
+        }
+        for (int s = 4; s < 8; ++s) {
+            const int F = md->nbr[8 * b + s];
+            if (F < 0 || lev[F] != lev[b] + 1) continue;
+            const bool east = s == 4 || s == 6, north = s == 4 || s == 5;
+            const BlockMeta &fm = c->meta_h[F];
+            for (int gj = 1; gj <= 2; ++gj)
+                for (int gi = 1; gi <= 2; ++gi) {
+                    const int i_f = east ? 2 + 2 * (gi - 1) : fm.ni + 2 - 2 * gi;
+                    const int j_f = north ? 2 + 2 * (gj - 1) : fm.nj + 2 - 2 * gj;
+                    rops.push_back(RestrictOp{cell(b, east ? ICu + gi : ICl - gi, north ? JCu + gj : JCl - gj),
+                                              src4(F, i_f, j_f), fm.pitch, 0});
+                }
+        }
+        // prolongation into b's ghosts from coarser faces (with the corner over
+        // the middle of the coarse face) and the remaining coarser corners
+        bool covered[8] = {false, false, false, false, false, false, false, false};
+        int cover_blk[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+        for (int f = 0; f < 4; ++f) {
+            const int C = md->nbr[8 * b + f];
+            if (C < 0 || lev[C] != lev[b] - 1) continue;
+            const int sig = sector(b, f);
+            const BlockMeta &cm = c->meta_h[C];
+            const int half = f < 2 ? m.ni / 2 : m.nj / 2;
+            const int k0 = sig == 0 ? 0 : -1, k1 = sig == 0 ? half : half - 1;
+            for (int kf = k0; kf <= k1; ++kf) {
+                ProlongOp op{};
+                int ic, jc;
+                if (f < 2) { ic = 2 + sig * half + kf; jc = f == 0 ? 2 : cm.nj + 1; }
+                else { ic = f == 2 ? 2 : cm.ni + 1; jc = 2 + sig * half + kf; }
+                op.src = cell(C, ic, jc);
+                op.blk = C;
+                for (int q = 0; q < 4; ++q) {
+                    const int qx = q & 1, qy = q >> 1;
+                    int i, j;
+                    if (f < 2) { i = ICl + 2 * kf + qx; j = f == 0 ? JCu + 1 + qy : JCl - 2 + qy; }
+                    else { i = f == 2 ? ICu + 1 + qx : ICl - 2 + qx; j = JCl + 2 * kf + qy; }
+                    op.dst[q] = cell(b, i, j);
+                    host_quarter_centroid(md, cm, ic, jc, q, &op.xq[q], &op.yq[q]);
+                }
+                pops.push_back(op);
+            }
+            static const int cov[4][2] = {{4, 5}, {6, 7}, {4, 6}, {5, 7}};
+            covered[cov[f][sig]] = true;
+            cover_blk[cov[f][sig]] = C;
+        }
+        for (int s = 4; s < 8; ++s) {
+            const int D = md->nbr[8 * b + s];
+            if (covered[s]) {
+                if (D >= 0 && D != cover_blk[s])
+                    return fail(CFFC_E_TOPOLOGY, "block %d corner %d: lists block %d inside coarser block %d", b, s, D,
+                                cover_blk[s]);
+                same[8 * b + s] = -1;
+                continue;
+            }
+            if (D < 0 || lev[D] != lev[b] - 1) continue;
+            const bool east = s == 4 || s == 6, north = s == 4 || s == 5;
+            const BlockMeta &dm = c->meta_h[D];
+            const int ic = east ? 2 : dm.ni + 1, jc = north ? 2 : dm.nj + 1;
+            const int i0 = east ? ICu + 1 : ICl - 2, j0 = north ? JCu + 1 : JCl - 2;
+            ProlongOp op{};
+            op.src = cell(D, ic, jc);
+            op.blk = D;
+            for (int q = 0; q < 4; ++q) {
+                op.dst[q] = cell(b, i0 + (q & 1), j0 + (q >> 1));
+                host_quarter_centroid(md, dm, ic, jc, q, &op.xq[q], &op.yq[q]);
+            }
+            pops.push_back(op);
+        }
+        // flux corrections of b's faces with finer neighbours, N, S, E, W
+        std::vector<std::pair<int64_t, int>> at;   // cell -> op index of this block
+        for (int f = 0; f < 4; ++f) {
+            if (!(c->fine_faces[b] & (1 << f))) continue;
+            const int n = f < 2 ? m.ni : m.nj, h = n / 2;
+            static const int face_off_n[4] = {0, 1, 2, 2};   // N, S, E, W offsets: 0, ni, 2ni, 2ni+nj
+            for (int q = 0; q < n; ++q) {
+                const int F = q < h ? md->nbr[8 * b + f] : n2[4 * b + f];
+                const BlockMeta &fm = c->meta_h[F];
+                auto off = [&](const BlockMeta &x, int face) -> int64_t {
+                    return x.eoff + (int64_t)face_off_n[face] * x.ni + (face == 3 ? x.nj : 0);
+                };
+                const int32_t own = (int32_t)(off(m, f) + q);
+                const int32_t fine = (int32_t)(off(fm, oppf[f]) + 2 * (q % h));
+                int64_t cc;
+                if (f == 0) cc = cell(b, 2 + q, JCu);
+                else if (f == 1) cc = cell(b, 2 + q, JCl);
+                else if (f == 2) cc = cell(b, ICu, 2 + q);
+                else cc = cell(b, ICl, 2 + q);
+                int idx = -1;
+                for (auto &pr : at)
+                    if (pr.first == cc) idx = pr.second;
+                if (idx < 0) {
+                    idx = (int)fops.size();
+                    fops.push_back(FluxFixOp{cc, 0, 0, {0, 0}, {0, 0}});
+                    at.push_back({cc, idx});
+                }
+                FluxFixOp &op = fops[idx];
+                op.own[op.n] = own;
+                op.fine[op.n] = fine;
+                op.n += 1;
+            }
+        }
+    }
+    if (nentries > INT32_MAX) return fail(CFFC_E_ARG, "too many edge-flux entries");
+    return 0;
+}
+
+// Block metadata (padded layout, offsets) and the same-level topology and BC
+// checks; host only.
+static int build_meta(cffc_ctx *c, const cffc_mesh_desc *md, int32_t &bcoff, int64_t &noff) {
+    int64_t coff = 0, ioff = 0;
+    noff = 0;
+    bcoff = 0;
+    c->meta_h.resize(md->nblk);
+    for (int b = 0; b < md->nblk; ++b) {
+        BlockMeta &m = c->meta_h[b];
+        m.ni = md->ni[b];
+        m.nj = md->nj[b];
+        if (m.ni < 2 || m.nj < 2 || m.ni > 4096 || m.nj > 4096) {
+            return fail(CFFC_E_ARG, "block %d: ni, nj must be in [2, 4096]", b);
+        }
+        m.nci = m.ni + 4;
+        m.ncj = m.nj + 4;
+        // rows pitch cells apart (a multiple of 16 doubles = 128 bytes) and
+        // the block placed so that interior cell (2, j) starts a line
+        m.pitch = (m.nci + 15) & ~15;
+        m.spare = 0;
+        m.coff = ((coff + 15) & ~(int64_t)15) + 14;
+        m.ioff = ioff;
+        m.noff = noff;
+        m.bcoff = bcoff;
+        m.has_bc = 0;
+        m.amr = 0;
+        m.pad = 0;
+        m.eoff = 0;
+        for (int q = 0; q < 2 * (m.ni + 4) + 2 * (m.nj + 4); ++q)
+            if (md->bc[bcoff + q] != CFFC_BC_NONE) m.has_bc = 1;
+        coff = m.coff + (int64_t)m.pitch * m.ncj;
+        ioff += (int64_t)m.ni * m.nj;
+        noff += (int64_t)(m.nci + 1) * (m.ncj + 1);
+        bcoff += 2 * m.ncj + 2 * m.nci;
+        c->max_int = std::max(c->max_int, m.ni * m.nj);
+        c->max_cells = std::max(c->max_cells, m.nci * m.ncj);
+    }
+    for (int b = 0; b < md->nblk; ++b) {
+        const BlockMeta &m = c->meta_h[b];
+        for (int s = 0; s < 8; ++s) {
+            const int nb = md->nbr[8 * b + s];
+            if (nb < -1 || nb >= md->nblk) {
+                return fail(CFFC_E_TOPOLOGY, "block %d slot %d: neighbour %d out of range", b, s, nb);
+            }
+            if (nb < 0) continue;
+            const BlockMeta &n = c->meta_h[nb];
+            if ((s == CFFC_NBR_N || s == CFFC_NBR_S) && n.ni != m.ni) {
+                return fail(CFFC_E_TOPOLOGY, "block %d: N/S neighbour %d has different ni", b, nb);
+            }
+            if ((s == CFFC_NBR_E || s == CFFC_NBR_W) && n.nj != m.nj) {
+                return fail(CFFC_E_TOPOLOGY, "block %d: E/W neighbour %d has different nj", b, nb);
+            }
+        }
+        const int32_t *bcb = md->bc + m.bcoff;
+        for (int q = 0; q < 2 * m.ncj + 2 * m.nci; ++q) {
+            const int v = bcb[q];
+            if (v != CFFC_BC_NONE && v != CFFC_BC_FIXED && v != CFFC_BC_CONSTANT_EXTRAPOLATION &&
+                v != CFFC_BC_REFLECTION && v != CFFC_BC_PERIODIC && v != CFFC_BC_CHARACTERISTIC) {
+                return fail(CFFC_E_ARG, "block %d: unsupported boundary condition %d", b, v);
+            }
+        }
+    }
+    c->ncp = (coff + 15) & ~(int64_t)15;   // every SoA plane starts 128-byte aligned
+    c->ncell = ioff;
+    return 0;
+}
+
 // ---------------------------------------------------------------------------
 extern "C" const char *cffc_last_error(void) { return g_err.c_str(); }
 
@@ -253,65 +626,23 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
         return fail(CFFC_E_HIP, "hipStreamCreate failed");
     }
     // block metadata and topology checks
-    int64_t coff = 0, ioff = 0, noff = 0;
     int32_t bcoff = 0;
-    c->meta_h.resize(md->nblk);
-    for (int b = 0; b < md->nblk; ++b) {
-        BlockMeta &m = c->meta_h[b];
-        m.ni = md->ni[b];
-        m.nj = md->nj[b];
-        if (m.ni < 2 || m.nj < 2 || m.ni > 4096 || m.nj > 4096) {
-            free_ctx(c);
-            return fail(CFFC_E_ARG, "block %d: ni, nj must be in [2, 4096]", b);
-        }
-        m.nci = m.ni + 4;
-        m.ncj = m.nj + 4;
-        // rows pitch cells apart (a multiple of 16 doubles = 128 bytes) and
-        // the block placed so that interior cell (2, j) starts a line
-        m.pitch = (m.nci + 15) & ~15;
-        m.spare = 0;
-        m.coff = ((coff + 15) & ~(int64_t)15) + 14;
-        m.ioff = ioff;
-        m.noff = noff;
-        m.bcoff = bcoff;
-        m.has_bc = 0;
-        for (int q = 0; q < 2 * (m.ni + 4) + 2 * (m.nj + 4); ++q)
-            if (md->bc[bcoff + q] != CFFC_BC_NONE) m.has_bc = 1;
-        coff = m.coff + (int64_t)m.pitch * m.ncj;
-        ioff += (int64_t)m.ni * m.nj;
-        noff += (int64_t)(m.nci + 1) * (m.ncj + 1);
-        bcoff += 2 * m.ncj + 2 * m.nci;
-        c->max_int = std::max(c->max_int, m.ni * m.nj);
-        c->max_cells = std::max(c->max_cells, m.nci * m.ncj);
+    int64_t noff = 0;
+    if (int ea = build_meta(c, md, bcoff, noff)) {
+        free_ctx(c);
+        return ea;
     }
-    for (int b = 0; b < md->nblk; ++b) {
-        const BlockMeta &m = c->meta_h[b];
-        for (int s = 0; s < 8; ++s) {
-            const int nb = md->nbr[8 * b + s];
-            if (nb < -1 || nb >= md->nblk) {
-                free_ctx(c);
-                return fail(CFFC_E_TOPOLOGY, "block %d slot %d: neighbour %d out of range", b, s, nb);
-            }
-            if (nb < 0) continue;
-            const BlockMeta &n = c->meta_h[nb];
-            if ((s == CFFC_NBR_N || s == CFFC_NBR_S) && n.ni != m.ni) {
-                free_ctx(c);
-                return fail(CFFC_E_TOPOLOGY, "block %d: N/S neighbour %d has different ni", b, nb);
-            }
-            if ((s == CFFC_NBR_E || s == CFFC_NBR_W) && n.nj != m.nj) {
-                free_ctx(c);
-                return fail(CFFC_E_TOPOLOGY, "block %d: E/W neighbour %d has different nj", b, nb);
-            }
-        }
-        const int32_t *bcb = md->bc + m.bcoff;
-        for (int q = 0; q < 2 * m.ncj + 2 * m.nci; ++q) {
-            const int v = bcb[q];
-            if (v != CFFC_BC_NONE && v != CFFC_BC_FIXED && v != CFFC_BC_CONSTANT_EXTRAPOLATION &&
-                v != CFFC_BC_REFLECTION && v != CFFC_BC_PERIODIC && v != CFFC_BC_CHARACTERISTIC) {
-                free_ctx(c);
-                return fail(CFFC_E_ARG, "block %d: unsupported boundary condition %d", b, v);
-            }
-        }
+    // levels: the same-level table (push / pull exchange) and, across
+    // resolution changes, the restriction / prolongation / correction lists
+    std::vector<int32_t> same;
+    std::vector<RestrictOp> rops;
+    std::vector<ProlongOp> pops;
+    std::vector<FluxFixOp> fops;
+    std::vector<uint8_t> ustore;
+    int64_t nentries = 0;
+    if (int ea = setup_levels(c, md, same, rops, pops, fops, ustore, nentries)) {
+        free_ctx(c);
+        return ea;
     }
     {
         // the push exchange needs every same-level link to be mutual
@@ -320,12 +651,16 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
         std::vector<int32_t> bl;
         for (int b = 0; b < md->nblk; ++b) {
             for (int s = 0; s < 8; ++s) {
-                const int nb = md->nbr[8 * b + s];
-                if (nb >= 0 && md->nbr[8 * nb + opp[s]] != b) sym = false;
+                const int nb = same[8 * b + s];
+                if (nb >= 0 && same[8 * nb + opp[s]] != b) sym = false;
             }
             if (c->meta_h[b].has_bc) bl.push_back(b);
         }
         c->push_ok = sym;
+        if (c->amr && !sym) {
+            free_ctx(c);
+            return fail(CFFC_E_TOPOLOGY, "a mesh with resolution changes needs mutual same-level neighbours");
+        }
         c->nbc = (int)bl.size();
         if (!bl.empty()) {
             if (int ea = dalloc(&c->bc_blocks, bl.size())) {
@@ -335,8 +670,6 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
             HIPCHK(hipMemcpy(c->bc_blocks, bl.data(), bl.size() * sizeof(int32_t), hipMemcpyHostToDevice));
         }
     }
-    c->ncp = (coff + 15) & ~(int64_t)15;   // every SoA plane starts 128-byte aligned
-    c->ncell = ioff;
     const int64_t ncp = c->ncp;
     int e = 0;
     if ((e = dalloc(&c->meta, md->nblk)) || (e = dalloc(&c->nbr, 8 * (size_t)md->nblk)) ||
@@ -350,7 +683,28 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
         return e;
     }
     HIPCHK(hipMemcpy(c->meta, c->meta_h.data(), md->nblk * sizeof(BlockMeta), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(c->nbr, md->nbr, 8 * md->nblk * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->nbr, same.data(), 8 * md->nblk * sizeof(int32_t), hipMemcpyHostToDevice));
+    if (c->amr) {
+        c->nrops = (int)rops.size();
+        c->npops = (int)pops.size();
+        c->nfops = (int)fops.size();
+        if ((c->nrops && (e = dalloc(&c->rops, rops.size()))) || (c->npops && (e = dalloc(&c->pops, pops.size()))) ||
+            (c->nfops && (e = dalloc(&c->fops, fops.size()))) || (e = dalloc(&c->eflux, 4 * (size_t)nentries + 4)) ||
+            (e = dalloc(&c->ustore, (size_t)ncp)) || (e = dalloc(&c->Ustage, 4 * (size_t)ncp)) ||
+            (e = dalloc(&c->Sd, 4 * (size_t)ncp))) {
+            free_ctx(c);
+            return e;
+        }
+        if (c->nrops) HIPCHK(hipMemcpy(c->rops, rops.data(), rops.size() * sizeof(RestrictOp), hipMemcpyHostToDevice));
+        if (c->npops) HIPCHK(hipMemcpy(c->pops, pops.data(), pops.size() * sizeof(ProlongOp), hipMemcpyHostToDevice));
+        if (c->nfops) HIPCHK(hipMemcpy(c->fops, fops.data(), fops.size() * sizeof(FluxFixOp), hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(c->ustore, ustore.data(), (size_t)ncp, hipMemcpyHostToDevice));
+        HIPCHK(hipMemset(c->eflux, 0, (4 * (size_t)nentries + 4) * sizeof(double)));
+        HIPCHK(hipMemset(c->Ustage, 0, 4 * (size_t)ncp * sizeof(double)));
+        HIPCHK(hipMemset(c->Sd, 0, 4 * (size_t)ncp * sizeof(double)));
+        // meta_h carries the AMR flags and edge-flux offsets set above
+        HIPCHK(hipMemcpy(c->meta, c->meta_h.data(), md->nblk * sizeof(BlockMeta), hipMemcpyHostToDevice));
+    }
     HIPCHK(hipMemcpy(c->bc, md->bc, bcoff * sizeof(int32_t), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->wo, md->wo, 4 * (size_t)bcoff * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(c->W[0], 0, 4 * ncp * sizeof(double)));
@@ -407,6 +761,43 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
     }
     *out = c;
     return 0;
+}
+
+extern "C" int cffc_check_mesh(const cffc_mesh_desc *md, int64_t counts[4]) {
+    if (!md) return fail(CFFC_E_ARG, "null argument");
+    if (md->nblk <= 0 || md->nghost != 2) return fail(CFFC_E_ARG, "nblk must be > 0 and nghost == 2");
+    cffc_ctx *c = new cffc_ctx();
+    c->nblk = md->nblk;
+    c->s.n_stage = 1;
+    int32_t bcoff = 0;
+    int64_t noff = 0;
+    std::vector<int32_t> same;
+    std::vector<RestrictOp> rops;
+    std::vector<ProlongOp> pops;
+    std::vector<FluxFixOp> fops;
+    std::vector<uint8_t> ustore;
+    int64_t nentries = 0;
+    int e = build_meta(c, md, bcoff, noff);
+    if (!e) e = setup_levels(c, md, same, rops, pops, fops, ustore, nentries);
+    if (!e && c->amr) {
+        static const int opp[8] = {1, 0, 3, 2, 7, 6, 5, 4};
+        for (int b = 0; b < md->nblk && !e; ++b)
+            for (int s = 0; s < 8; ++s) {
+                const int nb = same[8 * b + s];
+                if (nb >= 0 && same[8 * nb + opp[s]] != b) {
+                    e = fail(CFFC_E_TOPOLOGY, "a mesh with resolution changes needs mutual same-level neighbours");
+                    break;
+                }
+            }
+    }
+    if (!e && counts) {
+        counts[0] = (int64_t)rops.size();
+        counts[1] = (int64_t)pops.size();
+        counts[2] = (int64_t)fops.size();
+        counts[3] = c->amr ? 1 : 0;
+    }
+    delete c;
+    return e;
 }
 
 extern "C" void cffc_destroy(cffc_ctx *c) {
@@ -488,8 +879,24 @@ static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long lon
     p.reset_slot = reset_slot;
     p.blocks = nullptr;
     p.bcs_only = 0;
+    p.no_bcs = 0;
     p.Wf = nullptr;
     return p;
+}
+
+static int ensure_phi(cffc_ctx *c);
+
+// The resolution-change legs of Send_All_Messages on the W buffer (W or Wf)
+// that the next stage reads: restriction from U (the state U^(s) of the
+// restricted fine cells), then prolongation, which reads the ghosts the
+// restriction filled.
+static int amr_legs(cffc_ctx *c, const double *U, double *W, float *Wf, const int32_t *part_state,
+                    const int32_t *stop_flag) {
+    if (!c->amr) return 0;
+    HIPCHK(cffc_launch_restrict(c->rops, c->nrops, U, c->geom, c->ncp, Wf ? nullptr : W, Wf, stop_flag, c->stream));
+    HIPCHK(cffc_launch_prolong(c->pops, c->npops, c->meta, c->geom, c->phi, c->freeze, part_state, c->ncp,
+                               Wf ? nullptr : W, Wf, stop_flag, c->stream));
+    return 0;
 }
 
 // ---- granular entry points (one reference call each) -----------------------
@@ -537,7 +944,32 @@ extern "C" int cffc_exchange_and_bcs(cffc_ctx *c) {
     if (!c) return fail(CFFC_E_ARG, "null context");
     // the granular path keeps U current, so ghosts are always W(U)
     ExchangeParams p = exchange_params(c, 1, nullptr);
-    HIPCHK(cffc_launch_exchange(&p, c->nblk, c->stream));
+    if (c->amr) {
+        // same-level, fine-to-coarse and coarse-to-fine legs, then the BCs
+        if (int e = ensure_phi(c)) return e;
+        p.no_bcs = 1;
+        HIPCHK(cffc_launch_exchange(&p, c->nblk, c->stream));
+        if (int e = amr_legs(c, c->U, c->W[c->wcur], nullptr, nullptr, nullptr)) return e;
+        ExchangeParams q = exchange_params(c, 1, nullptr);
+        q.bcs_only = 1;
+        q.blocks = c->bc_blocks;
+        if (c->nbc) HIPCHK(cffc_launch_exchange(&q, c->nbc, c->stream));
+    } else {
+        HIPCHK(cffc_launch_exchange(&p, c->nblk, c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int cffc_apply_flux_corrections(cffc_ctx *c, int i_stage) {
+    if (!c) return fail(CFFC_E_ARG, "null context");
+    if (i_stage < 1 || i_stage > c->s.n_stage) return fail(CFFC_E_ARG, "bad stage %d", i_stage);
+    if (!c->amr || c->nfops == 0) return 0;
+    if (int e = ensure_granular(c)) return e;
+    const int k = k_residual(c->s, i_stage);
+    // the reference reads the cell's dt array (global stepping filled it)
+    HIPCHK(cffc_launch_flux_fix(c->fops, c->nfops, c->dUdt[k], c->eflux, c->geom, c->dt, c->scal, 1,
+                                c->s.cfl_number, c->ncp, nullptr, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     return 0;
 }
@@ -580,6 +1012,8 @@ static StageParams base_stage_params(cffc_ctx *c) {
     p.nbr = c->nbr;
     p.push = 0;
     p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE;
+    p.eflux = c->eflux;
+    p.ustore = c->ustore;
     return p;
 }
 
@@ -709,6 +1143,12 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         ex.stop_flag = c->stop_flag;
         ex.Wf = c->Wf[c->wcur];
         mark(c, 0, true);
+        // messages across resolution changes (the same-level ones were pushed
+        // by the previous stage), then the BCs: Euler2DQuadSolvers.cc:807-826.
+        // The restricted fine cells hold U^n at stage 1, U^(s-1) after.
+        if (int e = amr_legs(c, st == 1 ? c->U : c->Ustage, c->W[c->wcur], c->Wf[c->wcur],
+                             parts ? c->part_state : nullptr, c->stop_flag))
+            return e;
         if (c->push_ok) {
             // same-level ghosts were pushed by the previous stage kernel (or
             // filled by begin_run); only the physical BCs remain
@@ -739,16 +1179,39 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         else if (c->store_phi && st == N) p.phi_mode = PHI_COMPUTE_STORE;
         else p.phi_mode = PHI_COMPUTE;
         mark(c, 1, true);
+        p.Ustage = c->Ustage;
+        const bool fp32 = c->s.flux_precision == CFFC_FLUX_FP32;
         if (c->rs_iters > 0) {
-            // residual, implicit residual smoothing, update (Euler2DQuadSolvers.cc:855-877)
+            // residual, flux corrections, implicit residual smoothing, update
+            // (Euler2DQuadSolvers.cc:841-877)
             p.defer_update = 1;
+            p.Sd = c->dUdt[0];
             HIPCHK(launch_stage(c, p, 1));
+            if (c->nfops)
+                HIPCHK(cffc_launch_flux_fix(c->fops, c->nfops, c->dUdt[0], c->eflux, c->geom, c->dt, c->scal + par,
+                                            p.lts, c->s.cfl_number, c->ncp, c->stop_flag, c->stream));
             HIPCHK(cffc_launch_residual_smoothing(c->meta, c->nblk, c->dUdt[0], c->dUdt[1], c->ncp, c->rs_eps,
                                                   c->rs_iters, parts ? c->part_state : nullptr, c->stop_flag,
                                                   c->stream));
             p.defer_update = 0;
-            if (c->s.flux_precision == CFFC_FLUX_FP32) HIPCHK(cffc_launch_update_tile_mixed(&p, c->ti, c->tj, c->stream));
+            if (fp32) HIPCHK(cffc_launch_update_tile_mixed(&p, c->ti, c->tj, c->stream));
             else HIPCHK(cffc_launch_update_tile_fp64(&p, c->ti, c->tj, c->stream));
+        } else if (c->amr) {
+            // tiles next to a finer face store their residual; the coarse face
+            // fluxes are corrected (Euler2DQuadSolvers.cc:841-853), then those
+            // tiles update
+            p.tile_defer = c->tile_defer;
+            p.Sd = c->Sd;
+            HIPCHK(launch_stage(c, p, 1));
+            if (c->nfops)
+                HIPCHK(cffc_launch_flux_fix(c->fops, c->nfops, c->Sd, c->eflux, c->geom, c->dt, c->scal + par, p.lts,
+                                            c->s.cfl_number, c->ncp, c->stop_flag, c->stream));
+            if (c->ndefer) {
+                p.tile_list = c->defer_list;
+                p.nlist = c->ndefer;
+                if (fp32) HIPCHK(cffc_launch_update_tile_mixed(&p, c->ti, c->tj, c->stream));
+                else HIPCHK(cffc_launch_update_tile_fp64(&p, c->ti, c->tj, c->stream));
+            }
         } else {
             HIPCHK(launch_stage(c, p, 1));
         }
@@ -808,6 +1271,22 @@ static int begin_run(cffc_ctx *c, int nsteps) {
         ExchangeParams ex = exchange_params(c, 1, nullptr);
         ex.Wf = c->Wf[c->wcur];
         HIPCHK(cffc_launch_exchange(&ex, c->nblk, c->stream));
+    }
+    if (c->amr) {
+        // Ghosts of the new state across resolution changes.  The
+        // coarse-to-fine leg reads coarse ghosts that the BCs fill after it,
+        // and BC corner rules read ghosts of that leg one level coarser:
+        // max_level + 2 rounds of messages + BCs make every ghost a function
+        // of the interior state (the first stage then repeats the messages).
+        if (int e = ensure_phi(c)) return e;
+        for (int r = 0; r < c->max_level + 2; ++r) {
+            if (int e = amr_legs(c, c->U, c->W[c->wcur], c->Wf[c->wcur], nullptr, nullptr)) return e;
+            ExchangeParams q = exchange_params(c, 1, nullptr);
+            q.Wf = c->Wf[c->wcur];
+            q.bcs_only = 1;
+            q.blocks = c->bc_blocks;
+            if (c->nbc) HIPCHK(cffc_launch_exchange(&q, c->nbc, c->stream));
+        }
     }
     HIPCHK(cffc_launch_stamp(c->step_clock, c->stream));
     return 0;
@@ -902,6 +1381,7 @@ extern "C" int cffc_run_steps_graph(cffc_ctx *c, int nsteps, double *norms, doub
 // ---- batched steady runs ----------------------------------------------------
 extern "C" int cffc_set_parts(cffc_ctx *c, int npart, const int32_t *first_block) {
     if (!c || npart < 1 || npart > c->nblk || !first_block) return fail(CFFC_E_ARG, "bad argument");
+    if (c->amr && npart > 1) return fail(CFFC_E_ARG, "a mesh with resolution changes runs as one part");
     if (first_block[0] != 0 || first_block[npart] != c->nblk) return fail(CFFC_E_ARG, "parts must cover all blocks");
     std::vector<int32_t> pob(c->nblk);
     for (int q = 0; q < npart; ++q) {

@@ -44,7 +44,15 @@ struct BlockMeta {
     int64_t noff;    // first node of the block in the node arrays
     int32_t bcoff;   // first entry of the block in the bc / wo arrays
     int32_t has_bc;  // nonzero if any BCtypeN/S/E/W entry is not BC_NONE
+    int32_t amr;     // AMR_* flags (resolution changes)
+    int32_t pad;
+    int64_t eoff;    // first entry of the block's saved face fluxes (N [ni], S [ni], E [nj], W [nj])
 };
+
+// BlockMeta::amr: faces whose boundary fluxes the stage saves (the coarse side
+// of a resolution change for its correction, the fine side for the sums it
+// sends), and blocks some of whose cells feed a coarser neighbour's ghosts
+enum { AMR_FLUX_N = 1, AMR_FLUX_S = 2, AMR_FLUX_E = 4, AMR_FLUX_W = 8, AMR_FLUX_ANY = 15, AMR_USTORE = 16 };
 
 struct TileDesc {
     int32_t b, i0, j0, part;  // block, first interior cell of the tile, part (case) of the block

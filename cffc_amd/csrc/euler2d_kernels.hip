@@ -199,6 +199,7 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
             stW(W, ncp, cd, Wg);
         }
     }
+    if (p.no_bcs) return;   // uniform over the workgroup
     __syncthreads();
     const int32_t *bcW = p.bc + m.bcoff, *bcE = bcW + m.ncj, *bcS = bcE + m.ncj, *bcN = bcS + m.nci;
     const double *woW = p.wo + 4 * m.bcoff, *woE = woW + 4 * m.ncj, *woS = woE + 4 * m.ncj, *woN = woS + 4 * m.nci;
@@ -717,7 +718,8 @@ hipError_t cffc_launch_residual_smoothing(const BlockMeta *meta, int nblk, doubl
     return hipGetLastError();
 }
 hipError_t cffc_launch_update_tile_fp64(const StageParams *p, int ti, int tj, hipStream_t st) {
-    hipLaunchKernelGGL((update_tile_kernel<double>), dim3(p->ntiles), dim3(UPD_NT), 0, st, *p, ti, tj);
+    hipLaunchKernelGGL((update_tile_kernel<double>), dim3(p->tile_list ? p->nlist : p->ntiles), dim3(UPD_NT), 0, st, *p, ti,
+                       tj);
     return hipGetLastError();
 }
 hipError_t cffc_launch_stamp(unsigned long long *dst, hipStream_t st) {

@@ -24,6 +24,30 @@ struct ExchangeParams {
     unsigned long long *reset_slot;  // set to MILLION (next-step dt slot), may be null
     const int32_t *blocks;     // blocks to process (workgroup k -> blocks[k]), or null: workgroup = block
     int32_t bcs_only;          // skip the same-level exchange (ghosts already pushed by the stage kernel)
+    int32_t no_bcs;            // same-level exchange only (the resolution-change legs and BCs follow)
+};
+
+// Resolution-change operations (euler2d_amr.hip), built by the host per mesh.
+// restriction: coarse ghost dst <- fine cells src, src+1, src+pitch, src+pitch+1
+struct RestrictOp {
+    int64_t dst, src;
+    int32_t pitch, pad;
+};
+// prolongation: coarse cell src of block blk -> its quarters SW, SE, NW, NE,
+// centred at (xq, yq), into fine ghost cells dst
+struct ProlongOp {
+    int64_t src;
+    int64_t dst[4];
+    double xq[4], yq[4];
+    int32_t blk, pad;
+};
+// flux correction of coarse cell `cell` on n (1 or 2) faces, in the order N, S,
+// E, W: own = the cell's saved face flux entry, fine = the first of the two
+// fine entries it covers (entries of 4 doubles in the edge-flux buffer)
+struct FluxFixOp {
+    int64_t cell;
+    int32_t n, pad;
+    int32_t own[2], fine[2];
 };
 
 // end of a fused time step: reduce the last stage's per-tile partials,
@@ -92,6 +116,14 @@ hipError_t cffc_launch_update_tile_fp64(const cffc::StageParams *p, int ti, int 
 hipError_t cffc_launch_update_tile_mixed(const cffc::StageParams *p, int ti, int tj, hipStream_t st);
 hipError_t cffc_launch_w_from_u(const cffc::BlockMeta *meta, int nblk, int max_int, const double *U, double *W,
                                 float *Wf, int64_t ncp, hipStream_t st);
+hipError_t cffc_launch_restrict(const cffc::RestrictOp *ops, int nops, const double *U, const double *geom,
+                                int64_t ncp, double *W, float *Wf, const int32_t *stop_flag, hipStream_t st);
+hipError_t cffc_launch_prolong(const cffc::ProlongOp *ops, int nops, const cffc::BlockMeta *meta, const double *geom,
+                               const double *phi, int frozen, const int32_t *part_state, int64_t ncp, double *W,
+                               float *Wf, const int32_t *stop_flag, hipStream_t st);
+hipError_t cffc_launch_flux_fix(const cffc::FluxFixOp *ops, int nops, double *S, const double *eflux,
+                                const double *geom, const double *dt, const unsigned long long *dtmin, int lts,
+                                double cfl, int64_t ncp, const int32_t *stop_flag, hipStream_t st);
 hipError_t cffc_launch_stage_fp64(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 hipError_t cffc_launch_stage_mixed(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 }

@@ -22,6 +22,7 @@ extern "C" hipError_t cffc_launch_stage_mixed(const StageParams *p, int ti, int 
 }
 
 extern "C" hipError_t cffc_launch_update_tile_mixed(const StageParams *p, int ti, int tj, hipStream_t st) {
-    hipLaunchKernelGGL((update_tile_kernel<float>), dim3(p->ntiles), dim3(UPD_NT), 0, st, *p, ti, tj);
+    hipLaunchKernelGGL((update_tile_kernel<float>), dim3(p->tile_list ? p->nlist : p->ntiles), dim3(UPD_NT), 0, st, *p, ti,
+                       tj);
     return hipGetLastError();
 }

@@ -56,6 +56,14 @@ struct StageParams {
     const int32_t *nbr;     // 8 same-level neighbours per block (push exchange)
     int32_t push;           // fused: also write W^(s+1) into the neighbours' ghost cells
     int32_t defer_update;   // fused: store the residual only; update_tile_kernel follows
+    // resolution changes
+    const uint8_t *tile_defer;  // fused: per tile, nonzero = store the residual in Sd (flux correction follows)
+    double *Sd;             // residual of deferred tiles; update_tile_kernel reads it
+    const int32_t *tile_list;   // update_tile_kernel: the tiles to update (null: all)
+    int32_t nlist;
+    double *eflux;          // saved block-edge fluxes (AMR_FLUX_* faces)
+    const uint8_t *ustore;  // per padded cell: U^(s+1) is read by a coarser neighbour's restriction
+    double *Ustage;         // fused: U^(s+1) of those cells
 };
 
 // Push form of the same-level exchange: the cell (ip, jp) of block b that a
@@ -344,6 +352,10 @@ __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMe
         p.Wout[c] = Wn.d; p.Wout[p.ncp + c] = Wn.vx; p.Wout[2 * p.ncp + c] = Wn.vy; p.Wout[3 * p.ncp + c] = Wn.p;
         if (p.push) push_ghosts(p, p.Wout, m, t.b, ip, jp, Wn);
     }
+    if ((m.amr & AMR_USTORE) && p.Ustage && p.ustore[c]) {
+        // a coarser neighbour restricts U^(s+1) of this cell into its ghosts
+        p.Ustage[c] = Un.d; p.Ustage[p.ncp + c] = Un.mx; p.Ustage[2 * p.ncp + c] = Un.my; p.Ustage[3 * p.ncp + c] = Un.E;
+    }
     if (p.mode == MODE_ZERO_ACC2) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) p.S[v * p.ncp + c] = p.S[v * p.ncp + c] + 2.0 * dU[v];
@@ -440,6 +452,9 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     const double *G = p.geom;
     const int ICu = m.ni + 1, JCu = m.nj + 1;
     const int pi0 = t.i0, pj0 = t.j0;   // padded index of the LDS window origin
+    // the update waits (in update_tile_kernel) for residual smoothing, or
+    // for the flux correction of a tile next to a finer neighbour
+    const bool defer = FUSED && (p.defer_update || (p.tile_defer && p.tile_defer[tix]));
     const int32_t *bcW = p.bc + m.bcoff, *bcE = bcW + m.ncj, *bcS = bcE + m.ncj, *bcN = bcS + m.nci;
     // role of this thread: a tile cell (tc, tr), or a ring cell of side
     // 0 = W (column -1), 1 = E (column TI), 2 = S (row -1), 3 = N (row TJ)
@@ -661,15 +676,41 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                 dU[v] = d;
             }
         }
-        if (!FUSED || p.defer_update) {
-            // granular residual, or residual smoothing follows: store dUdt
+        if (m.amr & AMR_FLUX_ANY) {
+            // block-edge fluxes FluxN/S/E/W = (+/-) Flux*lface (Euler2DQuad.cc:
+            // 1518-1523, 1709-1715) for the resolution-change corrections
+            double *eb = p.eflux + 4 * m.eoff;
+            if ((m.amr & AMR_FLUX_N) && jp == JCu) {
+                double *e = eb + 4 * (ip - 2);
 #pragma unroll
-            for (int v = 0; v < 4; ++v) p.S[v * ncp + c] = dU[v];
+                for (int v = 0; v < 4; ++v) e[v] = (double)sFj[v][tr + 1][tc] * lN;
+            }
+            if ((m.amr & AMR_FLUX_S) && jp == 2) {
+                double *e = eb + 4 * (m.ni + ip - 2);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) e[v] = (-(double)sFj[v][tr][tc]) * lS;
+            }
+            if ((m.amr & AMR_FLUX_E) && ip == ICu) {
+                double *e = eb + 4 * (2 * m.ni + jp - 2);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) e[v] = (double)sFi[v][tr][tc + 1] * lE;
+            }
+            if ((m.amr & AMR_FLUX_W) && ip == 2) {
+                double *e = eb + 4 * (2 * m.ni + m.nj + jp - 2);
+#pragma unroll
+                for (int v = 0; v < 4; ++v) e[v] = (-(double)sFi[v][tr][tc]) * lW;
+            }
+        }
+        if (!FUSED || defer) {
+            // granular residual, or a correction / smoothing follows: store dUdt
+            double *Sx = FUSED ? p.Sd : p.S;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) Sx[v * ncp + c] = dU[v];
         } else {
             fused_update<FT>(p, m, t, c, ip, jp, A, dtc, dU, l1, l2, mx, dtn);
         }
     }
-    if (!(FUSED) || !p.last_stage || p.defer_update) return;
+    if (!(FUSED) || !p.last_stage || defer) return;
 
     // ---- last stage: workgroup partials of the norms and the next dt --------
     // wavefront reductions with DPP/permute shuffles, then across the 4 waves
@@ -706,7 +747,7 @@ template <typename FT>
 __global__ __launch_bounds__(UPD_NT) void update_tile_kernel(StageParams p, int TI, int TJ) {
     __shared__ double sRed[UPD_NT / 64][4];
     if (p.stop_flag && *p.stop_flag) return;
-    const int tix = xcd_tile(blockIdx.x, p.ntiles);
+    const int tix = p.tile_list ? p.tile_list[blockIdx.x] : xcd_tile(blockIdx.x, p.ntiles);
     const TileDesc t = p.tiles[tix];
     const BlockMeta m = p.meta[t.b];
     const int tid = threadIdx.x;
@@ -740,7 +781,7 @@ __global__ __launch_bounds__(UPD_NT) void update_tile_kernel(StageParams p, int 
         const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
         const double A = p.geom[GEO_A * ncp + c];
         const double dtc = p.lts ? p.dt[c] : bitsd(*p.dtmin_in);
-        double dU[4] = {p.S[c], p.S[ncp + c], p.S[2 * ncp + c], p.S[3 * ncp + c]};
+        double dU[4] = {p.Sd[c], p.Sd[ncp + c], p.Sd[2 * ncp + c], p.Sd[3 * ncp + c]};
         double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = MILLION;
         fused_update<FT>(p, m, t, c, ip, jp, A, dtc, dU, a1, a2, a3, a4);
         l1 += a1; l2 += a2; mx = dmax(mx, a3); dtn = dmin(dtn, a4);

@@ -54,6 +54,7 @@ def load():
         getattr(L, n).restype = i32
     L.cffc_get_dUdt.argtypes = [vp, i32, pd, i64]
     L.cffc_exchange_and_bcs.argtypes = [vp]
+    L.cffc_apply_flux_corrections.argtypes = [vp, i32]
     L.cffc_cfl.argtypes = [vp, pd, i32]
     L.cffc_dudt_stage.argtypes = [vp, i32]
     L.cffc_update_stage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64)]
@@ -155,6 +156,22 @@ class Euler2DQuadBlocks:
     def Send_All_Messages_and_BCs(self):
         """Send_All_Messages(..., NUM_VAR_EULER2D, OFF) followed by BCs()."""
         return _check(load().cffc_exchange_and_bcs(self.h), "cffc_exchange_and_bcs")
+
+    def prime_ghosts(self):
+        """Send_All_Messages + BCs repeated until every ghost derives from the
+        interior state: once on one level; max_level + 2 times across
+        resolution changes, where the coarse-to-fine leg reads coarse BC ghosts
+        filled after it.  Returns the number of rounds."""
+        lv = getattr(self.mesh, "level", None)
+        rounds = int(np.max(lv)) + 2 if lv is not None and np.any(lv != lv[0]) else 1
+        for _ in range(rounds):
+            self.Send_All_Messages_and_BCs()
+        return rounds
+
+    def Apply_Boundary_Flux_Corrections(self, i_stage):
+        """Send_Conservative_Flux_Corrections() + Apply_Boundary_Flux_Corrections_
+        Multistage_Explicit() (no-op without resolution changes)."""
+        return _check(load().cffc_apply_flux_corrections(self.h, i_stage), "cffc_apply_flux_corrections")
 
     def CFL(self, set_global=True):
         """CFL() + CFFC_Minimum_MPI() (+ Set_Global_TimeStep()); returns dTime."""
@@ -294,6 +311,7 @@ class Euler2DQuadBlocks:
             for st in range(1, s.n_stage + 1):
                 self.Send_All_Messages_and_BCs()
                 self.dUdt_Multistage_Explicit(st)
+                self.Apply_Boundary_Flux_Corrections(st)
                 rc = self.Update_Solution_Multistage_Explicit(st)
                 if rc:
                     return rc, np.array(norms)

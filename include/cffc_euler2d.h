@@ -140,6 +140,12 @@ int  cffc_create(const cffc_mesh_desc *mesh, const cffc_scheme_desc *scheme,
                  cffc_ctx **out);
 void cffc_destroy(cffc_ctx *ctx);
 
+/* Host-only check of a mesh descriptor: the metadata, topology, BC and
+   resolution-change checks of cffc_create, without a device.  On success
+   counts[0..3] receive the numbers of restriction, prolongation and
+   flux-correction operations and 1 if the mesh has resolution changes. */
+int  cffc_check_mesh(const cffc_mesh_desc *mesh, int64_t counts[4]);
+
 /* Number of interior cells over all blocks (the unit of the state arrays). */
 int64_t cffc_num_cells(const cffc_ctx *ctx);
 
@@ -163,6 +169,16 @@ int  cffc_get_dt(cffc_ctx *ctx, double *dt, int64_t ncell);
       layers of every block from its same-level neighbours (faces and
       corners), then applies the physical boundary conditions. */
 int  cffc_exchange_and_bcs(cffc_ctx *ctx);
+
+/* == Send_Conservative_Flux_Corrections(blocks, list, NUM_VAR_EULER2D)
+      (AdaptiveBlock2D_MessagePassing.h:8328) followed by
+      Apply_Boundary_Flux_Corrections_Multistage_Explicit(blocks, list, IP,
+      i_stage) (Euler2DQuadMultiBlock.cc:1307 -> SingleBlock.cc:4992-5080):
+      on every face with two finer neighbours, the coarse face flux saved by
+      cffc_dudt_stage becomes -(sum of the two fine fluxes) - own, and
+      dUdt[k] -= (CFL*dt)*Flux/A on the face's cells.  No-op without
+      resolution changes. */
+int  cffc_apply_flux_corrections(cffc_ctx *ctx, int i_stage);
 
 /* == CFL(blocks, list, IP) (Euler2DQuadMultiBlock.cc:963) followed by
       CFFC_Minimum_MPI (MPI.h:184) and, when set_global != 0,

@@ -1,0 +1,187 @@
+"""Resolution changes on the GPU: parity with the oracle on quadtree meshes.
+
+The HIP restriction, prolongation and flux-correction kernels (through the C
+ABI) against the oracle's restatement of the reference's fine-to-coarse /
+coarse-to-fine messages and conservative flux correction, on a disturbed
+3-level quadtree mesh with reflecting walls and the cylindrical explosion
+(plus a 1% density perturbation, so every resolution change carries flux
+differences from the first step).  FP64 runs are held to 1e-12 relative per
+cell (bit identity expected and reported); the mixed FP32-flux path to a
+loose bound against FP64.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from cffc_amd import abi
+from cffc_amd import mesh as M
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def E():
+    from cffc_amd import euler2d as E
+    E.load()
+    return E
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300)))
+
+
+def nonbitwise(a, b):
+    return int(np.sum(np.ascontiguousarray(a).view(np.uint64) != np.ascontiguousarray(b).view(np.uint64)))
+
+
+def case(n=8, jitter=0.1, seed=2, nroot=4, radius=0.1):
+    m = M.quadtree_box(nroot=nroot, n=n, max_level=2, refine=M.circle_refinement(radius), jitter=jitter, seed=seed)
+    W0 = M.density_perturbation(m, M.cylindrical_explosion(m, radius=radius), seed=seed)
+    return m, W0
+
+
+def scheme(**kw):
+    s = abi.Scheme(cfl_number=0.4)
+    for k, v in kw.items():
+        setattr(s, k, v)
+    return s
+
+
+def areas(m, orc):
+    return np.concatenate([orc.block_geometry(b)[2][2:int(m.ni[b]) + 2, 2:int(m.ni[b]) + 2].ravel()
+                           for b in range(m.nblk)])
+
+
+def test_granular_reference_order_matches_oracle(E):
+    """Send_All_Messages + BCs, dUdt, flux corrections, update: call by call,
+    then with the limiters frozen (the prolongation keeps the stored phi)."""
+    m, W0 = case()
+    s = scheme()
+    gpu = E.Euler2DQuadBlocks(m, s)
+    orc = oracle.Oracle(m, s)
+    gpu.set_W(W0)
+    orc.set_W(W0)
+    print("mesh: %d blocks, levels %s, %d cells" % (m.nblk, np.bincount(m.level).tolist(), m.ncell), flush=True)
+    assert gpu.prime_ghosts() == orc.prime_ghosts()
+    worst = 0.0
+    for n in range(4):
+        if n == 2:
+            gpu.Freeze_Limiters(True)
+            orc.freeze_limiters(True)
+        assert gpu.CFL() == orc.cfl()
+        for st in range(1, 5):
+            gpu.Send_All_Messages_and_BCs()
+            orc.exchange_and_bcs()
+            gpu.dUdt_Multistage_Explicit(st)
+            orc.dudt_stage(st)
+            gpu.Apply_Boundary_Flux_Corrections(st)
+            orc.apply_flux_corrections(st)
+            k = 0 if st == 4 else st - 1
+            e = relerr(gpu.get_dUdt(k), orc.get_dUdt(k))
+            worst = max(worst, e)
+            assert e <= REL, (n, st, e)
+            assert gpu.Update_Solution_Multistage_Explicit(st) == 0
+            assert orc.update_stage(st)[0] == 0
+    Wg, Wo = gpu.get_W(), orc.get_W()
+    print("dUdt worst rel %.3e; W rel %.3e, %d non-bitwise of %d" % (worst, relerr(Wg, Wo), nonbitwise(Wg, Wo), Wg.size))
+    assert relerr(Wg, Wo) <= REL
+    gpu.close()
+
+
+@pytest.mark.parametrize("n,tile", [(8, (8, 8)), (16, (16, 8)), (16, (8, 16)), (16, (16, 16))])
+def test_fused_run_matches_oracle(E, n, tile):
+    m, W0 = case(n=n)
+    s = scheme()
+    gpu = E.Euler2DQuadBlocks(m, s)
+    gpu.set_tile(*tile)
+    gpu.set_W(W0)
+    rc, steps, norms, dts = gpu.run_steps(12)
+    orc = oracle.Oracle(m, s)
+    orc.set_W(W0)
+    rc2, steps2, norms2, dts2 = orc.run(12, nthreads=4)
+    assert rc == 0 and rc2 == 0 and steps == steps2 == 12
+    np.testing.assert_array_equal(dts, dts2)
+    Wg, Wo = gpu.get_W(), orc.get_W()
+    print("n=%d tile %s: W rel %.3e, %d non-bitwise; norms rel %.3e" %
+          (n, tile, relerr(Wg, Wo), nonbitwise(Wg, Wo), relerr(norms, norms2)))
+    assert relerr(Wg, Wo) <= REL
+    assert relerr(gpu.get_U(), orc.get_U()) <= REL
+    assert relerr(norms, norms2) <= REL
+    gpu.close()
+
+
+def test_graph_replay_matches_stream_run(E):
+    m, W0 = case(n=16)
+    s = scheme()
+    a = E.Euler2DQuadBlocks(m, s)
+    a.set_W(W0)
+    a.run_steps(10)
+    b = E.Euler2DQuadBlocks(m, s)
+    b.set_W(W0)
+    b.run_steps(10, graph=True)
+    assert np.array_equal(a.get_W(), b.get_W())
+    a.close()
+    b.close()
+
+
+def test_local_time_stepping_optimal_smoothing_matches_oracle(E):
+    """The steady scheme: 5-stage optimal smoothing, local (per-cell) time
+    steps, which the flux correction reads per cell."""
+    m, W0 = case(n=8, seed=5)
+    s = scheme(i_time_integration=abi.TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING, n_stage=5, cfl_number=1.0,
+               local_time_stepping=abi.SCALAR_LOCAL_TIME_STEPPING)
+    gpu = E.Euler2DQuadBlocks(m, s)
+    gpu.set_W(W0)
+    rc, steps, norms, _ = gpu.run_steps(10)
+    orc = oracle.Oracle(m, s)
+    orc.set_W(W0)
+    rc2, steps2, norms2, _ = orc.run(10, nthreads=4)
+    assert rc == 0 and rc2 == 0
+    Wg, Wo = gpu.get_W(), orc.get_W()
+    print("LTS opt5: W rel %.3e, %d non-bitwise" % (relerr(Wg, Wo), nonbitwise(Wg, Wo)))
+    assert relerr(Wg, Wo) <= REL
+    gpu.close()
+
+
+def test_mass_and_energy_conserved_on_gpu(E):
+    m, W0 = case(n=16, seed=7)
+    s = scheme()
+    gpu = E.Euler2DQuadBlocks(m, s)
+    gpu.set_W(W0)
+    orc = oracle.Oracle(m, s)
+    A = areas(m, orc)
+    U0 = gpu.get_U()
+    rc, steps, _, _ = gpu.run_steps(40)
+    assert rc == 0 and steps == 40
+    U1 = gpu.get_U()
+    m0, m1 = np.sum(A * U0[:, 0]), np.sum(A * U1[:, 0])
+    e0, e1 = np.sum(A * U0[:, 3]), np.sum(A * U1[:, 3])
+    print("mass drift %.3e, energy drift %.3e (relative)" % (abs(m1 - m0) / m0, abs(e1 - e0) / e0))
+    assert abs(m1 - m0) <= 1e-13 * m0 and abs(e1 - e0) <= 1e-13 * e0
+    gpu.close()
+
+
+def test_mixed_precision_against_fp64(E):
+    m, W0 = case(n=16, seed=3)
+    s64, s32 = scheme(), scheme(flux_precision=abi.FLUX_FP32)
+    a = E.Euler2DQuadBlocks(m, s64)
+    a.set_W(W0)
+    a.run_steps(20)
+    b = E.Euler2DQuadBlocks(m, s32)
+    b.set_W(W0)
+    rc, steps, _, _ = b.run_steps(20)
+    assert rc == 0 and steps == 20
+    Wa, Wb = a.get_W(), b.get_W()
+    l1 = np.mean(np.abs(Wb[:, 0] - Wa[:, 0])) / np.mean(np.abs(Wa[:, 0]))
+    linf = np.max(np.abs(Wb[:, 0] - Wa[:, 0]))
+    orc = oracle.Oracle(m, s64)
+    A = areas(m, orc)
+    Ua, Ub = a.get_U(), b.get_U()
+    mdrift = abs(np.sum(A * Ub[:, 0]) - np.sum(A * W0[:, 0])) / np.sum(A * W0[:, 0])
+    print("mixed vs FP64 (rho): L1 rel %.3e, Linf %.3e; mixed mass drift %.3e" % (l1, linf, mdrift))
+    assert l1 < 1e-5 and linf < 1e-2
+    assert mdrift < 1e-6
+    a.close()
+    b.close()
