@@ -141,6 +141,74 @@ def steady_leg(ncases, seed=7):
             "all_converged": bool(np.all(cs > 0) and np.all(ws > 0))}
 
 
+def amr_case(seed=3):
+    """The quadtree workload: [0,1]^2 in 32 x 32 root blocks of 16^2 cells,
+    refined twice where blocks come within 0.02 of the circle r = 0.3 about
+    (0.5, 0.5), 2:1 balanced (3,088 blocks at levels 0/1/2, 790,528 cells),
+    interior nodes disturbed by 0.1 h; reflecting walls; the cylindrical
+    explosion (density ratio 8, pressure ratio 10) across that circle."""
+    from cffc_amd import abi
+    from cffc_amd import mesh as M
+    m = M.quadtree_box(nroot=32, n=16, max_level=2, refine=M.circle_refinement(0.3, band=0.02),
+                       jitter=0.1, seed=seed)
+    W0 = M.cylindrical_explosion(m, radius=0.3)
+    return m, W0
+
+
+def cell_areas(m):
+    """Interior cell areas from the nodes, in quadCentroid's operation order
+    (Math.h:311-350), so that they equal the solver's areas bit for bit."""
+    out = []
+    for b in range(m.nblk):
+        x, y = m.block_nodes(b)
+        n = int(m.ni[b])
+        X0x, X0y = x[2:n + 2, 2:n + 2], y[2:n + 2, 2:n + 2]
+        X1x, X1y = x[2:n + 2, 3:n + 3], y[2:n + 2, 3:n + 3]
+        X2x, X2y = x[3:n + 3, 3:n + 3], y[3:n + 3, 3:n + 3]
+        X3x, X3y = x[3:n + 3, 2:n + 2], y[3:n + 3, 2:n + 2]
+        a = (X3x * X0y - X0x * X3y) + (X0x * X1y - X1x * X0y)
+        a = a + (X1x * X2y - X2x * X1y)
+        a = a + (X2x * X3y - X3x * X2y)
+        out.append((0.5 * a).ravel())
+    return np.concatenate(out)
+
+
+def amr_leg(steps, warmup=10, autotune_tiles=True, fixed_tile=(16, 8)):
+    """Throughput on the quadtree mesh (resolution changes every stage:
+    restriction, prolongation, flux correction, deferred tile updates), both
+    precisions, graph-replayed; mass drift and mixed-vs-FP64 difference."""
+    t0 = time.perf_counter()
+    m, W0 = amr_case()
+    tmesh = time.perf_counter() - t0
+    A = cell_areas(m)
+    out = {"workload": "euler2d_cylindrical_explosion_quadtree_3088_blocks_of_16x16", "blocks": int(m.nblk),
+           "blocks_per_level": np.bincount(m.level).tolist(), "cells": int(m.ncell), "steps": steps,
+           "scheme": "RK4/Roe/LSQ/Venkatakrishnan/CFL0.5/global-dt", "mesh_seconds": tmesh}
+    Ws = {}
+    for prec in ("mixed", "fp64"):
+        tile, tune = fixed_tile, {}
+        if autotune_tiles:
+            tile, tune = autotune(m, W0, prec)
+        g, rc, dt, norms, W = gpu_run(m, W0, prec, steps, warmup, tile)
+        U = g.get_U()
+        g.close()
+        kt = kernel_time(m, W0, prec, 20, tile)
+        stage_ms = kt["stage_ms"] / max(kt["stage_launches"], 1)
+        rate = m.ncell * 4 * steps / dt
+        rho0 = np.sum(A * W0[:, 0])
+        out[prec] = {"value": rate, "unit": "cell-stage updates/s", "ms_per_step": dt / steps * 1e3,
+                     "tile": list(tile), "stage_kernel_ms": stage_ms, "rc": rc,
+                     "mass_drift_rel": float(abs(np.sum(A * U[:, 0]) - rho0) / rho0),
+                     "autotune_ms_per_step": {"%dx%d" % k: round(v, 4) for k, v in tune.items()}}
+        Ws[prec] = W
+        print("[amr %s] tile %dx%d: %.3f s for %d steps: %.3e cell-stage/s; stage kernel %.4f ms"
+              % (prec, tile[0], tile[1], dt, steps, rate, stage_ms), file=sys.stderr, flush=True)
+    d = np.abs(Ws["mixed"] - Ws["fp64"])
+    out["mixed_vs_fp64"] = {"L1": d.mean(axis=0).tolist(), "Linf": d.max(axis=0).tolist(),
+                            "vars": ["rho", "u", "v", "p"], "steps": warmup + steps}
+    return out
+
+
 def load_traffic(name):
     p = os.path.join(ROOT, "profiles", name)
     if os.path.exists(p):
@@ -181,9 +249,17 @@ def main():
                     help="held-out steady bump cases solved cold and warm (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES bytes (counter calibration)")
+    ap.add_argument("--amr-steps", type=int, default=1000,
+                    help="steps of the quadtree (resolution-change) workload (0: skip)")
+    ap.add_argument("--only-amr", action="store_true", help="run only the quadtree workload and print it")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.only_amr:
+        fixed = tuple(int(x) for x in args.tile.lower().split("x")) if args.tile else (16, 8)
+        print(json.dumps(amr_leg(args.amr_steps or 200, autotune_tiles=not args.no_autotune and not args.tile,
+                                 fixed_tile=fixed)))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole mesh on a single MI355X "
@@ -254,6 +330,11 @@ def main():
         out["mixed_vs_fp64"] = {"steps": args.warmup + args.steps,
                                 "L1": d.mean(axis=0).tolist(), "Linf": d.max(axis=0).tolist(),
                                 "vars": ["rho", "u", "v", "p"]}
+    if args.amr_steps > 0:
+        try:
+            out["amr"] = amr_leg(args.amr_steps)
+        except Exception as e:  # reported, never required for the throughput line
+            out["amr"] = {"error": str(e)}
     if args.steady_cases > 0:
         try:
             out["steady_warm_vs_cold"] = steady_leg(args.steady_cases)

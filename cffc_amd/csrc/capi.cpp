@@ -526,7 +526,7 @@ static int setup_levels(cffc_ctx *c, const cffc_mesh_desc *md, std::vector<int32
                     if (pr.first == cc) idx = pr.second;
                 if (idx < 0) {
                     idx = (int)fops.size();
-                    fops.push_back(FluxFixOp{cc, 0, 0, {0, 0}, {0, 0}});
+                    fops.push_back(FluxFixOp{cc, 0, b, {0, 0}, {0, 0}});
                     at.push_back({cc, idx});
                 }
                 FluxFixOp &op = fops[idx];

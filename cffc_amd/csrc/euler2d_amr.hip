@@ -1,8 +1,11 @@
 // euler2d_amr.hip -- resolution changes of quadtree block meshes on gfx950.
 //
-// Three small list-driven kernels, one thread per operation, launched once
-// per stage after the stage update (and repeatedly while the ghosts of a new
-// state are primed):
+// Three small list-driven kernels, one thread per operation.  The
+// restriction and prolongation run at the start of every stage, before the
+// BCs (and repeatedly while the ghosts of a new state are primed); the flux
+// correction runs between the residual and the update of the deferred tiles
+// (a separate launch: inside the update kernel its load latency would add
+// to every deferred tile's):
 //   restrict_kernel   fine-to-coarse leg of Send_All_Messages: a coarse ghost
 //                     cell takes the area-weighted average of the 2 x 2 fine
 //                     cells it covers (LoadSendBuffer_F2C, Euler2DQuad.h:
@@ -41,7 +44,7 @@ template <typename ST> __device__ __forceinline__ void amr_stW(ST *W, int64_t nc
 }
 
 template <typename ST>
-__global__ __launch_bounds__(256) void restrict_kernel(const RestrictOp *ops, int nops, const double *U,
+__global__ __launch_bounds__(64) void restrict_kernel(const RestrictOp *ops, int nops, const double *U,
                                                        const double *geom, int64_t ncp, ST *W,
                                                        const int32_t *stop_flag) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -62,7 +65,7 @@ __global__ __launch_bounds__(256) void restrict_kernel(const RestrictOp *ops, in
 }
 
 template <typename ST>
-__global__ __launch_bounds__(128) void prolong_kernel(const ProlongOp *ops, int nops, const BlockMeta *meta,
+__global__ __launch_bounds__(64) void prolong_kernel(const ProlongOp *ops, int nops, const BlockMeta *meta,
                                                       const double *geom, const double *phi, int frozen,
                                                       const int32_t *part_state, int64_t ncp, ST *W,
                                                       const int32_t *stop_flag) {
@@ -175,12 +178,13 @@ extern "C" hipError_t cffc_launch_restrict(const RestrictOp *ops, int nops, cons
                                            int64_t ncp, double *W, float *Wf, const int32_t *stop_flag,
                                            hipStream_t st) {
     if (nops <= 0) return hipSuccess;
-    const int nb = (nops + 255) / 256;
+    // one wavefront per workgroup: short lists spread over more CUs
+    const int nb = (nops + 63) / 64;
     if (Wf)
-        hipLaunchKernelGGL((restrict_kernel<float>), dim3(nb), dim3(256), 0, st, ops, nops, U, geom, ncp, Wf,
+        hipLaunchKernelGGL((restrict_kernel<float>), dim3(nb), dim3(64), 0, st, ops, nops, U, geom, ncp, Wf,
                            stop_flag);
     else
-        hipLaunchKernelGGL((restrict_kernel<double>), dim3(nb), dim3(256), 0, st, ops, nops, U, geom, ncp, W,
+        hipLaunchKernelGGL((restrict_kernel<double>), dim3(nb), dim3(64), 0, st, ops, nops, U, geom, ncp, W,
                            stop_flag);
     return hipGetLastError();
 }
@@ -189,12 +193,12 @@ extern "C" hipError_t cffc_launch_prolong(const ProlongOp *ops, int nops, const 
                                           const double *phi, int frozen, const int32_t *part_state, int64_t ncp,
                                           double *W, float *Wf, const int32_t *stop_flag, hipStream_t st) {
     if (nops <= 0) return hipSuccess;
-    const int nb = (nops + 127) / 128;
+    const int nb = (nops + 63) / 64;
     if (Wf)
-        hipLaunchKernelGGL((prolong_kernel<float>), dim3(nb), dim3(128), 0, st, ops, nops, meta, geom, phi, frozen,
+        hipLaunchKernelGGL((prolong_kernel<float>), dim3(nb), dim3(64), 0, st, ops, nops, meta, geom, phi, frozen,
                            part_state, ncp, Wf, stop_flag);
     else
-        hipLaunchKernelGGL((prolong_kernel<double>), dim3(nb), dim3(128), 0, st, ops, nops, meta, geom, phi, frozen,
+        hipLaunchKernelGGL((prolong_kernel<double>), dim3(nb), dim3(64), 0, st, ops, nops, meta, geom, phi, frozen,
                            part_state, ncp, W, stop_flag);
     return hipGetLastError();
 }

@@ -58,6 +58,29 @@ struct TileDesc {
     int32_t b, i0, j0, part;  // block, first interior cell of the tile, part (case) of the block
 };
 
+// Resolution-change operations (euler2d_amr.hip), built by the host per mesh.
+// restriction: coarse ghost dst <- fine cells src, src+1, src+pitch, src+pitch+1
+struct RestrictOp {
+    int64_t dst, src;
+    int32_t pitch, pad;
+};
+// prolongation: coarse cell src of block blk -> its quarters SW, SE, NW, NE,
+// centred at (xq, yq), into fine ghost cells dst
+struct ProlongOp {
+    int64_t src;
+    int64_t dst[4];
+    double xq[4], yq[4];
+    int32_t blk, pad;
+};
+// flux correction of coarse cell `cell` on n (1 or 2) faces, in the order N, S,
+// E, W: own = the cell's saved face flux entry, fine = the first of the two
+// fine entries it covers (entries of 4 doubles in the edge-flux buffer)
+struct FluxFixOp {
+    int64_t cell;
+    int32_t n, blk;
+    int32_t own[2], fine[2];
+};
+
 // per-part state of a batched steady run (one part = one independent case)
 enum PartState { PART_RUNNING = 0, PART_FROZEN = 1, PART_DONE = 2 };
 

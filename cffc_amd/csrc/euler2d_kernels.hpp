@@ -27,28 +27,7 @@ struct ExchangeParams {
     int32_t no_bcs;            // same-level exchange only (the resolution-change legs and BCs follow)
 };
 
-// Resolution-change operations (euler2d_amr.hip), built by the host per mesh.
-// restriction: coarse ghost dst <- fine cells src, src+1, src+pitch, src+pitch+1
-struct RestrictOp {
-    int64_t dst, src;
-    int32_t pitch, pad;
-};
-// prolongation: coarse cell src of block blk -> its quarters SW, SE, NW, NE,
-// centred at (xq, yq), into fine ghost cells dst
-struct ProlongOp {
-    int64_t src;
-    int64_t dst[4];
-    double xq[4], yq[4];
-    int32_t blk, pad;
-};
-// flux correction of coarse cell `cell` on n (1 or 2) faces, in the order N, S,
-// E, W: own = the cell's saved face flux entry, fine = the first of the two
-// fine entries it covers (entries of 4 doubles in the edge-flux buffer)
-struct FluxFixOp {
-    int64_t cell;
-    int32_t n, pad;
-    int32_t own[2], fine[2];
-};
+
 
 // end of a fused time step: reduce the last stage's per-tile partials,
 // record the step's norms and dt, set the next step's global dt, and raise
