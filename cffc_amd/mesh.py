@@ -225,6 +225,46 @@ def _quad_centroid(x, y):
     return xs / (3 * a), ys / (3 * a)
 
 
+def _tables_from_leaves(leaves, nri, nrj):
+    """Neighbour tables (nbr [n, 8], nbr2 [n, 4]) of quadtree leaves
+    (level, ix, iy) over nri x nrj roots, in the order given: a face's
+    neighbour at the same or a coarser level, or its two finer neighbours
+    (west/south one in nbr, the other in nbr2); a corner's neighbour is the
+    block holding the diagonal cell."""
+    keys = set(leaves)
+    bid = {k: b for b, k in enumerate(leaves)}
+
+    def inside(lv, x, y):
+        return 0 <= x < (nri << lv) and 0 <= y < (nrj << lv)
+
+    def find(lv, x, y):
+        for l in range(lv, -1, -1):
+            k = (l, x >> (lv - l), y >> (lv - l))
+            if k in keys:
+                return k
+        return None
+
+    n = len(leaves)
+    nbr = np.full((n, 8), -1, np.int32)
+    nbr2 = np.full((n, 4), -1, np.int32)
+    for b, (lv, x, y) in enumerate(leaves):
+        for f, (dx, dy) in enumerate(_FACE_DIR):
+            if not inside(lv, x + dx, y + dy):
+                continue
+            o = find(lv, x + dx, y + dy)
+            if o is not None:
+                nbr[b, f] = bid[o]
+            else:
+                c0, c1 = _children_on_face(lv, x, y, f)
+                nbr[b, f], nbr2[b, f] = bid[c0], bid[c1]
+        for c, (dx, dy) in enumerate(_CORNER_DIR):
+            if not inside(lv, x + dx, y + dy):
+                continue
+            o = find(lv, x + dx, y + dy)
+            nbr[b, 4 + c] = bid[o] if o is not None else bid[_child_at_corner(lv, x, y, c)]
+    return nbr, nbr2
+
+
 def quadtree_box(nroot=4, n=8, max_level=2, refine=None, x0=0.0, x1=1.0, y0=0.0, y1=1.0, bcs=None,
                  jitter=0.1, seed=0):
     """Square domain [x0,x1]x[y0,y1] split into nroot x nroot root blocks,
@@ -294,26 +334,9 @@ def quadtree_box(nroot=4, n=8, max_level=2, refine=None, x0=0.0, x1=1.0, y0=0.0,
                     split(o)
                     changed = True
     order = sorted(leaves, key=lambda k: _morton(k[1] << (max_level + 1 - k[0]), k[2] << (max_level + 1 - k[0])))
-    bid = {k: b for b, k in enumerate(order)}
     nblk = len(order)
     level = np.array([k[0] for k in order], np.int32)
-    nbr = np.full((nblk, 8), -1, np.int32)
-    nbr2 = np.full((nblk, 4), -1, np.int32)
-    for b, (lv, x, y) in enumerate(order):
-        for f, (dx, dy) in enumerate(_FACE_DIR):
-            if not inside(lv, x + dx, y + dy):
-                continue
-            o = find(lv, x + dx, y + dy)
-            if o is not None:
-                nbr[b, f] = bid[o]
-            else:
-                c0, c1 = _children_on_face(lv, x, y, f)
-                nbr[b, f], nbr2[b, f] = bid[c0], bid[c1]
-        for c, (dx, dy) in enumerate(_CORNER_DIR):
-            if not inside(lv, x + dx, y + dy):
-                continue
-            o = find(lv, x + dx, y + dy)
-            nbr[b, 4 + c] = bid[o] if o is not None else bid[_child_at_corner(lv, x, y, c)]
+    nbr, nbr2 = _tables_from_leaves(order, nroot, nroot)
 
     # own nodes: lattice (ghost nodes included), interior nodes disturbed
     rng = np.random.default_rng(seed)
