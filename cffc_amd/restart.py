@@ -103,24 +103,23 @@ def _read_text(path):
         return f.read()
 
 
-def read_soln(path):
-    """Parse one block restart file (gzip-compressed if it ends in .gz)."""
-    lines = _read_text(path).split("\n")
-    if lines and lines[-1] == "":
-        lines.pop()
-    k = 0
-    header, gas = lines[0], lines[1]
-    NNi, INl, INu = _ints(lines[2])
-    NNj, JNl, JNu = _ints(lines[3])
-    nghost = int(lines[4])
-    max_rec, ho_bnd = int(lines[5]), int(lines[6])
-    NCi, ICl, ICu = _ints(lines[7])
-    NCj, JCl, JCu = _ints(lines[8])
-    k = 9
+def _parse_grid(lines, k):
+    """The grid of one block (operator>> of Grid2D_Quad_Block_HO) starting
+    at line k: a dict of its fields and the line after it.  The grid ends
+    with its orthogonality flags (4 integers)."""
+    g = {}
+    g["NNi"], g["INl"], g["INu"] = _ints(lines[k])
+    g["NNj"], g["JNl"], g["JNu"] = _ints(lines[k + 1])
+    g["nghost"] = int(lines[k + 2])
+    g["max_rec_order"], g["ho_boundary"] = int(lines[k + 3]), int(lines[k + 4])
+    g["NCi"], g["ICl"], g["ICu"] = _ints(lines[k + 5])
+    g["NCj"], g["JCl"], g["JCu"] = _ints(lines[k + 6])
+    k += 7
+    NNi, NNj, NCi, NCj = g["NNi"], g["NNj"], g["NCi"], g["NCj"]
     nn = NNi * NNj
     xy = np.array([[float(t) for t in lines[k + q].split()] for q in range(nn)])
     k += nn
-    xn, yn = xy[:, 0].reshape(NNj, NNi), xy[:, 1].reshape(NNj, NNi)
+    g["xn"], g["yn"] = xy[:, 0].reshape(NNj, NNi), xy[:, 1].reshape(NNj, NNi)
     ncell = NCi * NCj
     cells = np.empty((ncell, 3))
     tails = []
@@ -128,7 +127,7 @@ def read_soln(path):
         tok = lines[k].split()
         i, j = int(tok[0]), int(tok[1])
         if (i, j) != (q % NCi, q // NCi):
-            raise ValueError("%s: cell record %d is (%d, %d)" % (path, q, i, j))
+            raise ValueError("cell record %d is (%d, %d)" % (q, i, j))
         cells[q] = [float(tok[2]), float(tok[3]), float(tok[4])]
         k += 1
         t0 = k
@@ -140,21 +139,55 @@ def read_soln(path):
             while len(lines[k].split()) != 2:
                 k += 1
         tails.append("\n".join(lines[t0:k]) + "\n")
+    g["cells"], g["cell_tails"] = cells.reshape(NCj, NCi, 3), tails
     bcNS = np.array([_ints(lines[k + q]) for q in range(NCi)], dtype=np.int32)
     k += NCi
     bcEW = np.array([_ints(lines[k + q]) for q in range(NCj)], dtype=np.int32)
     k += NCj
-    sol_head = "%d %d %d %d" % (NCi, ICl, ICu, nghost)
+    g["bcN"], g["bcS"], g["bcE"], g["bcW"] = bcNS[:, 0], bcNS[:, 1], bcEW[:, 0], bcEW[:, 1]
+    # twelve spline sections (kept verbatim), then the spline parameter
+    # ranges (two lines of 4 reals), the stretching parameters (two lines
+    # "int real real") and the orthogonality flags (4 integers)
     t0 = k
-    while lines[k] != sol_head:
+    while not _grid_end_at(lines, k):
         k += 1
-    grid_tail = "\n".join(lines[t0:k]) + "\n"
-    k += 1
-    if _ints(lines[k]) != [NCj, JCl, JCu]:
+    k += 5
+    g["grid_tail"] = "\n".join(lines[t0:k]) + "\n"
+    return g, k
+
+
+def _is_real(t):
+    return ("e" in t or "." in t) and t.lstrip("-+").replace(".", "", 1).replace("e", "", 1).replace("-", "").replace(
+        "+", "").isdigit()
+
+
+def _grid_end_at(lines, k):
+    """lines[k:k+5] are the ranges, stretching and orthogonality lines that
+    close a grid (HO_Grid2DQuad.cc:13989-13996)"""
+    if k + 4 >= len(lines):
+        raise ValueError("grid section does not end")
+    r1, r2, s1, s2, o = (lines[k + q].split() for q in range(5))
+    if len(r1) != 4 or len(r2) != 4 or len(s1) != 3 or len(s2) != 3 or len(o) != 4:
+        return False
+    if not all(_is_real(t) for t in r1 + r2 + s1[1:] + s2[1:]):
+        return False
+    return all(t.lstrip("-").isdigit() for t in o + [s1[0], s2[0]])
+
+
+def read_soln(path):
+    """Parse one block restart file (gzip-compressed if it ends in .gz)."""
+    lines = _read_text(path).split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    header, gas = lines[0], lines[1]
+    g, k = _parse_grid(lines, 2)
+    NCi, NCj, ICl, ICu, JCl, JCu = g["NCi"], g["NCj"], g["ICl"], g["ICu"], g["JCl"], g["JCu"]
+    if lines[k] != "%d %d %d %d" % (NCi, ICl, ICu, g["nghost"]) or _ints(lines[k + 1]) != [NCj, JCl, JCu]:
         raise ValueError("%s: unexpected solution header %r" % (path, lines[k]))
-    k += 1
+    k += 2
     axisym = int(lines[k])
     k += 1
+    ncell = NCi * NCj
     mant = lines[k].split()[0].lstrip("-").split("e")[0]
     digits = len(mant.split(".")[1]) if "." in mant else 15
     U = np.array([[float(t) for t in lines[k + q].split()] for q in range(ncell)]).reshape(NCj, NCi, 4)
@@ -164,17 +197,12 @@ def read_soln(path):
     wo2 = np.array([[float(t) for t in lines[k + q].split()] for q in range(2 * NCi)])
     k += 2 * NCi
     tail = "\n".join(lines[k:]) + "\n"
-    return SolnBlock(header=header, gas=gas, NNi=NNi, INl=INl, INu=INu, NNj=NNj, JNl=JNl, JNu=JNu, nghost=nghost,
-                     max_rec_order=max_rec, ho_boundary=ho_bnd, NCi=NCi, ICl=ICl, ICu=ICu, NCj=NCj, JCl=JCl, JCu=JCu,
-                     xn=xn, yn=yn, cells=cells.reshape(NCj, NCi, 3), cell_tails=tails, bcN=bcNS[:, 0],
-                     bcS=bcNS[:, 1], bcE=bcEW[:, 0], bcW=bcEW[:, 1], grid_tail=grid_tail, axisymmetric=axisym, U=U,
-                     WoW=wo[0::2], WoE=wo[1::2], WoS=wo2[0::2], WoN=wo2[1::2], tail=tail, state_digits=digits)
+    return SolnBlock(header=header, gas=gas, axisymmetric=axisym, U=U, WoW=wo[0::2], WoE=wo[1::2],
+                     WoS=wo2[0::2], WoN=wo2[1::2], tail=tail, state_digits=digits, **g)
 
 
-def format_soln(B):
-    """The text of a block restart file (inverse of read_soln)."""
-    out = [B.header + "\n", B.gas + "\n",
-           "%d %d %d\n" % (B.NNi, B.INl, B.INu), "%d %d %d\n" % (B.NNj, B.JNl, B.JNu),
+def _format_grid(B):
+    out = ["%d %d %d\n" % (B.NNi, B.INl, B.INu), "%d %d %d\n" % (B.NNj, B.JNl, B.JNu),
            "%d\n" % B.nghost, "%d\n" % B.max_rec_order, "%d\n" % B.ho_boundary,
            "%d %d %d\n" % (B.NCi, B.ICl, B.ICu), "%d %d %d\n" % (B.NCj, B.JCl, B.JCu)]
     out += [" %s %s\n" % (_e15(x), _e15(y)) for x, y in zip(B.xn.ravel(), B.yn.ravel())]
@@ -185,6 +213,12 @@ def format_soln(B):
     out += ["%d %d\n" % (n, s) for n, s in zip(B.bcN, B.bcS)]
     out += ["%d %d\n" % (e, w) for e, w in zip(B.bcE, B.bcW)]
     out.append(B.grid_tail)
+    return out
+
+
+def format_soln(B):
+    """The text of a block restart file (inverse of read_soln)."""
+    out = [B.header + "\n", B.gas + "\n"] + _format_grid(B)
     out += ["%d %d %d %d\n" % (B.NCi, B.ICl, B.ICu, B.nghost), "%d %d %d\n" % (B.NCj, B.JCl, B.JCu),
             "%d\n" % B.axisymmetric]
     fs = " %%.%de" % B.state_digits
@@ -203,6 +237,109 @@ def format_soln(B):
 def write_soln(path, B):
     with open(path, "w") as f:
         f.write(format_soln(B))
+
+
+# ---- grid data files (Grid_Type = Read_From_Data_File) -----------------------------
+@dataclass
+class GridBlock:
+    """One block of a grid data file: the grid fields of SolnBlock only."""
+    NNi: int
+    INl: int
+    INu: int
+    NNj: int
+    JNl: int
+    JNu: int
+    nghost: int
+    max_rec_order: int
+    ho_boundary: int
+    NCi: int
+    ICl: int
+    ICu: int
+    NCj: int
+    JCl: int
+    JCu: int
+    xn: np.ndarray
+    yn: np.ndarray
+    cells: np.ndarray
+    cell_tails: list
+    bcN: np.ndarray
+    bcS: np.ndarray
+    bcE: np.ndarray
+    bcW: np.ndarray
+    grid_tail: str
+
+
+@dataclass
+class GridFile:
+    grid_type: str
+    i_grid: int
+    nbi: int
+    nbj: int
+    blocks: list            # [nbj][nbi] GridBlock (j outer, as written)
+
+
+def read_grid_file(path):
+    """A multi-block grid data file (Write_Multi_Block_Grid_Using_IP /
+    Read_Multi_Block_Grid_Using_IP, HO_Grid2DQuadMultiBlock.h:3160-3210):
+    the grid type, i_Grid, "NBi NBj", then every block's grid, j outer."""
+    lines = _read_text(path).split("\n")
+    if lines and lines[-1] == "":
+        lines.pop()
+    grid_type, i_grid = lines[0], int(lines[1])
+    nbi, nbj = _ints(lines[2])
+    k = 3
+    rows = []
+    for _ in range(nbj):
+        row = []
+        for _ in range(nbi):
+            g, k = _parse_grid(lines, k)
+            row.append(GridBlock(**g))
+        rows.append(row)
+    if k != len(lines):
+        raise ValueError("%s: %d lines after the last block" % (path, len(lines) - k))
+    return GridFile(grid_type, i_grid, nbi, nbj, rows)
+
+
+def format_grid_file(G):
+    out = [G.grid_type + "\n", "%d\n" % G.i_grid, "%d %d\n" % (G.nbi, G.nbj)]
+    for row in G.blocks:
+        for B in row:
+            out += _format_grid(B)
+    return "".join(out)
+
+
+def grid_file_to_mesh(G, wo=None):
+    """The solver's Mesh of a grid data file's block array (two inner ghost
+    layers kept; same-level neighbours from the array; the reference states
+    Wo set to wo, or zero)."""
+    from . import mesh as M
+    nbi, nbj = G.nbi, G.nbj
+    xs, ys, bc, ni, nj = [], [], [], [], []
+    nbr = np.full((nbi * nbj, 8), -1, np.int32)
+    for jb in range(nbj):
+        for ib in range(nbi):
+            B = G.blocks[jb][ib]
+            g = B.nghost
+            if g < NG:
+                raise ValueError("a grid block with %d ghost layers: the solver needs %d" % (g, NG))
+            d = g - NG
+            xs.append(B.xn[d:B.NNj - d, d:B.NNi - d].ravel())
+            ys.append(B.yn[d:B.NNj - d, d:B.NNi - d].ravel())
+            ci, cj = slice(d, B.NCi - d), slice(d, B.NCj - d)
+            bc += [B.bcW[cj], B.bcE[cj], B.bcS[ci], B.bcN[ci]]
+            ni.append(B.NCi - 2 * g)
+            nj.append(B.NCj - 2 * g)
+            b = jb * nbi + ib
+
+            def nb(di, dj):
+                a, c = ib + di, jb + dj
+                return c * nbi + a if (0 <= a < nbi and 0 <= c < nbj) else -1
+            nbr[b] = [nb(0, 1), nb(0, -1), nb(1, 0), nb(-1, 0), nb(1, 1), nb(-1, 1), nb(1, -1), nb(-1, -1)]
+    bc = np.concatenate(bc).astype(np.int32)
+    wo_arr = np.zeros((bc.size, 4)) if wo is None else np.tile(np.asarray(wo, np.float64), (bc.size, 1))
+    return M.Mesh(nblk=nbi * nbj, ni=np.array(ni, np.int32), nj=np.array(nj, np.int32), xnode=np.concatenate(xs),
+                  ynode=np.concatenate(ys), nbr=nbr, bc=bc, wo=wo_arr,
+                  info={"kind": "grid_file", "grid_type": G.grid_type, "nbi": nbi, "nbj": nbj})
 
 
 # ---- quadtree files -------------------------------------------------------------

@@ -103,3 +103,47 @@ def test_quadtree_restart_reads_back_topology(tmp_path):
     np.testing.assert_allclose(U2, U, rtol=1e-15)
     with open(prefix + "_quadtree.tree") as f:
         assert restart.format_tree(restart.read_tree(prefix + "_quadtree.tree")) == f.read()
+
+
+GRIDFILE = os.path.join(G, "CartesianMesh_3x3.grid.gz")
+
+
+def test_grid_file_round_trip_is_byte_exact():
+    """The reference's own grid data file (HighOrder2D unit tests: 3 x 3
+    blocks, 5 ghost layers, third-order cell moments)."""
+    Gf = restart.read_grid_file(GRIDFILE)
+    with gzip.open(GRIDFILE, "rt") as f:
+        assert restart.format_grid_file(Gf) == f.read()
+    assert (Gf.grid_type, Gf.nbi, Gf.nbj) == ("Square", 3, 3)
+
+
+def test_grid_file_mesh_is_consistent():
+    """The converted mesh: its ghost nodes are the neighbouring blocks'
+    interior nodes to round-off (the reference's grid generator computed
+    them separately: -2.8e-17 against 0.0 on one line), and the host
+    checks of cffc_create accept it."""
+    import ctypes
+    Gf = restart.read_grid_file(GRIDFILE)
+    m = restart.grid_file_to_mesh(Gf)
+    assert m.nblk == 9 and set(m.ni.tolist()) == {10} and set(m.nj.tolist()) == {20}
+    for b in range(m.nblk):
+        x, y = m.block_nodes(b)
+        e = m.nbr[b, abi.NBR_E]
+        if e >= 0:
+            xe, ye = m.block_nodes(e)
+            n = int(m.ni[b])
+            # b's east ghost node columns n+3, n+4 = e's interior columns 3, 4
+            np.testing.assert_allclose(x[:, n + 3:n + 5], xe[:, 3:5], rtol=0, atol=1e-15)
+            np.testing.assert_allclose(y[:, n + 3:n + 5], ye[:, 3:5], rtol=0, atol=1e-15)
+        nn = m.nbr[b, abi.NBR_N]
+        if nn >= 0:
+            xn_, yn_ = m.block_nodes(nn)
+            n = int(m.nj[b])
+            np.testing.assert_allclose(y[n + 3:n + 5, :], yn_[3:5, :], rtol=0, atol=1e-15)
+    from cffc_amd import euler2d as E
+    L = E.load()
+    L.cffc_check_mesh.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)]
+    keep = []
+    md = abi.mesh_desc(m, keep)
+    counts = (ctypes.c_int64 * 4)()
+    assert L.cffc_check_mesh(ctypes.byref(md), counts) == 0, L.cffc_last_error()
