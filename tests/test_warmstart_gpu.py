@@ -3,7 +3,7 @@
 A U-Net is trained on the GPU, by the library's own BF16 MFMA forward and
 backward kernels, on steady bump states that the GPU solver produced; the
 held-out cases are then solved from the freestream and from the network's
-prediction to the same tolerance.  The solver judges the prediction: both
+prediction to the same tolerance.  The solver decides on the prediction: both
 runs must reach the tolerance, and the warm runs must need fewer steps.
 Reduced sizes (96 x 32 cells, tol 1e-8) keep the test to about a minute;
 the full 192 x 64, 512-case, 1e-10 study is run by
