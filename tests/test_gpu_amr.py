@@ -185,3 +185,50 @@ def test_mixed_precision_against_fp64(E):
     assert mdrift < 1e-6
     a.close()
     b.close()
+
+
+def _blob_case():
+    """Free stream (M 0.4 along x, fixed-state boundaries) with a density
+    blob: it convects out and the residual decays to a steady state."""
+    m = M.quadtree_box(nroot=4, n=8, max_level=2, refine=M.circle_refinement(0.1), jitter=0.1, seed=6)
+    W0s = M.freestream(0.4)
+    m.bc[:] = np.where(m.bc != abi.BC_NONE, abi.BC_FIXED, abi.BC_NONE)
+    m.wo[:] = W0s
+    W0 = M.uniform_state(m, W0s)
+    c = m.cell_centres()
+    W0[:, 0] *= 1.0 + 0.2 * np.exp(-((c[:, 0] - 0.45) ** 2 + (c[:, 1] - 0.5) ** 2) / 0.01)
+    return m, W0
+
+
+def test_steady_device_loop_equals_host_loop_and_oracle(E):
+    """Limiters frozen at 1e-4 then run to 1e-8: the device-driven loop (the
+    freeze reaches the prolongation through the part state), the host-driven
+    reference order (Freeze_Limiters between two runs) and the oracle take
+    the same steps to the same state."""
+    from cffc_amd import steady
+    m, W0 = _blob_case()
+    s = steady.bump_scheme()
+    g = E.Euler2DQuadBlocks(m, s)
+    g.set_W(W0)
+    ref = steady.solve_steady_host_loop(g, tol=1e-8, max_steps=20000, freeze_level=1e-4)
+    W_host = g.get_W()
+    g.close()
+    g = E.Euler2DQuadBlocks(m, s)
+    g.set_W(W0)
+    r = steady.solve_steady(g, tol=1e-8, max_steps=20000, freeze_level=1e-4)
+    W_dev = g.get_W()
+    g.close()
+    print("host loop %d steps (frozen at %d), device loop %d steps" % (ref.steps, ref.frozen_at, r.steps))
+    assert ref.rc == 0 and r.rc == 0 and ref.frozen_at > 0
+    assert r.steps == ref.steps
+    assert np.array_equal(W_dev, W_host)
+    orc = oracle.Oracle(m, s)
+    orc.set_W(W0)
+    rc1, n1, _, _ = orc.run(20000, tol=1e-4, nthreads=4)
+    orc.freeze_limiters(True)
+    rc2, n2, _, _ = orc.run(20000 - n1, tol=1e-8, nthreads=4)
+    print("oracle %d + %d steps" % (n1, n2))
+    assert n1 == ref.frozen_at and n1 + n2 == ref.steps
+    Wo = orc.get_W()
+    print("host loop vs oracle: W rel %.3e, %d non-bitwise" % (relerr(W_host, Wo), nonbitwise(W_host, Wo)))
+    assert relerr(W_host, Wo) <= REL
