@@ -99,15 +99,28 @@ def main(argv=None):
     ap.add_argument("--fp64", action="store_true", help="FP64 only (default: FP32 flux, FP64 finish)")
     ap.add_argument("--max-steps", type=int, default=400000)
     ap.add_argument("--out", default=None)
+    ap.add_argument("--only", choices=["both", "cold", "warm"], default="both",
+                    help="run only the cold or only the warm batch (warm needs --model)")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     cases = serving_cases(a.n, a.seed)
     log("%d cases, %d cells" % (len(cases), sum(96 * k * 32 * k for _, _, k in cases)))
-    if a.model:
+    kw = dict(mixed=not a.fp64, max_steps=a.max_steps, log=log)
+    cold = warm = None
+    if a.only == "warm" or (a.only == "both" and a.model):
+        if not a.model:
+            raise SystemExit("--only warm needs --model")
         net, norm = warmstart.load_model(a.model)
-        cold, warm = serve_cold_warm(net, norm, cases, mixed=not a.fp64, max_steps=a.max_steps, log=log)
+        if a.only == "both":
+            cold, warm = serve_cold_warm(net, norm, cases, **kw)
+        else:
+            t0 = time.perf_counter()
+            init = warm_states(net, norm, cases)
+            t_inf = time.perf_counter() - t0
+            warm = serve(cases, initial=init, **kw)
+            warm["inference_seconds"] = t_inf
     else:
-        cold, warm = serve(cases, mixed=not a.fp64, max_steps=a.max_steps, log=log), None
+        cold = serve(cases, **kw)
     for tag, r in (("cold", cold), ("warm", warm)):
         if r:
             log("%s: %d/%d converged, %.2f solves/s, p50 %.2f s, p99 %.2f s, wall %.1f s" %
