@@ -390,7 +390,10 @@ __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMe
 // so every thread reconstructs exactly one cell.
 template <int TI, int TJ> struct TileThreads {
     static constexpr int tile = TI * TJ, ring = 2 * (TI + TJ);
-    static constexpr int value = ((tile + ring + 63) / 64) * 64;
+    // tiles of 256 cells or more: one thread per tile cell, and the ring
+    // cells' reconstructions and ring-face fluxes in a second pass
+    static constexpr bool ring_pass = tile >= 256;
+    static constexpr int value = ring_pass ? ((tile + 63) / 64) * 64 : ((tile + ring + 63) / 64) * 64;
 };
 
 template <typename FT, int TI, int TJ, bool FUSED>
@@ -410,6 +413,10 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     __shared__ FT sPF[NP > NF ? NP : NF];
     FT(*sPx)[RJ][RI] = reinterpret_cast<FT(*)[RJ][RI]>(sPF);
     FT(*sPy)[RJ][RI] = reinterpret_cast<FT(*)[RJ][RI]>(sPF + 4 * RJ * RI);
+    // ring pass: the W and S ring-face fluxes, outside the shared buffer, so
+    // that they can be stored as soon as they are computed
+    constexpr bool RP = TileThreads<TI, TJ>::ring_pass;
+    __shared__ FT sRing[RP ? 4 * (TI + TJ) : 1];
     FT(*sFi)[TJ][TI + 1] = reinterpret_cast<FT(*)[TJ][TI + 1]>(sPF);
     FT(*sFj)[TJ + 1][TI] = reinterpret_cast<FT(*)[TJ + 1][TI]>(sPF + 4 * TJ * (TI + 1));
     __shared__ double sRed[NT / 64][4];
@@ -492,9 +499,17 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     // boundaries, and cells away from the ring next to a physical boundary,
     // use the full 8-point stencil; only the ring cells of boundary blocks
     // read the BC arrays (reduced stencils).
-    {
+    for (int role = tid; role < (RP ? NTILE + 2 * (TI + TJ) : tid + 1); role += NT) {
+        // role: a tile cell, then the W, E, S, N ring cells
         int ir = -1, jr = -1;
-        if (is_tile) { ir = tc + 1; jr = tr + 1; }
+        if (RP) {
+            int r = role - NTILE;
+            if (r < 0) { ir = role % TI + 1; jr = role / TI + 1; }
+            else if (r < TJ) { ir = 0; jr = r + 1; }
+            else if ((r -= TJ) < TJ) { ir = TI + 1; jr = r + 1; }
+            else if ((r -= TJ) < TI) { ir = r + 1; jr = 0; }
+            else { r -= TI; ir = r + 1; jr = TJ + 1; }
+        } else if (is_tile) { ir = tc + 1; jr = tr + 1; }
         else if (ring_side == 0) { ir = 0; jr = ring_k + 1; }
         else if (ring_side == 1) { ir = TI + 1; jr = ring_k + 1; }
         else if (ring_side == 2) { ir = ring_k + 1; jr = 0; }
@@ -600,6 +615,18 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
     // up to two faces per thread: (idir, f, g) of each
     CS<FT> F1, F2;
     bool h1 = false, h2 = false;
+    if (RP && tid < TI + TJ) {
+        // ring pass: W ring face (row tid) or S ring face (column tid - TJ),
+        // stored at once (first, so no flux is held while computing it)
+        CS<FT> Fr;
+        const bool wface = tid < TJ;
+        const int k = wface ? tid : tid - TJ;
+        if (face(wface, 0, k, Fr)) {
+            FT *dst = wface ? sRing + k : sRing + 4 * TJ + k;
+            const int st = wface ? TJ : TI;
+            dst[0] = Fr.d; dst[st] = Fr.mx; dst[2 * st] = Fr.my; dst[3 * st] = Fr.E;
+        }
+    }
     if (is_tile) {
         h1 = face(true, tc + 1, tr, F1);
         h2 = face(false, tr + 1, tc, F2);
@@ -652,22 +679,32 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                 for (int v = 0; v < 4; ++v) dU[v] = 0.0;
             }
         }
+        // W and S face fluxes of this cell (from the ring array on the tile's
+        // W column / S row in the ring pass)
+        auto fW = [&](int v) -> FT {
+            if constexpr (RP) return tc == 0 ? sRing[v * TJ + tr] : sFi[v][tr][tc];
+            else return sFi[v][tr][tc];
+        };
+        auto fS = [&](int v) -> FT {
+            if constexpr (RP) return tr == 0 ? sRing[4 * TJ + v * TI + tc] : sFj[v][tr][tc];
+            else return sFj[v][tr][tc];
+        };
         if constexpr (Fast<FT>::on) {
             // mixed path: net face flux in FP32, one FP64 scaling per cell
             const double s = cdt / A;
             const FT flW = (FT)lW, flE = (FT)lE, flS = (FT)lS, flN = (FT)lN;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const FT net = (sFi[v][tr][tc] * flW - sFi[v][tr][tc + 1] * flE) +
-                               (sFj[v][tr][tc] * flS - sFj[v][tr + 1][tc] * flN);
+                const FT net = (fW(v) * flW - sFi[v][tr][tc + 1] * flE) +
+                               (fS(v) * flS - sFj[v][tr + 1][tc] * flN);
                 dU[v] += s * (double)net;
             }
         } else {
             // ((CFL*dt)*F*l)/A per face, +W -E +S -N (Euler2DQuad.cc:1502-1509)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const double FW = (double)sFi[v][tr][tc], FE = (double)sFi[v][tr][tc + 1];
-                const double FS = (double)sFj[v][tr][tc], FN = (double)sFj[v][tr + 1][tc];
+                const double FW = (double)fW(v), FE = (double)sFi[v][tr][tc + 1];
+                const double FS = (double)fS(v), FN = (double)sFj[v][tr + 1][tc];
                 double d = dU[v];
                 d += cdt * FW * lW / A;
                 d -= cdt * FE * lE / A;
@@ -688,7 +725,7 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             if ((m.amr & AMR_FLUX_S) && jp == 2) {
                 double *e = eb + 4 * (m.ni + ip - 2);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) e[v] = (-(double)sFj[v][tr][tc]) * lS;
+                for (int v = 0; v < 4; ++v) e[v] = (-(double)fS(v)) * lS;
             }
             if ((m.amr & AMR_FLUX_E) && ip == ICu) {
                 double *e = eb + 4 * (2 * m.ni + jp - 2);
@@ -698,7 +735,7 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             if ((m.amr & AMR_FLUX_W) && ip == 2) {
                 double *e = eb + 4 * (2 * m.ni + m.nj + jp - 2);
 #pragma unroll
-                for (int v = 0; v < 4; ++v) e[v] = (-(double)sFi[v][tr][tc]) * lW;
+                for (int v = 0; v < 4; ++v) e[v] = (-(double)fW(v)) * lW;
             }
         }
         if (!FUSED || defer) {
