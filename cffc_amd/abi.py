@@ -22,6 +22,9 @@ LIMITER_ONE = -1
 LIMITER_ZERO = 0
 LIMITER_VENKATAKRISHNAN = 9
 FLUX_FUNCTION_ROE = 2
+FLUX_FUNCTION_HLLE = 4
+MULTIGRID_V_CYCLE = 1
+MULTIGRID_W_CYCLE = 2
 RECONSTRUCTION_LEAST_SQUARES = 2
 GLOBAL_TIME_STEPPING = 0
 SCALAR_LOCAL_TIME_STEPPING = 1
@@ -47,6 +50,16 @@ class SchemeDesc(ctypes.Structure):
                 ("cfl_number", ctypes.c_double), ("i_reconstruction", ctypes.c_int32),
                 ("i_limiter", ctypes.c_int32), ("i_flux_function", ctypes.c_int32),
                 ("local_time_stepping", ctypes.c_int32), ("flux_precision", ctypes.c_int32)]
+
+
+class MGDesc(ctypes.Structure):
+    _fields_ = [("levels", ctypes.c_int32), ("i_cycle", ctypes.c_int32),
+                ("smooths_on_finest_level", ctypes.c_int32), ("pre_smooths", ctypes.c_int32),
+                ("post_smooths", ctypes.c_int32), ("smooths_on_coarsest_level", ctypes.c_int32),
+                ("defect_correction", ctypes.c_int32), ("prolong_using_injection", ctypes.c_int32),
+                ("first_order_coarse", ctypes.c_int32), ("injection_at_dirichlet", ctypes.c_int32),
+                ("update_stability_switch", ctypes.c_int32), ("max_update_reductions", ctypes.c_int32),
+                ("absolute_tolerance", ctypes.c_double), ("relative_tolerance", ctypes.c_double)]
 
 
 @dataclass

@@ -112,6 +112,7 @@ struct cffc_ctx {
     size_t ev_used = 0;
     double t_ms[2] = {0, 0};
     long long t_n[2] = {0, 0};
+    bool own_stream = true;   // false: a multigrid level on its finest level's stream
 };
 
 // ---------------------------------------------------------------------------
@@ -126,7 +127,7 @@ static void free_ctx(cffc_ctx *c) {
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    if (c->stream && c->own_stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -880,6 +881,7 @@ static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long lon
     p.blocks = nullptr;
     p.bcs_only = 0;
     p.no_bcs = 0;
+    p.store_U = 0;
     p.Wf = nullptr;
     return p;
 }
@@ -1633,4 +1635,363 @@ extern "C" int cffc_kernel_times(cffc_ctx *c, double out[4]) {
     out[2] = c->t_ms[0];
     out[3] = (double)c->t_n[0];
     return 0;
+}
+
+// ---- FAS multigrid (FASMultigrid2D/FASMultigrid2D.h) -------------------------
+// One context per level on one stream, level 0 the finest; level l+1 halves
+// level l block by block.  The cycle is the reference's Coarse_Grid_Correction
+// with Top_Level = 0, every smoothing stage through the granular FP64 stage
+// kernel (HLLE and the zero limiter below the top level when
+// first_order_coarse is set), the forcing term folded into that kernel after
+// the face fluxes, and the level transfers of euler2d_mg.hip.  Per-cell dt on
+// every level (Smooth's CFL call does not apply the global minimum).
+struct cffc_mg {
+    std::vector<cffc_ctx *> lv;
+    cffc_mg_desc d{};
+    std::vector<double *> P, uo;     // per level (P unused on level 0)
+    std::vector<int> max_edge;       // per level: max ni + nj over blocks
+    std::vector<double> Ahost;       // scratch: fine-level areas for the Wo restriction
+    double rs_eps = 0.0;
+    int rs_iters = 0;
+    bool frozen = false;
+};
+
+static void mg_free(cffc_mg *mg) {
+    if (!mg) return;
+    for (size_t l = 0; l < mg->lv.size(); ++l) {
+        if (l < mg->P.size() && mg->P[l]) (void)hipFree(mg->P[l]);
+        if (l < mg->uo.size() && mg->uo[l]) (void)hipFree(mg->uo[l]);
+    }
+    // coarse levels share the finest level's stream: free them first
+    for (size_t l = mg->lv.size(); l-- > 0;)
+        if (mg->lv[l]) free_ctx(mg->lv[l]);
+    delete mg;
+}
+
+extern "C" void cffc_mg_destroy(cffc_mg *mg) { mg_free(mg); }
+
+extern "C" cffc_ctx *cffc_mg_level(cffc_mg *mg, int level) {
+    if (!mg || level < 0 || level >= (int)mg->lv.size()) return nullptr;
+    return mg->lv[level];
+}
+
+extern "C" int cffc_mg_create(const cffc_mesh_desc *levels, int nlevels, const cffc_scheme_desc *smoother,
+                              const cffc_mg_desc *d, cffc_mg **out) {
+    if (!levels || !smoother || !d || !out) return fail(CFFC_E_ARG, "null argument");
+    *out = nullptr;
+    if (nlevels < 2 || nlevels != d->levels) return fail(CFFC_E_ARG, "levels (%d) must be >= 2 and match the meshes (%d)",
+                                                         d->levels, nlevels);
+    if (d->i_cycle != CFFC_MULTIGRID_V_CYCLE && d->i_cycle != CFFC_MULTIGRID_W_CYCLE)
+        return fail(CFFC_E_ARG, "unknown multigrid cycle %d", d->i_cycle);
+    if (smoother->flux_precision != CFFC_FLUX_FP64) return fail(CFFC_E_ARG, "multigrid runs the FP64 path only");
+    if (d->smooths_on_finest_level < 0 || d->pre_smooths < 0 || d->post_smooths < 0 ||
+        d->smooths_on_coarsest_level < 0 || d->max_update_reductions < 0)
+        return fail(CFFC_E_ARG, "negative smoothing or reduction count");
+    for (int l = 0; l < nlevels; ++l) {
+        if (levels[l].level || levels[l].nbr2) return fail(CFFC_E_ARG, "multigrid levels must be single-level meshes");
+        if (levels[l].nblk != levels[0].nblk) return fail(CFFC_E_ARG, "level %d has a different block count", l);
+        for (int b = 0; b < levels[0].nblk && l > 0; ++b)
+            if (levels[l - 1].ni[b] != 2 * levels[l].ni[b] || levels[l - 1].nj[b] != 2 * levels[l].nj[b])
+                return fail(CFFC_E_ARG, "block %d of level %d does not halve level %d", b, l, l - 1);
+    }
+    cffc_mg *mg = new cffc_mg();
+    mg->d = *d;
+    mg->lv.assign(nlevels, nullptr);
+    mg->P.assign(nlevels, nullptr);
+    mg->uo.assign(nlevels, nullptr);
+    mg->max_edge.assign(nlevels, 0);
+    for (int l = 0; l < nlevels; ++l) {
+        if (int e = cffc_create(&levels[l], smoother, &mg->lv[l])) {
+            mg_free(mg);
+            return e;
+        }
+        cffc_ctx *c = mg->lv[l];
+        if (l > 0) {
+            (void)hipStreamDestroy(c->stream);
+            c->stream = mg->lv[0]->stream;
+            c->own_stream = false;
+            if (d->first_order_coarse) {
+                // dUdt_Multistage_Explicit_for_Multigrid :2502-2509
+                c->s.i_flux_function = CFFC_FLUX_FUNCTION_HLLE;
+                c->s.i_limiter = CFFC_LIMITER_ZERO;
+            }
+        }
+        int e;
+        if ((e = ensure_granular(c)) || (e = ensure_phi(c))) {
+            mg_free(mg);
+            return e;
+        }
+        const size_t n = 4 * (size_t)c->ncp;
+        if ((e = dalloc(&mg->uo[l], n)) || (l > 0 && (e = dalloc(&mg->P[l], n)))) {
+            mg_free(mg);
+            return e;
+        }
+        if (l > 0 && hipMemsetAsync(mg->P[l], 0, n * sizeof(double), c->stream) != hipSuccess) {
+            mg_free(mg);
+            return fail(CFFC_E_HIP, "hipMemsetAsync failed");
+        }
+        for (int b = 0; b < c->nblk; ++b) mg->max_edge[l] = std::max(mg->max_edge[l], c->meta_h[b].ni + c->meta_h[b].nj);
+    }
+    if (hipStreamSynchronize(mg->lv[0]->stream) != hipSuccess) {
+        mg_free(mg);
+        return fail(CFFC_E_HIP, "multigrid set-up failed");
+    }
+    *out = mg;
+    return 0;
+}
+
+extern "C" int cffc_mg_set_residual_smoothing(cffc_mg *mg, double epsilon, int iterations) {
+    if (!mg || iterations < 0 || !(epsilon >= 0.0)) return fail(CFFC_E_ARG, "bad argument");
+    mg->rs_eps = epsilon;
+    mg->rs_iters = iterations;
+    return 0;
+}
+
+// Exchange_Solution_Information + BCs of one level (U sent, W = W(U))
+static int mg_exchange(cffc_ctx *c) {
+    ExchangeParams p = exchange_params(c, 1, nullptr);
+    p.store_U = 1;
+    HIPCHK(cffc_launch_exchange(&p, c->nblk, c->stream));
+    return 0;
+}
+
+static const double *area(const cffc_ctx *c) { return c->geom + (size_t)GEO_A * c->ncp; }
+
+// one granular FP64 stage kernel: a smoothing stage (residual = 0) or
+// dUdt_Residual_Evaluation into dUdt[0] (residual = 1); P: forcing term or null
+static int mg_stage(cffc_ctx *c, int i_stage, int residual, const double *P) {
+    StageParams p = base_stage_params(c);
+    p.Win = c->W[c->wcur];
+    p.Wout = c->W[c->wcur];
+    const int k = residual ? 0 : k_residual(c->s, i_stage);
+    p.S = c->dUdt[k];
+    p.S1 = c->dUdt[1];
+    p.S2 = c->dUdt[2];
+    p.mode = residual ? GINIT_ZERO : granular_init(c->s, i_stage);
+    p.dtmin_in = c->scal;
+    p.lts = 1;
+    p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE_STORE;
+    p.hlle = c->s.i_flux_function == CFFC_FLUX_FUNCTION_HLLE;
+    p.residual = residual;
+    p.P = P;
+    HIPCHK(cffc_launch_stage_fp64(&p, c->ti, c->tj, 0, c->stream));
+    return 0;
+}
+
+// Smooth :2865-3110 (Top_Level = 0)
+static int mg_smooth(cffc_mg *mg, int l) {
+    cffc_ctx *c = mg->lv[l];
+    HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, 0, 1, c->stream));
+    HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
+                           c->stream));
+    if (l > 0) {
+        cffc_ctx *f = mg->lv[l - 1];
+        HIPCHK(cffc_launch_mg_cfl(f->meta, c->meta, c->nblk, c->max_int, f->dt, area(f), c->dt, area(c), c->stream));
+    }
+    for (int st = 1; st <= c->s.n_stage; ++st) {
+        if (int e = mg_exchange(c)) return e;
+        if (st == 1) HIPCHK(cffc_launch_copy_interior(c->meta, c->nblk, c->max_int, c->U, c->Uo, c->ncp, c->stream));
+        if (int e = mg_stage(c, st, 0, l > 0 ? mg->P[l] : nullptr)) return e;
+        if (mg->rs_iters > 0)
+            HIPCHK(cffc_launch_residual_smoothing(c->meta, c->nblk, c->dUdt[0], c->dUdt[1], c->ncp, mg->rs_eps,
+                                                  mg->rs_iters, nullptr, nullptr, c->stream));
+        const int k = k_residual(c->s, st);
+        HIPCHK(cffc_launch_update(c->meta, c->nblk, c->max_int, c->W[c->wcur], c->U, c->Uo, c->dUdt[k], c->dt, c->ncp,
+                                  stage_omega(c->s, st),
+                                  c->s.local_time_stepping == CFFC_SCALAR_LOCAL_TIME_STEPPING, c->scal + 2,
+                                  c->stream));
+    }
+    if (int e = mg_exchange(c)) return e;
+    if (int e = mg_stage(c, 1, 1, l > 0 ? mg->P[l] : nullptr)) return e;
+    if (!c->freeze) c->phi_valid = 1;
+    return 0;
+}
+
+static int mg_store_uo(cffc_mg *mg, int l) {
+    cffc_ctx *c = mg->lv[l];
+    HIPCHK(hipMemcpyAsync(mg->uo[l], c->U, 4 * (size_t)c->ncp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    return 0;
+}
+
+// Coarse_Grid_Correction :3118-3370 (Top_Level = 0)
+static int mg_cgc(cffc_mg *mg, int l) {
+    const cffc_mg_desc &d = mg->d;
+    const int L = (int)mg->lv.size();
+    cffc_ctx *c = mg->lv[l];
+    int e;
+    if (l == L - 1) {
+        if ((e = mg_store_uo(mg, l))) return e;
+        for (int nu = 1; nu <= d.smooths_on_coarsest_level; ++nu)
+            if ((e = mg_smooth(mg, l))) return e;
+    } else {
+        cffc_ctx *cc = mg->lv[l + 1];
+        const int m_cycle = (d.i_cycle == CFFC_MULTIGRID_V_CYCLE || l == 0) ? 1 : 2;
+        for (int m = 1; m <= m_cycle; ++m) {
+            if ((e = mg_store_uo(mg, l))) return e;
+            const int n_pre = l == 0 ? d.smooths_on_finest_level : d.pre_smooths;
+            for (int nu = 0; nu < n_pre; ++nu)
+                if ((e = mg_smooth(mg, l))) return e;
+            HIPCHK(cffc_launch_mg_restrict(c->meta, cc->meta, c->nblk, cc->max_int, c->U, area(c), c->ncp, cc->U,
+                                           area(cc), cc->ncp, c->stream));
+            HIPCHK(cffc_launch_mg_primitive(cc->meta, cc->nblk, cc->max_int, cc->U, cc->W[cc->wcur], cc->ncp,
+                                            c->stream));
+            if ((e = mg_exchange(cc))) return e;
+            if (d.defect_correction) {
+                HIPCHK(cffc_launch_mg_restrict(c->meta, cc->meta, c->nblk, cc->max_int, c->dUdt[0], area(c), c->ncp,
+                                               mg->P[l + 1], area(cc), cc->ncp, c->stream));
+                if ((e = mg_stage(cc, 1, 1, nullptr))) return e;
+                HIPCHK(cffc_launch_mg_subtract(cc->meta, cc->nblk, cc->max_int, mg->P[l + 1], cc->dUdt[0], cc->ncp,
+                                               c->stream));
+            }
+            if ((e = mg_cgc(mg, l + 1))) return e;
+            HIPCHK(cffc_launch_mg_prolong(cc->meta, c->meta, c->nblk, cc->max_int, mg->max_edge[l + 1], cc->bc, cc->U,
+                                          cc->geom, cc->ncp, c->U, c->geom, c->ncp, !d.prolong_using_injection,
+                                          d.injection_at_dirichlet, d.prolong_using_injection,
+                                          d.update_stability_switch, d.max_update_reductions, c->stream));
+            HIPCHK(cffc_launch_mg_primitive(c->meta, c->nblk, c->max_int, c->U, c->W[c->wcur], c->ncp, c->stream));
+            if ((e = mg_exchange(c))) return e;
+            const int n_post = l == 0 ? 0 : d.post_smooths;
+            for (int nu = 0; nu < n_post; ++nu)
+                if ((e = mg_smooth(mg, l))) return e;
+        }
+    }
+    if (l != 0) HIPCHK(cffc_launch_mg_changes(c->U, mg->uo[l], 4 * c->ncp, c->stream));
+    return 0;
+}
+
+// Restrict_Boundary_Ref_States :1112-1186 on the host (the fine areas read
+// back once): coarse WoN/S/E/W over the interior span
+static int mg_restrict_wo(cffc_mg *mg, int lf) {
+    cffc_ctx *f = mg->lv[lf], *c = mg->lv[lf + 1];
+    std::vector<double> Af(f->ncp);
+    HIPCHK(hipMemcpyAsync(Af.data(), area(f), f->ncp * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    const size_t nwf = 4 * (size_t)(f->meta_h.back().bcoff + 2 * f->meta_h.back().ncj + 2 * f->meta_h.back().nci);
+    const size_t nwc = 4 * (size_t)(c->meta_h.back().bcoff + 2 * c->meta_h.back().ncj + 2 * c->meta_h.back().nci);
+    std::vector<double> wf(nwf), wc(nwc);
+    HIPCHK(hipMemcpyAsync(wf.data(), f->wo, nwf * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipMemcpyAsync(wc.data(), c->wo, nwc * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+    HIPCHK(hipStreamSynchronize(f->stream));
+    for (int b = 0; b < c->nblk; ++b) {
+        const BlockMeta &F = f->meta_h[b], &C = c->meta_h[b];
+        auto A = [&](int i, int j) { return Af[F.coff + (int64_t)j * F.pitch + i]; };
+        // entry (side, k) -> AoS offset; sides W, E (ncj entries), S, N (nci)
+        auto fo = [&](int side, int k) {
+            const int base = side == 0 ? 0 : side == 1 ? F.ncj : side == 2 ? 2 * F.ncj : 2 * F.ncj + F.nci;
+            return 4 * (size_t)(F.bcoff + base + k);
+        };
+        auto co = [&](int side, int k) {
+            const int base = side == 0 ? 0 : side == 1 ? C.ncj : side == 2 ? 2 * C.ncj : 2 * C.ncj + C.nci;
+            return 4 * (size_t)(C.bcoff + base + k);
+        };
+        auto avg = [&](int side_f, int k0, double A0, double A1, int side_c, int kc) {
+            const double *w0 = &wf[fo(side_f, k0)], *w1 = &wf[fo(side_f, k0 + 1)];
+            double *o = &wc[co(side_c, kc)];
+            for (int v = 0; v < 4; ++v) o[v] = (w0[v] * A0 + w1[v] * A1) / (A0 + A1);
+        };
+        const int JCl = 2, JCu = F.nj + 1, ICl = 2, ICu = F.ni + 1;
+        for (int ic = 2; ic <= C.ni + 1; ++ic) {
+            const int i0 = 2 * (ic - 2) + 2;
+            avg(3, i0, A(i0, JCu), A(i0 + 1, JCu), 3, ic);
+            avg(2, i0, A(i0, JCl), A(i0 + 1, JCl), 2, ic);
+        }
+        for (int jc = 2; jc <= C.nj + 1; ++jc) {
+            const int j0 = 2 * (jc - 2) + 2;
+            avg(0, j0, A(ICl, j0), A(ICl, j0 + 1), 0, jc);
+            avg(1, j0, A(ICu, j0), A(ICu, j0 + 1), 1, jc);
+        }
+    }
+    HIPCHK(hipMemcpy(c->wo, wc.data(), nwc * sizeof(double), hipMemcpyHostToDevice));
+    return 0;
+}
+
+// The start of Execute :3395-3420 for a state set on level 0: coarse levels
+// restricted from the level above (the IC_RESTART path of Apply_ICs
+// :719-733) with their boundary reference states, then messages and BCs.
+extern "C" int cffc_mg_initialize(cffc_mg *mg) {
+    if (!mg) return fail(CFFC_E_ARG, "null multigrid solver");
+    if (int e = mg_exchange(mg->lv[0])) return e;
+    for (size_t l = 1; l < mg->lv.size(); ++l) {
+        cffc_ctx *f = mg->lv[l - 1], *c = mg->lv[l];
+        HIPCHK(cffc_launch_mg_restrict(f->meta, c->meta, c->nblk, c->max_int, f->U, area(f), f->ncp, c->U, area(c),
+                                       c->ncp, c->stream));
+        HIPCHK(cffc_launch_mg_primitive(c->meta, c->nblk, c->max_int, c->U, c->W[c->wcur], c->ncp, c->stream));
+        if (int e = mg_restrict_wo(mg, (int)l - 1)) return e;
+        if (int e = mg_exchange(c)) return e;
+    }
+    HIPCHK(hipStreamSynchronize(mg->lv[0]->stream));
+    return 0;
+}
+
+static int mg_reset_errors(cffc_mg *mg) {
+    for (cffc_ctx *c : mg->lv) HIPCHK(cffc_launch_set_u64(c->scal + 2, ERR_NONE, 0, 1, c->stream));
+    return 0;
+}
+// the first failing update of the last cycle, finest level first: level in
+// bits 56-63 of *bad_cell, then the update kernel's block/j/i key
+static int mg_check_errors(cffc_mg *mg, int64_t *bad_cell) {
+    std::vector<unsigned long long> keys(mg->lv.size());
+    for (size_t l = 0; l < mg->lv.size(); ++l)
+        HIPCHK(hipMemcpyAsync(&keys[l], mg->lv[l]->scal + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                              mg->lv[0]->stream));
+    HIPCHK(hipStreamSynchronize(mg->lv[0]->stream));
+    for (size_t l = 0; l < keys.size(); ++l)
+        if (keys[l] != ERR_NONE) {
+            if (bad_cell) *bad_cell = (int64_t)(keys[l] | ((unsigned long long)l << 56));
+            return (int)(keys[l] & 0xFFFF);
+        }
+    if (bad_cell) *bad_cell = -1;
+    return 0;
+}
+
+// one FAS cycle from level 0 (Coarse_Grid_Correction(0, 0)), synchronous
+extern "C" int cffc_mg_cycle(cffc_mg *mg, int64_t *bad_cell) {
+    if (!mg) return fail(CFFC_E_ARG, "null multigrid solver");
+    if (int e = mg_reset_errors(mg)) return e;
+    if (int e = mg_cgc(mg, 0)) return e;
+    return mg_check_errors(mg, bad_cell);
+}
+
+extern "C" int cffc_mg_get_P(cffc_mg *mg, int level, double *P, int64_t ncell) {
+    if (!mg || level < 1 || level >= (int)mg->lv.size()) return fail(CFFC_E_ARG, "bad level");
+    return get_field(mg->lv[level], mg->P[level], P, ncell, 4);
+}
+
+// Execute's regular-cycle loop :3440-3705: the level-0 residual norms (of the
+// dUdt[0] the last smoothing left) before each cycle, the relative norm
+// against the second check's, limiter freezing at freeze_level (<= 0: off),
+// and the absolute / relative / cycle-count exits.  norms: 3 per check.
+extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done,
+                               int *checks) {
+    if (!mg || max_cycles < 0) return fail(CFFC_E_ARG, "bad argument");
+    cffc_ctx *f = mg->lv[0];
+    double init_l2 = -1.0, ratio = -1.0;
+    bool first_step = true;
+    int n = 0, nchk = 0, rc = 0;
+    for (int cycles = 1;; ++cycles) {
+        double nr[3];
+        HIPCHK(cffc_launch_norms(f->meta, f->tiles, f->ntiles, f->ti, f->tj, f->dUdt[0], f->ncp, 1, f->norm_part,
+                                 f->norm_small, f->stream));
+        HIPCHK(hipMemcpyAsync(nr, f->norm_small, 3 * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+        HIPCHK(hipStreamSynchronize(f->stream));
+        if (norms) memcpy(norms + 3 * nchk, nr, sizeof nr);
+        nchk++;
+        if (cycles == 2) init_l2 = nr[1];
+        else if (cycles > 2 && fabs(init_l2) > 1e-9) ratio = nr[1] / init_l2;
+        bool stop = n >= max_cycles;
+        if (cycles > 2 && (nr[1] < mg->d.absolute_tolerance || ratio < mg->d.relative_tolerance)) stop = true;
+        if (stop) break;
+        if (!first_step && cycles > 1 && freeze_level > 0 && !mg->frozen && f->s.i_limiter != CFFC_LIMITER_ZERO &&
+            nr[1] <= freeze_level) {
+            for (cffc_ctx *c : mg->lv) c->freeze = 1;
+            mg->frozen = true;
+        }
+        if ((rc = cffc_mg_cycle(mg, nullptr))) break;
+        first_step = false;
+        n++;
+    }
+    if (cycles_done) *cycles_done = n;
+    if (checks) *checks = nchk;
+    if (int e = mg_exchange(f)) return e;
+    HIPCHK(hipStreamSynchronize(f->stream));
+    return rc;
 }

@@ -193,6 +193,9 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
             if (p.from_U) {
                 const CS<double> U = {p.U[cs], p.U[ncp + cs], p.U[2 * ncp + cs], p.U[3 * ncp + cs]};
                 Wg = W_of_U(U);
+                if (p.store_U) {
+                    p.U[cd] = U.d; p.U[ncp + cd] = U.mx; p.U[2 * ncp + cd] = U.my; p.U[3 * ncp + cd] = U.E;
+                }
             } else {
                 Wg = ldW(W, ncp, cs);
             }
@@ -205,6 +208,18 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     const double *woW = p.wo + 4 * m.bcoff, *woE = woW + 4 * m.ncj, *woS = woE + 4 * m.ncj, *woN = woS + 4 * m.nci;
     auto cidx = [&](int i, int j) { return m.coff + (int64_t)j * m.pitch + i; };
     auto wo = [&](const double *a, int k) { PS<double> r = {a[4 * k], a[4 * k + 1], a[4 * k + 2], a[4 * k + 3]}; return r; };
+    // ghost U next to the ghost W (store_U): U(W) of a state the BC sets, or
+    // the source cell's U where the BC copies a cell (the reference's BCs
+    // keep U and W of every ghost cell)
+    auto putU = [&](int64_t c, const PS<double> &w) {
+        if (!p.store_U) return;
+        const CS<double> u = U_of_W(w);
+        p.U[c] = u.d; p.U[ncp + c] = u.mx; p.U[2 * ncp + c] = u.my; p.U[3 * ncp + c] = u.E;
+    };
+    auto cpyU = [&](int64_t cd, int64_t cs) {
+        if (!p.store_U) return;
+        for (int v = 0; v < 4; ++v) p.U[v * ncp + cd] = p.U[v * ncp + cs];
+    };
     // phase 1: West and East boundaries, j over all rows (corner rows when the
     // South/North BC of the ghost column is BC_NONE), SingleBlock.cc:2362-2676
     for (int q = tid; q < 2 * m.ncj; q += NT) {
@@ -221,26 +236,34 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
         const int64_t cf = west ? cidx(ICl - 1, j) : cidx(ICu, j);
         const double nx = west ? -p.geom[GEO_NEX * ncp + cf] : p.geom[GEO_NEX * ncp + cf];
         const double ny = west ? -p.geom[GEO_NEY * ncp + cf] : p.geom[GEO_NEY * ncp + cf];
+        const int64_t g1c = cidx(in + sg, j), g2c = cidx(in + 2 * sg, j);
         switch (bcv) {
         case 11000: {  // FIXED
             const PS<double> w0 = wo(west ? woW : woE, j);
-            stW(W, ncp, cidx(in + sg, j), w0); stW(W, ncp, cidx(in + 2 * sg, j), w0);
+            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            putU(g1c, w0); putU(g2c, w0);
         } break;
-        case 7001:  // REFLECTION: ghost g mirrors cell in-(g-1)*sg
-            stW(W, ncp, cidx(in + sg, j), reflect(ldW(W, ncp, cidx(in, j)), nx, ny));
-            stW(W, ncp, cidx(in + 2 * sg, j), reflect(ldW(W, ncp, cidx(in - sg, j)), nx, ny));
-            break;
-        case 6000:  // PERIODIC: W ghosts take ICu-g+1 (E ghosts take ICl+g-1)
-            stW(W, ncp, cidx(in + sg, j), ldW(W, ncp, cidx(west ? ICu : ICl, j)));
-            stW(W, ncp, cidx(in + 2 * sg, j), ldW(W, ncp, cidx(west ? ICu - 1 : ICl + 1, j)));
-            break;
+        case 7001: {  // REFLECTION: ghost g mirrors cell in-(g-1)*sg
+            const PS<double> r1 = reflect(ldW(W, ncp, cidx(in, j)), nx, ny);
+            const PS<double> r2 = reflect(ldW(W, ncp, cidx(in - sg, j)), nx, ny);
+            stW(W, ncp, g1c, r1); stW(W, ncp, g2c, r2);
+            putU(g1c, r1); putU(g2c, r2);
+        } break;
+        case 6000: {  // PERIODIC: W ghosts take ICu-g+1 (E ghosts take ICl+g-1)
+            const int64_t s1 = cidx(west ? ICu : ICl, j), s2 = cidx(west ? ICu - 1 : ICl + 1, j);
+            stW(W, ncp, g1c, ldW(W, ncp, s1));
+            stW(W, ncp, g2c, ldW(W, ncp, s2));
+            cpyU(g1c, s1); cpyU(g2c, s2);
+        } break;
         case 8001: {  // CHARACTERISTIC: first layer from BC_Characteristic_Pressure, second copied
             const PS<double> g1 = bc_characteristic_pressure(ldW(W, ncp, cidx(in, j)), wo(west ? woW : woE, j), nx, ny);
-            stW(W, ncp, cidx(in + sg, j), g1); stW(W, ncp, cidx(in + 2 * sg, j), g1);
+            stW(W, ncp, g1c, g1); stW(W, ncp, g2c, g1);
+            putU(g1c, g1); cpyU(g2c, g1c);
         } break;
         default: {  // CONSTANT_EXTRAPOLATION (and the reference's default)
             const PS<double> w0 = ldW(W, ncp, cidx(in, j));
-            stW(W, ncp, cidx(in + sg, j), w0); stW(W, ncp, cidx(in + 2 * sg, j), w0);
+            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            cpyU(g1c, cidx(in, j)); cpyU(g2c, cidx(in, j));
         } break;
         }
     }
@@ -256,26 +279,34 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
         const int64_t cf = south ? cidx(i, JCl - 1) : cidx(i, JCu);
         const double nx = south ? -p.geom[GEO_NNX * ncp + cf] : p.geom[GEO_NNX * ncp + cf];
         const double ny = south ? -p.geom[GEO_NNY * ncp + cf] : p.geom[GEO_NNY * ncp + cf];
+        const int64_t g1c = cidx(i, in + sg), g2c = cidx(i, in + 2 * sg);
         switch (bcv) {
         case 11000: {
             const PS<double> w0 = wo(south ? woS : woN, i);
-            stW(W, ncp, cidx(i, in + sg), w0); stW(W, ncp, cidx(i, in + 2 * sg), w0);
+            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            putU(g1c, w0); putU(g2c, w0);
         } break;
-        case 7001:
-            stW(W, ncp, cidx(i, in + sg), reflect(ldW(W, ncp, cidx(i, in)), nx, ny));
-            stW(W, ncp, cidx(i, in + 2 * sg), reflect(ldW(W, ncp, cidx(i, in - sg)), nx, ny));
-            break;
-        case 6000:
-            stW(W, ncp, cidx(i, in + sg), ldW(W, ncp, cidx(i, south ? JCu : JCl)));
-            stW(W, ncp, cidx(i, in + 2 * sg), ldW(W, ncp, cidx(i, south ? JCu - 1 : JCl + 1)));
-            break;
+        case 7001: {
+            const PS<double> r1 = reflect(ldW(W, ncp, cidx(i, in)), nx, ny);
+            const PS<double> r2 = reflect(ldW(W, ncp, cidx(i, in - sg)), nx, ny);
+            stW(W, ncp, g1c, r1); stW(W, ncp, g2c, r2);
+            putU(g1c, r1); putU(g2c, r2);
+        } break;
+        case 6000: {
+            const int64_t s1 = cidx(i, south ? JCu : JCl), s2 = cidx(i, south ? JCu - 1 : JCl + 1);
+            stW(W, ncp, g1c, ldW(W, ncp, s1));
+            stW(W, ncp, g2c, ldW(W, ncp, s2));
+            cpyU(g1c, s1); cpyU(g2c, s2);
+        } break;
         case 8001: {
             const PS<double> g1 = bc_characteristic_pressure(ldW(W, ncp, cidx(i, in)), wo(south ? woS : woN, i), nx, ny);
-            stW(W, ncp, cidx(i, in + sg), g1); stW(W, ncp, cidx(i, in + 2 * sg), g1);
+            stW(W, ncp, g1c, g1); stW(W, ncp, g2c, g1);
+            putU(g1c, g1); cpyU(g2c, g1c);
         } break;
         default: {
             const PS<double> w0 = ldW(W, ncp, cidx(i, in));
-            stW(W, ncp, cidx(i, in + sg), w0); stW(W, ncp, cidx(i, in + 2 * sg), w0);
+            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            cpyU(g1c, cidx(i, in)); cpyU(g2c, cidx(i, in));
         } break;
         }
     }

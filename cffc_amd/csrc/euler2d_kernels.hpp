@@ -18,8 +18,10 @@ struct ExchangeParams {
     int64_t ncp;
     double *W;                 // W buffer whose ghosts are filled (FP64 storage)
     float *Wf;                 // or, if not null, the FP32 W buffer of the fused mixed path
-    const double *U;           // conserved state (stage 1: ghosts = W(U))
+    double *U;                 // conserved state (stage 1: ghosts = W(U))
     int32_t from_U;
+    int32_t store_U;           // also fill the ghost U: copied with the neighbour's U, U(W) of a BC state
+                               // (the multigrid levels read ghost U; needs from_U, FP64 storage)
     int32_t *stop_flag;
     unsigned long long *reset_slot;  // set to MILLION (next-step dt slot), may be null
     const int32_t *blocks;     // blocks to process (workgroup k -> blocks[k]), or null: workgroup = block
@@ -104,5 +106,20 @@ hipError_t cffc_launch_flux_fix(const cffc::FluxFixOp *ops, int nops, double *S,
                                 const double *geom, const double *dt, const unsigned long long *dtmin, int lts,
                                 double cfl, int64_t ncp, const int32_t *stop_flag, hipStream_t st);
 hipError_t cffc_launch_stage_fp64(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
+// multigrid level transfers (euler2d_mg.hip)
+hipError_t cffc_launch_mg_restrict(const cffc::BlockMeta *mf, const cffc::BlockMeta *mc, int nblk, int max_int_c,
+                                   const double *src, const double *Af, int64_t ncpf, double *dst, const double *Ac,
+                                   int64_t ncpc, hipStream_t st);
+hipError_t cffc_launch_mg_primitive(const cffc::BlockMeta *meta, int nblk, int max_int, const double *U, double *W,
+                                    int64_t ncp, hipStream_t st);
+hipError_t cffc_launch_mg_subtract(const cffc::BlockMeta *meta, int nblk, int max_int, double *P, const double *R,
+                                   int64_t ncp, hipStream_t st);
+hipError_t cffc_launch_mg_cfl(const cffc::BlockMeta *mf, const cffc::BlockMeta *mc, int nblk, int max_int_c,
+                              const double *dtf, const double *Af, double *dtc, const double *Ac, hipStream_t st);
+hipError_t cffc_launch_mg_changes(double *U, const double *uo, int64_t n, hipStream_t st);
+hipError_t cffc_launch_mg_prolong(const cffc::BlockMeta *mc, const cffc::BlockMeta *mf, int nblk, int max_int_c,
+                                  int max_edge_c, const int32_t *bc, double *Uc, const double *gc, int64_t ncpc,
+                                  double *Uf, const double *gf, int64_t ncpf, int neumann, int inj_dirichlet,
+                                  int inj_always, int stability, int max_red, hipStream_t st);
 hipError_t cffc_launch_stage_mixed(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 }

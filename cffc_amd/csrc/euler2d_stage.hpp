@@ -64,6 +64,10 @@ struct StageParams {
     double *eflux;          // saved block-edge fluxes (AMR_FLUX_* faces)
     const uint8_t *ustore;  // per padded cell: U^(s+1) is read by a coarser neighbour's restriction
     double *Ustage;         // fused: U^(s+1) of those cells
+    // multigrid levels (granular FP64 kernel only)
+    int32_t hlle;           // HLLE flux instead of Roe
+    int32_t residual;       // steady residual (F*l)/A, no CFL*dt factor (dUdt_Residual_Evaluation)
+    const double *P;        // forcing term added after the fluxes: +(CFL*dt)*P, or +P with residual
 };
 
 // Push form of the same-level exchange: the cell (ip, jp) of block b that a
@@ -609,7 +613,11 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             load_state(rl_i, rl_j, Wl);
             load_state(rr_i, rr_j, Wr);
         }
-        Fout = flux_roe_n(Wl, Wr, nx, ny);
+        if constexpr (!FUSED && !Fast<FT>::on) {
+            Fout = p.hlle ? flux_hlle_n(Wl, Wr, nx, ny) : flux_roe_n(Wl, Wr, nx, ny);
+        } else {
+            Fout = flux_roe_n(Wl, Wr, nx, ny);
+        }
         return true;
     };
     // up to two faces per thread: (idir, f, g) of each
@@ -701,16 +709,33 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             }
         } else {
             // ((CFL*dt)*F*l)/A per face, +W -E +S -N (Euler2DQuad.cc:1502-1509)
+            bool res = false;
+            if constexpr (!FUSED) res = p.residual != 0;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const double FW = (double)fW(v), FE = (double)sFi[v][tr][tc + 1];
                 const double FS = (double)fS(v), FN = (double)sFj[v][tr + 1][tc];
                 double d = dU[v];
-                d += cdt * FW * lW / A;
-                d -= cdt * FE * lE / A;
-                d += cdt * FS * lS / A;
-                d -= cdt * FN * lN / A;
+                if (res) {
+                    // (Flux*l)/A of dUdt_Residual_Evaluation, Euler2DQuad.cc:1001-1006
+                    d += FW * lW / A;
+                    d -= FE * lE / A;
+                    d += FS * lS / A;
+                    d -= FN * lN / A;
+                } else {
+                    d += cdt * FW * lW / A;
+                    d -= cdt * FE * lE / A;
+                    d += cdt * FS * lS / A;
+                    d -= cdt * FN * lN / A;
+                }
                 dU[v] = d;
+            }
+            if constexpr (!FUSED) {
+                // FAS forcing term (FASMultigrid2D.h:2736-2739, 2840-2847)
+                if (p.P) {
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) dU[v] += res ? p.P[v * ncp + c] : cdt * p.P[v * ncp + c];
+                }
             }
         }
         if (m.amr & AMR_FLUX_ANY) {

@@ -246,6 +246,34 @@ template <typename T> CFFC_HD CS<T> flux_roe_n(const PS<T> &Wl, const PS<T> &Wr,
     return F;
 }
 
+// FluxHLLE_x / FluxHLLE_n, Euler2DState.cc:2921-2963, 3046-3095: wave speeds
+// from the left, right and Roe-average eigenvalues u -/+ a, then
+// (sR*Fl - sL*Fr + sL*sR*(Ur - Ul)) / (sR - sL).  FP64 only (the coarse
+// levels of the multigrid solver).
+template <typename T> CFFC_HD CS<T> flux_hlle_x(const PS<T> &Wl, const PS<T> &Wr) {
+    const PS<T> Wa = roe_average(Wl, Wr);
+    const CS<T> Ul = U_of_W(Wl), Ur = U_of_W(Wr);
+    const T sl = dmin(Wl.vx - W_a(Wl), Wa.vx - W_a(Wa));
+    const T sr = dmax(Wr.vx + W_a(Wr), Wa.vx + W_a(Wa));
+    if (sl >= T(0)) return W_Fx(Wl);
+    if (sr <= T(0)) return W_Fx(Wr);
+    const CS<T> Fl = W_Fx(Wl), Fr = W_Fx(Wr);
+    const T slr = sl * sr, den = sr - sl;
+    CS<T> F;
+    F.d = ((sr * Fl.d - sl * Fr.d) + slr * (Ur.d - Ul.d)) / den;
+    F.mx = ((sr * Fl.mx - sl * Fr.mx) + slr * (Ur.mx - Ul.mx)) / den;
+    F.my = ((sr * Fl.my - sl * Fr.my) + slr * (Ur.my - Ul.my)) / den;
+    F.E = ((sr * Fl.E - sl * Fr.E) + slr * (Ur.E - Ul.E)) / den;
+    return F;
+}
+template <typename T> CFFC_HD CS<T> flux_hlle_n(const PS<T> &Wl, const PS<T> &Wr, T nx, T ny) {
+    PS<T> a = {Wl.d, Wl.vx * nx + Wl.vy * ny, -Wl.vx * ny + Wl.vy * nx, Wl.p};
+    PS<T> b = {Wr.d, Wr.vx * nx + Wr.vy * ny, -Wr.vx * ny + Wr.vy * nx, Wr.p};
+    CS<T> Fr = flux_hlle_x(a, b);
+    CS<T> F = {Fr.d, Fr.mx * nx - Fr.my * ny, Fr.mx * ny + Fr.my * nx, Fr.E};
+    return F;
+}
+
 // Reflect, Euler2DState.cc:751-787
 template <typename T> CFFC_HD PS<T> reflect(const PS<T> &W, T nx, T ny) {
     T ur = W.vx * nx + W.vy * ny;

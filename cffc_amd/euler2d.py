@@ -80,6 +80,15 @@ def load():
     L.cffc_residual_smoothing.argtypes = [vp, i32]
     pint = ctypes.POINTER(ctypes.c_int)
     L.cffc_autotune_tile.argtypes = [vp, i32, pint, pint, pd, pint]
+    L.cffc_mg_create.argtypes = [vp, i32, vp, vp, ctypes.POINTER(vp)]
+    L.cffc_mg_destroy.argtypes = [vp]
+    L.cffc_mg_level.argtypes = [vp, i32]
+    L.cffc_mg_level.restype = vp
+    L.cffc_mg_set_residual_smoothing.argtypes = [vp, d, i32]
+    L.cffc_mg_initialize.argtypes = [vp]
+    L.cffc_mg_cycle.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+    L.cffc_mg_execute.argtypes = [vp, i32, d, pd, pint, pint]
+    L.cffc_mg_get_P.argtypes = [vp, i32, pd, ctypes.c_int64]
     L.cffc_last_error.argtypes = []
     L.cffc_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -113,9 +122,9 @@ class Euler2DQuadBlocks:
         self.ncell = L.cffc_num_cells(h)
 
     def close(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owned", True):
             load().cffc_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

@@ -55,8 +55,12 @@ extern "C" {
 #define CFFC_LIMITER_ONE             (-1)
 #define CFFC_LIMITER_ZERO              0
 #define CFFC_LIMITER_VENKATAKRISHNAN   9
-/* flux function, CFD.h:769 */
+/* flux functions, CFD.h:769-771 */
 #define CFFC_FLUX_FUNCTION_ROE         2
+#define CFFC_FLUX_FUNCTION_HLLE        4
+/* multigrid cycles, CFD.h:689-690 */
+#define CFFC_MULTIGRID_V_CYCLE         1
+#define CFFC_MULTIGRID_W_CYCLE         2
 /* reconstruction, CFD.h */
 #define CFFC_RECONSTRUCTION_LEAST_SQUARES 2
 /* time-step type, CFD.h:801-802 */
@@ -126,10 +130,33 @@ typedef struct cffc_scheme_desc {
     double  cfl_number;           /* CFL_Number                             */
     int32_t i_reconstruction;     /* CFFC_RECONSTRUCTION_LEAST_SQUARES      */
     int32_t i_limiter;            /* CFFC_LIMITER_*                         */
-    int32_t i_flux_function;      /* CFFC_FLUX_FUNCTION_ROE                 */
+    int32_t i_flux_function;      /* CFFC_FLUX_FUNCTION_ROE / _HLLE         */
     int32_t local_time_stepping;  /* CFFC_GLOBAL/SCALAR_LOCAL_TIME_STEPPING */
     int32_t flux_precision;       /* CFFC_FLUX_FP64 / CFFC_FLUX_FP32        */
 } cffc_scheme_desc;
+
+/* The FAS multigrid parameters of Multigrid_Input_Parameters
+   (FASMultigrid2DInput.h:93-114; the reference's defaults in brackets).
+   Coarse-mesh boundary conditions are always applied
+   (Apply_Coarse_Mesh_Boundary_Conditions = ON, the default); full
+   multigrid start-up cycles are not offered. */
+typedef struct cffc_mg_desc {
+    int32_t levels;                      /* Levels, >= 2                     [2] */
+    int32_t i_cycle;                     /* CFFC_MULTIGRID_V/W_CYCLE         [V] */
+    int32_t smooths_on_finest_level;     /* Number_of_Smooths_on_Finest_Level[1] */
+    int32_t pre_smooths;                 /* Number_of_Pre_Smooths            [1] */
+    int32_t post_smooths;                /* Number_of_Post_Smooths           [0] */
+    int32_t smooths_on_coarsest_level;   /* Number_of_Smooths_on_Coarsest_Level [3] */
+    int32_t defect_correction;           /* Defect_Correction                [1] */
+    int32_t prolong_using_injection;     /* Prolong_Using_Injection          [0] */
+    int32_t first_order_coarse;          /* First_Order_Coarse_Mesh_Reconstruction: HLLE
+                                            flux, zero limiter below the top [1] */
+    int32_t injection_at_dirichlet;      /* Injection_at_Dirichlet_Boundary_Conditions [1] */
+    int32_t update_stability_switch;     /* Update_Stability_Switch          [1] */
+    int32_t max_update_reductions;       /* Maximum_Number_of_Update_Reductions [5] */
+    double  absolute_tolerance;          /* Absolute_Convergence_Tolerance  [-1] */
+    double  relative_tolerance;          /* Relative_Convergence_Tolerance  [-1] */
+} cffc_mg_desc;
 
 typedef struct cffc_ctx cffc_ctx;
 
@@ -306,6 +333,40 @@ int  cffc_autotune_tile(cffc_ctx *ctx, int nsteps, int *tile_i, int *tile_j, dou
    since timing was switched on.  (Not applied inside cffc_run_steps_graph.) */
 int  cffc_set_kernel_timing(cffc_ctx *ctx, int on);
 int  cffc_kernel_times(cffc_ctx *ctx, double out[4]);
+
+/* ---- FAS multigrid: FAS_Multigrid2D_Solver (FASMultigrid2D/FASMultigrid2D.h) ----
+   levels[0..nlevels-1]: the finest mesh and its coarsenings, level l+1
+   halving every block of level l (Half_Mesh_Resolution, built by the host
+   as the reference's allocate does, FASMultigrid2D.h:418-658; single-level
+   meshes, FP64).  smoother: the scheme of Multigrid_IP.i_Smoothing (the
+   reference overrides i_Time_Integration with it while smoothing); coarse
+   levels switch to HLLE and the zero limiter when first_order_coarse is
+   set.  Each level is a cffc_ctx on one stream; cffc_mg_level returns it
+   (cffc_set_W / cffc_get_W on level 0). */
+typedef struct cffc_mg cffc_mg;
+/* allocate, FASMultigrid2D.h:418 */
+int  cffc_mg_create(const cffc_mesh_desc *levels, int nlevels, const cffc_scheme_desc *smoother,
+                    const cffc_mg_desc *mg, cffc_mg **out);
+void cffc_mg_destroy(cffc_mg *mg);
+cffc_ctx *cffc_mg_level(cffc_mg *mg, int level);
+/* Residual_Smoothing inside Smooth (:2987-2993) */
+int  cffc_mg_set_residual_smoothing(cffc_mg *mg, double epsilon, int iterations);
+/* after cffc_set_W on level 0: the coarse levels restricted from it with
+   their boundary reference states (Apply_ICs with IC_RESTART :719-733,
+   Restrict_Boundary_Ref_States :1112), messages and BCs on every level */
+int  cffc_mg_initialize(cffc_mg *mg);
+/* one cycle, Coarse_Grid_Correction(0, 0) :3118; nonzero = the first
+   unphysical update's i index, *bad_cell its level << 56 | block << 32 |
+   j << 16 | i (may be NULL) */
+int  cffc_mg_cycle(cffc_mg *mg, int64_t *bad_cell);
+/* Execute's regular-cycle loop :3440-3705: norms (L1, L2, max of the
+   level-0 density residual; 3 per check, room for max_cycles + 1 checks)
+   before each cycle, limiter freezing at freeze_level (<= 0: never), exits
+   on max_cycles or the absolute / relative tolerances of cffc_mg_desc */
+int  cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done,
+                     int *checks);
+/* the forcing term P of a coarse level, interior cells in block order */
+int  cffc_mg_get_P(cffc_mg *mg, int level, double *P, int64_t ncell);
 
 /* Last error message of this thread (static storage). */
 const char *cffc_last_error(void);

@@ -75,6 +75,18 @@ def lib():
         L.orc_set_residual_smoothing.argtypes = [vp, d, ctypes.c_int]
         L.orc_residual_smoothing.argtypes = [vp, ctypes.c_int]
         L.orc_run.argtypes = [vp, ctypes.c_int, d, ctypes.c_int, pd, pd, ctypes.POINTER(ctypes.c_int)]
+        L.orc_flux_hlle_n.argtypes = [pd, pd, d, d, pd]
+        L.orc_bilinear.argtypes = [pd, pd, d, d, pd]
+        L.orc_mg_create.argtypes = [vp, ctypes.c_int, vp, vp, ctypes.POINTER(vp)]
+        L.orc_mg_destroy.argtypes = [vp]
+        L.orc_mg_level.argtypes = [vp, ctypes.c_int]
+        L.orc_mg_level.restype = vp
+        L.orc_mg_set_residual_smoothing.argtypes = [vp, d, ctypes.c_int]
+        L.orc_mg_initialize.argtypes = [vp]
+        L.orc_mg_cycle.argtypes = [vp]
+        pi = ctypes.POINTER(ctypes.c_int)
+        L.orc_mg_execute.argtypes = [vp, ctypes.c_int, d, pd, pi, pi]
+        L.orc_mg_get_P.argtypes = [vp, ctypes.c_int, pd]
         _lib = L
     return _lib
 
@@ -164,6 +176,22 @@ def quad_centroid(x4, y4):
     return out
 
 
+def flux_hlle_n(Wl, Wr, nx, ny):
+    out = np.zeros(4)
+    lib().orc_flux_hlle_n(_p(_arr(Wl)), _p(_arr(Wr)), nx, ny, _p(out))
+    return out
+
+
+def bilinear(U4, xy4, x0, y0):
+    """Bilinear_Interpolation of four conserved states at the clockwise
+    quadrilateral xy4 ([4,2]); returns (error_flag, U0)."""
+    out = np.zeros(4)
+    U = np.ascontiguousarray(np.asarray(U4, dtype=np.float64).reshape(16))
+    P = np.ascontiguousarray(np.asarray(xy4, dtype=np.float64).reshape(8))
+    e = lib().orc_bilinear(_p(U), _p(P), x0, y0, _p(out))
+    return e, out
+
+
 # ---- multi-block solver -----------------------------------------------------
 class Oracle:
     """The oracle's multi-block solver on a cffc_amd.mesh.Mesh."""
@@ -183,9 +211,9 @@ class Oracle:
         self.mesh = mesh
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self, "_owner", None) is None:
             lib().orc_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def set_W(self, W):
         W = np.ascontiguousarray(W, dtype=np.float64).reshape(self.ncell, 4)
@@ -283,3 +311,72 @@ class Oracle:
         rc = lib().orc_run(self.h, nsteps, tol, nthreads, _p(norms), _p(dts), ctypes.byref(done))
         n = done.value
         return rc, n, norms[:n], dts[:n]
+
+
+def _level_view(h, mesh, owner):
+    """An Oracle over one multigrid level's context (owned by OracleMG)."""
+    o = Oracle.__new__(Oracle)
+    o.h = h
+    o.ncell = lib().orc_num_cells(h)
+    o.mesh = mesh
+    o._owner = owner
+    return o
+
+
+class OracleMG:
+    """The oracle's FAS multigrid (FAS_Multigrid2D_Solver) over a list of
+    level meshes (finest first, cffc_amd.multigrid.hierarchy)."""
+
+    def __init__(self, meshes, scheme, params):
+        from cffc_amd import abi
+        from cffc_amd.multigrid import mg_desc
+        self._keep = []
+        arr = (abi.MeshDesc * len(meshes))(*[abi.mesh_desc(m, self._keep) for m in meshes])
+        self._arr = arr
+        sd = abi.scheme_desc(scheme)
+        md = mg_desc(params)
+        self._sd, self._mgd = sd, md
+        h = ctypes.c_void_p()
+        rc = lib().orc_mg_create(ctypes.cast(arr, ctypes.c_void_p), len(meshes), ctypes.byref(sd), ctypes.byref(md),
+                                 ctypes.byref(h))
+        if rc != 0:
+            raise ValueError("orc_mg_create failed: %d" % rc)
+        self.h = h
+        self.meshes = meshes
+        self.levels = [_level_view(ctypes.c_void_p(lib().orc_mg_level(h, l)), meshes[l], self)
+                       for l in range(len(meshes))]
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_mg_destroy(self.h)
+            self.h = None
+
+    def set_W(self, W):
+        self.levels[0].set_W(W)
+
+    def get_W(self):
+        return self.levels[0].get_W()
+
+    def get_U(self):
+        return self.levels[0].get_U()
+
+    def set_residual_smoothing(self, epsilon, iterations):
+        lib().orc_mg_set_residual_smoothing(self.h, epsilon, iterations)
+
+    def initialize(self):
+        return lib().orc_mg_initialize(self.h)
+
+    def cycle(self):
+        return lib().orc_mg_cycle(self.h)
+
+    def get_P(self, level):
+        out = np.zeros((self.levels[level].ncell, 4))
+        lib().orc_mg_get_P(self.h, level, _p(out))
+        return out
+
+    def execute(self, max_cycles, freeze_level=0.0):
+        """Execute's regular-cycle loop; returns (rc, cycles, norms[checks, 3])."""
+        norms = np.zeros((max_cycles + 1, 3))
+        done, chk = ctypes.c_int(0), ctypes.c_int(0)
+        rc = lib().orc_mg_execute(self.h, max_cycles, freeze_level, _p(norms), ctypes.byref(done), ctypes.byref(chk))
+        return rc, done.value, norms[:chk.value]

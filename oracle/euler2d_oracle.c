@@ -297,6 +297,39 @@ static cst flux_roe_n(pst Wl, pst Wr, double nx, double ny) {
     cst F = {Fr.d, Fr.mx * cos_angle - Fr.my * sin_angle, Fr.mx * sin_angle + Fr.my * cos_angle, Fr.E};
     return F;
 }
+/* FluxHLLE_x Euler2DState.cc:2921-2963: wave speeds from the left, right and
+   Roe-average eigenvalues, (sR*Fl - sL*Fr + sL*sR*(Ur - Ul)) / (sR - sL) */
+static cst flux_hlle_x(pst Wl, pst Wr) {
+    pst Wa = roe_average(Wl, Wr);
+    cst Ul = U_of_W(Wl), Ur = U_of_W(Wr);
+    cst dU = {Ur.d - Ul.d, Ur.mx - Ul.mx, Ur.my - Ul.my, Ur.E - Ul.E};
+    pst ll = W_lambda_x(Wl), lr = W_lambda_x(Wr), la = W_lambda_x(Wa);
+    double sl = dmin(ll.d, la.d), sr = dmax(lr.p, la.p);
+    if (sl >= ZERO) return W_Fx(Wl);
+    if (sr <= ZERO) return W_Fx(Wr);
+    cst Fl = W_Fx(Wl), Fr = W_Fx(Wr), F;
+    double slr = sl * sr, den = sr - sl;
+    F.d = ((sr * Fl.d - sl * Fr.d) + slr * dU.d) / den;
+    F.mx = ((sr * Fl.mx - sl * Fr.mx) + slr * dU.mx) / den;
+    F.my = ((sr * Fl.my - sl * Fr.my) + slr * dU.my) / den;
+    F.E = ((sr * Fl.E - sl * Fr.E) + slr * dU.E) / den;
+    return F;
+}
+/* FluxHLLE_n Euler2DState.cc:3046-3095 (the rotation of FluxRoe_n) */
+static cst flux_hlle_n(pst Wl, pst Wr, double nx, double ny) {
+    double cos_angle = nx, sin_angle = ny;
+    pst Wlr = {Wl.d, Wl.vx * cos_angle + Wl.vy * sin_angle, -Wl.vx * sin_angle + Wl.vy * cos_angle, Wl.p};
+    pst Wrr = {Wr.d, Wr.vx * cos_angle + Wr.vy * sin_angle, -Wr.vx * sin_angle + Wr.vy * cos_angle, Wr.p};
+    cst Fr = flux_hlle_x(Wlr, Wrr);
+    cst F = {Fr.d, Fr.mx * cos_angle - Fr.my * sin_angle, Fr.mx * sin_angle + Fr.my * cos_angle, Fr.E};
+    return F;
+}
+/* the flux-function switch of dUdt_Multistage_Explicit / dUdt_Residual_
+   Evaluation (Euler2DQuad.cc:958-992) for the two functions built here */
+static cst flux_n(const cffc_scheme_desc *s, pst Wl, pst Wr, double nx, double ny) {
+    if (s->i_flux_function == CFFC_FLUX_FUNCTION_HLLE) return flux_hlle_n(Wl, Wr, nx, ny);
+    return flux_roe_n(Wl, Wr, nx, ny);
+}
 /* Reflect Euler2DState.cc:751-787 */
 static pst reflect(pst W, double nx, double ny) {
     double cos_angle = nx, sin_angle = ny;
@@ -356,6 +389,7 @@ typedef struct {
     int level;             /* AdaptiveBlock2D_List Block[].info.level */
     int nbr2[4];           /* second neighbour of a face with two finer neighbours (infoX[1]) */
     cst *FluxN, *FluxS, *FluxE, *FluxW; /* block-edge fluxes, Euler2DQuad.h:140-205 */
+    cst *P, *uo;           /* FAS_Multigrid_Quad_Block forcing term and uo_MG (multigrid levels) */
 } oblk;
 
 typedef struct orc_ctx {
@@ -869,7 +903,7 @@ static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
                     Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
                 }
                 nfaceE(B, i, j, &nx, &ny);
-                cst F = flux_roe_n(Wl, Wr, nx, ny);
+                cst F = flux_n(s, Wl, Wr, nx, ny);
                 acc_sub(&B->dUdt[k][cL], CFL * B->dt[cL], F, lfaceE(B, i, j), B->A[cL]);
                 acc_add(&B->dUdt[k][cR], CFL * B->dt[cR], F, lfaceW(B, i + 1, j), B->A[cR]);
                 /* west and east boundary fluxes :1518-1523 */
@@ -907,7 +941,7 @@ static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
                 Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
             }
             nfaceN(B, i, j, &nx, &ny);
-            cst F = flux_roe_n(Wl, Wr, nx, ny);
+            cst F = flux_n(s, Wl, Wr, nx, ny);
             acc_sub(&B->dUdt[k][cL], CFL * B->dt[cL], F, lfaceN(B, i, j), B->A[cL]);
             acc_add(&B->dUdt[k][cR], CFL * B->dt[cR], F, lfaceS(B, i, j + 1), B->A[cR]);
             /* south and north boundary fluxes :1709-1715 */
@@ -916,6 +950,98 @@ static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
         }
         B->dUdt[k][CI(B, i, JCl - 1)] = Z;
         B->dUdt[k][CI(B, i, JCu + 1)] = Z;
+    }
+}
+
+/* (Flux*l)/A of the residual evaluation (Euler2DQuad.cc:1001-1006) */
+static inline void res_sub(cst *d, cst F, double l, double A) {
+    d->d -= F.d * l / A; d->mx -= F.mx * l / A; d->my -= F.my * l / A; d->E -= F.E * l / A;
+}
+static inline void res_add(cst *d, cst F, double l, double A) {
+    d->d += F.d * l / A; d->mx += F.mx * l / A; d->my += F.my * l / A; d->E += F.E * l / A;
+}
+
+/* Euler2D_Quad_Block::dUdt_Residual_Evaluation, Euler2DQuad.cc:814-1217:
+   the steady residual R(U) = -sum(Flux*l)/A into dUdt[0] (no CFL*dt factor),
+   with the faces, BC states and accumulation order of the multistage
+   residual. */
+static void block_residual(oblk *B, const cffc_scheme_desc *s) {
+    int ICl = B->ICl, ICu = B->ICu, JCl = B->JCl, JCu = B->JCu, Ng = B->ng;
+    const cst Z = {0, 0, 0, 0};
+    cst *R = B->dUdt[0];
+    for (int j = JCl - Ng + 1; j <= JCu + Ng - 1; ++j)
+        for (int i = ICl - Ng + 1; i <= ICu + Ng - 1; ++i) lsq_reconstruct(B, i, j, s->i_limiter);
+    for (int j = JCl - 1; j <= JCu + 1; ++j) {
+        R[CI(B, ICl - 1, j)] = Z;
+        for (int i = ICl - 1; i <= ICu; ++i) {
+            int cL = CI(B, i, j), cR = CI(B, i + 1, j);
+            R[cR] = Z;
+            if (j > JCl - 1 && j < JCu + 1) {
+                pst Wl, Wr;
+                double nx, ny, xf, yf;
+                if (i == ICl - 1 && is_face_bc(B->bcW[j])) {
+                    xfaceW(B, i + 1, j, &xf, &yf);
+                    Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+                    nfaceW(B, i + 1, j, &nx, &ny);
+                    if (B->bcW[j] == CFFC_BC_REFLECTION) Wl = reflect(Wr, nx, ny);
+                    else Wl = bc_characteristic_pressure(Wr, B->WoW[j], nx, ny);
+                } else if (i == ICu && is_face_bc(B->bcE[j])) {
+                    xfaceE(B, i, j, &xf, &yf);
+                    Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                    nfaceE(B, i, j, &nx, &ny);
+                    if (B->bcE[j] == CFFC_BC_REFLECTION) Wr = reflect(Wl, nx, ny);
+                    else Wr = bc_characteristic_pressure(Wl, B->WoE[j], nx, ny);
+                } else {
+                    xfaceE(B, i, j, &xf, &yf);
+                    Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                    xfaceW(B, i + 1, j, &xf, &yf);
+                    Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+                }
+                nfaceE(B, i, j, &nx, &ny);
+                cst F = flux_n(s, Wl, Wr, nx, ny);
+                res_sub(&R[cL], F, lfaceE(B, i, j), B->A[cL]);
+                res_add(&R[cR], F, lfaceW(B, i + 1, j), B->A[cR]);
+                if (i == ICl - 1) B->FluxW[j - JCl] = flux_times(F, -ONE, lfaceW(B, i + 1, j));
+                else if (i == ICu) B->FluxE[j - JCl] = flux_times(F, ONE, lfaceE(B, i, j));
+            }
+        }
+        if (j > JCl - 1 && j < JCu + 1) {
+            R[CI(B, ICl - 1, j)] = Z;
+            R[CI(B, ICu + 1, j)] = Z;
+        }
+    }
+    for (int i = ICl; i <= ICu; ++i) {
+        for (int j = JCl - 1; j <= JCu; ++j) {
+            int cL = CI(B, i, j), cR = CI(B, i, j + 1);
+            pst Wl, Wr;
+            double nx, ny, xf, yf;
+            if (j == JCl - 1 && is_face_bc(B->bcS[i])) {
+                xfaceS(B, i, j + 1, &xf, &yf);
+                Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+                nfaceS(B, i, j + 1, &nx, &ny);
+                if (B->bcS[i] == CFFC_BC_REFLECTION) Wl = reflect(Wr, nx, ny);
+                else Wl = bc_characteristic_pressure(Wr, B->WoS[i], nx, ny);
+            } else if (j == JCu && is_face_bc(B->bcN[i])) {
+                xfaceN(B, i, j, &xf, &yf);
+                Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                nfaceN(B, i, j, &nx, &ny);
+                if (B->bcN[i] == CFFC_BC_REFLECTION) Wr = reflect(Wl, nx, ny);
+                else Wr = bc_characteristic_pressure(Wl, B->WoN[i], nx, ny);
+            } else {
+                xfaceN(B, i, j, &xf, &yf);
+                Wl = face_state(B, cL, xf - B->xc[cL], yf - B->yc[cL]);
+                xfaceS(B, i, j + 1, &xf, &yf);
+                Wr = face_state(B, cR, xf - B->xc[cR], yf - B->yc[cR]);
+            }
+            nfaceN(B, i, j, &nx, &ny);
+            cst F = flux_n(s, Wl, Wr, nx, ny);
+            res_sub(&R[cL], F, lfaceN(B, i, j), B->A[cL]);
+            res_add(&R[cR], F, lfaceS(B, i, j + 1), B->A[cR]);
+            if (j == JCl - 1) B->FluxS[i - ICl] = flux_times(F, -ONE, lfaceS(B, i, j + 1));
+            else if (j == JCu) B->FluxN[i - ICl] = flux_times(F, ONE, lfaceN(B, i, j));
+        }
+        R[CI(B, i, JCl - 1)] = Z;
+        R[CI(B, i, JCu + 1)] = Z;
     }
 }
 
@@ -1286,6 +1412,7 @@ static void blk_free(oblk *B) {
     free(B->U); free(B->Uo); free(B->dUdt[0]); free(B->dUdt[1]); free(B->dUdt[2]);
     free(B->dt);
     free(B->FluxN); free(B->FluxS); free(B->FluxE); free(B->FluxW);
+    free(B->P); free(B->uo);
 }
 
 void orc_destroy(orc_ctx *ctx) {
@@ -1762,4 +1889,464 @@ void orc_quad_centroid(const double *x4, const double *y4, double *out3) {
     B.nci = 1;
     B.xn = xs; B.yn = ys; /* node (i,j) at j*(nci+1)+i: (0,0)=SW (1,0)=SE (0,1)=NW (1,1)=NE */
     quad_centroid(&B, 0, 0, &out3[0], &out3[1], &out3[2]);
+}
+
+/* ====================================================================== */
+/* FAS multigrid, FASMultigrid2D/FASMultigrid2D.h (regular V and W cycles) */
+/* ====================================================================== */
+/* One orc_ctx per level (level 0 the finest), each on its own mesh: level
+   l+1 halves the resolution of level l block by block (Half_Mesh_Resolution,
+   HO_Grid2DQuad.cc:12614), same block order.  Level schemes: the smoother's
+   on level 0; below it HLLE and the zero limiter when first_order_coarse is
+   set (dUdt_Multistage_Explicit_for_Multigrid :2491-2540 switch them for
+   Level > Top_Level), which also selects the first-order optimal-smoothing
+   coefficients.  Every level steps with its own per-cell dt (Smooth's CFL
+   call, :2877; the global minimum is not applied there). */
+typedef struct orc_mg orc_mg;
+void orc_mg_destroy(orc_mg *mg);
+struct orc_mg {
+    int L;
+    orc_ctx **lv;
+    cffc_mg_desc d;
+    int rs_iters;
+    double rs_eps;
+};
+
+#define FINE_IDX(Ng, ic) (2 * ((ic) - (Ng)) + (Ng))
+
+/* Restrict_Solution_Blocks :996-1037 (coarse interior cells: coarse-mesh BCs
+   are applied, so nghost = 0) and Restrict_Residuals :1053-1095 into P */
+static void mg_restrict(orc_ctx *F, orc_ctx *C, int residual) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *f = &F->b[b], *c = &C->b[b];
+        const int Ng = f->ng;
+        const cst *src = residual ? f->dUdt[0] : f->U;
+        cst *dst = residual ? c->P : c->U;
+        for (int ic = c->ICl; ic <= c->ICu; ++ic)
+            for (int jc = c->JCl; jc <= c->JCu; ++jc) {
+                const int i0 = FINE_IDX(Ng, ic), j0 = FINE_IDX(Ng, jc);
+                const int q0 = CI(f, i0, j0), q1 = CI(f, i0 + 1, j0), q2 = CI(f, i0, j0 + 1), q3 = CI(f, i0 + 1, j0 + 1);
+                const cst a = src[q0], bb = src[q1], cc = src[q2], dd = src[q3];
+                const double A0 = f->A[q0], A1 = f->A[q1], A2 = f->A[q2], A3 = f->A[q3], Ac = c->A[CI(c, ic, jc)];
+                cst r;
+                r.d = (a.d * A0 + bb.d * A1 + cc.d * A2 + dd.d * A3) / Ac;
+                r.mx = (a.mx * A0 + bb.mx * A1 + cc.mx * A2 + dd.mx * A3) / Ac;
+                r.my = (a.my * A0 + bb.my * A1 + cc.my * A2 + dd.my * A3) / Ac;
+                r.E = (a.E * A0 + bb.E * A1 + cc.E * A2 + dd.E * A3) / Ac;
+                dst[CI(c, ic, jc)] = r;
+            }
+    }
+}
+/* Restrict_Boundary_Ref_States :1112-1186: the coarse WoN/S/E/W of the
+   interior span from the two fine boundary cells' states, area weighted */
+static void mg_restrict_wo(orc_ctx *F, orc_ctx *C) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *f = &F->b[b], *c = &C->b[b];
+        const int Ng = f->ng;
+#define WAVG(Wa, Wb, Aa, Ab) { ((Wa).d * (Aa) + (Wb).d * (Ab)) / ((Aa) + (Ab)), ((Wa).vx * (Aa) + (Wb).vx * (Ab)) / ((Aa) + (Ab)), \
+                               ((Wa).vy * (Aa) + (Wb).vy * (Ab)) / ((Aa) + (Ab)), ((Wa).p * (Aa) + (Wb).p * (Ab)) / ((Aa) + (Ab)) }
+        for (int ic = c->ICl; ic <= c->ICu; ++ic) {
+            const int i0 = FINE_IDX(Ng, ic);
+            double An0 = f->A[CI(f, i0, f->JCu)], An1 = f->A[CI(f, i0 + 1, f->JCu)];
+            double As0 = f->A[CI(f, i0, f->JCl)], As1 = f->A[CI(f, i0 + 1, f->JCl)];
+            pst wn = WAVG(f->WoN[i0], f->WoN[i0 + 1], An0, An1);
+            pst ws = WAVG(f->WoS[i0], f->WoS[i0 + 1], As0, As1);
+            c->WoN[ic] = wn; c->WoS[ic] = ws;
+        }
+        for (int jc = c->JCl; jc <= c->JCu; ++jc) {
+            const int j0 = FINE_IDX(Ng, jc);
+            double Aw0 = f->A[CI(f, f->ICl, j0)], Aw1 = f->A[CI(f, f->ICl, j0 + 1)];
+            double Ae0 = f->A[CI(f, f->ICu, j0)], Ae1 = f->A[CI(f, f->ICu, j0 + 1)];
+            pst ww = WAVG(f->WoW[j0], f->WoW[j0 + 1], Aw0, Aw1);
+            pst we = WAVG(f->WoE[j0], f->WoE[j0 + 1], Ae0, Ae1);
+            c->WoW[jc] = ww; c->WoE[jc] = we;
+        }
+#undef WAVG
+    }
+}
+/* Update_Primitive_Variables :2423-2445 */
+static void mg_update_primitive(orc_ctx *C) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *B = &C->b[b];
+        for (int i = B->ICl; i <= B->ICu; ++i)
+            for (int j = B->JCl; j <= B->JCu; ++j) B->W[CI(B, i, j)] = W_of_U(B->U[CI(B, i, j)]);
+    }
+}
+/* Subtract_dUdt_from_P :2331-2352 */
+static void mg_subtract_dudt(orc_ctx *C) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *B = &C->b[b];
+        for (int i = B->ICl; i <= B->ICu; ++i)
+            for (int j = B->JCl; j <= B->JCu; ++j) {
+                cst *P = &B->P[CI(B, i, j)];
+                const cst r = B->dUdt[0][CI(B, i, j)];
+                P->d -= r.d; P->mx -= r.mx; P->my -= r.my; P->E -= r.E;
+            }
+    }
+}
+/* CFL_Multigrid :2364-2407: a coarse cell's dt no larger than its fine
+   cells' dt scaled by sqrt(A_coarse/A_fine) */
+static void mg_cfl_multigrid(orc_ctx *F, orc_ctx *C) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *f = &F->b[b], *c = &C->b[b];
+        const int Ng = f->ng;
+        for (int ic = c->ICl; ic <= c->ICu; ++ic)
+            for (int jc = c->JCl; jc <= c->JCu; ++jc) {
+                const int i0 = FINE_IDX(Ng, ic), j0 = FINE_IDX(Ng, jc);
+                const double Ac = c->A[CI(c, ic, jc)];
+                const int ne = CI(f, i0 + 1, j0 + 1), se = CI(f, i0 + 1, j0), nw = CI(f, i0, j0 + 1), sw = CI(f, i0, j0);
+                const double dt_NE = sqrt(Ac / f->A[ne]) * f->dt[ne];
+                const double dt_SE = sqrt(Ac / f->A[se]) * f->dt[se];
+                const double dt_NW = sqrt(Ac / f->A[nw]) * f->dt[nw];
+                const double dt_SW = sqrt(Ac / f->A[sw]) * f->dt[sw];
+                double *dt = &c->dt[CI(c, ic, jc)];
+                *dt = dmin(*dt, dmin(dt_NE, dmin(dt_SE, dmin(dt_NW, dt_SW))));
+            }
+    }
+}
+/* Store_Current_Solution_in_uo_MG :2294-2316 and Evaluate_Solution_Changes
+   :2454-2478 (all cells, ghosts included) */
+static void mg_store_uo(orc_ctx *C) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *B = &C->b[b];
+        memcpy(B->uo, B->U, sizeof(cst) * B->nci * B->ncj);
+    }
+}
+static void mg_solution_changes(orc_ctx *C) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *B = &C->b[b];
+        for (int q = 0; q < B->nci * B->ncj; ++q) {
+            B->U[q].d -= B->uo[q].d; B->U[q].mx -= B->uo[q].mx;
+            B->U[q].my -= B->uo[q].my; B->U[q].E -= B->uo[q].E;
+        }
+    }
+}
+
+/* Bilinear_Interpolation, CFD.h:1367-1416 (Zingg and Yarrow): the point P0
+   in the quadrilateral P1..P4 (clockwise from P1), 1 if the solve fails or
+   zeta falls outside [0, 1] */
+static int bilinear(const cst *U1, double x1, double y1, const cst *U2, double x2, double y2, const cst *U3, double x3,
+                    double y3, const cst *U4, double x4, double y4, double x0, double y0, cst *U0) {
+    double a0 = x1, a1 = x4 - x1, a2 = x2 - x1, a3 = x1 + x3 - x2 - x4;
+    double b0 = y1, b1 = y4 - y1, b2 = y2 - y1, b3 = y1 + y3 - y2 - y4;
+    double a = a1 * b3 - a3 * b1;
+    double b = a0 * b3 - b3 * x0 + a1 * b2 - a2 * b1 + a3 * y0 - a3 * b0;
+    double c = a0 * b2 - b2 * x0 + a2 * y0 - a2 * b0;
+    double zeta0, eta0;
+    if (a > TOLER || a < -TOLER) {
+        if (b * b - 4 * a * c < 0) return 1;
+        zeta0 = (-b + sqrt(b * b - 4 * a * c)) / (2 * a);
+        if (zeta0 < 0 || zeta0 > 1) {
+            zeta0 = (-b - sqrt(b * b - 4 * a * c)) / (2 * a);
+            if (zeta0 < 0 || zeta0 > 1) return 1;
+        }
+    } else if (b > TOLER || b < -TOLER) {
+        zeta0 = -c / b;
+        if (zeta0 < 0 || zeta0 > 1) return 1;
+    } else {
+        return 1;
+    }
+    if (b2 + b3 * zeta0 < TOLER && b2 + b3 * zeta0 > -TOLER) return 1;
+    eta0 = (y0 - b0 - b1 * zeta0) / (b2 + b3 * zeta0);
+#define BIL(m) U0->m = U1->m + (U4->m - U1->m) * zeta0 + (U2->m - U1->m) * eta0 + (U1->m + U3->m - U2->m - U4->m) * zeta0 * eta0
+    BIL(d); BIL(mx); BIL(my); BIL(E);
+#undef BIL
+    return 0;
+}
+int orc_bilinear(const double *U, const double *xy, double x0, double y0, double *out) {
+    cst u[4], r;
+    memcpy(u, U, sizeof u);
+    int e = bilinear(&u[0], xy[0], xy[1], &u[1], xy[2], xy[3], &u[2], xy[4], xy[5], &u[3], xy[6], xy[7], x0, y0, &r);
+    memcpy(out, &r, sizeof r);
+    return e;
+}
+
+/* the interpolation quadrilaterals each fine cell tries, by the offset of
+   the quadrilateral's SW coarse cell from the parent: fine SW, SE, NE, NW in
+   the order of Prolong_and_Update_Solution_Blocks :1718-2211 */
+static const int MG_QUAD[4][4][2] = {
+    {{-1, -1}, {-1, 0}, {0, -1}, {0, 0}},   /* SW */
+    {{0, -1}, {0, 0}, {-1, -1}, {-1, 0}},   /* SE */
+    {{0, 0}, {0, -1}, {-1, 0}, {-1, -1}},   /* NE */
+    {{-1, 0}, {0, 0}, {-1, -1}, {0, -1}},   /* NW */
+};
+static const int MG_FINE[4][2] = {{0, 0}, {1, 0}, {1, 1}, {0, 1}};
+
+static int unphysical(cst U) { return U.d <= ZERO || U.E <= ZERO || U_e(U) <= ZERO; }
+
+/* near a Dirichlet-type boundary (BC_FIXED is the only one on this path):
+   prolong by injection, :1675-1716 */
+static int next_to_dirichlet(const oblk *c, int ic, int jc) {
+    return (ic == c->ICl && c->bcW[jc] == CFFC_BC_FIXED) || (ic == c->ICu && c->bcE[jc] == CFFC_BC_FIXED) ||
+           (jc == c->JCl && c->bcS[ic] == CFFC_BC_FIXED) || (jc == c->JCu && c->bcN[ic] == CFFC_BC_FIXED);
+}
+
+/* Prolong_and_Update_Solution_Blocks :1627-2213: coarse U holds the solution
+   change (Evaluate_Solution_Changes); add it to the fine cells, bilinear or
+   injected, halving an update that leaves the state unphysical.  A failed
+   interpolation restarts the coarse cell with injection from its SW fine
+   cell, which then receives a second update, as in the reference. */
+static void mg_prolong_update(orc_ctx *C, orc_ctx *F, const cffc_mg_desc *d) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *c = &C->b[b], *f = &F->b[b];
+        const int Ng = c->ng;
+        if (!d->prolong_using_injection) {
+            for (int jc = c->JCl; jc <= c->JCu; ++jc) {
+                if (c->bcE[jc] == CFFC_BC_CONSTANT_EXTRAPOLATION) c->U[CI(c, c->ICu + 1, jc)] = c->U[CI(c, c->ICu, jc)];
+                if (c->bcW[jc] == CFFC_BC_CONSTANT_EXTRAPOLATION) c->U[CI(c, c->ICl - 1, jc)] = c->U[CI(c, c->ICl, jc)];
+            }
+            for (int ic = c->ICl; ic <= c->ICu; ++ic) {
+                if (c->bcS[ic] == CFFC_BC_CONSTANT_EXTRAPOLATION) c->U[CI(c, ic, c->JCl - 1)] = c->U[CI(c, ic, c->JCl)];
+                if (c->bcN[ic] == CFFC_BC_CONSTANT_EXTRAPOLATION) c->U[CI(c, ic, c->JCu + 1)] = c->U[CI(c, ic, c->JCu)];
+            }
+        }
+        for (int ic = c->ICl; ic <= c->ICu; ++ic)
+            for (int jc = c->JCl; jc <= c->JCu; ++jc) {
+                const int i0 = FINE_IDX(Ng, ic), j0 = FINE_IDX(Ng, jc);
+                const cst Uc = c->U[CI(c, ic, jc)];
+                if (d->injection_at_dirichlet && next_to_dirichlet(c, ic, jc)) {
+                    for (int q = 0; q < 4; ++q) {
+                        cst *Uf = &f->U[CI(f, i0 + MG_FINE[q][0], j0 + MG_FINE[q][1])];
+                        Uf->d += Uc.d; Uf->mx += Uc.mx; Uf->my += Uc.my; Uf->E += Uc.E;
+                    }
+                    continue;
+                }
+                int inject = d->prolong_using_injection;
+                for (int q = 0; q < 4; ++q) {
+                    const int fi = i0 + MG_FINE[q][0], fj = j0 + MG_FINE[q][1];
+                    cst dU;
+                    if (!inject) {
+                        int err = 1;
+                        for (int t = 0; t < 4 && err; ++t) {
+                            const int a = ic + MG_QUAD[q][t][0], e = jc + MG_QUAD[q][t][1];
+                            const int p1 = CI(c, a, e), p2 = CI(c, a, e + 1), p3 = CI(c, a + 1, e + 1), p4 = CI(c, a + 1, e);
+                            err = bilinear(&c->U[p1], c->xc[p1], c->yc[p1], &c->U[p2], c->xc[p2], c->yc[p2], &c->U[p3],
+                                           c->xc[p3], c->yc[p3], &c->U[p4], c->xc[p4], c->yc[p4], f->xc[CI(f, fi, fj)],
+                                           f->yc[CI(f, fi, fj)], &dU);
+                        }
+                        if (err) {
+                            inject = 1;
+                            q = -1;   /* goto restart */
+                            continue;
+                        }
+                    } else {
+                        dU = Uc;
+                    }
+                    cst *Uf = &f->U[CI(f, fi, fj)];
+                    if (d->update_stability_switch) {
+                        cst Un = {Uf->d + dU.d, Uf->mx + dU.mx, Uf->my + dU.my, Uf->E + dU.E};
+                        if (unphysical(Un)) {
+                            int n = 1;
+                            while (n <= d->max_update_reductions) {
+                                dU.d *= HALF; dU.mx *= HALF; dU.my *= HALF; dU.E *= HALF;
+                                cst Um = {Uf->d + dU.d, Uf->mx + dU.mx, Uf->my + dU.my, Uf->E + dU.E};
+                                if (!unphysical(Um)) break;
+                                n++;
+                            }
+                            if (n > d->max_update_reductions) { dU.d = 0; dU.mx = 0; dU.my = 0; dU.E = 0; }
+                        }
+                    }
+                    Uf->d += dU.d; Uf->mx += dU.mx; Uf->my += dU.my; Uf->E += dU.E;
+                }
+            }
+    }
+}
+
+/* dUdt[0] of the whole level: Residual_Evaluation_for_Multigrid :2755-2858
+   (no resolution changes within a level, so no flux corrections), plus the
+   forcing term P on coarse levels */
+static void mg_residual(orc_ctx *C, int add_P) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *B = &C->b[b];
+        block_residual(B, &C->s);
+        if (add_P)
+            for (int j = B->JCl; j <= B->JCu; ++j)
+                for (int i = B->ICl; i <= B->ICu; ++i) {
+                    cst *r = &B->dUdt[0][CI(B, i, j)];
+                    const cst P = B->P[CI(B, i, j)];
+                    r->d += P.d; r->mx += P.mx; r->my += P.my; r->E += P.E;
+                }
+    }
+}
+
+/* Smooth :2865-3110 (Top_Level = 0) */
+static int mg_smooth(orc_mg *mg, int l) {
+    orc_ctx *C = mg->lv[l];
+    double dtmin;
+    orc_cfl(C, &dtmin, 0);
+    if (l != 0) mg_cfl_multigrid(mg->lv[l - 1], C);
+    for (int st = 1; st <= C->s.n_stage; ++st) {
+        orc_exchange_and_bcs(C);
+        orc_dudt_stage(C, st);
+        if (l > 0) {
+            /* Apply_the_FAS_Multigrid_Forcing_Term :2678-2745 */
+            const int k = stage_k_residual(&C->s, st);
+            for (int b = 0; b < C->nblk; ++b) {
+                oblk *B = &C->b[b];
+                for (int j = B->JCl; j <= B->JCu; ++j)
+                    for (int i = B->ICl; i <= B->ICu; ++i) {
+                        const int q = CI(B, i, j);
+                        const double cdt = C->s.cfl_number * B->dt[q];
+                        cst *r = &B->dUdt[k][q];
+                        r->d += cdt * B->P[q].d; r->mx += cdt * B->P[q].mx;
+                        r->my += cdt * B->P[q].my; r->E += cdt * B->P[q].E;
+                    }
+            }
+        }
+        /* Residual_Smoothing under TIME_STEPPING_MULTIGRID smooths dUdt[0] */
+        if (mg->rs_iters > 0)
+            for (int b = 0; b < C->nblk; ++b) block_residual_smoothing(&C->b[b], 0, mg->rs_eps, mg->rs_iters);
+        int64_t bad;
+        int e = orc_update_stage(C, st, &bad);
+        if (e) return e;
+    }
+    orc_exchange_and_bcs(C);
+    mg_residual(C, l > 0);
+    return 0;
+}
+
+/* Coarse_Grid_Correction :3118-3370 (Top_Level = 0) */
+static int mg_cgc(orc_mg *mg, int l) {
+    const cffc_mg_desc *d = &mg->d;
+    int e;
+    if (l == mg->L - 1) {
+        mg_store_uo(mg->lv[l]);
+        for (int nu = 1; nu <= d->smooths_on_coarsest_level; ++nu)
+            if ((e = mg_smooth(mg, l))) return e;
+    } else {
+        const int m_cycle = (d->i_cycle == CFFC_MULTIGRID_V_CYCLE || l == 0) ? 1 : 2;
+        for (int m = 1; m <= m_cycle; ++m) {
+            mg_store_uo(mg->lv[l]);
+            const int n_pre = (l == 0) ? d->smooths_on_finest_level : d->pre_smooths;
+            for (int nu = 0; nu < n_pre; ++nu)
+                if ((e = mg_smooth(mg, l))) return e;
+            mg_restrict(mg->lv[l], mg->lv[l + 1], 0);
+            mg_update_primitive(mg->lv[l + 1]);
+            orc_exchange_and_bcs(mg->lv[l + 1]);
+            if (d->defect_correction) {
+                mg_restrict(mg->lv[l], mg->lv[l + 1], 1);
+                mg_residual(mg->lv[l + 1], 0);
+                mg_subtract_dudt(mg->lv[l + 1]);
+            }
+            if ((e = mg_cgc(mg, l + 1))) return e;
+            mg_prolong_update(mg->lv[l + 1], mg->lv[l], d);
+            mg_update_primitive(mg->lv[l]);
+            orc_exchange_and_bcs(mg->lv[l]);
+            const int n_post = (l == 0) ? 0 : d->post_smooths;
+            for (int nu = 0; nu < n_post; ++nu)
+                if ((e = mg_smooth(mg, l))) return e;
+        }
+    }
+    if (l != 0) mg_solution_changes(mg->lv[l]);
+    return 0;
+}
+
+int orc_mg_create(const cffc_mesh_desc *meshes, int nlevels, const cffc_scheme_desc *smoother,
+                  const cffc_mg_desc *d, orc_mg **out) {
+    if (nlevels < 2 || nlevels != d->levels) return -1;
+    orc_mg *mg = (orc_mg *)calloc(1, sizeof(orc_mg));
+    mg->L = nlevels;
+    mg->d = *d;
+    mg->lv = (orc_ctx **)calloc(nlevels, sizeof(orc_ctx *));
+    for (int l = 0; l < nlevels; ++l) {
+        cffc_scheme_desc s = *smoother;
+        if (l > 0 && d->first_order_coarse) {
+            s.i_flux_function = CFFC_FLUX_FUNCTION_HLLE;
+            s.i_limiter = CFFC_LIMITER_ZERO;
+        }
+        int e = orc_create(&meshes[l], &s, &mg->lv[l]);
+        if (e) { orc_mg_destroy(mg); return e; }
+        if (l > 0 && (mg->lv[l]->nblk != mg->lv[0]->nblk)) { orc_mg_destroy(mg); return -1; }
+        for (int b = 0; b < mg->lv[l]->nblk; ++b) {
+            oblk *B = &mg->lv[l]->b[b];
+            if (l > 0) {
+                const oblk *Fb = &mg->lv[l - 1]->b[b];
+                if (Fb->ni != 2 * B->ni || Fb->nj != 2 * B->nj) { orc_mg_destroy(mg); return -1; }
+            }
+            B->P = (cst *)calloc(B->nci * B->ncj, sizeof(cst));
+            B->uo = (cst *)calloc(B->nci * B->ncj, sizeof(cst));
+        }
+    }
+    *out = mg;
+    return 0;
+}
+void orc_mg_destroy(orc_mg *mg) {
+    if (!mg) return;
+    for (int l = 0; l < mg->L; ++l) orc_destroy(mg->lv[l]);
+    free(mg->lv);
+    free(mg);
+}
+orc_ctx *orc_mg_level(orc_mg *mg, int l) { return (l >= 0 && l < mg->L) ? mg->lv[l] : 0; }
+int orc_mg_set_residual_smoothing(orc_mg *mg, double epsilon, int iterations) {
+    mg->rs_eps = epsilon;
+    mg->rs_iters = iterations;
+    return 0;
+}
+/* the start of Execute :3395-3420 for a state set on level 0: every coarse
+   level restricted from the one above (the IC_RESTART path of Apply_ICs
+   :719-733) with its boundary reference states, then messages and BCs */
+int orc_mg_initialize(orc_mg *mg) {
+    orc_exchange_and_bcs(mg->lv[0]);
+    for (int l = 1; l < mg->L; ++l) {
+        mg_restrict(mg->lv[l - 1], mg->lv[l], 0);
+        mg_update_primitive(mg->lv[l]);
+        mg_restrict_wo(mg->lv[l - 1], mg->lv[l]);
+        orc_exchange_and_bcs(mg->lv[l]);
+    }
+    return 0;
+}
+/* one FAS cycle from level 0 */
+int orc_mg_cycle(orc_mg *mg) { return mg_cgc(mg, 0); }
+/* Execute's regular-cycle loop :3440-3705: the residual norms of level 0
+   (the dUdt[0] the last cycle's smoothing left) before each cycle, the
+   relative norm against the second cycle's, limiter freezing, and the
+   absolute / relative / cycle-count exits.  norms: 3 per check. */
+int orc_mg_execute(orc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done, int *checks) {
+    double init_l2 = -1.0, ratio = -1.0;
+    int first_step = 1, frozen = 0, n = 0, e = 0;
+    int nchk = 0;
+    for (int cycles = 1;; ++cycles) {
+        double nr[3];
+        orc_residual_norms(mg->lv[0], 1, nr);
+        if (norms) { norms[3 * nchk] = nr[0]; norms[3 * nchk + 1] = nr[1]; norms[3 * nchk + 2] = nr[2]; }
+        nchk++;
+        if (cycles == 2) init_l2 = nr[1];
+        else if (cycles > 2 && fabs(init_l2) > NANO) ratio = nr[1] / init_l2;
+        int stop = n >= max_cycles;
+        if (cycles > 2) {
+            if (nr[1] < mg->d.absolute_tolerance) stop = 1;
+            if (ratio < mg->d.relative_tolerance) stop = 1;
+        }
+        if (stop) break;
+        if (!first_step && cycles > 1 && freeze_level > 0 && !frozen && mg->lv[0]->s.i_limiter != CFFC_LIMITER_ZERO &&
+            nr[1] <= freeze_level) {
+            for (int l = 0; l < mg->L; ++l) orc_freeze_limiters(mg->lv[l], 1);
+            frozen = 1;
+        }
+        if ((e = mg_cgc(mg, 0))) break;
+        first_step = 0;
+        n++;
+    }
+    if (cycles_done) *cycles_done = n;
+    if (checks) *checks = nchk;
+    orc_exchange_and_bcs(mg->lv[0]);
+    return e;
+}
+/* P and uo of one level, interior cells in block order (tests) */
+int orc_mg_get_P(orc_mg *mg, int l, double *out) {
+    orc_ctx *C = mg->lv[l];
+    int64_t q = 0;
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *B = &C->b[b];
+        for (int j = B->JCl; j <= B->JCu; ++j)
+            for (int i = B->ICl; i <= B->ICu; ++i, ++q) memcpy(out + 4 * q, &B->P[CI(B, i, j)], sizeof(cst));
+    }
+    return 0;
+}
+/* HLLE flux (known-answer tests) */
+void orc_flux_hlle_n(const double *Wl, const double *Wr, double nx, double ny, double *F) {
+    pst l, r;
+    memcpy(&l, Wl, sizeof l); memcpy(&r, Wr, sizeof r);
+    cst f = flux_hlle_n(l, r, nx, ny);
+    memcpy(F, &f, sizeof f);
 }

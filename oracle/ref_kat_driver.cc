@@ -12,7 +12,8 @@
 // VENKAT (uQuad[4] u0 umin umax -> phi), HARTEN (la ll lr -> pos neg),
 // REFLECT (W[4] nx ny -> W[4]), BCCHARP (Wi[4] Wo[4] nx ny -> W[4]),
 // W_OF_U (U[4] -> W[4]), U_OF_W (W[4] -> U[4]), FX (W[4] -> F[4]),
-// RK (i n -> beta), OPTSM (i n limiter -> beta).
+// RK (i n -> beta), OPTSM (i n limiter -> beta), HLLE_N (Wl[4] Wr[4] nx ny ->
+// F[4]), BILIN (U1..U4 x1 y1 .. x4 y4 x0 y0 -> flag U0, scalar states).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -85,6 +86,18 @@ int main() {
             Euler2D_cState F = P(v, 0).Fx();
             for (int k = 0; k < 4; ++k) out[k] = F[k + 1];
             emit(tag, v, out, 4);
+        } else if (t == "HLLE_N") {
+            Euler2D_cState F = FluxHLLE_n(P(v, 0), P(v, 4), Vector2D(v[8], v[9]));
+            for (int k = 0; k < 4; ++k) out[k] = F[k + 1];
+            emit(tag, v, out, 4);
+        } else if (t == "BILIN") {
+            double U0 = 0.0;
+            int e = Bilinear_Interpolation(v[0], Vector2D(v[4], v[5]), v[1], Vector2D(v[6], v[7]), v[2],
+                                           Vector2D(v[8], v[9]), v[3], Vector2D(v[10], v[11]),
+                                           Vector2D(v[12], v[13]), U0);
+            out[0] = (double)e;
+            out[1] = e ? 0.0 : U0;
+            emit(tag, v, out, 2);
         } else if (t == "RK") {
             out[0] = Runge_Kutta((int)v[0], (int)v[1]);
             emit(tag, v, out, 1);

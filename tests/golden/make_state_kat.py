@@ -1,7 +1,8 @@
 """Write tests/golden/state_kat.txt: known-answer vectors for the reference's
 state and CFD routines (FluxRoe_n, RoeAverage, Limiter_Venkatakrishnan,
 HartenFixPos/Neg, Reflect, BC_Characteristic_Pressure, W(U), U(W), Fx,
-Runge_Kutta, MultiStage_Optimally_Smoothing).
+Runge_Kutta, MultiStage_Optimally_Smoothing, FluxHLLE_n and
+Bilinear_Interpolation).
 
 Inputs are generated here from a fixed seed (plus hand-picked edge cases: the
 limiter dead band at |uQuad-u0| = TOLER, the Harten switch, sonic and
@@ -130,6 +131,33 @@ def cases():
         for lim in (0, 9):
             for i in range(1, nst + 1):
                 lines.append(fmt("OPTSM", [i, nst, lim]))
+    # HLLE flux: random pairs, supersonic both ways (upwind branches), near-equal
+    Wl, Wr = rand_W(rng, 400), rand_W(rng, 400)
+    nn = unit(rng, 400)
+    for k in range(400):
+        lines.append(fmt("HLLE_N", list(Wl[k]) + list(Wr[k]) + list(nn[k])))
+    Wl, Wr = rand_W(rng, 100, True), rand_W(rng, 100, True)
+    for k in range(100):
+        lines.append(fmt("HLLE_N", list(Wl[k]) + list(Wr[k]) + [1.0, 0.0]))
+        lines.append(fmt("HLLE_N", list(Wl[k]) + list(Wr[k]) + [0.0, -1.0]))
+    W = rand_W(rng, 50)
+    nn = unit(rng, 50)
+    for k in range(50):
+        lines.append(fmt("HLLE_N", list(W[k]) + list(W[k] * (1 + rng.uniform(-1e-6, 1e-6, 4))) + list(nn[k])))
+    # Bilinear_Interpolation (scalar states): points inside, on the edges and
+    # outside skewed quadrilaterals, plus parallelograms (a == 0 branch)
+    for k in range(600):
+        base = np.array([[0.0, 0.0], [0.0, 1.0], [1.0, 1.0], [1.0, 0.0]])
+        quad = base + rng.normal(0, 0.15, (4, 2)) * rng.choice([0.0, 1.0], p=[0.2, 0.8])
+        if k % 5 == 0:
+            quad[2] = quad[1] + quad[3] - quad[0]   # parallelogram
+        zeta, eta = rng.uniform(-0.3, 1.3, 2) if k % 3 == 0 else rng.uniform(0, 1, 2)
+        if k % 11 == 0:
+            zeta = float(rng.choice([0.0, 1.0]))
+        p0 = (quad[0] * (1 - zeta) * (1 - eta) + quad[3] * zeta * (1 - eta) + quad[2] * zeta * eta +
+              quad[1] * (1 - zeta) * eta)
+        U = rng.normal(0, 1, 4)
+        lines.append(fmt("BILIN", list(U) + list(quad.ravel()) + list(p0)))
     return lines
 
 

@@ -175,3 +175,36 @@ def test_quad_centroid_closed_form(oracle_lib):
     # trapezoid with parallel sides 2 (bottom) and 1 (top), height 1
     x, y, A = oracle.quad_centroid([0, 2, 1.5, 0.5], [0, 0, 1, 1])
     assert abs(A - 1.5) < 1e-15 and abs(x - 1.0) < 1e-15 and abs(y - 4.0 / 9.0) < 1e-15
+
+
+def test_kat_flux_hlle_n(oracle_lib):
+    _check("HLLE_N", 10, lambda v: oracle.flux_hlle_n(v[0:4], v[4:8], v[8], v[9]))
+
+
+def _bilin_scalar(v):
+    U = np.zeros((4, 4))
+    U[:, 0] = v[0:4]
+    e, out = oracle.bilinear(U, np.asarray(v[4:12]).reshape(4, 2), v[12], v[13])
+    return [float(e), 0.0 if e else out[0]]
+
+
+def test_kat_bilinear_interpolation(oracle_lib):
+    _check("BILIN", 14, _bilin_scalar)
+
+
+def test_hlle_consistency_and_upwinding(oracle_lib):
+    """F_HLLE(W, W) = Fn(W); supersonic to the right (left) gives the left
+    (right) state's normal flux exactly."""
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        W = np.array([rng.uniform(0.2, 2), rng.normal(), rng.normal(), rng.uniform(0.2, 2)])
+        th = rng.uniform(0, 2 * np.pi)
+        nx, ny = np.cos(th), np.sin(th)
+        F = oracle.flux_hlle_n(W, W, nx, ny)
+        un = W[1] * nx + W[2] * ny
+        H = 1.4 / 0.4 * W[3] + 0.5 * W[0] * (W[1] ** 2 + W[2] ** 2)
+        ref = [W[0] * un, W[0] * un * W[1] + W[3] * nx, W[0] * un * W[2] + W[3] * ny, un * H]
+        np.testing.assert_allclose(F, ref, rtol=1e-13, atol=1e-13)
+    Wl = np.array([1.0, 3.0, 0.2, 1.0])
+    Wr = np.array([0.5, 2.5, -0.1, 0.4])
+    np.testing.assert_array_equal(oracle.flux_hlle_n(Wl, Wr, 1.0, 0.0), oracle.Fx(Wl))
