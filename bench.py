@@ -141,6 +141,50 @@ def steady_leg(ncases, seed=7):
             "all_converged": bool(np.all(cs > 0) and np.all(ws > 0))}
 
 
+def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None):
+    """The same held-out steady bump cases, cold, solved together (one mesh
+    of disjoint parts) by the FAS multigrid solver (FASMultigrid2D: V cycle,
+    HLLE first-order coarse levels, 5-stage optimal smoothing).  Multigrid
+    stops on the L2 norm of the unscaled density residual R (the
+    reference's Absolute_Convergence_Tolerance); the single-grid runs stop
+    on the L2 norm of (CFL*dt)*R at 1e-10.  The multigrid tolerance is that
+    1e-10 divided by the median CFL*dt of the single-grid scheme over the
+    cells at the initial state, and the freeze level is converted the same
+    way."""
+    from cffc_amd import euler2d, multigrid, steady
+    from cffc_amd import mesh as M
+    cases = steady.sample_cases(ncases, seed)
+    parts = [steady.bump_case(h, mach) for h, mach in cases]
+    m, _ = M.concat([p[0] for p in parts])
+    W0 = np.concatenate([p[1] for p in parts])
+    sg = steady.bump_scheme()
+    f = euler2d.Euler2DQuadBlocks(m, sg)
+    f.set_W(W0)
+    f.Send_All_Messages_and_BCs()
+    f.CFL(set_global=False)
+    scale = float(np.median(sg.cfl_number * f.get_dt()))
+    f.close()
+    tol = 1e-10 / scale
+    freeze = steady.STEADY_FREEZE_LEVEL / scale
+    p = multigrid.MGParams(levels=levels, absolute_tolerance=tol)
+    g = multigrid.FASMultigrid(m, steady.bump_scheme(cfl=cfl), p)
+    g.set_W(W0)
+    g.initialize()
+    t0 = time.perf_counter()
+    rc, cycles, norms = g.execute(20000, freeze_level=freeze)
+    dt = time.perf_counter() - t0
+    g.close()
+    out = {"cases": ncases, "seed": seed, "levels": levels, "cycle": "V", "cfl": cfl,
+           "mesh": "%d x (192x64 cells, 12x4 blocks of 16^2), coarsened to 2^2-cell blocks" % ncases,
+           "scale_median_cfl_dt": scale, "tol_R": tol, "freeze_R": freeze, "rc": rc, "cycles": cycles,
+           "final_R": float(norms[-1, 1]), "seconds": dt, "ms_per_cycle": 1e3 * dt / max(cycles, 1),
+           "converged": bool(rc == 0 and norms[-1, 1] < tol)}
+    if cold_batch_s:
+        out["cold_single_grid_batch_s"] = cold_batch_s
+        out["time_ratio_vs_cold_single_grid"] = cold_batch_s / dt
+    return out
+
+
 def amr_case(seed=3):
     """The quadtree workload: [0,1]^2 in 32 x 32 root blocks of 16^2 cells,
     refined twice where blocks come within 0.02 of the circle r = 0.3 about
@@ -252,6 +296,9 @@ def main():
     ap.add_argument("--amr-steps", type=int, default=1000,
                     help="steps of the quadtree (resolution-change) workload (0: skip)")
     ap.add_argument("--only-amr", action="store_true", help="run only the quadtree workload and print it")
+    ap.add_argument("--mg-cases", type=int, default=8, help="steady cases solved by FAS multigrid (0: skip)")
+    ap.add_argument("--only-steady", action="store_true",
+                    help="run only the steady legs (cold/warm single grid, multigrid) and print them")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
@@ -260,6 +307,13 @@ def main():
         fixed = tuple(int(x) for x in args.tile.lower().split("x")) if args.tile else (16, 8)
         print(json.dumps(amr_leg(args.amr_steps or 200, autotune_tiles=not args.no_autotune and not args.tile,
                                  fixed_tile=fixed)))
+        return
+    if args.only_steady:
+        o = {"steady_warm_vs_cold": steady_leg(args.steady_cases)} if args.steady_cases > 0 else {}
+        if args.mg_cases > 0:
+            o["steady_multigrid"] = multigrid_leg(args.mg_cases, cold_batch_s=o.get("steady_warm_vs_cold", {}).get(
+                "cold_batch_s"))
+        print(json.dumps(o))
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build runs the whole mesh on a single MI355X "
@@ -340,6 +394,12 @@ def main():
             out["steady_warm_vs_cold"] = steady_leg(args.steady_cases)
         except Exception as e:  # reported, never required for the throughput line
             out["steady_warm_vs_cold"] = {"error": str(e)}
+    if args.mg_cases > 0:
+        try:
+            out["steady_multigrid"] = multigrid_leg(args.mg_cases, cold_batch_s=out.get(
+                "steady_warm_vs_cold", {}).get("cold_batch_s"))
+        except Exception as e:  # reported, never required for the throughput line
+            out["steady_multigrid"] = {"error": str(e)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_threads)
