@@ -1654,10 +1654,18 @@ struct cffc_mg {
     double rs_eps = 0.0;
     int rs_iters = 0;
     bool frozen = false;
+    // Execute replays one captured cycle (kernels only); recaptured when the
+    // limiters freeze (the stage kernels' phi mode changes)
+    bool use_graph = true;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int graph_frozen = -1;
 };
 
 static void mg_free(cffc_mg *mg) {
     if (!mg) return;
+    if (mg->gexec) (void)hipGraphExecDestroy(mg->gexec);
+    if (mg->graph) (void)hipGraphDestroy(mg->graph);
     for (size_t l = 0; l < mg->lv.size(); ++l) {
         if (l < mg->P.size() && mg->P[l]) (void)hipFree(mg->P[l]);
         if (l < mg->uo.size() && mg->uo[l]) (void)hipFree(mg->uo[l]);
@@ -1809,7 +1817,7 @@ static int mg_smooth(cffc_mg *mg, int l) {
 
 static int mg_store_uo(cffc_mg *mg, int l) {
     cffc_ctx *c = mg->lv[l];
-    HIPCHK(hipMemcpyAsync(mg->uo[l], c->U, 4 * (size_t)c->ncp * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(cffc_launch_mg_copy(mg->uo[l], c->U, 4 * c->ncp, c->stream));
     return 0;
 }
 
@@ -1951,6 +1959,38 @@ extern "C" int cffc_mg_cycle(cffc_mg *mg, int64_t *bad_cell) {
     return mg_check_errors(mg, bad_cell);
 }
 
+extern "C" int cffc_mg_set_graph(cffc_mg *mg, int on) {
+    if (!mg) return fail(CFFC_E_ARG, "null multigrid solver");
+    mg->use_graph = on != 0;
+    return 0;
+}
+
+// enqueue one cycle: a replay of the captured cycle, captured first (and
+// again after the limiters freeze), or the launches themselves
+static int mg_enqueue_cycle(cffc_mg *mg) {
+    cffc_ctx *f = mg->lv[0];
+    if (!mg->use_graph) {
+        if (int e = mg_reset_errors(mg)) return e;
+        return mg_cgc(mg, 0);
+    }
+    if (!mg->gexec || mg->graph_frozen != (int)mg->frozen) {
+        if (mg->gexec) (void)hipGraphExecDestroy(mg->gexec);
+        if (mg->graph) (void)hipGraphDestroy(mg->graph);
+        mg->gexec = nullptr;
+        mg->graph = nullptr;
+        HIPCHK(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
+        int e = mg_reset_errors(mg);
+        if (!e) e = mg_cgc(mg, 0);
+        hipError_t ce = hipStreamEndCapture(f->stream, &mg->graph);
+        if (e) return e;
+        if (ce != hipSuccess) return fail(CFFC_E_HIP, "multigrid cycle capture failed: %s", hipGetErrorString(ce));
+        HIPCHK(hipGraphInstantiate(&mg->gexec, mg->graph, nullptr, nullptr, 0));
+        mg->graph_frozen = (int)mg->frozen;
+    }
+    HIPCHK(hipGraphLaunch(mg->gexec, f->stream));
+    return 0;
+}
+
 extern "C" int cffc_mg_get_P(cffc_mg *mg, int level, double *P, int64_t ncell) {
     if (!mg || level < 1 || level >= (int)mg->lv.size()) return fail(CFFC_E_ARG, "bad level");
     return get_field(mg->lv[level], mg->P[level], P, ncell, 4);
@@ -1967,12 +2007,22 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
     double init_l2 = -1.0, ratio = -1.0;
     bool first_step = true;
     int n = 0, nchk = 0, rc = 0;
+    std::vector<unsigned long long> keys(mg->lv.size());
+    if (int e = mg_reset_errors(mg)) return e;
     for (int cycles = 1;; ++cycles) {
+        // the norms of the last cycle's level-0 residual and every level's
+        // error key, read with one synchronisation
         double nr[3];
         HIPCHK(cffc_launch_norms(f->meta, f->tiles, f->ntiles, f->ti, f->tj, f->dUdt[0], f->ncp, 1, f->norm_part,
                                  f->norm_small, f->stream));
         HIPCHK(hipMemcpyAsync(nr, f->norm_small, 3 * sizeof(double), hipMemcpyDeviceToHost, f->stream));
+        for (size_t l = 0; l < mg->lv.size(); ++l)
+            HIPCHK(hipMemcpyAsync(&keys[l], mg->lv[l]->scal + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  f->stream));
         HIPCHK(hipStreamSynchronize(f->stream));
+        for (size_t l = 0; l < keys.size() && !rc; ++l)
+            if (keys[l] != ERR_NONE) rc = (int)(keys[l] & 0xFFFF);
+        if (rc) break;
         if (norms) memcpy(norms + 3 * nchk, nr, sizeof nr);
         nchk++;
         if (cycles == 2) init_l2 = nr[1];
@@ -1985,7 +2035,7 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
             for (cffc_ctx *c : mg->lv) c->freeze = 1;
             mg->frozen = true;
         }
-        if ((rc = cffc_mg_cycle(mg, nullptr))) break;
+        if (int e = mg_enqueue_cycle(mg)) return e;
         first_step = false;
         n++;
     }

@@ -94,6 +94,13 @@ __global__ void mg_changes_kernel(double *U, const double *uo, int64_t n) {
     if (k < n) U[k] -= uo[k];
 }
 
+// uo = U over every padded cell (Store_Current_Solution_in_uo_MG); a kernel
+// rather than a copy node, so that a captured cycle holds kernels only
+__global__ void mg_copy_kernel(double *dst, const double *src, int64_t n) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) dst[k] = src[k];
+}
+
 // Neumann conditions on the coarse solution change before bilinear
 // prolongation (:1640-1669): ghost = interior next to constant-extrapolation
 // boundaries; one thread per boundary cell of the interior span
@@ -259,6 +266,10 @@ extern "C" hipError_t cffc_launch_mg_cfl(const BlockMeta *mf, const BlockMeta *m
 }
 extern "C" hipError_t cffc_launch_mg_changes(double *U, const double *uo, int64_t n, hipStream_t st) {
     hipLaunchKernelGGL(mg_changes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, uo, n);
+    return hipGetLastError();
+}
+extern "C" hipError_t cffc_launch_mg_copy(double *dst, const double *src, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(mg_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
     return hipGetLastError();
 }
 extern "C" hipError_t cffc_launch_mg_prolong(const BlockMeta *mc, const BlockMeta *mf, int nblk, int max_int_c, int max_edge_c,

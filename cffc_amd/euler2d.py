@@ -89,6 +89,7 @@ def load():
     L.cffc_mg_cycle.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
     L.cffc_mg_execute.argtypes = [vp, i32, d, pd, pint, pint]
     L.cffc_mg_get_P.argtypes = [vp, i32, pd, ctypes.c_int64]
+    L.cffc_mg_set_graph.argtypes = [vp, i32]
     L.cffc_last_error.argtypes = []
     L.cffc_last_error.restype = ctypes.c_char_p
     _lib = L

@@ -324,6 +324,11 @@ class FASMultigrid:
         bad = ctypes.c_int64(0)
         return _check(_lib().cffc_mg_cycle(self.h, ctypes.byref(bad)), "cffc_mg_cycle")
 
+    def set_graph(self, on):
+        """execute replays a captured cycle (default) or issues the launches."""
+        from .euler2d import _check
+        _check(_lib().cffc_mg_set_graph(self.h, 1 if on else 0), "cffc_mg_set_graph")
+
     def get_P(self, level):
         from .euler2d import _check, _p
         out = np.empty((self.levels[level].ncell, 4))

@@ -365,6 +365,11 @@ int  cffc_mg_cycle(cffc_mg *mg, int64_t *bad_cell);
    on max_cycles or the absolute / relative tolerances of cffc_mg_desc */
 int  cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done,
                      int *checks);
+/* cffc_mg_execute replays one captured cycle as a HIP graph (on by
+   default; recaptured once when the limiters freeze); off: the launches
+   are issued one by one, with the same arithmetic.  No reference
+   counterpart: launch overhead. */
+int  cffc_mg_set_graph(cffc_mg *mg, int on);
 /* the forcing term P of a coarse level, interior cells in block order */
 int  cffc_mg_get_P(cffc_mg *mg, int level, double *P, int64_t ncell);
 

@@ -150,3 +150,28 @@ def test_multigrid_solution_is_the_fine_steady_state():
     print("single-grid residual of the multigrid state: %.3e" % sg[0, 1], flush=True)
     assert rc == 0 and sg[0, 1] < 1e-12
     f.close()
+
+
+def test_graph_replays_equal_eager_launches():
+    """Execute replays one captured cycle (recaptured when the limiters
+    freeze), with the norm kernels and host reads between the replays; the
+    same runs issued launch by launch give the same bits."""
+    m, W0 = steady.bump_case(0.08, 0.5, nbi=3, nbj=1, ni=8, nj=8)
+    for freeze in (0.0, 1e-4):
+        out = []
+        for graph in (True, False):
+            g = MG.FASMultigrid(m, _scheme(), MG.MGParams(levels=3))
+            g.set_graph(graph)
+            g.set_W(W0)
+            g.initialize()
+            rc, n, norms = g.execute(120, freeze_level=freeze)
+            out.append((rc, n, norms, g.get_W(), [g.levels[l].get_U() for l in range(3)]))
+            g.close()
+        (rc1, n1, nr1, W1, U1), (rc2, n2, nr2, W2, U2) = out
+        print("freeze %g: graph %d cycles, eager %d cycles; W %d non-bitwise" % (freeze, n1, n2, _nonbitwise(W1, W2)),
+              flush=True)
+        assert rc1 == rc2 == 0 and n1 == n2 == 120
+        np.testing.assert_array_equal(nr1, nr2)
+        np.testing.assert_array_equal(W1, W2)
+        for a, b in zip(U1, U2):
+            np.testing.assert_array_equal(a, b)
