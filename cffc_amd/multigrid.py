@@ -67,12 +67,13 @@ def _normal(xa, ya, xb, yb):
     """unit normal of the edge a -> b, (dy, -dx) / l (nfaceN/S/E/W,
     HO_Grid2DQuad.h:1972-2000: the right of the direction of travel)"""
     dx, dy = xb - xa, yb - ya
-    l = np.sqrt(dx * dx + dy * dy)
-    return dy / l, -dx / l
+    l = (dx * dx + dy * dy) ** 0.5
+    return (dy / l, -dx / l)
 
 
 class _Nodes:
-    """One block's node arrays X[j, i], Y[j, i] (ghosts included)."""
+    """One block's node arrays X[j, i], Y[j, i] (ghosts included), with
+    scalar (x, y) access."""
 
     def __init__(self, ni, nj):
         self.ni, self.nj = ni, nj
@@ -80,32 +81,53 @@ class _Nodes:
         self.Y = np.zeros_like(self.X)
 
     def get(self, i, j):
-        return np.array([self.X[j, i], self.Y[j, i]])
+        return (float(self.X[j, i]), float(self.Y[j, i]))
 
     def put(self, i, j, v):
         self.X[j, i], self.Y[j, i] = v[0], v[1]
 
     # outward face normals of cell (i, j) from its nodes
     def nW(self, i, j):
-        return np.array(_normal(self.X[j + 1, i], self.Y[j + 1, i], self.X[j, i], self.Y[j, i]))
+        return _normal(self.X[j + 1, i], self.Y[j + 1, i], self.X[j, i], self.Y[j, i])
 
     def nE(self, i, j):
-        return np.array(_normal(self.X[j, i + 1], self.Y[j, i + 1], self.X[j + 1, i + 1], self.Y[j + 1, i + 1]))
+        return _normal(self.X[j, i + 1], self.Y[j, i + 1], self.X[j + 1, i + 1], self.Y[j + 1, i + 1])
 
     def nS(self, i, j):
-        return np.array(_normal(self.X[j, i], self.Y[j, i], self.X[j, i + 1], self.Y[j, i + 1]))
+        return _normal(self.X[j, i], self.Y[j, i], self.X[j, i + 1], self.Y[j, i + 1])
 
     def nN(self, i, j):
-        return np.array(_normal(self.X[j + 1, i + 1], self.Y[j + 1, i + 1], self.X[j + 1, i], self.Y[j + 1, i]))
+        return _normal(self.X[j + 1, i + 1], self.Y[j + 1, i + 1], self.X[j + 1, i], self.Y[j + 1, i])
+
+
+def _avg(a, b, sign):
+    """sign * HALF * (a + b) for normals a, b"""
+    return (sign * 0.5 * (a[0] + b[0]), sign * 0.5 * (a[1] + b[1]))
+
+
+def _ext(base, inner):
+    """point reflection: base + (base - inner)"""
+    return (base[0] + (base[0] - inner[0]), base[1] + (base[1] - inner[1]))
 
 
 def _mirror(base, inner, n, sign):
     """base - X_norm + X_tan (sign -1) or base + X_norm - X_tan (sign +1) with
     d = inner - base (sign -1) or base - inner (sign +1)"""
-    d = inner - base if sign < 0 else base - inner
-    xn = np.dot(d, n) * n
-    xt = d - xn
-    return base - xn + xt if sign < 0 else base + xn - xt
+    if sign < 0:
+        d = (inner[0] - base[0], inner[1] - base[1])
+    else:
+        d = (base[0] - inner[0], base[1] - inner[1])
+    dn = d[0] * n[0] + d[1] * n[1]
+    xn = (dn * n[0], dn * n[1])
+    xt = (d[0] - xn[0], d[1] - xn[1])
+    if sign < 0:
+        return (base[0] - xn[0] + xt[0], base[1] - xn[1] + xt[1])
+    return (base[0] + xn[0] - xt[0], base[1] + xn[1] - xt[1])
+
+
+def _periodic(base, far, far_in):
+    """base + (far_in - far): the opposite edge's interior offset"""
+    return (base[0] + (far_in[0] - far[0]), base[1] + (far_in[1] - far[1]))
 
 
 def _exterior_nodes(N, bcW, bcE, bcS, bcN):
@@ -122,15 +144,14 @@ def _exterior_nodes(N, bcW, bcE, bcS, bcN):
             I0 = INl if side < 0 else INu
             for g in G:
                 if t == abi.BC_PERIODIC:
-                    # the opposite edge's interior offset
                     I1 = INu if side < 0 else INl
-                    N.put(I0 + side * g, j, N.get(I0, j) + (N.get(I1 + side * g, j) - N.get(I1, j)))
+                    N.put(I0 + side * g, j, _periodic(N.get(I0, j), N.get(I1, j), N.get(I1 + side * g, j)))
                 elif t in _WALLS and JNl < j < JNu:
-                    n = (-0.5 * (N.nW(ICl, j) + N.nW(ICl, j - 1)) if side < 0 else
-                         0.5 * (N.nE(ICu, j) + N.nE(ICu, j - 1)))
+                    n = (_avg(N.nW(ICl, j), N.nW(ICl, j - 1), -1) if side < 0 else
+                         _avg(N.nE(ICu, j), N.nE(ICu, j - 1), 1))
                     N.put(I0 + side * g, j, _mirror(N.get(I0, j), N.get(I0 - side * g, j), n, side))
                 else:
-                    N.put(I0 + side * g, j, N.get(I0, j) + (N.get(I0, j) - N.get(I0 - side * g, j)))
+                    N.put(I0 + side * g, j, _ext(N.get(I0, j), N.get(I0 - side * g, j)))
     for i in range(INl, INu + 1):
         for bc, side in ((bcS, -1), (bcN, 1)):
             t = bc[i]
@@ -140,13 +161,13 @@ def _exterior_nodes(N, bcW, bcE, bcS, bcN):
             for g in G:
                 if t == abi.BC_PERIODIC:
                     J1 = JNu if side < 0 else JNl
-                    N.put(i, J0 + side * g, N.get(i, J0) + (N.get(i, J1 + side * g) - N.get(i, J1)))
+                    N.put(i, J0 + side * g, _periodic(N.get(i, J0), N.get(i, J1), N.get(i, J1 + side * g)))
                 elif t in _WALLS and INl < i < INu:
-                    n = (-0.5 * (N.nS(i, JCl) + N.nS(i - 1, JCl)) if side < 0 else
-                         0.5 * (N.nN(i, JCu) + N.nN(i - 1, JCu)))
+                    n = (_avg(N.nS(i, JCl), N.nS(i - 1, JCl), -1) if side < 0 else
+                         _avg(N.nN(i, JCu), N.nN(i - 1, JCu), 1))
                     N.put(i, J0 + side * g, _mirror(N.get(i, J0), N.get(i, J0 - side * g), n, side))
                 else:
-                    N.put(i, J0 + side * g, N.get(i, J0) + (N.get(i, J0) - N.get(i, J0 - side * g)))
+                    N.put(i, J0 + side * g, _ext(N.get(i, J0), N.get(i, J0 - side * g)))
     # SW/SE then NW/NE corner rows: the W/E rule on the ghost rows, with the
     # BC type of the row (the cell below the node row for the north corners)
     for rows, below in ((range(JNl - NG, JNl + 1), 0), (range(JNu, JNu + NG + 1), 1)):
@@ -160,16 +181,16 @@ def _exterior_nodes(N, bcW, bcE, bcS, bcN):
                 for g in G:
                     if t == abi.BC_PERIODIC:
                         I1 = INu if side < 0 else INl
-                        N.put(I0 + side * g, j, N.get(I0, j) + (N.get(I1 + side * g, j) - N.get(I1, j)))
+                        N.put(I0 + side * g, j, _periodic(N.get(I0, j), N.get(I1, j), N.get(I1 + side * g, j)))
                     elif t in _WALLS:
                         if below == 0:
-                            nn = nf(ic, j) if j == JNl - NG else 0.5 * (nf(ic, j) + nf(ic, j - 1))
+                            nn = nf(ic, j) if j == JNl - NG else _avg(nf(ic, j), nf(ic, j - 1), 1)
                         else:
-                            nn = nf(ic, j - 1) if j == JNu + NG else 0.5 * (nf(ic, j) + nf(ic, j - 1))
-                        n = -nn if side < 0 else nn
+                            nn = nf(ic, j - 1) if j == JNu + NG else _avg(nf(ic, j), nf(ic, j - 1), 1)
+                        n = (-nn[0], -nn[1]) if side < 0 else nn
                         N.put(I0 + side * g, j, _mirror(N.get(I0, j), N.get(I0 - side * g, j), n, side))
                     else:
-                        N.put(I0 + side * g, j, N.get(I0, j) + (N.get(I0, j) - N.get(I0 - side * g, j)))
+                        N.put(I0 + side * g, j, _ext(N.get(I0, j), N.get(I0 - side * g, j)))
 
 
 _SLOT = {(0, 1): 0, (0, -1): 1, (1, 0): 2, (-1, 0): 3, (1, 1): 4, (-1, 1): 5, (1, -1): 6, (-1, -1): 7}
@@ -213,25 +234,28 @@ def coarsen(mesh):
     for b in range(nb):
         N = nodes[b]
         _exterior_nodes(N, *bcs[b][0])
-    # geometry messages: ghost nodes inside a neighbour's region
+    # geometry messages: ghost nodes inside a neighbour's region (faces over
+    # the interior span of node lines, corners 2 x 2)
     out = [(nodes[b].X.copy(), nodes[b].Y.copy()) for b in range(nb)]
     for b in range(nb):
         N = nodes[b]
         X, Y = out[b]
         INu, JNu = NG + N.ni, NG + N.nj
-        for j in range(N.nj + 2 * NG + 1):
-            for i in range(N.ni + 2 * NG + 1):
-                di = -1 if i < NG else (1 if i > INu else 0)
-                dj = -1 if j < NG else (1 if j > JNu else 0)
-                if di == 0 and dj == 0:
-                    continue
-                o = int(mesh.nbr[b, _SLOT[(di, dj)]])
-                if o < 0:
-                    continue
-                M = nodes[o]
-                si = i + M.ni if di < 0 else (i - N.ni if di > 0 else i)
-                sj = j + M.nj if dj < 0 else (j - N.nj if dj > 0 else j)
-                X[j, i], Y[j, i] = M.X[sj, si], M.Y[sj, si]
+        spans = {-1: slice(0, NG), 0: None, 1: slice(INu + 1, INu + NG + 1)}
+        jspans = {-1: slice(0, NG), 0: None, 1: slice(JNu + 1, JNu + NG + 1)}
+        for (di, dj), slot in _SLOT.items():
+            o = int(mesh.nbr[b, slot])
+            if o < 0:
+                continue
+            M = nodes[o]
+            si = spans[di] if di else slice(NG, INu + 1)
+            sj = jspans[dj] if dj else slice(NG, JNu + 1)
+            shift_i = M.ni if di < 0 else (-N.ni if di > 0 else 0)
+            shift_j = M.nj if dj < 0 else (-N.nj if dj > 0 else 0)
+            src_i = slice(si.start + shift_i, si.stop + shift_i)
+            src_j = slice(sj.start + shift_j, sj.stop + shift_j)
+            X[sj, si] = M.X[src_j, src_i]
+            Y[sj, si] = M.Y[src_j, src_i]
     xs = np.concatenate([out[b][0].ravel() for b in range(nb)])
     ys = np.concatenate([out[b][1].ravel() for b in range(nb)])
     bc = np.concatenate([np.concatenate(bcs[b][0]) for b in range(nb)]).astype(np.int32)

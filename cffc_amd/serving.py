@@ -78,6 +78,62 @@ def serve(cases, initial=None, tol=1e-10, max_steps=400000, freeze_level=steady.
                 **phases)
 
 
+def serve_multigrid(cases, initial=None, tol=1e-10, max_cycles=20000, freeze_level=steady.STEADY_FREEZE_LEVEL,
+                    levels=4, cfl=1.0, segment=500, log=None):
+    """Solve all cases in one batch with the FAS multigrid solver (one
+    mesh of disjoint parts; FP64).  Multigrid stops on the L2 norm of the
+    unscaled residual R over the whole batch, so every case finishes with
+    the batch.  The single-grid criterion (L2 of (CFL*dt)*R <= tol per case)
+    is converted with the largest per-case median of CFL*dt (the strictest
+    case), and so is the freeze level."""
+    from . import euler2d, multigrid
+    batch = []
+    for q, (h, m, k) in enumerate(cases):
+        mesh, W0 = case_mesh(h, m, k)
+        batch.append((mesh, W0 if initial is None else initial[q]))
+    mesh, starts = M.concat([b[0] for b in batch])
+    W0 = np.concatenate([b[1] for b in batch])
+    sg = steady.bump_scheme()
+    f = euler2d.Euler2DQuadBlocks(mesh, sg)
+    f.set_W(W0)
+    f.Send_All_Messages_and_BCs()
+    f.CFL(set_global=False)
+    dt = f.get_dt()
+    f.close()
+    offs = np.concatenate([[0], np.cumsum([b[0].ncell for b in batch])])
+    scale = max(float(np.median(sg.cfl_number * dt[offs[q]:offs[q + 1]])) for q in range(len(batch)))
+    t0 = time.perf_counter()
+    g = multigrid.FASMultigrid(mesh, steady.bump_scheme(cfl=cfl),
+                               multigrid.MGParams(levels=levels, absolute_tolerance=tol / scale))
+    g.set_W(W0)
+    g.initialize()
+    t_setup = time.perf_counter() - t0
+    # Execute in segments, one progress line each; a diverged batch stops
+    cycles, parts, rc = 0, [], 0
+    while cycles < max_cycles:
+        rc, n, nr = g.execute(min(segment, max_cycles - cycles), freeze_level=freeze_level / scale)
+        cycles += n
+        parts.append(nr[1:] if parts else nr)
+        last = nr[-1, 1] if len(nr) else float("nan")
+        if log:
+            log("  multigrid: %d cycles, R %.3e, %.1f s" % (cycles, last, time.perf_counter() - t0))
+        if rc != 0 or n < segment or not np.isfinite(last) or last > 1e12:
+            break
+    norms = np.concatenate(parts) if parts else np.zeros((0, 3))
+    wall = time.perf_counter() - t0
+    g.close()
+    ok = rc == 0 and len(norms) and norms[-1, 1] < tol / scale
+    if log:
+        log("multigrid: %d cycles, R %.3e (tol %.3e), %.2f s (set-up %.2f s)" %
+            (cycles, norms[-1, 1] if len(norms) else -1, tol / scale, wall, t_setup))
+    lat = [wall] * len(cases)
+    return dict(rc=rc, n=len(cases), converged=len(cases) if ok else 0, wall_seconds=wall,
+                solves_per_s=len(cases) / wall if wall > 0 else None, p50_s=wall if ok else None,
+                p99_s=wall if ok else None, cycles=cycles, setup_seconds=t_setup, levels=levels, cfl=cfl,
+                tol_R=tol / scale, final_R=float(norms[-1, 1]) if len(norms) else None,
+                latency_s=lat, cells=int(mesh.ncell), precision="fp64", solver="FAS multigrid, V cycle")
+
+
 def serve_cold_warm(net, norm, cases, **kw):
     cold = serve(cases, **kw)
     t0 = time.perf_counter()
@@ -101,10 +157,31 @@ def main(argv=None):
     ap.add_argument("--out", default=None)
     ap.add_argument("--only", choices=["both", "cold", "warm"], default="both",
                     help="run only the cold or only the warm batch (warm needs --model)")
+    ap.add_argument("--solver", choices=["single", "multigrid"], default="single",
+                    help="single-grid batched steady runs, or FAS multigrid over the batch")
+    ap.add_argument("--levels", type=int, default=4, help="multigrid levels")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     cases = serving_cases(a.n, a.seed)
     log("%d cases, %d cells" % (len(cases), sum(96 * k * 32 * k for _, _, k in cases)))
+    if a.solver == "multigrid":
+        out = {}
+        if a.only in ("both", "cold"):
+            out["cold"] = serve_multigrid(cases, levels=a.levels, log=log)
+        if a.only in ("both", "warm") and a.model:
+            net, norm = warmstart.load_model(a.model)
+            t0 = time.perf_counter()
+            init = warm_states(net, norm, cases)
+            t_inf = time.perf_counter() - t0
+            out["warm"] = serve_multigrid(cases, initial=init, levels=a.levels, log=log)
+            out["warm"]["inference_seconds"] = t_inf
+        for tag, r in out.items():
+            log("%s (multigrid): %d/%d converged, %.2f solves/s, wall %.1f s, %d cycles" %
+                (tag, r["converged"], r["n"], r["solves_per_s"] or 0, r["wall_seconds"], r["cycles"]))
+        if a.out:
+            with open(a.out, "w") as fo:
+                json.dump({"cases": cases, **out}, fo)
+        return
     kw = dict(mixed=not a.fp64, max_steps=a.max_steps, log=log)
     cold = warm = None
     if a.only == "warm" or (a.only == "both" and a.model):
