@@ -141,7 +141,7 @@ def steady_leg(ncases, seed=7):
             "all_converged": bool(np.all(cs > 0) and np.all(ws > 0))}
 
 
-def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None):
+def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=True):
     """The same held-out steady bump cases, cold, solved together (one mesh
     of disjoint parts) by the FAS multigrid solver (FASMultigrid2D: V cycle,
     HLLE first-order coarse levels, 5-stage optimal smoothing).  Multigrid
@@ -168,13 +168,14 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None):
     freeze = steady.STEADY_FREEZE_LEVEL / scale
     p = multigrid.MGParams(levels=levels, absolute_tolerance=tol)
     g = multigrid.FASMultigrid(m, steady.bump_scheme(cfl=cfl), p)
+    g.set_graph(graph)
     g.set_W(W0)
     g.initialize()
     t0 = time.perf_counter()
     rc, cycles, norms = g.execute(20000, freeze_level=freeze)
     dt = time.perf_counter() - t0
     g.close()
-    out = {"cases": ncases, "seed": seed, "levels": levels, "cycle": "V", "cfl": cfl,
+    out = {"cases": ncases, "seed": seed, "levels": levels, "cycle": "V", "cfl": cfl, "graph": graph,
            "mesh": "%d x (192x64 cells, 12x4 blocks of 16^2), coarsened to 2^2-cell blocks" % ncases,
            "scale_median_cfl_dt": scale, "tol_R": tol, "freeze_R": freeze, "rc": rc, "cycles": cycles,
            "final_R": float(norms[-1, 1]), "seconds": dt, "ms_per_cycle": 1e3 * dt / max(cycles, 1),
@@ -297,6 +298,8 @@ def main():
                     help="steps of the quadtree (resolution-change) workload (0: skip)")
     ap.add_argument("--only-amr", action="store_true", help="run only the quadtree workload and print it")
     ap.add_argument("--mg-cases", type=int, default=8, help="steady cases solved by FAS multigrid (0: skip)")
+    ap.add_argument("--mg-eager", action="store_true",
+                    help="multigrid cycles launched one by one instead of replayed as a graph (profiling)")
     ap.add_argument("--only-steady", action="store_true",
                     help="run only the steady legs (cold/warm single grid, multigrid) and print them")
     args = ap.parse_args()
@@ -312,7 +315,7 @@ def main():
         o = {"steady_warm_vs_cold": steady_leg(args.steady_cases)} if args.steady_cases > 0 else {}
         if args.mg_cases > 0:
             o["steady_multigrid"] = multigrid_leg(args.mg_cases, cold_batch_s=o.get("steady_warm_vs_cold", {}).get(
-                "cold_batch_s"))
+                "cold_batch_s"), graph=not args.mg_eager)
         print(json.dumps(o))
         return
     if args.gpus != 1:
@@ -397,7 +400,7 @@ def main():
     if args.mg_cases > 0:
         try:
             out["steady_multigrid"] = multigrid_leg(args.mg_cases, cold_batch_s=out.get(
-                "steady_warm_vs_cold", {}).get("cold_batch_s"))
+                "steady_warm_vs_cold", {}).get("cold_batch_s"), graph=not args.mg_eager)
         except Exception as e:  # reported, never required for the throughput line
             out["steady_multigrid"] = {"error": str(e)}
     if not args.no_cpu:
