@@ -167,14 +167,19 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=Tr
     tol = 1e-10 / scale
     freeze = steady.STEADY_FREEZE_LEVEL / scale
     p = multigrid.MGParams(levels=levels, absolute_tolerance=tol)
-    g = multigrid.FASMultigrid(m, steady.bump_scheme(cfl=cfl), p)
-    g.set_graph(graph)
-    g.set_W(W0)
-    g.initialize()
-    t0 = time.perf_counter()
-    rc, cycles, norms = g.execute(20000, freeze_level=freeze)
-    dt = time.perf_counter() - t0
-    g.close()
+
+    def solve(W):
+        g = multigrid.FASMultigrid(m, steady.bump_scheme(cfl=cfl), p)
+        g.set_graph(graph)
+        g.set_W(W)
+        g.initialize()
+        t0 = time.perf_counter()
+        rc, cycles, norms = g.execute(20000, freeze_level=freeze)
+        dt = time.perf_counter() - t0
+        g.close()
+        return rc, cycles, norms, dt
+
+    rc, cycles, norms, dt = solve(W0)
     out = {"cases": ncases, "seed": seed, "levels": levels, "cycle": "V", "cfl": cfl, "graph": graph,
            "mesh": "%d x (192x64 cells, 12x4 blocks of 16^2), coarsened to 2^2-cell blocks" % ncases,
            "scale_median_cfl_dt": scale, "tol_R": tol, "freeze_R": freeze, "rc": rc, "cycles": cycles,
@@ -183,6 +188,19 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=Tr
     if cold_batch_s:
         out["cold_single_grid_batch_s"] = cold_batch_s
         out["time_ratio_vs_cold_single_grid"] = cold_batch_s / dt
+    # the same from the U-Net's predicted states (warm multigrid)
+    model = os.path.join(ROOT, "cffc_amd", "data", "unet_bump.npz")
+    if os.path.exists(model):
+        from cffc_amd import warmstart
+        net, norm = warmstart.load_model(model)
+        t0 = time.perf_counter()
+        Ww = np.concatenate(warmstart.predict_states(net, norm, cases))
+        t_inf = time.perf_counter() - t0
+        rcw, cw, nw, dtw = solve(Ww)
+        out["warm"] = {"rc": rcw, "cycles": cw, "seconds": dtw + t_inf, "inference_s": t_inf,
+                       "final_R": float(nw[-1, 1]), "converged": bool(rcw == 0 and nw[-1, 1] < tol)}
+        if cold_batch_s:
+            out["warm"]["time_ratio_vs_cold_single_grid"] = cold_batch_s / (dtw + t_inf)
     return out
 
 
