@@ -1698,9 +1698,15 @@ extern "C" int cffc_mg_create(const cffc_mesh_desc *levels, int nlevels, const c
     for (int l = 0; l < nlevels; ++l) {
         if (levels[l].level || levels[l].nbr2) return fail(CFFC_E_ARG, "multigrid levels must be single-level meshes");
         if (levels[l].nblk != levels[0].nblk) return fail(CFFC_E_ARG, "level %d has a different block count", l);
-        for (int b = 0; b < levels[0].nblk && l > 0; ++b)
+        for (int b = 0; b < levels[0].nblk && l > 0; ++b) {
             if (levels[l - 1].ni[b] != 2 * levels[l].ni[b] || levels[l - 1].nj[b] != 2 * levels[l].nj[b])
                 return fail(CFFC_E_ARG, "block %d of level %d does not halve level %d", b, l, l - 1);
+            // Half_Mesh_Resolution (HO_Grid2DQuad.cc:12618-12621): a coarse
+            // block keeps at least Nghost cells a side
+            if (levels[l].ni[b] < levels[l].nghost || levels[l].nj[b] < levels[l].nghost)
+                return fail(CFFC_E_ARG, "block %d of level %d has fewer than %d cells a side", b, l,
+                            levels[l].nghost);
+        }
     }
     cffc_mg *mg = new cffc_mg();
     mg->d = *d;

@@ -204,6 +204,11 @@ def coarsen(mesh):
     nb = mesh.nblk
     if np.any(mesh.ni % 2) or np.any(mesh.nj % 2):
         raise ValueError("every block needs even ni and nj to coarsen (FASMultigrid2D.h:470-486)")
+    if np.any(mesh.ni < 2 * NG) or np.any(mesh.nj < 2 * NG):
+        # Half_Mesh_Resolution refuses blocks of fewer than 2*Nghost cells a
+        # side (HO_Grid2DQuad.cc:12618-12621): the coarse block must still
+        # fill its neighbours' two ghost layers
+        raise ValueError("blocks need at least %d cells a side to coarsen" % (2 * NG))
     cni, cnj = mesh.ni // 2, mesh.nj // 2
     noff = mesh.node_offsets()
     boff = mesh.bc_offsets()

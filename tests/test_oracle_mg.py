@@ -76,9 +76,12 @@ def test_coarsen_areas_and_interface_ghosts():
     np.testing.assert_array_equal(Y[2:n + 3, n + 3:n + 5], Ye[2:n + 3, 3:5])
 
 
-def test_coarsen_rejects_odd_blocks():
+def test_coarsen_rejects_odd_and_small_blocks():
     with pytest.raises(ValueError):
         MG.coarsen(M.cartesian(2, 2, 6, 5))
+    with pytest.raises(ValueError):
+        MG.coarsen(M.cartesian(2, 2, 2, 2))   # 1-cell coarse blocks
+    assert MG.coarsen(M.cartesian(2, 2, 4, 4)).ni[0] == 2
 
 
 def test_restriction_exact_for_linear_fields():
