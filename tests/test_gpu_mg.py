@@ -175,3 +175,15 @@ def test_graph_replays_equal_eager_launches():
         np.testing.assert_array_equal(W1, W2)
         for a, b in zip(U1, U2):
             np.testing.assert_array_equal(a, b)
+
+
+def test_serving_batch_with_multigrid_converges():
+    """serving.serve_multigrid on 4 cases of two mesh sizes (96 x 32 and
+    192 x 64 cells in blocks of 32^2): one multigrid batch, run in segments,
+    reaches every case's single-grid-equivalent tolerance."""
+    from cffc_amd import serving
+    cases = [(0.05, 0.4, 1), (0.08, 0.55, 2), (0.06, 0.35, 1), (0.09, 0.5, 2)]
+    r = serving.serve_multigrid(cases, levels=3, segment=200, log=lambda s: print(s, flush=True))
+    print("batch: %d cycles, %.2f s, R %.3e (tol %.3e)" % (r["cycles"], r["wall_seconds"], r["final_R"], r["tol_R"]),
+          flush=True)
+    assert r["rc"] == 0 and r["converged"] == 4 and r["final_R"] < r["tol_R"]
