@@ -111,11 +111,11 @@ def test_execute_bump_matches_oracle():
 
 def test_execute_bump_with_freezing_matches_oracle():
     """The same with the limiters frozen at 1e-4: same cycle count, state
-    within 1e-9 relative.  On three levels a frozen fine level takes the
-    device run off bit identity two cycles after the freeze, by 1e-15 on
-    the coarsest level first, and the difference grows to about 3e-11 by
-    convergence; two levels, frozen coarse levels only and unfrozen runs
-    stay bitwise (150 cycles).  The cause is not found."""
+    within 1e-9 relative.  On this bump with three levels the frozen run
+    leaves bit identity two cycles after the freeze (1e-15 on every level
+    at once, about 3e-11 at convergence), as injection prolongation and
+    second-order coarse levels do after a few cycles; the box case, two
+    levels and the unfrozen defaults stay bitwise.  The cause is not found."""
     m, W0 = steady.bump_case(0.08, 0.5, nbi=3, nbj=1, ni=8, nj=8)
     p = MG.MGParams(levels=3, absolute_tolerance=1e-9)
     g, o = _pair(m, W0, _scheme(), p)
