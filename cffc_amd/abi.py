@@ -59,7 +59,9 @@ class MGDesc(ctypes.Structure):
                 ("defect_correction", ctypes.c_int32), ("prolong_using_injection", ctypes.c_int32),
                 ("first_order_coarse", ctypes.c_int32), ("injection_at_dirichlet", ctypes.c_int32),
                 ("update_stability_switch", ctypes.c_int32), ("max_update_reductions", ctypes.c_int32),
-                ("absolute_tolerance", ctypes.c_double), ("relative_tolerance", ctypes.c_double)]
+                ("absolute_tolerance", ctypes.c_double), ("relative_tolerance", ctypes.c_double),
+                ("fmg_cycles", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("fmg_absolute_tolerance", ctypes.c_double), ("fmg_relative_tolerance", ctypes.c_double)]
 
 
 @dataclass

@@ -1640,11 +1640,12 @@ extern "C" int cffc_kernel_times(cffc_ctx *c, double out[4]) {
 // ---- FAS multigrid (FASMultigrid2D/FASMultigrid2D.h) -------------------------
 // One context per level on one stream, level 0 the finest; level l+1 halves
 // level l block by block.  The cycle is the reference's Coarse_Grid_Correction
-// with Top_Level = 0, every smoothing stage through the granular FP64 stage
-// kernel (HLLE and the zero limiter below the top level when
-// first_order_coarse is set), the forcing term folded into that kernel after
-// the face fluxes, and the level transfers of euler2d_mg.hip.  Per-cell dt on
-// every level (Smooth's CFL call does not apply the global minimum).
+// from a top level (0, or a coarser one during the full-multigrid start-up),
+// every smoothing stage through the granular FP64 stage kernel (HLLE and the
+// zero limiter below the top level when first_order_coarse is set), the
+// forcing term folded into that kernel after the face fluxes, and the level
+// transfers of euler2d_mg.hip.  Per-cell dt on every level (Smooth's CFL call
+// does not apply the global minimum).
 struct cffc_mg {
     std::vector<cffc_ctx *> lv;
     cffc_mg_desc d{};
@@ -1654,13 +1655,28 @@ struct cffc_mg {
     double rs_eps = 0.0;
     int rs_iters = 0;
     bool frozen = false;
+    // the smoother's scheme (top level) and the one below the top level
+    cffc_scheme_desc fine_s{}, coarse_s{};
     // Execute replays one captured cycle (kernels only); recaptured when the
-    // limiters freeze (the stage kernels' phi mode changes)
+    // limiters freeze (the stage kernels' phi mode changes) or the top level
+    // changes
     bool use_graph = true;
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
-    int graph_frozen = -1;
+    int graph_frozen = -1, graph_top = -1;
 };
+
+// the flux function and limiter of every level for a top level: the
+// smoother's on the top level and above, HLLE and the zero limiter below it
+// when first_order_coarse is set (dUdt_Multistage_Explicit_for_Multigrid
+// :2502-2509, Level > Top_Level)
+static void mg_set_top(cffc_mg *mg, int top) {
+    for (int l = 0; l < (int)mg->lv.size(); ++l) {
+        const cffc_scheme_desc &s = l > top ? mg->coarse_s : mg->fine_s;
+        mg->lv[l]->s.i_flux_function = s.i_flux_function;
+        mg->lv[l]->s.i_limiter = s.i_limiter;
+    }
+}
 
 static void mg_free(cffc_mg *mg) {
     if (!mg) return;
@@ -1710,6 +1726,12 @@ extern "C" int cffc_mg_create(const cffc_mesh_desc *levels, int nlevels, const c
     }
     cffc_mg *mg = new cffc_mg();
     mg->d = *d;
+    mg->fine_s = *smoother;
+    mg->coarse_s = *smoother;
+    if (d->first_order_coarse) {
+        mg->coarse_s.i_flux_function = CFFC_FLUX_FUNCTION_HLLE;
+        mg->coarse_s.i_limiter = CFFC_LIMITER_ZERO;
+    }
     mg->lv.assign(nlevels, nullptr);
     mg->P.assign(nlevels, nullptr);
     mg->uo.assign(nlevels, nullptr);
@@ -1724,11 +1746,6 @@ extern "C" int cffc_mg_create(const cffc_mesh_desc *levels, int nlevels, const c
             (void)hipStreamDestroy(c->stream);
             c->stream = mg->lv[0]->stream;
             c->own_stream = false;
-            if (d->first_order_coarse) {
-                // dUdt_Multistage_Explicit_for_Multigrid :2502-2509
-                c->s.i_flux_function = CFFC_FLUX_FUNCTION_HLLE;
-                c->s.i_limiter = CFFC_LIMITER_ZERO;
-            }
         }
         int e;
         if ((e = ensure_granular(c)) || (e = ensure_phi(c))) {
@@ -1746,6 +1763,7 @@ extern "C" int cffc_mg_create(const cffc_mesh_desc *levels, int nlevels, const c
         }
         for (int b = 0; b < c->nblk; ++b) mg->max_edge[l] = std::max(mg->max_edge[l], c->meta_h[b].ni + c->meta_h[b].nj);
     }
+    mg_set_top(mg, 0);
     if (hipStreamSynchronize(mg->lv[0]->stream) != hipSuccess) {
         mg_free(mg);
         return fail(CFFC_E_HIP, "multigrid set-up failed");
@@ -1792,20 +1810,20 @@ static int mg_stage(cffc_ctx *c, int i_stage, int residual, const double *P) {
     return 0;
 }
 
-// Smooth :2865-3110 (Top_Level = 0)
-static int mg_smooth(cffc_mg *mg, int l) {
+// Smooth :2865-3110
+static int mg_smooth(cffc_mg *mg, int top, int l) {
     cffc_ctx *c = mg->lv[l];
     HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, 0, 1, c->stream));
     HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
                            c->stream));
-    if (l > 0) {
+    if (l > top) {
         cffc_ctx *f = mg->lv[l - 1];
         HIPCHK(cffc_launch_mg_cfl(f->meta, c->meta, c->nblk, c->max_int, f->dt, area(f), c->dt, area(c), c->stream));
     }
     for (int st = 1; st <= c->s.n_stage; ++st) {
         if (int e = mg_exchange(c)) return e;
         if (st == 1) HIPCHK(cffc_launch_copy_interior(c->meta, c->nblk, c->max_int, c->U, c->Uo, c->ncp, c->stream));
-        if (int e = mg_stage(c, st, 0, l > 0 ? mg->P[l] : nullptr)) return e;
+        if (int e = mg_stage(c, st, 0, l > top ? mg->P[l] : nullptr)) return e;
         if (mg->rs_iters > 0)
             HIPCHK(cffc_launch_residual_smoothing(c->meta, c->nblk, c->dUdt[0], c->dUdt[1], c->ncp, mg->rs_eps,
                                                   mg->rs_iters, nullptr, nullptr, c->stream));
@@ -1816,7 +1834,7 @@ static int mg_smooth(cffc_mg *mg, int l) {
                                   c->stream));
     }
     if (int e = mg_exchange(c)) return e;
-    if (int e = mg_stage(c, 1, 1, l > 0 ? mg->P[l] : nullptr)) return e;
+    if (int e = mg_stage(c, 1, 1, l > top ? mg->P[l] : nullptr)) return e;
     if (!c->freeze) c->phi_valid = 1;
     return 0;
 }
@@ -1827,8 +1845,8 @@ static int mg_store_uo(cffc_mg *mg, int l) {
     return 0;
 }
 
-// Coarse_Grid_Correction :3118-3370 (Top_Level = 0)
-static int mg_cgc(cffc_mg *mg, int l) {
+// Coarse_Grid_Correction :3118-3370
+static int mg_cgc(cffc_mg *mg, int top, int l) {
     const cffc_mg_desc &d = mg->d;
     const int L = (int)mg->lv.size();
     cffc_ctx *c = mg->lv[l];
@@ -1836,15 +1854,17 @@ static int mg_cgc(cffc_mg *mg, int l) {
     if (l == L - 1) {
         if ((e = mg_store_uo(mg, l))) return e;
         for (int nu = 1; nu <= d.smooths_on_coarsest_level; ++nu)
-            if ((e = mg_smooth(mg, l))) return e;
+            if ((e = mg_smooth(mg, top, l))) return e;
     } else {
         cffc_ctx *cc = mg->lv[l + 1];
+        // the W cycle's two passes apply below the finest level, whatever
+        // the top level (:3151-3158)
         const int m_cycle = (d.i_cycle == CFFC_MULTIGRID_V_CYCLE || l == 0) ? 1 : 2;
         for (int m = 1; m <= m_cycle; ++m) {
             if ((e = mg_store_uo(mg, l))) return e;
-            const int n_pre = l == 0 ? d.smooths_on_finest_level : d.pre_smooths;
+            const int n_pre = l == top ? d.smooths_on_finest_level : d.pre_smooths;
             for (int nu = 0; nu < n_pre; ++nu)
-                if ((e = mg_smooth(mg, l))) return e;
+                if ((e = mg_smooth(mg, top, l))) return e;
             HIPCHK(cffc_launch_mg_restrict(c->meta, cc->meta, c->nblk, cc->max_int, c->U, area(c), c->ncp, cc->U,
                                            area(cc), cc->ncp, c->stream));
             HIPCHK(cffc_launch_mg_primitive(cc->meta, cc->nblk, cc->max_int, cc->U, cc->W[cc->wcur], cc->ncp,
@@ -1857,19 +1877,19 @@ static int mg_cgc(cffc_mg *mg, int l) {
                 HIPCHK(cffc_launch_mg_subtract(cc->meta, cc->nblk, cc->max_int, mg->P[l + 1], cc->dUdt[0], cc->ncp,
                                                c->stream));
             }
-            if ((e = mg_cgc(mg, l + 1))) return e;
+            if ((e = mg_cgc(mg, top, l + 1))) return e;
             HIPCHK(cffc_launch_mg_prolong(cc->meta, c->meta, c->nblk, cc->max_int, mg->max_edge[l + 1], cc->bc, cc->U,
                                           cc->geom, cc->ncp, c->U, c->geom, c->ncp, !d.prolong_using_injection,
                                           d.injection_at_dirichlet, d.prolong_using_injection,
-                                          d.update_stability_switch, d.max_update_reductions, c->stream));
+                                          d.update_stability_switch, d.max_update_reductions, 0, c->stream));
             HIPCHK(cffc_launch_mg_primitive(c->meta, c->nblk, c->max_int, c->U, c->W[c->wcur], c->ncp, c->stream));
             if ((e = mg_exchange(c))) return e;
-            const int n_post = l == 0 ? 0 : d.post_smooths;
+            const int n_post = l == top ? 0 : d.post_smooths;
             for (int nu = 0; nu < n_post; ++nu)
-                if ((e = mg_smooth(mg, l))) return e;
+                if ((e = mg_smooth(mg, top, l))) return e;
         }
     }
-    if (l != 0) HIPCHK(cffc_launch_mg_changes(c->U, mg->uo[l], 4 * c->ncp, c->stream));
+    if (l != top) HIPCHK(cffc_launch_mg_changes(c->U, mg->uo[l], 4 * c->ncp, c->stream));
     return 0;
 }
 
@@ -1960,8 +1980,9 @@ static int mg_check_errors(cffc_mg *mg, int64_t *bad_cell) {
 // one FAS cycle from level 0 (Coarse_Grid_Correction(0, 0)), synchronous
 extern "C" int cffc_mg_cycle(cffc_mg *mg, int64_t *bad_cell) {
     if (!mg) return fail(CFFC_E_ARG, "null multigrid solver");
+    mg_set_top(mg, 0);
     if (int e = mg_reset_errors(mg)) return e;
-    if (int e = mg_cgc(mg, 0)) return e;
+    if (int e = mg_cgc(mg, 0, 0)) return e;
     return mg_check_errors(mg, bad_cell);
 }
 
@@ -1971,27 +1992,29 @@ extern "C" int cffc_mg_set_graph(cffc_mg *mg, int on) {
     return 0;
 }
 
-// enqueue one cycle: a replay of the captured cycle, captured first (and
-// again after the limiters freeze), or the launches themselves
-static int mg_enqueue_cycle(cffc_mg *mg) {
+// enqueue one cycle from a top level: a replay of the captured cycle,
+// captured first (and again after the limiters freeze or the top level
+// changes), or the launches themselves
+static int mg_enqueue_cycle(cffc_mg *mg, int top) {
     cffc_ctx *f = mg->lv[0];
     if (!mg->use_graph) {
         if (int e = mg_reset_errors(mg)) return e;
-        return mg_cgc(mg, 0);
+        return mg_cgc(mg, top, top);
     }
-    if (!mg->gexec || mg->graph_frozen != (int)mg->frozen) {
+    if (!mg->gexec || mg->graph_frozen != (int)mg->frozen || mg->graph_top != top) {
         if (mg->gexec) (void)hipGraphExecDestroy(mg->gexec);
         if (mg->graph) (void)hipGraphDestroy(mg->graph);
         mg->gexec = nullptr;
         mg->graph = nullptr;
         HIPCHK(hipStreamBeginCapture(f->stream, hipStreamCaptureModeThreadLocal));
         int e = mg_reset_errors(mg);
-        if (!e) e = mg_cgc(mg, 0);
+        if (!e) e = mg_cgc(mg, top, top);
         hipError_t ce = hipStreamEndCapture(f->stream, &mg->graph);
         if (e) return e;
         if (ce != hipSuccess) return fail(CFFC_E_HIP, "multigrid cycle capture failed: %s", hipGetErrorString(ce));
         HIPCHK(hipGraphInstantiate(&mg->gexec, mg->graph, nullptr, nullptr, 0));
         mg->graph_frozen = (int)mg->frozen;
+        mg->graph_top = top;
     }
     HIPCHK(hipGraphLaunch(mg->gexec, f->stream));
     return 0;
@@ -2002,52 +2025,80 @@ extern "C" int cffc_mg_get_P(cffc_mg *mg, int level, double *P, int64_t ncell) {
     return get_field(mg->lv[level], mg->P[level], P, ncell, 4);
 }
 
-// Execute's regular-cycle loop :3440-3705: the level-0 residual norms (of the
-// dUdt[0] the last smoothing left) before each cycle, the relative norm
-// against the second check's, limiter freezing at freeze_level (<= 0: off),
-// and the absolute / relative / cycle-count exits.  norms: 3 per check.
+// Execute :3375-3745: with fmg_cycles > 0 the full-multigrid start-up first
+// (top level Levels-2 up to 1, fmg_cycles cycles each, stopping early on the
+// FMG tolerances, then Prolong_Solution_Blocks to the next level with
+// messages and BCs), then the regular cycles from level 0.  Before each
+// cycle: the residual norms of the top level (the dUdt[0] the last
+// smoothing left) and every level's error key, read with one
+// synchronisation; the relative norm against the level's second check;
+// limiter freezing at freeze_level (<= 0: never; unfrozen again at a new
+// top level, :3462-3476); and the exits: max_cycles cycles in all, or the
+// absolute / relative tolerance (the FMG ones above level 0).  A spent
+// budget still prolongs through the remaining start-up levels, each
+// stopping at its first check.  norms: 3 per check.
 extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done,
                                int *checks) {
     if (!mg || max_cycles < 0) return fail(CFFC_E_ARG, "bad argument");
-    cffc_ctx *f = mg->lv[0];
-    double init_l2 = -1.0, ratio = -1.0;
+    const int L = (int)mg->lv.size();
     bool first_step = true;
     int n = 0, nchk = 0, rc = 0;
     std::vector<unsigned long long> keys(mg->lv.size());
     if (int e = mg_reset_errors(mg)) return e;
-    for (int cycles = 1;; ++cycles) {
-        // the norms of the last cycle's level-0 residual and every level's
-        // error key, read with one synchronisation
-        double nr[3];
-        HIPCHK(cffc_launch_norms(f->meta, f->tiles, f->ntiles, f->ti, f->tj, f->dUdt[0], f->ncp, 1, f->norm_part,
-                                 f->norm_small, f->stream));
-        HIPCHK(hipMemcpyAsync(nr, f->norm_small, 3 * sizeof(double), hipMemcpyDeviceToHost, f->stream));
-        for (size_t l = 0; l < mg->lv.size(); ++l)
-            HIPCHK(hipMemcpyAsync(&keys[l], mg->lv[l]->scal + 2, sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                                  f->stream));
-        HIPCHK(hipStreamSynchronize(f->stream));
-        for (size_t l = 0; l < keys.size() && !rc; ++l)
-            if (keys[l] != ERR_NONE) rc = (int)(keys[l] & 0xFFFF);
-        if (rc) break;
-        if (norms) memcpy(norms + 3 * nchk, nr, sizeof nr);
-        nchk++;
-        if (cycles == 2) init_l2 = nr[1];
-        else if (cycles > 2 && fabs(init_l2) > 1e-9) ratio = nr[1] / init_l2;
-        bool stop = n >= max_cycles;
-        if (cycles > 2 && (nr[1] < mg->d.absolute_tolerance || ratio < mg->d.relative_tolerance)) stop = true;
-        if (stop) break;
-        if (!first_step && cycles > 1 && freeze_level > 0 && !mg->frozen && f->s.i_limiter != CFFC_LIMITER_ZERO &&
-            nr[1] <= freeze_level) {
-            for (cffc_ctx *c : mg->lv) c->freeze = 1;
-            mg->frozen = true;
+    const int top0 = mg->d.fmg_cycles > 0 ? L - 2 : 0;
+    for (int top = top0; top >= 0 && !rc; --top) {
+        mg_set_top(mg, top);
+        cffc_ctx *t = mg->lv[top];
+        if (!first_step && freeze_level > 0 && mg->frozen && mg->fine_s.i_limiter != CFFC_LIMITER_ZERO) {
+            for (int l = top; l < L; ++l) mg->lv[l]->freeze = 0;
+            mg->frozen = false;
         }
-        if (int e = mg_enqueue_cycle(mg)) return e;
-        first_step = false;
-        n++;
+        const long long ncyc = top != 0 ? mg->d.fmg_cycles : 0x7fffffffffffffffLL;
+        double init_l2 = -1.0, ratio = -1.0;
+        for (long long cycles = 1; cycles <= ncyc; ++cycles) {
+            double nr[3];
+            HIPCHK(cffc_launch_norms(t->meta, t->tiles, t->ntiles, t->ti, t->tj, t->dUdt[0], t->ncp, 1,
+                                     t->norm_part, t->norm_small, t->stream));
+            HIPCHK(hipMemcpyAsync(nr, t->norm_small, 3 * sizeof(double), hipMemcpyDeviceToHost, t->stream));
+            for (size_t l = 0; l < mg->lv.size(); ++l)
+                HIPCHK(hipMemcpyAsync(&keys[l], mg->lv[l]->scal + 2, sizeof(unsigned long long),
+                                      hipMemcpyDeviceToHost, t->stream));
+            HIPCHK(hipStreamSynchronize(t->stream));
+            for (size_t l = 0; l < keys.size() && !rc; ++l)
+                if (keys[l] != ERR_NONE) rc = (int)(keys[l] & 0xFFFF);
+            if (rc) break;
+            if (norms) memcpy(norms + 3 * nchk, nr, sizeof nr);
+            nchk++;
+            if (cycles == 2) init_l2 = nr[1];
+            else if (cycles > 2 && fabs(init_l2) > 1e-9) ratio = nr[1] / init_l2;
+            bool stop = n >= max_cycles;
+            if (cycles > 2) {
+                const double at = top == 0 ? mg->d.absolute_tolerance : mg->d.fmg_absolute_tolerance;
+                const double rt = top == 0 ? mg->d.relative_tolerance : mg->d.fmg_relative_tolerance;
+                if (nr[1] < at || ratio < rt) stop = true;
+            }
+            if (stop) break;
+            if (!first_step && cycles > 1 && freeze_level > 0 && !mg->frozen &&
+                mg->fine_s.i_limiter != CFFC_LIMITER_ZERO && nr[1] <= freeze_level) {
+                for (int l = top; l < L; ++l) mg->lv[l]->freeze = 1;
+                mg->frozen = true;
+            }
+            if (int e = mg_enqueue_cycle(mg, top)) return e;
+            first_step = false;
+            n++;
+        }
+        if (top != 0 && !rc) {
+            cffc_ctx *f = mg->lv[top - 1];
+            HIPCHK(cffc_launch_mg_prolong(t->meta, f->meta, t->nblk, t->max_int, mg->max_edge[top], t->bc, t->U,
+                                          t->geom, t->ncp, f->U, f->geom, f->ncp, 0, 0, mg->d.prolong_using_injection,
+                                          0, 0, 1, t->stream));
+            if (int e = mg_exchange(f)) return e;
+        }
     }
+    mg_set_top(mg, 0);
     if (cycles_done) *cycles_done = n;
     if (checks) *checks = nchk;
-    if (int e = mg_exchange(f)) return e;
-    HIPCHK(hipStreamSynchronize(f->stream));
+    if (int e = mg_exchange(mg->lv[0])) return e;
+    HIPCHK(hipStreamSynchronize(mg->lv[0]->stream));
     return rc;
 }

@@ -121,6 +121,6 @@ hipError_t cffc_launch_mg_copy(double *dst, const double *src, int64_t n, hipStr
 hipError_t cffc_launch_mg_prolong(const cffc::BlockMeta *mc, const cffc::BlockMeta *mf, int nblk, int max_int_c,
                                   int max_edge_c, const int32_t *bc, double *Uc, const double *gc, int64_t ncpc,
                                   double *Uf, const double *gf, int64_t ncpf, int neumann, int inj_dirichlet,
-                                  int inj_always, int stability, int max_red, hipStream_t st);
+                                  int inj_always, int stability, int max_red, int assign, hipStream_t st);
 hipError_t cffc_launch_stage_mixed(const cffc::StageParams *p, int ti, int tj, int fused, hipStream_t st);
 }

@@ -171,9 +171,13 @@ __constant__ int8_t kQuad[4][4][2] = {
 // the coarse cell with injection from the SW fine cell (cells already
 // updated are updated again, as in the reference).  Updates that leave the
 // state unphysical are halved up to max_red times, then dropped.
+//
+// assign (Prolong_Solution_Blocks :1200-1625, full multigrid): Uc is the
+// coarse solution; the fine cells take it (same quadrilateral order and
+// injection fall-back), with neither Dirichlet injection nor reduction.
 __global__ void mg_prolong_kernel(const BlockMeta *mc, const BlockMeta *mf, const int32_t *bc, const double *Uc,
                                   const double *gc, int64_t ncpc, double *Uf, const double *gf, int64_t ncpf,
-                                  int inj_dirichlet, int inj_always, int stability, int max_red) {
+                                  int inj_dirichlet, int inj_always, int stability, int max_red, int assign) {
     const BlockMeta c = mc[blockIdx.y], f = mf[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= c.ni * c.nj) return;
@@ -184,7 +188,7 @@ __global__ void mg_prolong_kernel(const BlockMeta *mc, const BlockMeta *mf, cons
     const int fdi[4] = {0, 1, 1, 0}, fdj[4] = {0, 0, 1, 1};
     const bool dirichlet = (ic == 2 && bW[jc] == 11000) || (ic == c.ni + 1 && bE[jc] == 11000) ||
                            (jc == 2 && bS[ic] == 11000) || (jc == c.nj + 1 && bN[ic] == 11000);
-    if (inj_dirichlet && dirichlet) {
+    if (!assign && inj_dirichlet && dirichlet) {
         for (int k = 0; k < 4; ++k) {
             const int64_t fc = cell(f, i0 + fdi[k], j0 + fdj[k]);
             CS<double> u = ldU(Uf, ncpf, fc);
@@ -214,6 +218,10 @@ __global__ void mg_prolong_kernel(const BlockMeta *mc, const BlockMeta *mf, cons
             }
         } else {
             dU = Ucc;
+        }
+        if (assign) {
+            stU(Uf, ncpf, fc, dU);
+            continue;
         }
         CS<double> u = ldU(Uf, ncpf, fc);
         if (stability) {
@@ -275,10 +283,10 @@ extern "C" hipError_t cffc_launch_mg_copy(double *dst, const double *src, int64_
 extern "C" hipError_t cffc_launch_mg_prolong(const BlockMeta *mc, const BlockMeta *mf, int nblk, int max_int_c, int max_edge_c,
                                   const int32_t *bc, double *Uc, const double *gc, int64_t ncpc, double *Uf,
                                   const double *gf, int64_t ncpf, int neumann, int inj_dirichlet, int inj_always,
-                                  int stability, int max_red, hipStream_t st) {
+                                  int stability, int max_red, int assign, hipStream_t st) {
     if (neumann)
         hipLaunchKernelGGL(mg_neumann_kernel, dim3((max_edge_c + 255) / 256, nblk), dim3(256), 0, st, mc, bc, Uc, ncpc);
     hipLaunchKernelGGL(mg_prolong_kernel, dim3((max_int_c + 255) / 256, nblk), dim3(256), 0, st, mc, mf, bc, Uc, gc,
-                       ncpc, Uf, gf, ncpf, inj_dirichlet, inj_always, stability, max_red);
+                       ncpc, Uf, gf, ncpf, inj_dirichlet, inj_always, stability, max_red, assign);
     return hipGetLastError();
 }

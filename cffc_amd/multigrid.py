@@ -51,13 +51,17 @@ class MGParams:
     max_update_reductions: int = 5
     absolute_tolerance: float = -1.0
     relative_tolerance: float = -1.0
+    fmg_cycles: int = 0
+    fmg_absolute_tolerance: float = -1.0
+    fmg_relative_tolerance: float = -1.0
 
 
 def mg_desc(p):
     return abi.MGDesc(p.levels, p.i_cycle, p.smooths_on_finest_level, p.pre_smooths, p.post_smooths,
                       p.smooths_on_coarsest_level, p.defect_correction, p.prolong_using_injection,
                       p.first_order_coarse, p.injection_at_dirichlet, p.update_stability_switch,
-                      p.max_update_reductions, p.absolute_tolerance, p.relative_tolerance)
+                      p.max_update_reductions, p.absolute_tolerance, p.relative_tolerance,
+                      p.fmg_cycles, 0, p.fmg_absolute_tolerance, p.fmg_relative_tolerance)
 
 
 _WALLS = (abi.BC_REFLECTION,)
@@ -365,9 +369,11 @@ class FASMultigrid:
         return out
 
     def execute(self, max_cycles, freeze_level=0.0):
-        """Execute's regular-cycle loop; returns (rc, cycles, norms[checks, 3])."""
+        """Execute (full-multigrid start-up when fmg_cycles > 0, then the
+        regular cycles); returns (rc, cycles, norms[checks, 3]), the norms
+        of each check on that check's top level."""
         from .euler2d import _check, _p
-        norms = np.zeros((max_cycles + 1, 3))
+        norms = np.zeros((max_cycles + len(self.levels), 3))
         done, chk = ctypes.c_int(0), ctypes.c_int(0)
         rc = _check(_lib().cffc_mg_execute(self.h, max_cycles, freeze_level, _p(norms), ctypes.byref(done),
                                            ctypes.byref(chk)), "cffc_mg_execute")

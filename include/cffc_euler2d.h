@@ -138,8 +138,7 @@ typedef struct cffc_scheme_desc {
 /* The FAS multigrid parameters of Multigrid_Input_Parameters
    (FASMultigrid2DInput.h:93-114; the reference's defaults in brackets).
    Coarse-mesh boundary conditions are always applied
-   (Apply_Coarse_Mesh_Boundary_Conditions = ON, the default); full
-   multigrid start-up cycles are not offered. */
+   (Apply_Coarse_Mesh_Boundary_Conditions = ON, the default). */
 typedef struct cffc_mg_desc {
     int32_t levels;                      /* Levels, >= 2                     [2] */
     int32_t i_cycle;                     /* CFFC_MULTIGRID_V/W_CYCLE         [V] */
@@ -156,6 +155,10 @@ typedef struct cffc_mg_desc {
     int32_t max_update_reductions;       /* Maximum_Number_of_Update_Reductions [5] */
     double  absolute_tolerance;          /* Absolute_Convergence_Tolerance  [-1] */
     double  relative_tolerance;          /* Relative_Convergence_Tolerance  [-1] */
+    int32_t fmg_cycles;                  /* Number_of_Cycles_per_Stage_for_Full_Multigrid [0] */
+    int32_t pad;
+    double  fmg_absolute_tolerance;      /* FMG_Absolute_Convergence_Tolerance [-1] */
+    double  fmg_relative_tolerance;      /* FMG_Relative_Convergence_Tolerance [-1] */
 } cffc_mg_desc;
 
 typedef struct cffc_ctx cffc_ctx;
@@ -359,10 +362,14 @@ int  cffc_mg_initialize(cffc_mg *mg);
    unphysical update's i index, *bad_cell its level << 56 | block << 32 |
    j << 16 | i (may be NULL) */
 int  cffc_mg_cycle(cffc_mg *mg, int64_t *bad_cell);
-/* Execute's regular-cycle loop :3440-3705: norms (L1, L2, max of the
-   level-0 density residual; 3 per check, room for max_cycles + 1 checks)
-   before each cycle, limiter freezing at freeze_level (<= 0: never), exits
-   on max_cycles or the absolute / relative tolerances of cffc_mg_desc */
+/* Execute :3375-3745: with fmg_cycles > 0 the full-multigrid start-up
+   (top level levels-2 up to 1, at most fmg_cycles cycles each or to the
+   fmg tolerances, then the solution prolonged to the next level,
+   Prolong_Solution_Blocks :1200), then the regular cycles from level 0.
+   norms: L1, L2, max of the top level's density residual before each
+   cycle, 3 per check, room for max_cycles + levels checks; limiter
+   freezing at freeze_level (<= 0: never; unfrozen at each new top level);
+   exits on max_cycles cycles in all or the tolerances of cffc_mg_desc */
 int  cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done,
                      int *checks);
 /* cffc_mg_execute replays one captured cycle as a HIP graph (on by

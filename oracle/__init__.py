@@ -375,8 +375,9 @@ class OracleMG:
         return out
 
     def execute(self, max_cycles, freeze_level=0.0):
-        """Execute's regular-cycle loop; returns (rc, cycles, norms[checks, 3])."""
-        norms = np.zeros((max_cycles + 1, 3))
+        """Execute (full-multigrid start-up when fmg_cycles > 0, then the
+        regular cycles); returns (rc, cycles, norms[checks, 3])."""
+        norms = np.zeros((max_cycles + len(self.levels), 3))
         done, chk = ctypes.c_int(0), ctypes.c_int(0)
         rc = lib().orc_mg_execute(self.h, max_cycles, freeze_level, _p(norms), ctypes.byref(done), ctypes.byref(chk))
         return rc, done.value, norms[:chk.value]

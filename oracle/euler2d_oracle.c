@@ -1910,7 +1910,15 @@ struct orc_mg {
     cffc_mg_desc d;
     int rs_iters;
     double rs_eps;
+    cffc_scheme_desc fine_s, coarse_s;   /* top-level and below-top schemes */
 };
+
+/* the scheme of every level for a top level: the smoother's on the top
+   level, HLLE and the zero limiter below it when first_order_coarse is set
+   (dUdt_Multistage_Explicit_for_Multigrid :2502-2509, Level > Top_Level) */
+static void mg_set_top(orc_mg *mg, int top) {
+    for (int l = 0; l < mg->L; ++l) mg->lv[l]->s = (l > top) ? mg->coarse_s : mg->fine_s;
+}
 
 #define FINE_IDX(Ng, ic) (2 * ((ic) - (Ng)) + (Ng))
 
@@ -2169,16 +2177,16 @@ static void mg_residual(orc_ctx *C, int add_P) {
     }
 }
 
-/* Smooth :2865-3110 (Top_Level = 0) */
-static int mg_smooth(orc_mg *mg, int l) {
+/* Smooth :2865-3110 */
+static int mg_smooth(orc_mg *mg, int top, int l) {
     orc_ctx *C = mg->lv[l];
     double dtmin;
     orc_cfl(C, &dtmin, 0);
-    if (l != 0) mg_cfl_multigrid(mg->lv[l - 1], C);
+    if (l != top) mg_cfl_multigrid(mg->lv[l - 1], C);
     for (int st = 1; st <= C->s.n_stage; ++st) {
         orc_exchange_and_bcs(C);
         orc_dudt_stage(C, st);
-        if (l > 0) {
+        if (l > top) {
             /* Apply_the_FAS_Multigrid_Forcing_Term :2678-2745 */
             const int k = stage_k_residual(&C->s, st);
             for (int b = 0; b < C->nblk; ++b) {
@@ -2201,25 +2209,27 @@ static int mg_smooth(orc_mg *mg, int l) {
         if (e) return e;
     }
     orc_exchange_and_bcs(C);
-    mg_residual(C, l > 0);
+    mg_residual(C, l > top);
     return 0;
 }
 
-/* Coarse_Grid_Correction :3118-3370 (Top_Level = 0) */
-static int mg_cgc(orc_mg *mg, int l) {
+/* Coarse_Grid_Correction :3118-3370 */
+static int mg_cgc(orc_mg *mg, int top, int l) {
     const cffc_mg_desc *d = &mg->d;
     int e;
     if (l == mg->L - 1) {
         mg_store_uo(mg->lv[l]);
         for (int nu = 1; nu <= d->smooths_on_coarsest_level; ++nu)
-            if ((e = mg_smooth(mg, l))) return e;
+            if ((e = mg_smooth(mg, top, l))) return e;
     } else {
+        /* the W cycle's two passes apply below the finest level, whatever
+           the top level (:3151-3158) */
         const int m_cycle = (d->i_cycle == CFFC_MULTIGRID_V_CYCLE || l == 0) ? 1 : 2;
         for (int m = 1; m <= m_cycle; ++m) {
             mg_store_uo(mg->lv[l]);
-            const int n_pre = (l == 0) ? d->smooths_on_finest_level : d->pre_smooths;
+            const int n_pre = (l == top) ? d->smooths_on_finest_level : d->pre_smooths;
             for (int nu = 0; nu < n_pre; ++nu)
-                if ((e = mg_smooth(mg, l))) return e;
+                if ((e = mg_smooth(mg, top, l))) return e;
             mg_restrict(mg->lv[l], mg->lv[l + 1], 0);
             mg_update_primitive(mg->lv[l + 1]);
             orc_exchange_and_bcs(mg->lv[l + 1]);
@@ -2228,17 +2238,56 @@ static int mg_cgc(orc_mg *mg, int l) {
                 mg_residual(mg->lv[l + 1], 0);
                 mg_subtract_dudt(mg->lv[l + 1]);
             }
-            if ((e = mg_cgc(mg, l + 1))) return e;
+            if ((e = mg_cgc(mg, top, l + 1))) return e;
             mg_prolong_update(mg->lv[l + 1], mg->lv[l], d);
             mg_update_primitive(mg->lv[l]);
             orc_exchange_and_bcs(mg->lv[l]);
-            const int n_post = (l == 0) ? 0 : d->post_smooths;
+            const int n_post = (l == top) ? 0 : d->post_smooths;
             for (int nu = 0; nu < n_post; ++nu)
-                if ((e = mg_smooth(mg, l))) return e;
+                if ((e = mg_smooth(mg, top, l))) return e;
         }
     }
-    if (l != 0) mg_solution_changes(mg->lv[l]);
+    if (l != top) mg_solution_changes(mg->lv[l]);
     return 0;
+}
+
+/* Prolong_Solution_Blocks :1200-1625 (full multigrid): the fine cells take
+   the coarse solution, bilinear in the same quadrilateral order as the
+   correction (injection when none holds the cell, or always), assigned
+   rather than added; no Dirichlet injection and no update reduction.  The
+   fine W is left as it was (Execute follows with messages and BCs only,
+   :3716-3728). */
+static void mg_prolong_solution(orc_ctx *C, orc_ctx *F, const cffc_mg_desc *d) {
+    for (int b = 0; b < C->nblk; ++b) {
+        oblk *c = &C->b[b], *f = &F->b[b];
+        const int Ng = c->ng;
+        for (int ic = c->ICl; ic <= c->ICu; ++ic)
+            for (int jc = c->JCl; jc <= c->JCu; ++jc) {
+                const int i0 = FINE_IDX(Ng, ic), j0 = FINE_IDX(Ng, jc);
+                const cst Uc = c->U[CI(c, ic, jc)];
+                int inject = d->prolong_using_injection;
+                for (int q = 0; q < 4; ++q) {
+                    const int fi = i0 + MG_FINE[q][0], fj = j0 + MG_FINE[q][1];
+                    cst Uf = Uc;
+                    if (!inject) {
+                        int err = 1;
+                        for (int t = 0; t < 4 && err; ++t) {
+                            const int a = ic + MG_QUAD[q][t][0], e = jc + MG_QUAD[q][t][1];
+                            const int p1 = CI(c, a, e), p2 = CI(c, a, e + 1), p3 = CI(c, a + 1, e + 1), p4 = CI(c, a + 1, e);
+                            err = bilinear(&c->U[p1], c->xc[p1], c->yc[p1], &c->U[p2], c->xc[p2], c->yc[p2], &c->U[p3],
+                                           c->xc[p3], c->yc[p3], &c->U[p4], c->xc[p4], c->yc[p4], f->xc[CI(f, fi, fj)],
+                                           f->yc[CI(f, fi, fj)], &Uf);
+                        }
+                        if (err) {
+                            inject = 1;
+                            q = -1;
+                            continue;
+                        }
+                    }
+                    f->U[CI(f, fi, fj)] = Uf;
+                }
+            }
+    }
 }
 
 int orc_mg_create(const cffc_mesh_desc *meshes, int nlevels, const cffc_scheme_desc *smoother,
@@ -2247,6 +2296,12 @@ int orc_mg_create(const cffc_mesh_desc *meshes, int nlevels, const cffc_scheme_d
     orc_mg *mg = (orc_mg *)calloc(1, sizeof(orc_mg));
     mg->L = nlevels;
     mg->d = *d;
+    mg->fine_s = *smoother;
+    mg->coarse_s = *smoother;
+    if (d->first_order_coarse) {
+        mg->coarse_s.i_flux_function = CFFC_FLUX_FUNCTION_HLLE;
+        mg->coarse_s.i_limiter = CFFC_LIMITER_ZERO;
+    }
     mg->lv = (orc_ctx **)calloc(nlevels, sizeof(orc_ctx *));
     for (int l = 0; l < nlevels; ++l) {
         cffc_scheme_desc s = *smoother;
@@ -2296,37 +2351,63 @@ int orc_mg_initialize(orc_mg *mg) {
     return 0;
 }
 /* one FAS cycle from level 0 */
-int orc_mg_cycle(orc_mg *mg) { return mg_cgc(mg, 0); }
-/* Execute's regular-cycle loop :3440-3705: the residual norms of level 0
-   (the dUdt[0] the last cycle's smoothing left) before each cycle, the
-   relative norm against the second cycle's, limiter freezing, and the
-   absolute / relative / cycle-count exits.  norms: 3 per check. */
+int orc_mg_cycle(orc_mg *mg) {
+    mg_set_top(mg, 0);
+    return mg_cgc(mg, 0, 0);
+}
+/* Execute :3375-3745: with fmg_cycles > 0 the full-multigrid start-up
+   first (top level Levels-2 up to 1, fmg_cycles cycles each, stopping early
+   on the FMG tolerances, then Prolong_Solution_Blocks to the next level
+   with messages and BCs), then the regular cycles from level 0.  Before
+   each cycle the residual norms of the top level (the dUdt[0] the last
+   cycle's smoothing left), the relative norm against the second check of
+   that level, limiter freezing at freeze_level (<= 0: never; unfrozen
+   again at a new top level, :3462-3476), and the exits: max_cycles cycles
+   in all, or the absolute / relative tolerance (FMG ones above level 0).
+   norms: 3 per check. */
 int orc_mg_execute(orc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done, int *checks) {
-    double init_l2 = -1.0, ratio = -1.0;
-    int first_step = 1, frozen = 0, n = 0, e = 0;
-    int nchk = 0;
-    for (int cycles = 1;; ++cycles) {
-        double nr[3];
-        orc_residual_norms(mg->lv[0], 1, nr);
-        if (norms) { norms[3 * nchk] = nr[0]; norms[3 * nchk + 1] = nr[1]; norms[3 * nchk + 2] = nr[2]; }
-        nchk++;
-        if (cycles == 2) init_l2 = nr[1];
-        else if (cycles > 2 && fabs(init_l2) > NANO) ratio = nr[1] / init_l2;
-        int stop = n >= max_cycles;
-        if (cycles > 2) {
-            if (nr[1] < mg->d.absolute_tolerance) stop = 1;
-            if (ratio < mg->d.relative_tolerance) stop = 1;
+    int first_step = 1, frozen = 0, n = 0, e = 0, nchk = 0;
+    const int top0 = mg->d.fmg_cycles > 0 ? mg->L - 2 : 0;
+    for (int top = top0; top >= 0 && !e; --top) {
+        mg_set_top(mg, top);
+        if (!first_step && freeze_level > 0 && frozen && mg->fine_s.i_limiter != CFFC_LIMITER_ZERO) {
+            for (int l = top; l < mg->L; ++l) orc_freeze_limiters(mg->lv[l], 0);
+            frozen = 0;
         }
-        if (stop) break;
-        if (!first_step && cycles > 1 && freeze_level > 0 && !frozen && mg->lv[0]->s.i_limiter != CFFC_LIMITER_ZERO &&
-            nr[1] <= freeze_level) {
-            for (int l = 0; l < mg->L; ++l) orc_freeze_limiters(mg->lv[l], 1);
-            frozen = 1;
+        const long long ncyc = top != 0 ? mg->d.fmg_cycles : 0x7fffffffffffffffLL;
+        double init_l2 = -1.0, ratio = -1.0;
+        for (long long cycles = 1; cycles <= ncyc; ++cycles) {
+            double nr[3];
+            orc_residual_norms(mg->lv[top], 1, nr);
+            if (norms) { norms[3 * nchk] = nr[0]; norms[3 * nchk + 1] = nr[1]; norms[3 * nchk + 2] = nr[2]; }
+            nchk++;
+            if (cycles == 2) init_l2 = nr[1];
+            else if (cycles > 2 && fabs(init_l2) > NANO) ratio = nr[1] / init_l2;
+            int stop = n >= max_cycles;
+            if (cycles > 2) {
+                const double at = top == 0 ? mg->d.absolute_tolerance : mg->d.fmg_absolute_tolerance;
+                const double rt = top == 0 ? mg->d.relative_tolerance : mg->d.fmg_relative_tolerance;
+                if (nr[1] < at) stop = 1;
+                if (ratio < rt) stop = 1;
+            }
+            if (stop) break;
+            if (!first_step && cycles > 1 && freeze_level > 0 && !frozen &&
+                mg->fine_s.i_limiter != CFFC_LIMITER_ZERO && nr[1] <= freeze_level) {
+                for (int l = top; l < mg->L; ++l) orc_freeze_limiters(mg->lv[l], 1);
+                frozen = 1;
+            }
+            if ((e = mg_cgc(mg, top, top))) break;
+            first_step = 0;
+            n++;
         }
-        if ((e = mg_cgc(mg, 0))) break;
-        first_step = 0;
-        n++;
+        /* a spent cycle budget still prolongs through the remaining
+           levels, each stopping at its first check, as the reference does */
+        if (top != 0 && !e) {
+            mg_prolong_solution(mg->lv[top], mg->lv[top - 1], &mg->d);
+            orc_exchange_and_bcs(mg->lv[top - 1]);
+        }
     }
+    mg_set_top(mg, 0);
     if (cycles_done) *cycles_done = n;
     if (checks) *checks = nchk;
     orc_exchange_and_bcs(mg->lv[0]);

@@ -157,3 +157,62 @@ def test_multigrid_converges_faster_than_single_grid():
     sg_hit = int(np.argmax(sg[:, 1] < 1e-3 * first)) + 1 if np.any(sg[:, 1] < 1e-3 * first) else len(sg) + 1
     print("multigrid: 1e-3 drop after %d cycles; single grid after %d steps" % (hit, sg_hit))
     assert 4 * hit < sg_hit
+
+
+def test_full_multigrid_prolongs_linear_fields_exactly():
+    """Execute with fmg_cycles > 0 and no cycle budget: level 1 (restricted
+    from level 0 at initialisation) is prolonged onto level 0 and nothing
+    else runs.  On a uniform mesh a linear conserved field comes back exactly
+    in every fine cell whose interpolation quadrilateral is interior (all but
+    the outermost fine layer along the physical boundary): a fine cell
+    written at the wrong index, an added rather than assigned solution or a
+    coarse centroid in place of the fine one breaks it."""
+    m, _ = _box(n=8)
+    mg = oracle.OracleMG(MG.hierarchy(m, 3), _scheme(), MG.MGParams(levels=3, fmg_cycles=4))
+    c = m.cell_centres()
+    W = np.stack([1.0 + 0.2 * c[:, 0] - 0.1 * c[:, 1], 0 * c[:, 0], 0 * c[:, 0], 2.0 + 0.3 * c[:, 1]], axis=1)
+    mg.set_W(W)
+    mg.initialize()
+    U0 = mg.levels[0].get_U()
+    U0[:] = 7.0                          # the prolongation must overwrite it
+    mg.levels[0].set_U(U0)
+    rc, n, norms = mg.execute(0)
+    assert rc == 0 and n == 0 and len(norms) == 2
+    U = mg.levels[0].get_U()
+    h = 1.0 / 16                         # 32 x 16 cells on [0, 2] x [0, 1]
+    inner = (c[:, 0] > h) & (c[:, 0] < 2.0 - h) & (c[:, 1] > h) & (c[:, 1] < 1.0 - h)
+    assert inner.sum() == 30 * 14
+    np.testing.assert_allclose(U[inner, 0], W[inner, 0], rtol=0, atol=1e-14)
+    np.testing.assert_allclose(U[inner, 3], W[inner, 3] / 0.4, rtol=0, atol=1e-13)
+    assert np.all(U[~inner] != 7.0)
+
+
+def test_full_multigrid_start_reaches_the_regular_fixed_point():
+    """Bump channel, unlimited Roe: the full-multigrid start (five cycles
+    with level 1 on top, checked on level 1, then the solution prolonged)
+    converges to the state the regular cycles reach, within round-off; its
+    first checks are level-1 norms, and free stream stays free stream
+    through the start-up on a uniform mesh.  (Not on a jittered one:
+    Restrict_Solution_Blocks divides the area-weighted sum by the coarse
+    cell's area, which there differs from the four fine areas, so the
+    coarse start-up level begins from a non-uniform state.)"""
+    m, W0 = steady.bump_case(0.08, 0.5, nbi=3, nbj=1, ni=8, nj=8)
+    s = _scheme(limiter=abi.LIMITER_ONE)
+    out = {}
+    for fmg in (0, 5):
+        mg = oracle.OracleMG(MG.hierarchy(m, 3), s, MG.MGParams(levels=3, fmg_cycles=fmg))
+        mg.set_W(W0)
+        mg.initialize()
+        rc, n, norms = mg.execute(320)
+        assert rc == 0 and n == 320 and len(norms) == 321
+        assert norms[-1, 1] < 1e-11 * norms[1, 1]
+        out[fmg] = (mg.get_W(), norms)
+    np.testing.assert_allclose(out[5][0], out[0][0], rtol=0, atol=1e-12)
+    assert not np.allclose(out[5][1][:5], out[0][1][:5])
+    mb, Wb = _box(n=8, jitter=0.0, blob=0.0)
+    mg = oracle.OracleMG(MG.hierarchy(mb, 3), _scheme(), MG.MGParams(levels=3, fmg_cycles=3))
+    mg.set_W(Wb)
+    mg.initialize()
+    rc, n, _ = mg.execute(6)
+    assert rc == 0 and n == 6
+    assert np.max(np.abs(mg.get_W() - Wb)) < 1e-13
