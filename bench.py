@@ -141,7 +141,7 @@ def steady_leg(ncases, seed=7):
             "all_converged": bool(np.all(cs > 0) and np.all(ws > 0))}
 
 
-def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=True):
+def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=True, fmg_cycles=50):
     """The same held-out steady bump cases, cold, solved together (one mesh
     of disjoint parts) by the FAS multigrid solver (FASMultigrid2D: V cycle,
     HLLE first-order coarse levels, 5-stage optimal smoothing).  Multigrid
@@ -150,7 +150,10 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=Tr
     on the L2 norm of (CFL*dt)*R at 1e-10.  The multigrid tolerance is that
     1e-10 divided by the median CFL*dt of the single-grid scheme over the
     cells at the initial state, and the freeze level is converted the same
-    way."""
+    way.  A third run starts with the full-multigrid start-up at
+    fmg_cycles cycles per coarse level (the reference's Euler2D examples
+    set Full_Multigrid = 50, examples/Euler2D/missile/missile.in:86)."""
+    import dataclasses
     from cffc_amd import euler2d, multigrid, steady
     from cffc_amd import mesh as M
     cases = steady.sample_cases(ncases, seed)
@@ -168,8 +171,8 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=Tr
     freeze = steady.STEADY_FREEZE_LEVEL / scale
     p = multigrid.MGParams(levels=levels, absolute_tolerance=tol)
 
-    def solve(W):
-        g = multigrid.FASMultigrid(m, steady.bump_scheme(cfl=cfl), p)
+    def solve(W, fmg=0):
+        g = multigrid.FASMultigrid(m, steady.bump_scheme(cfl=cfl), dataclasses.replace(p, fmg_cycles=fmg))
         g.set_graph(graph)
         g.set_W(W)
         g.initialize()
@@ -188,6 +191,12 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=Tr
     if cold_batch_s:
         out["cold_single_grid_batch_s"] = cold_batch_s
         out["time_ratio_vs_cold_single_grid"] = cold_batch_s / dt
+    if fmg_cycles > 0:
+        rcf, cf, nf, dtf = solve(W0, fmg_cycles)
+        out["fmg"] = {"fmg_cycles": fmg_cycles, "rc": rcf, "cycles": cf, "seconds": dtf,
+                      "final_R": float(nf[-1, 1]), "converged": bool(rcf == 0 and nf[-1, 1] < tol)}
+        if cold_batch_s:
+            out["fmg"]["time_ratio_vs_cold_single_grid"] = cold_batch_s / dtf
     # the same from the U-Net's predicted states (warm multigrid)
     model = os.path.join(ROOT, "cffc_amd", "data", "unet_bump.npz")
     if os.path.exists(model):
