@@ -378,8 +378,8 @@ def main():
     traffic = load_traffic("stage_kernel_%s_riemann4q_%dx%d_blocks_of_%dsq_traffic.json"
                            % (head_prec, args.blocks, args.blocks, args.cells))
     kname = "stage_kernel<%s,%d,%d,fused>" % ("float" if head_prec == "mixed" else "double", *head["tile"])
-    if traffic and traffic.get("kernel") != kname:
-        traffic = None   # the profiled kernel is another tile shape
+    # per-launch HBM bytes measured for this kernel and tile shape (None: tile not profiled)
+    traffic = traffic.get("kernels", {}).get(kname) if traffic else None
     out = {
         "metric": "cell-stage updates/s",
         "value": head["rate"],
