@@ -330,6 +330,10 @@ def main():
     ap.add_argument("--only-steady", action="store_true",
                     help="run only the steady legs (cold/warm single grid, multigrid) and print them")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build runs the whole mesh on a single MI355X "
+              "(multi-GPU block sharding is a design note only); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
@@ -345,11 +349,6 @@ def main():
                 "cold_batch_s"), graph=not args.mg_eager)
         print(json.dumps(o))
         return
-    if args.gpus != 1:
-        print("bench.py measures one GPU: this build runs the whole mesh on a single MI355X "
-              "(multi-GPU block sharding is a design note only); use --gpus 1", file=sys.stderr)
-        sys.exit(2)
-
     m, W0 = make_case(args.blocks, args.cells)
     ncell = m.ncell
     res = {}
