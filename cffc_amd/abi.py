@@ -49,7 +49,8 @@ class SchemeDesc(ctypes.Structure):
     _fields_ = [("i_time_integration", ctypes.c_int32), ("n_stage", ctypes.c_int32),
                 ("cfl_number", ctypes.c_double), ("i_reconstruction", ctypes.c_int32),
                 ("i_limiter", ctypes.c_int32), ("i_flux_function", ctypes.c_int32),
-                ("local_time_stepping", ctypes.c_int32), ("flux_precision", ctypes.c_int32)]
+                ("local_time_stepping", ctypes.c_int32), ("flux_precision", ctypes.c_int32),
+                ("axisymmetric", ctypes.c_int32)]
 
 
 class MGDesc(ctypes.Structure):
@@ -77,6 +78,7 @@ class Scheme:
     i_flux_function: int = FLUX_FUNCTION_ROE
     local_time_stepping: int = GLOBAL_TIME_STEPPING
     flux_precision: int = FLUX_FP64
+    axisymmetric: int = 0
 
 
 def _keep(arr, dtype, keep):
@@ -106,4 +108,5 @@ def mesh_desc(mesh, keep):
 
 def scheme_desc(s):
     return SchemeDesc(s.i_time_integration, s.n_stage, s.cfl_number, s.i_reconstruction,
-                      s.i_limiter, s.i_flux_function, s.local_time_stepping, s.flux_precision)
+                      s.i_limiter, s.i_flux_function, s.local_time_stepping, s.flux_precision,
+                      s.axisymmetric)

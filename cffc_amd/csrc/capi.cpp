@@ -225,6 +225,7 @@ static int check_scheme(const cffc_scheme_desc *s) {
         return fail(CFFC_E_ARG, "unsupported Local_Time_Stepping %d", s->local_time_stepping);
     if (s->flux_precision != CFFC_FLUX_FP64 && s->flux_precision != CFFC_FLUX_FP32)
         return fail(CFFC_E_ARG, "unsupported flux precision %d", s->flux_precision);
+    if (s->axisymmetric != 0 && s->axisymmetric != 1) return fail(CFFC_E_ARG, "Axisymmetric must be 0 or 1");
     return 0;
 }
 
@@ -737,6 +738,22 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
         free_ctx(c);
         return fail(CFFC_E_ARG, "mesh has %u degenerate faces (length <= 1e-9 x cell perimeter)", nd);
     }
+    if (c->s.axisymmetric) {
+        // the source divides by the centroid radius: every interior cell
+        // must lie off the axis (y > 0)
+        std::vector<double> yc(ncp);
+        HIPCHK(hipMemcpy(yc.data(), c->geom + (size_t)GEO_YC * ncp, ncp * sizeof(double), hipMemcpyDeviceToHost));
+        for (int b = 0; b < c->nblk; ++b) {
+            const BlockMeta &m = c->meta_h[b];
+            for (int j = 2; j < m.nj + 2; ++j)
+                for (int i = 2; i < m.ni + 2; ++i)
+                    if (!(yc[m.coff + (int64_t)j * m.pitch + i] > 0.0)) {
+                        free_ctx(c);
+                        return fail(CFFC_E_ARG, "axisymmetric: block %d cell (%d, %d) has centroid y <= 0", b, i - 2,
+                                    j - 2);
+                    }
+        }
+    }
     if (c->s.flux_precision == CFFC_FLUX_FP32) {
         if ((e = dalloc(&c->geomf, GF_COUNT * (size_t)ncp))) {
             free_ctx(c);
@@ -1016,6 +1033,7 @@ static StageParams base_stage_params(cffc_ctx *c) {
     p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE;
     p.eflux = c->eflux;
     p.ustore = c->ustore;
+    p.axisym = c->s.axisymmetric;
     return p;
 }
 

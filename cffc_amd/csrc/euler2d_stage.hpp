@@ -68,6 +68,7 @@ struct StageParams {
     int32_t hlle;           // HLLE flux instead of Roe
     int32_t residual;       // steady residual (F*l)/A, no CFL*dt factor (dUdt_Residual_Evaluation)
     const double *P;        // forcing term added after the fluxes: +(CFL*dt)*P, or +P with residual
+    int32_t axisym;         // axisymmetric source (CFL*dt)*S(W, Xc) after the east face
 };
 
 // Push form of the same-level exchange: the cell (ip, jp) of block b that a
@@ -697,6 +698,23 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
             if constexpr (RP) return tr == 0 ? sRing[4 * TJ + v * TI + tc] : sFj[v][tr][tc];
             else return sFj[v][tr][tc];
         };
+        // axisymmetric source terms (Euler2DQuad.cc:1511-1516, 1016-1020):
+        // S(W, Xc) of the cell's W^(s) and centroid radius
+        double Sx[4] = {0.0, 0.0, 0.0, 0.0};
+        const bool axi = p.axisym != 0;
+        if (axi) {
+            // the stage's W as the tile loaded it (the fused mixed path
+            // keeps W^(s) in FP32 only)
+            PS<double> Wc;
+            if constexpr (FUSED && Fast<FT>::on) {
+                Wc = {(double)p.Winf[c], (double)p.Winf[ncp + c], (double)p.Winf[2 * ncp + c],
+                      (double)p.Winf[3 * ncp + c]};
+            } else {
+                Wc = {p.Win[c], p.Win[ncp + c], p.Win[2 * ncp + c], p.Win[3 * ncp + c]};
+            }
+            const CS<double> s = axisym_source(Wc, G[GEO_YC * ncp + c]);
+            Sx[0] = s.d; Sx[1] = s.mx; Sx[2] = s.my; Sx[3] = s.E;
+        }
         if constexpr (Fast<FT>::on) {
             // mixed path: net face flux in FP32, one FP64 scaling per cell
             const double s = cdt / A;
@@ -706,6 +724,7 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                 const FT net = (fW(v) * flW - sFi[v][tr][tc + 1] * flE) +
                                (fS(v) * flS - sFj[v][tr + 1][tc] * flN);
                 dU[v] += s * (double)net;
+                if (axi) dU[v] += cdt * Sx[v];
             }
         } else {
             // ((CFL*dt)*F*l)/A per face, +W -E +S -N (Euler2DQuad.cc:1502-1509)
@@ -720,11 +739,13 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
                     // (Flux*l)/A of dUdt_Residual_Evaluation, Euler2DQuad.cc:1001-1006
                     d += FW * lW / A;
                     d -= FE * lE / A;
+                    if (axi) d += Sx[v];
                     d += FS * lS / A;
                     d -= FN * lN / A;
                 } else {
                     d += cdt * FW * lW / A;
                     d -= cdt * FE * lE / A;
+                    if (axi) d += cdt * Sx[v];
                     d += cdt * FS * lS / A;
                     d -= cdt * FN * lN / A;
                 }

@@ -84,6 +84,15 @@ template <typename T> CFFC_HD T W_H(const PS<T> &W) {
     return gam<T>() * gm1i<T>() * W.p + Consts<T>::HALF * W.d * (W.vx * W.vx + W.vy * W.vy);
 }
 template <typename T> CFFC_HD T W_a(const PS<T> &W) { return qsqrt(qdiv(gam<T>() * W.p, W.d)); }
+// axisymmetric source S(W, X), Euler2DState.h:3179-3182 (y the radius)
+template <typename T> CFFC_HD CS<T> axisym_source(const PS<T> &W, T y) {
+    CS<T> S;
+    S.d = -W.d * W.vy / y;
+    S.mx = -W.d * W.vx * W.vy / y;
+    S.my = -W.d * (W.vy * W.vy) / y;
+    S.E = -W.vy * W_H(W) / y;
+    return S;
+}
 template <typename T> CFFC_HD T W_a2(const PS<T> &W) { return qdiv(gam<T>() * W.p, W.d); }
 template <typename T> CFFC_HD T U_e(const CS<T> &U) {
     return U.E / U.d - Consts<T>::HALF * (U.mx * U.mx + U.my * U.my) / sqr(U.d);

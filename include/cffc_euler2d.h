@@ -133,6 +133,9 @@ typedef struct cffc_scheme_desc {
     int32_t i_flux_function;      /* CFFC_FLUX_FUNCTION_ROE / _HLLE         */
     int32_t local_time_stepping;  /* CFFC_GLOBAL/SCALAR_LOCAL_TIME_STEPPING */
     int32_t flux_precision;       /* CFFC_FLUX_FP64 / CFFC_FLUX_FP32        */
+    int32_t axisymmetric;         /* Axisymmetric: 0 planar, 1 y is the radius
+                                     (source S(W, X), Euler2DState.h:3179; every
+                                     cell centroid must have y > 0)          */
 } cffc_scheme_desc;
 
 /* The FAS multigrid parameters of Multigrid_Input_Parameters

@@ -36,6 +36,7 @@ def lib():
         vp = ctypes.c_void_p
         L.orc_flux_roe_n.argtypes = [pd, pd, d, d, pd]
         L.orc_flux_roe_x.argtypes = [pd, pd, pd]
+        L.orc_axisym_source.argtypes = [pd, d, d, pd]
         L.orc_roe_average.argtypes = [pd, pd, pd]
         L.orc_limiter_venkatakrishnan.argtypes = [pd, d, d, d]
         L.orc_limiter_venkatakrishnan.restype = d
@@ -173,6 +174,12 @@ def multistage_optimal_smoothing(i, n, limiter):
 def quad_centroid(x4, y4):
     out = np.zeros(3)
     lib().orc_quad_centroid(_p(_arr(x4)), _p(_arr(y4)), _p(out))
+    return out
+
+
+def axisym_source(W, x, y):
+    out = np.zeros(4)
+    lib().orc_axisym_source(_p(_arr(W)), x, y, _p(out))
     return out
 
 

@@ -86,6 +86,11 @@ static inline cst U_of_W(pst W) {
 static inline double W_h(pst W) { return g * W.p / (gm1 * W.d) + HALF * (W.vx * W.vx + W.vy * W.vy); }
 /* H() :1081 */
 static inline double W_H(pst W) { return g * gm1i * W.p + HALF * W.d * (W.vx * W.vx + W.vy * W.vy); }
+/* axisymmetric source S(W, X), Euler2DState.h:3179-3182 (y the radius) */
+static inline cst axisym_source(pst W, double y) {
+    cst S = {-W.d * W.vy / y, -W.d * W.vx * W.vy / y, -W.d * (W.vy * W.vy) / y, -W.vy * W_H(W) / y};
+    return S;
+}
 /* a() :1092, a2() :1103 */
 static inline double W_a(pst W) { return sqrt(g * W.p / W.d); }
 static inline double W_a2(pst W) { return g * W.p / W.d; }
@@ -906,6 +911,14 @@ static void block_dudt(oblk *B, const cffc_scheme_desc *s, int i_stage) {
                 cst F = flux_n(s, Wl, Wr, nx, ny);
                 acc_sub(&B->dUdt[k][cL], CFL * B->dt[cL], F, lfaceE(B, i, j), B->A[cL]);
                 acc_add(&B->dUdt[k][cR], CFL * B->dt[cR], F, lfaceW(B, i + 1, j), B->A[cR]);
+                /* axisymmetric source terms :1511-1516, after cell i's east
+                   face (the ghost cell ICl-1 is cleared below anyway) */
+                if (s->axisymmetric && i >= ICl) {
+                    const double cdt = CFL * B->dt[cL];
+                    const cst Sx = axisym_source(B->W[cL], B->yc[cL]);
+                    cst *d = &B->dUdt[k][cL];
+                    d->d += cdt * Sx.d; d->mx += cdt * Sx.mx; d->my += cdt * Sx.my; d->E += cdt * Sx.E;
+                }
                 /* west and east boundary fluxes :1518-1523 */
                 if (i == ICl - 1) B->FluxW[j - JCl] = flux_times(F, -ONE, lfaceW(B, i + 1, j));
                 else if (i == ICu) B->FluxE[j - JCl] = flux_times(F, ONE, lfaceE(B, i, j));
@@ -1001,6 +1014,12 @@ static void block_residual(oblk *B, const cffc_scheme_desc *s) {
                 cst F = flux_n(s, Wl, Wr, nx, ny);
                 res_sub(&R[cL], F, lfaceE(B, i, j), B->A[cL]);
                 res_add(&R[cR], F, lfaceW(B, i + 1, j), B->A[cR]);
+                /* axisymmetric source terms :1016-1020 */
+                if (s->axisymmetric && i >= ICl) {
+                    const cst Sx = axisym_source(B->W[cL], B->yc[cL]);
+                    cst *d = &R[cL];
+                    d->d += Sx.d; d->mx += Sx.mx; d->my += Sx.my; d->E += Sx.E;
+                }
                 if (i == ICl - 1) B->FluxW[j - JCl] = flux_times(F, -ONE, lfaceW(B, i + 1, j));
                 else if (i == ICu) B->FluxE[j - JCl] = flux_times(F, ONE, lfaceE(B, i, j));
             }
@@ -2430,4 +2449,10 @@ void orc_flux_hlle_n(const double *Wl, const double *Wr, double nx, double ny, d
     memcpy(&l, Wl, sizeof l); memcpy(&r, Wr, sizeof r);
     cst f = flux_hlle_n(l, r, nx, ny);
     memcpy(F, &f, sizeof f);
+}
+void orc_axisym_source(const double *W, double x, double y, double *out) {
+    (void)x;
+    pst w; memcpy(&w, W, sizeof w);
+    cst S = axisym_source(w, y);
+    memcpy(out, &S, sizeof S);
 }

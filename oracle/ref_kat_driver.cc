@@ -13,7 +13,8 @@
 // REFLECT (W[4] nx ny -> W[4]), BCCHARP (Wi[4] Wo[4] nx ny -> W[4]),
 // W_OF_U (U[4] -> W[4]), U_OF_W (W[4] -> U[4]), FX (W[4] -> F[4]),
 // RK (i n -> beta), OPTSM (i n limiter -> beta), HLLE_N (Wl[4] Wr[4] nx ny ->
-// F[4]), BILIN (U1..U4 x1 y1 .. x4 y4 x0 y0 -> flag U0, scalar states).
+// F[4]), BILIN (U1..U4 x1 y1 .. x4 y4 x0 y0 -> flag U0, scalar states),
+// AXISRC (W[4] x y -> S[4], the axisymmetric source S(W, X)).
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -85,6 +86,10 @@ int main() {
         } else if (t == "FX") {
             Euler2D_cState F = P(v, 0).Fx();
             for (int k = 0; k < 4; ++k) out[k] = F[k + 1];
+            emit(tag, v, out, 4);
+        } else if (t == "AXISRC") {
+            Euler2D_cState Sx = S(P(v, 0), Vector2D(v[4], v[5]));
+            for (int k = 0; k < 4; ++k) out[k] = Sx[k + 1];
             emit(tag, v, out, 4);
         } else if (t == "HLLE_N") {
             Euler2D_cState F = FluxHLLE_n(P(v, 0), P(v, 4), Vector2D(v[8], v[9]));

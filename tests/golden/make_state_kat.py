@@ -158,6 +158,12 @@ def cases():
               quad[1] * (1 - zeta) * eta)
         U = rng.normal(0, 1, 4)
         lines.append(fmt("BILIN", list(U) + list(quad.ravel()) + list(p0)))
+    # axisymmetric source S(W, X) (own stream: the vectors above are unchanged)
+    rng2 = np.random.default_rng(2024)
+    W = rand_W(rng2, 300)
+    for k in range(300):
+        x, y = rng2.uniform(-2.0, 2.0), rng2.uniform(0.01, 3.0) * (1.0 if k % 10 else -1.0)
+        lines.append(fmt("AXISRC", list(W[k]) + [x, y]))
     return lines
 
 

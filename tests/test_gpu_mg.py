@@ -96,6 +96,19 @@ def test_cycles_match_oracle(cycle):
     g.close()
 
 
+def test_axisymmetric_cycles_match_oracle():
+    """Axisymmetric levels (the source in every smoothing stage and in the
+    residual evaluation, dUdt_Residual_Evaluation :1016-1020)."""
+    m, W0 = _box(seed=6)
+    s = _scheme()
+    s.axisymmetric = 1
+    g, o = _pair(m, W0, s, MG.MGParams(levels=3))
+    for n in range(3):
+        assert g.cycle() == o.cycle() == 0
+    _compare_levels(g, o, "axisymmetric")
+    g.close()
+
+
 def test_variants_match_oracle():
     """Second-order coarse levels (Roe + limiter), injection prolongation,
     no defect correction, residual smoothing."""

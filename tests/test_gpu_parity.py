@@ -249,3 +249,49 @@ def test_two_level_step_reduction_large_mesh(euler2d):
     np.testing.assert_array_equal(g[3], o[3])
     assert relerr(g[2], o[2]) <= REL
     assert nonbitwise(gpu.get_W(), orc.get_W()) == 0
+
+
+def _axisymmetric_case():
+    bcs = dict(W=abi.BC_FIXED, E=abi.BC_CONSTANT_EXTRAPOLATION, S=abi.BC_REFLECTION, N=abi.BC_REFLECTION)
+    m = M.cartesian(3, 2, 8, 8, x0=0, x1=1.5, y0=0.2, y1=1.2, bcs=bcs, jitter=0.1, seed=13)
+    W0 = M.riemann_4quadrant(m, split=(0.75, 0.7))
+    m.set_wo_from_initial(W0)
+    return m, W0
+
+
+def test_axisymmetric_fp64_matches_oracle(euler2d):
+    """Axisymmetric (y the radius, source (CFL*dt)*S(W, Xc) after each
+    cell's east face): the fused loop and the granular reference-named
+    calls agree with the oracle bit for bit; the mixed path stays close."""
+    m, W0 = _axisymmetric_case()
+    s = abi.Scheme(axisymmetric=1)
+    gpu, orc, g, o = run_both(euler2d, m, W0, s, 30)
+    assert g[0] == 0 and o[0] == 0 and g[1] == o[1] == 30
+    Wg, Wo = gpu.get_W(), orc.get_W()
+    print("axisymmetric, 30 steps: rel %.3e, %d non-bitwise" % (relerr(Wg, Wo), nonbitwise(Wg, Wo)), flush=True)
+    assert relerr(Wg, Wo) <= REL
+    assert relerr(g[2], o[2]) <= REL
+    gr = euler2d.Euler2DQuadBlocks(m, s)
+    gr.set_W(W0)
+    rc, _ = gr.step_reference_order(5)
+    o5 = oracle.Oracle(m, s)
+    o5.set_W(W0)
+    o5.run(5)
+    assert rc == 0 and nonbitwise(gr.get_W(), o5.get_W()) == 0
+    mx = euler2d.Euler2DQuadBlocks(m, abi.Scheme(axisymmetric=1, flux_precision=abi.FLUX_FP32))
+    mx.set_W(W0)
+    rc, n, _, _ = mx.run_steps(30)
+    d = np.abs(mx.get_W() - Wo)
+    print("axisymmetric mixed vs FP64 oracle: L1 %s Linf %s" % (d.mean(axis=0), d.max(axis=0)), flush=True)
+    assert rc == 0 and np.all(d.mean(axis=0) < 1e-4) and np.all(d.max(axis=0) < 5e-3)
+    # the planar run differs: the source is really applied
+    pl = oracle.Oracle(m, abi.Scheme())
+    pl.set_W(W0)
+    pl.run(30)
+    assert np.max(np.abs(pl.get_W() - Wo)) > 1e-3
+
+
+def test_axisymmetric_mesh_on_the_axis_is_refused(euler2d):
+    m = M.cartesian(2, 2, 4, 4, y0=-0.5, y1=0.5)
+    with pytest.raises(euler2d.CffcError, match="centroid y <= 0"):
+        euler2d.Euler2DQuadBlocks(m, abi.Scheme(axisymmetric=1))

@@ -250,3 +250,30 @@ def test_riemann_4quadrant_diagonal_mirror_symmetry():
     Wm = W[mirror][:, [0, 2, 1, 3]]
     assert np.max(np.abs(W - W0)) > 0.1                       # the waves have moved
     np.testing.assert_allclose(W, Wm, rtol=0, atol=1e-12)
+
+
+def test_axisymmetric_source_is_the_only_residual_of_a_uniform_flow():
+    """Axisymmetric (y the radius): a uniform state has zero net face flux,
+    so the first stage's residual is (CFL*dt)*S(W, Xc) in every cell, with
+    S the reference's source (Euler2DState.h:3179, bitwise against it in
+    test_oracle_kat.py) at the cell's area centroid; planar, it is zero."""
+    bcs = dict(W=abi.BC_FIXED, E=abi.BC_FIXED, S=abi.BC_FIXED, N=abi.BC_FIXED)
+    m = M.cartesian(3, 2, 6, 6, x1=1.5, y0=0.3, y1=1.3, bcs=bcs, jitter=0.15, seed=12)
+    ws = np.array([1.2, 0.35, -0.25, 0.9])
+    m.wo[:] = ws
+    W = np.broadcast_to(ws, (m.ncell, 4)).copy()
+    for axi in (0, 1):
+        s = abi.Scheme(local_time_stepping=abi.SCALAR_LOCAL_TIME_STEPPING, axisymmetric=axi)
+        o = oracle.Oracle(m, s)
+        o.set_W(W)
+        o.exchange_and_bcs()
+        o.cfl(set_global=False)
+        assert o.dudt_stage(1) == 0
+        d = o.get_dUdt(0)
+        if axi:
+            yc = _centroids(o, m)[:, 1]
+            ref = np.stack([s.cfl_number * dt * oracle.axisym_source(ws, 0.0, y) for dt, y in zip(o.get_dt(), yc)])
+            np.testing.assert_allclose(d, ref, rtol=1e-12, atol=1e-15)
+            assert np.abs(ref).min() > 1e-4
+        else:
+            assert np.abs(d).max() < 1e-14

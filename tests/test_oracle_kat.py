@@ -181,6 +181,10 @@ def test_kat_flux_hlle_n(oracle_lib):
     _check("HLLE_N", 10, lambda v: oracle.flux_hlle_n(v[0:4], v[4:8], v[8], v[9]))
 
 
+def test_kat_axisymmetric_source(oracle_lib):
+    _check("AXISRC", 6, lambda v: oracle.axisym_source(v[0:4], v[4], v[5]))
+
+
 def _bilin_scalar(v):
     U = np.zeros((4, 4))
     U[:, 0] = v[0:4]
