@@ -1008,6 +1008,15 @@ extern "C" int cffc_cfl(cffc_ctx *c, double *dt_min, int set_global) {
     return 0;
 }
 
+extern "C" int cffc_set_global_timestep(cffc_ctx *c, double dt) {
+    if (!c || !(dt > 0.0)) return fail(CFFC_E_ARG, "bad argument");
+    unsigned long long bits;
+    memcpy(&bits, &dt, sizeof bits);
+    HIPCHK(cffc_launch_set_u64(c->scal, bits, 0, 1, c->stream));
+    HIPCHK(cffc_launch_set_global_dt(c->meta, c->nblk, c->max_int, c->dt, c->scal, c->stream));
+    return 0;
+}
+
 static StageParams base_stage_params(cffc_ctx *c) {
     StageParams p;
     memset(&p, 0, sizeof p);
@@ -2118,5 +2127,41 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
     if (checks) *checks = nchk;
     if (int e = mg_exchange(mg->lv[0])) return e;
     HIPCHK(hipStreamSynchronize(mg->lv[0]->stream));
+    return rc;
+}
+
+// Time-accurate explicit loop, Euler2DQuadSolvers.cc:549-898 (Time_Accurate
+// = 1): the reference's call order through the granular entry points, one
+// host synchronisation per step for the clipped global step.
+extern "C" int cffc_run_time_accurate(cffc_ctx *c, double t_max, int max_steps, double *time, int *steps_done) {
+    if (!c || max_steps < 0 || !(t_max >= 0.0)) return fail(CFFC_E_ARG, "bad argument");
+    if (c->s.local_time_stepping != CFFC_GLOBAL_TIME_STEPPING)
+        return fail(CFFC_E_ARG, "time-accurate runs use global time stepping");
+    const bool optsm = c->s.i_time_integration == CFFC_TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING;
+    const double beta = optsm ? optimal_smoothing(c->s.n_stage, c->s.n_stage, c->s.i_limiter) : 1.0;
+    const double CFL = c->s.cfl_number;
+    double Time = 0.0;
+    int n = 0, rc = 0;
+    while (Time < t_max && n < max_steps) {
+        double dTime;
+        if (int e = cffc_cfl(c, &dTime, 0)) return e;
+        // :662-680
+        if (!optsm && Time + CFL * dTime > t_max) dTime = (t_max - Time) / CFL;
+        else if (optsm && Time + CFL * dTime * beta > t_max) dTime = (t_max - Time) / (CFL * beta);
+        if (int e = cffc_set_global_timestep(c, dTime)) return e;
+        for (int st = 1; st <= c->s.n_stage && !rc; ++st) {
+            int e;
+            if ((e = cffc_exchange_and_bcs(c)) || (e = cffc_dudt_stage(c, st)) ||
+                (e = cffc_apply_flux_corrections(c, st)))
+                return e;
+            rc = cffc_update_stage(c, st, nullptr);
+            if (rc < 0) return rc;
+        }
+        if (rc) break;
+        ++n;
+        Time = optsm ? Time + CFL * dTime * beta : Time + CFL * dTime;
+    }
+    if (time) *time = Time;
+    if (steps_done) *steps_done = n;
     return rc;
 }

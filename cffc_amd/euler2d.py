@@ -56,6 +56,8 @@ def load():
     L.cffc_exchange_and_bcs.argtypes = [vp]
     L.cffc_apply_flux_corrections.argtypes = [vp, i32]
     L.cffc_cfl.argtypes = [vp, pd, i32]
+    L.cffc_set_global_timestep.argtypes = [vp, ctypes.c_double]
+    L.cffc_run_time_accurate.argtypes = [vp, ctypes.c_double, i32, pd, ctypes.POINTER(ctypes.c_int)]
     L.cffc_dudt_stage.argtypes = [vp, i32]
     L.cffc_update_stage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64)]
     L.cffc_residual_norms.argtypes = [vp, i32, pd]
@@ -188,6 +190,17 @@ class Euler2DQuadBlocks:
         d = ctypes.c_double()
         _check(load().cffc_cfl(self.h, ctypes.byref(d), 1 if set_global else 0), "cffc_cfl")
         return d.value
+
+    def Set_Global_TimeStep(self, dt):
+        _check(load().cffc_set_global_timestep(self.h, ctypes.c_double(dt)), "cffc_set_global_timestep")
+
+    def run_time_accurate(self, t_max, max_steps=1000000):
+        """Time-accurate loop to t_max (the last step clipped); returns
+        (rc, time, steps)."""
+        t, n = ctypes.c_double(0.0), ctypes.c_int(0)
+        rc = _check(load().cffc_run_time_accurate(self.h, ctypes.c_double(t_max), max_steps, ctypes.byref(t),
+                                                  ctypes.byref(n)), "cffc_run_time_accurate")
+        return rc, t.value, n.value
 
     def dUdt_Multistage_Explicit(self, i_stage):
         return _check(load().cffc_dudt_stage(self.h, i_stage), "cffc_dudt_stage")

@@ -217,6 +217,17 @@ int  cffc_apply_flux_corrections(cffc_ctx *ctx, int i_stage);
       CFFC_Minimum_MPI (MPI.h:184) and, when set_global != 0,
       Set_Global_TimeStep (Euler2DQuadMultiBlock.cc:994). */
 int  cffc_cfl(cffc_ctx *ctx, double *dt_min, int set_global);
+/* Set_Global_TimeStep (Euler2DQuadMultiBlock.cc:994): every cell's dt and
+   the global step become dt (the time-accurate loop's clipped step) */
+int  cffc_set_global_timestep(cffc_ctx *ctx, double dt);
+/* The time-accurate explicit loop (Euler2DQuadSolvers.cc:549-898 with
+   Time_Accurate = 1, global time stepping): per step CFL + global minimum,
+   the step clipped so that Time reaches t_max exactly (:662-680), then the
+   stages through the calls above; Time += CFL*dTime (times
+   MultiStage_Optimally_Smoothing(N, N, limiter) for that scheme, :884-896);
+   stops when Time >= t_max or after max_steps.  *time / *steps_done may be
+   NULL; a nonzero return is the update's failing-cell code. */
+int  cffc_run_time_accurate(cffc_ctx *ctx, double t_max, int max_steps, double *time, int *steps_done);
 
 /* == dUdt_Multistage_Explicit(blocks, global, list, IP, i_stage)
       (Euler2DQuadMultiBlock.cc:1346 -> Euler2DQuad.cc:1235). */

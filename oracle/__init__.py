@@ -288,6 +288,9 @@ class Oracle:
         lib().orc_cfl(self.h, ctypes.byref(d), 1 if set_global else 0)
         return d.value
 
+    def set_global_dt(self, dt):
+        return lib().orc_set_global_dt(self.h, ctypes.c_double(dt))
+
     def dudt_stage(self, i_stage):
         return lib().orc_dudt_stage(self.h, i_stage)
 

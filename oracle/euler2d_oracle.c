@@ -2456,3 +2456,8 @@ void orc_axisym_source(const double *W, double x, double y, double *out) {
     cst S = axisym_source(w, y);
     memcpy(out, &S, sizeof S);
 }
+/* Set_Global_TimeStep (Euler2DQuadMultiBlock.cc:994) with a given step */
+int orc_set_global_dt(orc_ctx *ctx, double dt) {
+    for (int b = 0; b < ctx->nblk; ++b) block_set_global_dt(&ctx->b[b], dt);
+    return 0;
+}

@@ -295,3 +295,44 @@ def test_axisymmetric_mesh_on_the_axis_is_refused(euler2d):
     m = M.cartesian(2, 2, 4, 4, y0=-0.5, y1=0.5)
     with pytest.raises(euler2d.CffcError, match="centroid y <= 0"):
         euler2d.Euler2DQuadBlocks(m, abi.Scheme(axisymmetric=1))
+
+
+def _oracle_time_accurate(m, s, W0, t_max):
+    """The reference's time-accurate loop (Euler2DQuadSolvers.cc:655-680,
+    788-792, 884-896) driven through the oracle's calls."""
+    o = oracle.Oracle(m, s)
+    o.set_W(W0)
+    beta = oracle.multistage_optimal_smoothing(s.n_stage, s.n_stage, s.i_limiter) \
+        if s.i_time_integration == abi.TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING else 1.0
+    time, n = 0.0, 0
+    while time < t_max:
+        dt = o.cfl(set_global=False)
+        if time + s.cfl_number * dt * beta > t_max:
+            dt = (t_max - time) / (s.cfl_number * beta) if beta != 1.0 else (t_max - time) / s.cfl_number
+        o.set_global_dt(dt)
+        for st in range(1, s.n_stage + 1):
+            o.exchange_and_bcs()
+            o.dudt_stage(st)
+            assert o.update_stage(st)[0] == 0
+        n += 1
+        time = time + s.cfl_number * dt * beta if beta != 1.0 else time + s.cfl_number * dt
+    return o, time, n
+
+
+def test_time_accurate_run_reaches_t_max_like_the_oracle(euler2d):
+    """Time_Accurate: the loop stops at exactly the oracle's step count and
+    final time, the last step clipped to reach t_max; the state is bit for
+    bit the oracle's."""
+    m = M.cartesian(3, 3, 8, 8, jitter=0.1, seed=21)
+    W0 = M.riemann_4quadrant(m)
+    m.set_wo_from_initial(W0)
+    t_max = 0.05
+    s = abi.Scheme()
+    g = euler2d.Euler2DQuadBlocks(m, s)
+    g.set_W(W0)
+    rc, t, n = g.run_time_accurate(t_max)
+    o, t2, n2 = _oracle_time_accurate(m, s, W0, t_max)
+    print("time-accurate to %g: %d / %d steps, time %.17g / %.17g, %d non-bitwise" %
+          (t_max, n, n2, t, t2, nonbitwise(g.get_W(), o.get_W())), flush=True)
+    assert rc == 0 and n == n2 > 3 and t == t2 and abs(t - t_max) <= 1e-15
+    assert nonbitwise(g.get_W(), o.get_W()) == 0

@@ -277,3 +277,13 @@ def test_axisymmetric_source_is_the_only_residual_of_a_uniform_flow():
             assert np.abs(ref).min() > 1e-4
         else:
             assert np.abs(d).max() < 1e-14
+
+
+def test_set_global_timestep_sets_every_cell():
+    m = M.cartesian(2, 2, 4, 4, jitter=0.1, seed=3)
+    o = oracle.Oracle(m, abi.Scheme())
+    o.set_W(_smooth_W(m.cell_centres()))
+    dtmin = o.cfl(set_global=False)
+    assert np.ptp(o.get_dt()) > 0
+    o.set_global_dt(0.5 * dtmin)
+    np.testing.assert_array_equal(o.get_dt(), 0.5 * dtmin)
