@@ -31,7 +31,7 @@ SCALAR_LOCAL_TIME_STEPPING = 1
 FLUX_FP64 = 0
 FLUX_FP32 = 1
 # runtime errors (CFFC_E_*)
-E_ARG, E_HIP, E_NOMEM, E_TOPOLOGY, E_NODEVICE = -1, -2, -3, -4, -5
+E_ARG, E_HIP, E_NOMEM, E_TOPOLOGY, E_NODEVICE, E_STEP_LIMIT = -1, -2, -3, -4, -5, -6
 
 NBR_N, NBR_S, NBR_E, NBR_W, NBR_NE, NBR_NW, NBR_SE, NBR_SW = range(8)
 

@@ -2130,38 +2130,58 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
     return rc;
 }
 
-// Time-accurate explicit loop, Euler2DQuadSolvers.cc:549-898 (Time_Accurate
+// Time-accurate explicit loop, Euler2DQuadSolvers.cc:526-898 (Time_Accurate
 // = 1): the reference's call order through the granular entry points, one
-// host synchronisation per step for the clipped global step.
-extern "C" int cffc_run_time_accurate(cffc_ctx *c, double t_max, int max_steps, double *time, int *steps_done) {
-    if (!c || max_steps < 0 || !(t_max >= 0.0)) return fail(CFFC_E_ARG, "bad argument");
+// host synchronisation per step for the clipped global step.  *time carries
+// the start time in (the restart's Time) and the reached time out; the
+// reference's loop has no step cap, max_steps > 0 adds one.
+extern "C" int cffc_run_time_accurate(cffc_ctx *c, double t_max, int max_steps, double freeze_level, double *time,
+                                      int *steps_done) {
+    if (!c || !time || !(t_max >= 0.0) || !(*time >= 0.0) || !(freeze_level >= 0.0))
+        return fail(CFFC_E_ARG, "bad argument");
     if (c->s.local_time_stepping != CFFC_GLOBAL_TIME_STEPPING)
         return fail(CFFC_E_ARG, "time-accurate runs use global time stepping");
     const bool optsm = c->s.i_time_integration == CFFC_TIME_STEPPING_MULTISTAGE_OPTIMAL_SMOOTHING;
-    const double beta = optsm ? optimal_smoothing(c->s.n_stage, c->s.n_stage, c->s.i_limiter) : 1.0;
+    // MultiStage_Optimally_Smoothing(N, N, limiter) enters the clip for every
+    // scheme (:668-679) but the time update only for optimal smoothing (:888-896)
+    const double beta = optimal_smoothing(c->s.n_stage, c->s.n_stage, c->s.i_limiter);
     const double CFL = c->s.cfl_number;
-    double Time = 0.0;
+    double Time = *time;
     int n = 0, rc = 0;
-    while (Time < t_max && n < max_steps) {
+    bool first_step = true;
+    while (Time < t_max) {
+        if (max_steps > 0 && n >= max_steps) {
+            *time = Time;
+            if (steps_done) *steps_done = n;
+            return fail(CFFC_E_STEP_LIMIT, "max_steps reached before t_max");
+        }
         double dTime;
         if (int e = cffc_cfl(c, &dTime, 0)) return e;
-        // :662-680
+        // :662-680, the reference's if / else-if as written
         if (!optsm && Time + CFL * dTime > t_max) dTime = (t_max - Time) / CFL;
-        else if (optsm && Time + CFL * dTime * beta > t_max) dTime = (t_max - Time) / (CFL * beta);
+        else if (Time + CFL * dTime * beta > t_max) dTime = (t_max - Time) / (CFL * beta);
         if (int e = cffc_set_global_timestep(c, dTime)) return e;
+        // :796-803 Freeze_Limiters once the (lagged) L2 norm is low enough
+        if (freeze_level > 0.0 && !first_step && !c->freeze) {
+            double nr[3];
+            if (int e = cffc_residual_norms(c, 1, nr)) return e;
+            if (nr[1] <= freeze_level)
+                if (int e = cffc_freeze_limiters(c, 1)) return e;
+        }
         for (int st = 1; st <= c->s.n_stage && !rc; ++st) {
             int e;
             if ((e = cffc_exchange_and_bcs(c)) || (e = cffc_dudt_stage(c, st)) ||
-                (e = cffc_apply_flux_corrections(c, st)))
+                (e = cffc_apply_flux_corrections(c, st)) || (e = cffc_residual_smoothing(c, st)))
                 return e;
             rc = cffc_update_stage(c, st, nullptr);
             if (rc < 0) return rc;
         }
         if (rc) break;
+        first_step = false;
         ++n;
         Time = optsm ? Time + CFL * dTime * beta : Time + CFL * dTime;
     }
-    if (time) *time = Time;
+    *time = Time;
     if (steps_done) *steps_done = n;
     return rc;
 }

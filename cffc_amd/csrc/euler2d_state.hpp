@@ -20,6 +20,10 @@
 
 #define CFFC_HD __host__ __device__ __forceinline__
 
+#define CFFC_POW_FN __device__ __forceinline__
+#define CFFC_POW_TAB static __constant__ const
+#include "glibc_pow.h"
+
 namespace cffc {
 
 template <typename T> struct Consts {
@@ -292,6 +296,12 @@ template <typename T> CFFC_HD PS<T> reflect(const PS<T> &W, T nx, T ny) {
     return r;
 }
 
+// pow of the characteristic boundary state (Euler2DState.cc:1432).  FP64:
+// glibc's own method and roundings (glibc_pow.h), so the boundary density
+// has the reference's bits; FP32 (mixed path): the device powf.
+__device__ __forceinline__ double bc_pow(double x, double y) { return cffc_pow_glibc(x, y); }
+__device__ __forceinline__ float bc_pow(float x, float y) { return powf(x, y); }
+
 // BC_Characteristic_Pressure, Euler2DState.cc:1383-1465
 template <typename T> CFFC_HD PS<T> bc_characteristic_pressure(const PS<T> &Wi, const PS<T> &Wo, T nx, T ny) {
     typedef Consts<T> K;
@@ -303,7 +313,7 @@ template <typename T> CFFC_HD PS<T> bc_characteristic_pressure(const PS<T> &Wi, 
         db = Wir.d; ubr = Wir.vx; vbr = Wir.vy; pb = Wir.p;
     } else if (mi >= K::ZERO) {
         pb = Wor.p;
-        db = Wir.d * pow(pb / Wir.p, K::ONE / gam<T>());
+        db = Wir.d * bc_pow(pb / Wir.p, K::ONE / gam<T>());
         ab = tsqrt(gam<T>() * pb / db);
         ubr = Wir.vx + K::TWO * (W_a(Wir) - ab) * gm1i<T>();
         vbr = Wir.vy;

@@ -57,7 +57,7 @@ def load():
     L.cffc_apply_flux_corrections.argtypes = [vp, i32]
     L.cffc_cfl.argtypes = [vp, pd, i32]
     L.cffc_set_global_timestep.argtypes = [vp, ctypes.c_double]
-    L.cffc_run_time_accurate.argtypes = [vp, ctypes.c_double, i32, pd, ctypes.POINTER(ctypes.c_int)]
+    L.cffc_run_time_accurate.argtypes = [vp, ctypes.c_double, i32, ctypes.c_double, pd, ctypes.POINTER(ctypes.c_int)]
     L.cffc_dudt_stage.argtypes = [vp, i32]
     L.cffc_update_stage.argtypes = [vp, i32, ctypes.POINTER(ctypes.c_int64)]
     L.cffc_residual_norms.argtypes = [vp, i32, pd]
@@ -194,12 +194,15 @@ class Euler2DQuadBlocks:
     def Set_Global_TimeStep(self, dt):
         _check(load().cffc_set_global_timestep(self.h, ctypes.c_double(dt)), "cffc_set_global_timestep")
 
-    def run_time_accurate(self, t_max, max_steps=1000000):
-        """Time-accurate loop to t_max (the last step clipped); returns
-        (rc, time, steps)."""
-        t, n = ctypes.c_double(0.0), ctypes.c_int(0)
-        rc = _check(load().cffc_run_time_accurate(self.h, ctypes.c_double(t_max), max_steps, ctypes.byref(t),
-                                                  ctypes.byref(n)), "cffc_run_time_accurate")
+    def run_time_accurate(self, t_max, max_steps=0, time=0.0, freeze_level=0.0):
+        """Time-accurate loop from `time` to t_max (the last step clipped);
+        returns (rc, time, steps).  max_steps > 0 caps the steps (rc is then
+        CFFC_E_STEP_LIMIT if the cap ends the run first)."""
+        t, n = ctypes.c_double(time), ctypes.c_int(0)
+        rc = load().cffc_run_time_accurate(self.h, ctypes.c_double(t_max), max_steps, ctypes.c_double(freeze_level),
+                                           ctypes.byref(t), ctypes.byref(n))
+        if rc != abi.E_STEP_LIMIT:
+            _check(rc, "cffc_run_time_accurate")
         return rc, t.value, n.value
 
     def dUdt_Multistage_Explicit(self, i_stage):

@@ -468,11 +468,12 @@ def to_mesh(blocks, nbr=None, level=None, nbr2=None):
     return m, np.concatenate(U)
 
 
-def blocks_from_mesh(mesh, U, nsteps=0, time=0.0, cpu_time=0.0, gas="AIR"):
+def blocks_from_mesh(mesh, U, nsteps=0, time=0.0, cpu_time=0.0, gas="AIR", axisymmetric=0):
     """SolnBlocks of a Mesh and its interior state U (ghost cells of U are
     written from the nearest interior cell; the reference rebuilds them by
     messages + BCs after reading).  The grids carry no boundary splines and
-    zeroth-order geometric moments."""
+    zeroth-order geometric moments.  `axisymmetric` is the block's
+    Axisymmetric flag (Euler2DQuad.h:1314), written after the grid."""
     U = np.asarray(U, np.float64).reshape(mesh.ncell, 4)
     co = mesh.cell_offsets()
     header = "%d %.14e  %.14e" % (nsteps, time, cpu_time)
@@ -502,7 +503,7 @@ def blocks_from_mesh(mesh, U, nsteps=0, time=0.0, cpu_time=0.0, gas="AIR"):
             JCu=NG + n_j - 1, xn=x.copy(), yn=y.copy(), cells=cells,
             cell_tails=["    0\n   0   0                        1\n\n"] * (NCi * NCj),
             bcN=np.asarray(bcd["N"]), bcS=np.asarray(bcd["S"]), bcE=np.asarray(bcd["E"]), bcW=np.asarray(bcd["W"]),
-            grid_tail=grid_tail, axisymmetric=0, U=Ub[jj][:, ii], WoW=wo[:NCj], WoE=wo[NCj:2 * NCj],
+            grid_tail=grid_tail, axisymmetric=int(axisymmetric), U=Ub[jj][:, ii], WoW=wo[:NCj], WoE=wo[NCj:2 * NCj],
             WoS=wo[2 * NCj:2 * NCj + NCi], WoN=wo[2 * NCj + NCi:], tail="0\n"))
     return out
 
@@ -518,10 +519,10 @@ def _centroid_area(x, y):
     return xs / (3 * a), ys / (3 * a), 0.5 * a
 
 
-def write_restart(prefix, mesh, U, nsteps=0, time=0.0, cpu_time=0.0, gas="AIR"):
+def write_restart(prefix, mesh, U, nsteps=0, time=0.0, cpu_time=0.0, gas="AIR", axisymmetric=0):
     """<prefix>_blkNNNNNN.soln for every block and, for meshes built by
     mesh.cartesian or mesh.quadtree_box, <prefix>_quadtree.tree."""
-    blocks = blocks_from_mesh(mesh, U, nsteps, time, cpu_time, gas)
+    blocks = blocks_from_mesh(mesh, U, nsteps, time, cpu_time, gas, axisymmetric)
     paths = []
     for b, B in enumerate(blocks):
         p = "%s_blk%06d.soln" % (prefix, b)
@@ -604,16 +605,28 @@ def read_restart(prefix):
 
 def checkpoint(prefix, solver, nsteps=0, time=0.0, cpu_time=0.0, gas="AIR"):
     """Write_Restart_Solution (+ Write_QuadTree) for a running
-    euler2d.Euler2DQuadBlocks: its mesh and current conserved state."""
-    return write_restart(prefix, solver.mesh, solver.get_U(), nsteps, time, cpu_time, gas)
+    euler2d.Euler2DQuadBlocks: its mesh, current conserved state and the
+    scheme's Axisymmetric flag."""
+    return write_restart(prefix, solver.mesh, solver.get_U(), nsteps, time, cpu_time, gas,
+                         axisymmetric=solver.scheme.axisymmetric)
 
 
 def resume(prefix, scheme=None):
     """Read_Restart_Solution (+ Read_QuadTree): a new Euler2DQuadBlocks on
     the restart's mesh with its state loaded (U, then W = W(U)), and the
-    step count and time of the file."""
-    from . import euler2d
+    step count and time of the file.  The blocks' Axisymmetric flag
+    (Euler2DQuad.h:1347) sets the scheme's when no scheme is given; a given
+    scheme that disagrees with it is refused."""
+    from . import abi, euler2d
     blocks, _, m, U = read_restart(prefix)
+    axi = blocks[0].axisymmetric
+    if any(B.axisymmetric != axi for B in blocks):
+        raise ValueError("restart blocks disagree on the Axisymmetric flag")
+    if scheme is None:
+        scheme = abi.Scheme(axisymmetric=axi)
+    elif int(scheme.axisymmetric) != axi:
+        raise ValueError("scheme axisymmetric=%d but the restart was written with Axisymmetric=%d"
+                         % (scheme.axisymmetric, axi))
     g = euler2d.Euler2DQuadBlocks(m, scheme)
     g.set_U(U)
     return g, blocks[0].nsteps, blocks[0].time

@@ -37,6 +37,7 @@ extern "C" {
                                    reports these as 2200-2205 from
                                    Load_Send_Message_Buffers_NoResChange)     */
 #define CFFC_E_NODEVICE   (-5)  /* no HIP device visible                      */
+#define CFFC_E_STEP_LIMIT (-6)  /* a step cap ended a run before its end time  */
 
 /* ---- reference constants (src_2D/CFD/CFD.h) ---------------------------- */
 /* boundary-condition codes, CFD.h:273-351 */
@@ -220,14 +221,23 @@ int  cffc_cfl(cffc_ctx *ctx, double *dt_min, int set_global);
 /* Set_Global_TimeStep (Euler2DQuadMultiBlock.cc:994): every cell's dt and
    the global step become dt (the time-accurate loop's clipped step) */
 int  cffc_set_global_timestep(cffc_ctx *ctx, double dt);
-/* The time-accurate explicit loop (Euler2DQuadSolvers.cc:549-898 with
-   Time_Accurate = 1, global time stepping): per step CFL + global minimum,
-   the step clipped so that Time reaches t_max exactly (:662-680), then the
-   stages through the calls above; Time += CFL*dTime (times
-   MultiStage_Optimally_Smoothing(N, N, limiter) for that scheme, :884-896);
-   stops when Time >= t_max or after max_steps.  *time / *steps_done may be
-   NULL; a nonzero return is the update's failing-cell code. */
-int  cffc_run_time_accurate(cffc_ctx *ctx, double t_max, int max_steps, double *time, int *steps_done);
+/* The time-accurate explicit loop (Euler2DQuadSolvers.cc:526-898 with
+   Time_Accurate = 1, global time stepping), from *time (0, or the Time of a
+   restart) to t_max: per step CFL + global minimum, the step clipped by the
+   reference's if / else-if (:662-680; the second branch, with
+   MultiStage_Optimally_Smoothing(N, N, limiter), applies to every scheme),
+   Set_Global_TimeStep, Freeze_Limiters when freeze_level > 0 and the lagged
+   L2 norm of the density residual is <= freeze_level (not on the first
+   step, :796-803), then per stage exchange + BCs, dUdt, flux corrections,
+   residual smoothing (when set) and the update; Time += CFL*dTime (times
+   MultiStage_Optimally_Smoothing(N, N, limiter) for that scheme, :884-896).
+   Stops when Time >= t_max, like the reference, which has no step cap;
+   max_steps > 0 adds one and returns CFFC_E_STEP_LIMIT when it ends the run
+   first.  *time (required) receives the time reached, *steps_done (may be
+   NULL) the steps taken; a positive return is the update's failing-cell
+   code. */
+int  cffc_run_time_accurate(cffc_ctx *ctx, double t_max, int max_steps, double freeze_level, double *time,
+                            int *steps_done);
 
 /* == dUdt_Multistage_Explicit(blocks, global, list, IP, i_stage)
       (Euler2DQuadMultiBlock.cc:1346 -> Euler2DQuad.cc:1235). */

@@ -259,6 +259,22 @@ class Oracle:
         lib().orc_get_block_W_padded(self.h, b, _p(out))
         return out
 
+    def set_block_W_padded(self, b, W):
+        """Set block b's padded W, ghosts included (no messages or BCs)."""
+        W = np.ascontiguousarray(W, dtype=np.float64)
+        assert W.shape == (self.mesh.nj[b] + 4, self.mesh.ni[b] + 4, 4)
+        lib().orc_set_block_W_padded(self.h, b, _p(W))
+
+    def reconstruct_block(self, b):
+        """The least-squares reconstruction of every cell dUdt reconstructs."""
+        lib().orc_reconstruct_block(self.h, b)
+
+    def block_recon_padded(self, b, which):
+        """Padded dWdx (0), dWdy (1) or phi (2) of block b."""
+        out = np.zeros((self.mesh.nj[b] + 4, self.mesh.ni[b] + 4, 4))
+        lib().orc_get_block_recon_padded(self.h, b, which, _p(out))
+        return out
+
     def block_geometry(self, b):
         nci, ncj = self.mesh.ni[b] + 4, self.mesh.nj[b] + 4
         xc, yc, A = (np.zeros((ncj, nci)) for _ in range(3))

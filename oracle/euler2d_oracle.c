@@ -1585,6 +1585,27 @@ int orc_get_block_W_padded(orc_ctx *ctx, int b, double *W) {
     memcpy(W, B->W, sizeof(pst) * B->nci * B->ncj);
     return 0;
 }
+/* Test access to one block's reconstruction: set its padded W (ghosts
+   included, no BCs or messages), run Linear_Reconstruction_LeastSquares over
+   the cells block_dudt reconstructs (SingleBlock.cc:4467-4482), and read
+   the padded dWdx / dWdy / phi back (which = 0, 1, 2). */
+int orc_set_block_W_padded(orc_ctx *ctx, int b, const double *W) {
+    oblk *B = &ctx->b[b];
+    memcpy(B->W, W, sizeof(pst) * B->nci * B->ncj);
+    return 0;
+}
+int orc_reconstruct_block(orc_ctx *ctx, int b) {
+    oblk *B = &ctx->b[b];
+    for (int j = B->JCl - B->ng + 1; j <= B->JCu + B->ng - 1; ++j)
+        for (int i = B->ICl - B->ng + 1; i <= B->ICu + B->ng - 1; ++i) lsq_reconstruct(B, i, j, ctx->s.i_limiter);
+    return 0;
+}
+int orc_get_block_recon_padded(orc_ctx *ctx, int b, int which, double *out) {
+    oblk *B = &ctx->b[b];
+    const pst *src = which == 0 ? B->dWdx : which == 1 ? B->dWdy : B->phi;
+    memcpy(out, src, sizeof(pst) * B->nci * B->ncj);
+    return 0;
+}
 /* cell geometry of one block (padded): xc, yc, A */
 int orc_get_block_geometry(orc_ctx *ctx, int b, double *xc, double *yc, double *A) {
     oblk *B = &ctx->b[b];

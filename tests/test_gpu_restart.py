@@ -42,3 +42,44 @@ def test_ringleb_restart_state_matches_oracle():
                                                                                     norms[-1, 1]))
     assert rel <= 1e-12
     gpu.close()
+
+
+def test_axisymmetric_checkpoint_resume_continues_the_run(tmp_path):
+    """Checkpoint an axisymmetric run after 6 steps, resume it with the
+    scheme taken from the file (its Axisymmetric flag) and run 6 more: the
+    state follows the uninterrupted 12-step run (the restart text keeps 15
+    significant digits, hence a tolerance, not bit identity), while the same
+    file resumed as a planar run departs from it."""
+    from cffc_amd import euler2d as E
+    from cffc_amd import mesh as M
+    m = M.cartesian(2, 2, 8, 8, y0=0.2, y1=1.2, jitter=0.05, seed=4)
+    W0 = M.uniform_state(m, M.freestream(0.3))
+    c = m.cell_centres()
+    W0[:, 0] *= 1.0 + 0.1 * np.exp(-((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.7) ** 2) / 0.02)
+    m.set_wo_from_initial(W0)
+    s = abi.Scheme(axisymmetric=1)
+    ref = E.Euler2DQuadBlocks(m, s)
+    ref.set_W(W0)
+    assert ref.run_steps(12)[0] == 0
+    a = E.Euler2DQuadBlocks(m, s)
+    a.set_W(W0)
+    assert a.run_steps(6)[0] == 0
+    prefix = str(tmp_path / "axi")
+    restart.checkpoint(prefix, a, nsteps=6)
+    a.close()
+    g, n, _ = restart.resume(prefix)
+    assert g.scheme.axisymmetric == 1 and n == 6
+    assert g.run_steps(6)[0] == 0
+    Ur, Ug = ref.get_U(), g.get_U()
+    rel = float(np.max(np.abs(Ug - Ur) / np.abs(Ur)))
+    # the file's state and mesh advanced as a planar flow (what resume() did
+    # before it read the flag): clearly not the same run
+    _, _, m2, U2 = restart.read_restart(prefix)
+    planar = E.Euler2DQuadBlocks(m2, abi.Scheme())
+    planar.set_U(U2)
+    assert planar.run_steps(6)[0] == 0
+    rel_planar = float(np.max(np.abs(planar.get_U() - Ur) / np.abs(Ur)))
+    print("axisymmetric resume: rel %.3e (resumed as planar: %.3e)" % (rel, rel_planar))
+    assert rel <= 1e-10 and rel_planar > 1e-6
+    for x in (ref, g, planar):
+        x.close()

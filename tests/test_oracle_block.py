@@ -189,6 +189,25 @@ def test_boundary_ghosts_by_definition_on_straight_walls():
     np.testing.assert_allclose(P[nj + 3, cols], P[nj, cols] * flip_v, rtol=1e-15)
 
 
+def test_periodic_ghosts_by_definition():
+    """BC_PERIODIC wraps the block onto itself (SingleBlock.cc:2421-2425,
+    2579-2583, 2734-2738, 2891-2895): W ghost g takes cell ICu-g+1, E ghost g
+    cell ICl+g-1, and the S/N loop, which runs over whole rows after the W/E
+    one (:2362, :2678), wraps rows, so the corners come out doubly periodic."""
+    P_ = abi.BC_PERIODIC
+    m = M.cartesian(1, 1, 5, 4, bcs=dict(W=P_, E=P_, S=P_, N=P_))
+    W = np.random.default_rng(12).uniform(0.5, 1.5, size=(m.ncell, 4))
+    o = oracle.Oracle(m, abi.Scheme())
+    o.set_W(W)
+    o.exchange_and_bcs()
+    P = o.block_W_padded(0)
+    ni, nj = 5, 4
+    inner = P[2:nj + 2, 2:ni + 2]
+    # the padded block equals the interior tiled periodically
+    wrap = np.take(np.take(inner, (np.arange(ni + 4) - 2) % ni, axis=1), (np.arange(nj + 4) - 2) % nj, axis=0)
+    np.testing.assert_array_equal(P, wrap)
+
+
 def test_residual_norms_closed_form():
     m = M.cartesian(2, 2, 6, 6, jitter=0.1, seed=8)
     o = _stage1(m, _smooth_W(m.cell_centres(), seed=6))
@@ -287,3 +306,130 @@ def test_set_global_timestep_sets_every_cell():
     assert np.ptp(o.get_dt()) > 0
     o.set_global_dt(0.5 * dtmin)
     np.testing.assert_array_equal(o.get_dt(), 0.5 * dtmin)
+
+
+# Reduced least-squares stencils of the first ghost ring at physical
+# boundaries, as the reference lists them (Euler2DQuadSingleBlock.cc:3692-3906,
+# the same table as the Green-Gauss one at :3195-3410): offsets (di, dj) of
+# the points used, per side, per BC class and per position along the side.
+# "extrap" = PERIODIC, CONSTANT_EXTRAPOLATION, LINEAR_EXTRAPOLATION,
+# CHARACTERISTIC; "other" = any other physical BC.
+_ALL8 = {(-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)}
+_REF_STENCILS = {
+    ("W", "extrap"): ({(-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)}, {(-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0)}, _ALL8),
+    ("W", "other"): ({(1, 0), (0, 1), (1, 1)}, {(0, -1), (1, -1), (1, 0)}, {(0, -1), (1, -1), (1, 0), (0, 1), (1, 1)}),
+    ("E", "extrap"): ({(-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)}, {(-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0)}, _ALL8),
+    ("E", "other"): ({(-1, 0), (-1, 1), (0, 1)}, {(-1, -1), (0, -1), (-1, 0)},
+                     {(-1, -1), (0, -1), (-1, 0), (-1, 1), (0, 1)}),
+    ("S", "extrap"): ({(0, -1), (1, -1), (1, 0), (0, 1), (1, 1)}, {(-1, -1), (0, -1), (-1, 0), (-1, 1), (0, 1)}, _ALL8),
+    ("S", "other"): ({(1, 0), (0, 1), (1, 1)}, {(-1, 0), (-1, 1), (0, 1)}, {(-1, 0), (1, 0), (-1, 1), (0, 1), (1, 1)}),
+    ("N", "extrap"): ({(0, -1), (1, -1), (1, 0), (0, 1), (1, 1)}, {(-1, -1), (0, -1), (-1, 0), (-1, 1), (0, 1)}, _ALL8),
+    ("N", "other"): ({(0, -1), (1, -1), (1, 0)}, {(-1, -1), (0, -1), (-1, 0)},
+                     {(-1, -1), (0, -1), (1, -1), (-1, 0), (1, 0)}),
+}
+
+
+@pytest.mark.parametrize("bc,cls", [(abi.BC_CONSTANT_EXTRAPOLATION, "extrap"), (abi.BC_CHARACTERISTIC, "extrap"),
+                                    (abi.BC_PERIODIC, "extrap"), (abi.BC_REFLECTION, "other"),
+                                    (abi.BC_FIXED, "other")])
+def test_boundary_ring_reduced_stencils(bc, cls):
+    """Which cells the reconstruction of each first-ring ghost cell reads:
+    on a linear field every stencil gives the exact gradient; perturbing one
+    neighbour changes the gradient exactly when that neighbour is in the
+    stencil.  The sets found must be the reference's table above; the
+    outermost ring and the first ring's corners get no reconstruction."""
+    n = 6
+    m = M.cartesian(1, 1, n, n, bcs=dict(W=bc, E=bc, S=bc, N=bc), jitter=0.1, seed=9)
+    W0 = M.uniform_state(m, M.freestream(0.4))
+    m.set_wo_from_initial(W0)
+    o = oracle.Oracle(m, abi.Scheme())
+    xc, yc, _ = o.block_geometry(0)
+    grad = np.array([[0.3, -0.2], [0.1, 0.05], [-0.07, 0.2], [0.4, 0.1]])
+    base = np.stack([1.0 + grad[v, 0] * xc + grad[v, 1] * yc for v in range(4)], axis=-1)
+    o.set_block_W_padded(0, base)
+    o.reconstruct_block(0)
+    gx0, gy0 = o.block_recon_padded(0, 0), o.block_recon_padded(0, 1)
+    lo, hi = 2, n + 1                      # ICl, ICu (= JCl, JCu)
+    ring = [(i, j) for j in range(1, n + 3) for i in range(1, n + 3) if i in (1, n + 2) or j in (1, n + 2)]
+    for i, j in ring:
+        corner = i in (1, n + 2) and j in (1, n + 2)
+        if corner:
+            assert not gx0[j, i].any() and not gy0[j, i].any(), (i, j)
+            continue
+        np.testing.assert_allclose(gx0[j, i], grad[:, 0], rtol=0, atol=1e-11)
+        np.testing.assert_allclose(gy0[j, i], grad[:, 1], rtol=0, atol=1e-11)
+    for j in range(n + 4):                 # outermost ring: n_pts = 0
+        for i in range(n + 4):
+            if i in (0, n + 3) or j in (0, n + 3):
+                assert not gx0[j, i].any() and not gy0[j, i].any()
+    for i, j in ring:
+        if i in (1, n + 2) and j in (1, n + 2):
+            continue
+        side = "W" if i == 1 else "E" if i == n + 2 else "S" if j == 1 else "N"
+        along = j if side in "WE" else i
+        pos = 0 if along == lo else 1 if along == hi else 2
+        used = set()
+        for di, dj in _ALL8:
+            W = base.copy()
+            W[j + dj, i + di, 0] += 0.37
+            o.set_block_W_padded(0, W)
+            o.reconstruct_block(0)
+            if o.block_recon_padded(0, 0)[j, i, 0] != gx0[j, i, 0] or \
+                    o.block_recon_padded(0, 1)[j, i, 0] != gy0[j, i, 0]:
+                used.add((di, dj))
+        assert used == _REF_STENCILS[(side, cls)][pos], (side, cls, i, j, sorted(used))
+
+
+def _lts_halving_case():
+    """A strong symmetric expansion (u = -/+3, p = 0.1) under scalar local
+    time stepping at CFL 0.9: some RK4 stage updates leave rho, E or e <= 0."""
+    m = M.cartesian(4, 1, 8, 8, x1=1.0, y1=0.25)
+    c = m.cell_centres()
+    W0 = np.zeros((m.ncell, 4))
+    W0[:, 0], W0[:, 3] = 1.0, 0.1
+    W0[:, 1] = np.where(c[:, 0] < 0.5, -3.0, 3.0)
+    m.set_wo_from_initial(W0)
+    s = abi.Scheme(local_time_stepping=abi.SCALAR_LOCAL_TIME_STEPPING, cfl_number=0.9)
+    return m, W0, s
+
+
+def _unphysical(U):
+    e = U[:, 3] / U[:, 0] - 0.5 * (U[:, 1] ** 2 + U[:, 2] ** 2) / U[:, 0] ** 2
+    return (U[:, 0] <= 0) | (U[:, 3] <= 0) | (e <= 0)
+
+
+def test_local_time_stepping_halving_rule():
+    """Update_Solution_Multistage_Explicit's scalar-local-time-stepping
+    branch (SingleBlock.cc:5387-5425): a cell whose update is unphysical has
+    its dt and dUdt[k] scaled by a factor that is halved before each of up to
+    10 tries (so by 1/2, 1/8, 1/64, ... in total) until U = Uo + omega*dUdt is
+    physical.  Checked cell by cell, stage by stage, from the values the
+    oracle holds before and after each update."""
+    m, W0, s = _lts_halving_case()
+    o = oracle.Oracle(m, s)
+    o.set_W(W0)
+    omega = {1: 0.5, 2: 0.5, 3: 1.0, 4: 1.0 / 6.0}
+    totals = [2.0 ** -(t * (t + 1) // 2) for t in range(1, 11)]
+    halved = 0
+    for step in range(12):
+        o.cfl(set_global=False)
+        Uo = o.get_U()
+        for st in range(1, 5):
+            k = 0 if st == 4 else st - 1
+            o.exchange_and_bcs()
+            o.dudt_stage(st)
+            dt_b, d_b = o.get_dt(), o.get_dUdt(k)
+            assert o.update_stage(st)[0] == 0
+            dt_a, d_a, U_a = o.get_dt(), o.get_dUdt(k), o.get_U()
+            r = dt_a / dt_b
+            same = r == 1.0
+            np.testing.assert_array_equal(d_a[same], d_b[same])
+            for c in np.nonzero(~same)[0]:
+                t = totals.index(r[c]) + 1            # the factor is one of the compounded halvings
+                np.testing.assert_array_equal(d_a[c], d_b[c] * r[c])
+                prev = 1.0 if t == 1 else totals[t - 2]
+                assert _unphysical((Uo[c] + omega[st] * (d_b[c] * prev))[None])[0]
+                halved += 1
+            np.testing.assert_array_equal(U_a, Uo + omega[st] * d_a)
+            assert not _unphysical(U_a).any()
+    assert halved > 10

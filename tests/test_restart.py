@@ -147,3 +147,23 @@ def test_grid_file_mesh_is_consistent():
     md = abi.mesh_desc(m, keep)
     counts = (ctypes.c_int64 * 4)()
     assert L.cffc_check_mesh(ctypes.byref(md), counts) == 0, L.cffc_last_error()
+
+
+def test_axisymmetric_flag_round_trips_and_is_enforced(tmp_path):
+    """The Axisymmetric flag each .soln block carries after its grid
+    (Euler2DQuad.h:1314 writes it, :1347 reads it) survives a write / read,
+    and resume() refuses a scheme that disagrees with it (checked before any
+    device work)."""
+    m = M.cartesian(2, 1, 4, 4, y0=0.5, y1=1.5)
+    W = M.uniform_state(m, M.freestream(0.4))
+    U = np.stack([W[:, 0], W[:, 0] * W[:, 1], W[:, 0] * W[:, 2],
+                  W[:, 3] / 0.4 + 0.5 * W[:, 0] * (W[:, 1] ** 2 + W[:, 2] ** 2)], axis=1)
+    for axi in (0, 1):
+        prefix = str(tmp_path / ("axi%d" % axi))
+        restart.write_restart(prefix, m, U, axisymmetric=axi)
+        blocks, _, m2, U2 = restart.read_restart(prefix)
+        assert [B.axisymmetric for B in blocks] == [axi, axi]
+        with open(prefix + "_blk000000.soln") as f:
+            assert restart.format_soln(restart.read_soln(prefix + "_blk000000.soln")) == f.read()
+        with pytest.raises(ValueError):
+            restart.resume(prefix, abi.Scheme(axisymmetric=1 - axi))
