@@ -55,7 +55,10 @@ def test_axisymmetric_checkpoint_resume_continues_the_run(tmp_path):
     m = M.cartesian(2, 2, 8, 8, y0=0.2, y1=1.2, jitter=0.05, seed=4)
     W0 = M.uniform_state(m, M.freestream(0.3))
     c = m.cell_centres()
-    W0[:, 0] *= 1.0 + 0.1 * np.exp(-((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.7) ** 2) / 0.02)
+    bump = np.exp(-((c[:, 0] - 0.5) ** 2 + (c[:, 1] - 0.7) ** 2) / 0.02)
+    # a pressure bump drives a radial velocity, so the source terms act
+    W0[:, 0] *= 1.0 + 0.1 * bump
+    W0[:, 3] *= 1.0 + 0.2 * bump
     m.set_wo_from_initial(W0)
     s = abi.Scheme(axisymmetric=1)
     ref = E.Euler2DQuadBlocks(m, s)
@@ -71,14 +74,16 @@ def test_axisymmetric_checkpoint_resume_continues_the_run(tmp_path):
     assert g.scheme.axisymmetric == 1 and n == 6
     assert g.run_steps(6)[0] == 0
     Ur, Ug = ref.get_U(), g.get_U()
-    rel = float(np.max(np.abs(Ug - Ur) / np.abs(Ur)))
+    # per-variable scale: the transverse momentum is exactly zero in places
+    scale = np.maximum(np.abs(Ur), 1e-3 * np.abs(Ur).max(axis=0))
+    rel = float(np.max(np.abs(Ug - Ur) / scale))
     # the file's state and mesh advanced as a planar flow (what resume() did
     # before it read the flag): clearly not the same run
     _, _, m2, U2 = restart.read_restart(prefix)
     planar = E.Euler2DQuadBlocks(m2, abi.Scheme())
     planar.set_U(U2)
     assert planar.run_steps(6)[0] == 0
-    rel_planar = float(np.max(np.abs(planar.get_U() - Ur) / np.abs(Ur)))
+    rel_planar = float(np.max(np.abs(planar.get_U() - Ur) / scale))
     print("axisymmetric resume: rel %.3e (resumed as planar: %.3e)" % (rel, rel_planar))
     assert rel <= 1e-10 and rel_planar > 1e-6
     for x in (ref, g, planar):
