@@ -42,7 +42,8 @@ _f64p = ctypes.POINTER(ctypes.c_double)
 class MeshDesc(ctypes.Structure):
     _fields_ = [("nblk", ctypes.c_int32), ("nghost", ctypes.c_int32),
                 ("ni", _i32p), ("nj", _i32p), ("xnode", _f64p), ("ynode", _f64p),
-                ("nbr", _i32p), ("bc", _i32p), ("wo", _f64p), ("level", _i32p), ("nbr2", _i32p)]
+                ("nbr", _i32p), ("bc", _i32p), ("wo", _f64p), ("level", _i32p), ("nbr2", _i32p),
+                ("lattice", _i32p)]
 
 
 class SchemeDesc(ctypes.Structure):
@@ -101,9 +102,14 @@ def mesh_desc(mesh, keep):
     nbr2 = getattr(mesh, "nbr2", None)
     lv = _keep(level, np.int32, keep).ctypes.data_as(_i32p) if level is not None else None
     n2 = _keep(nbr2, np.int32, keep).ctypes.data_as(_i32p) if nbr2 is not None else None
+    # lattice positions are passed for meshes that need them (blocks of
+    # uneven size); quadtree_box meshes record them but use the neighbour tables
+    lat = getattr(mesh, "lattice", None)
+    use_lat = lat is not None and (mesh.info.get("kind") != "quadtree" or mesh.info.get("use_lattice", False))
+    la = _keep(lat, np.int32, keep).ctypes.data_as(_i32p) if use_lat else None
     return MeshDesc(int(mesh.nblk), 2, ni.ctypes.data_as(_i32p), nj.ctypes.data_as(_i32p),
                     xn.ctypes.data_as(_f64p), yn.ctypes.data_as(_f64p), nbr.ctypes.data_as(_i32p),
-                    bc.ctypes.data_as(_i32p), wo.ctypes.data_as(_f64p), lv, n2)
+                    bc.ctypes.data_as(_i32p), wo.ctypes.data_as(_f64p), lv, n2, la)
 
 
 def scheme_desc(s):

@@ -10,7 +10,9 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/cffc_euler2d.h"
@@ -542,6 +544,192 @@ static int setup_levels(cffc_ctx *c, const cffc_mesh_desc *md, std::vector<int32
     return 0;
 }
 
+// Blocks placed on per-level cell lattices (cffc_mesh_desc.lattice): blocks
+// of any size, several per face.  Every ghost cell is filled from the block
+// that holds its place, by the rule the reference applies between quadtree
+// blocks of equal size (AdaptiveBlock2D_MessagePassing.h:32-1183 same level,
+// 1185-2398 fine to coarse, 2399-5679 coarse to fine): a copy of the
+// same-level cell, the restriction of the 2 x 2 finer cells, or the quarter
+// of the coarser cell from its limited reconstruction; the BCs set ghost
+// cells no block holds.  Coarse faces next to finer cells get the flux
+// correction (MessagePassing.h:8328).  Same-level copies are RestrictOp
+// entries of kind 1 in the restriction launch.
+static int64_t floor_div(int64_t a, int64_t b) { return a >= 0 ? a / b : -((-a + b - 1) / b); }
+
+static int setup_lattice(cffc_ctx *c, const cffc_mesh_desc *md, std::vector<int32_t> &same,
+                         std::vector<RestrictOp> &rops, std::vector<ProlongOp> &pops, std::vector<FluxFixOp> &fops,
+                         std::vector<uint8_t> &ustore, int64_t &nentries) {
+    const int nb = md->nblk;
+    if (!md->level) return fail(CFFC_E_ARG, "a lattice mesh needs block levels");
+    std::vector<int32_t> lev(nb), I0(nb), J0(nb);
+    int64_t g = 0;
+    auto gcd = [](int64_t a, int64_t b) { while (b) { int64_t t = a % b; a = b; b = t; } return a < 0 ? -a : a; };
+    c->max_level = 0;
+    for (int b = 0; b < nb; ++b) {
+        lev[b] = md->level[b];
+        I0[b] = md->lattice[2 * b];
+        J0[b] = md->lattice[2 * b + 1];
+        if (lev[b] < 0 || lev[b] > 24) return fail(CFFC_E_ARG, "block %d: level %d out of range", b, lev[b]);
+        const BlockMeta &m = c->meta_h[b];
+        if ((m.ni & 1) || (m.nj & 1) || (I0[b] & 1) || (J0[b] & 1) || m.ni < 4 || m.nj < 4)
+            return fail(CFFC_E_TOPOLOGY, "block %d: lattice blocks need even ni, nj >= 4 and even origins", b);
+        g = gcd(g, gcd(gcd(m.ni, m.nj), gcd(I0[b], J0[b])));
+        c->max_level = std::max(c->max_level, lev[b]);
+    }
+    if (g <= 0) g = 2;
+    // index: squares of g x g cells of each level -> block
+    auto key = [](int64_t l, int64_t x, int64_t y) {
+        return ((uint64_t)l << 58) ^ ((uint64_t)(x + (1ll << 28)) << 29) ^ (uint64_t)(y + (1ll << 28));
+    };
+    std::unordered_map<uint64_t, int32_t> index;
+    for (int b = 0; b < nb; ++b) {
+        const BlockMeta &m = c->meta_h[b];
+        for (int64_t y = J0[b] / g; y < (J0[b] + m.nj) / g; ++y)
+            for (int64_t x = I0[b] / g; x < (I0[b] + m.ni) / g; ++x)
+                if (!index.emplace(key(lev[b], x, y), b).second)
+                    return fail(CFFC_E_TOPOLOGY, "block %d overlaps another block of level %d", b, lev[b]);
+    }
+    auto locate = [&](int64_t l, int64_t I, int64_t J) -> int {
+        if (l < 0) return -1;
+        auto it = index.find(key(l, floor_div(I, g), floor_div(J, g)));
+        return it == index.end() ? -1 : it->second;
+    };
+    auto cell = [&](int b, int64_t i, int64_t j) { const BlockMeta &m = c->meta_h[b]; return m.coff + j * m.pitch + i; };
+    same.assign(8 * (size_t)nb, -1);
+    c->amr = true;
+    c->fine_faces.assign(nb, 0);
+    std::vector<int32_t> amr(nb, 0);
+    // flux-correction faces first (they set the edge-flux flags and offsets)
+    static const int oppf[4] = {1, 0, 3, 2};
+    struct FixFace { int b, f, q, F; int64_t qf; };
+    std::vector<FixFace> fixes;
+    for (int b = 0; b < nb; ++b) {
+        const BlockMeta &m = c->meta_h[b];
+        const int L = lev[b];
+        for (int f = 0; f < 4; ++f) {
+            const int n = f < 2 ? m.ni : m.nj;
+            for (int q = 0; q < n; ++q) {
+                // the level-L cell across face f of boundary cell q
+                int64_t I, J;
+                if (f == 0) { I = I0[b] + q; J = J0[b] + m.nj; }
+                else if (f == 1) { I = I0[b] + q; J = J0[b] - 1; }
+                else if (f == 2) { I = I0[b] + m.ni; J = J0[b] + q; }
+                else { I = I0[b] - 1; J = J0[b] + q; }
+                if (locate(L, I, J) >= 0) continue;
+                // the two finer cells along the face
+                int64_t Ia = 2 * I, Ja = 2 * J, Ib, Jb;
+                if (f == 0) { Ib = Ia + 1; Jb = Ja; }
+                else if (f == 1) { Ja += 1; Ib = Ia + 1; Jb = Ja; }
+                else if (f == 2) { Ib = Ia; Jb = Ja + 1; }
+                else { Ia += 1; Ib = Ia; Jb = Ja + 1; }
+                const int F = locate(L + 1, Ia, Ja);
+                if (F < 0) continue;
+                if (locate(L + 1, Ib, Jb) != F) return fail(CFFC_E_TOPOLOGY, "block %d face %d: finer cells split", b, f);
+                const int64_t qf = (f < 2) ? Ia - I0[F] : Ja - J0[F];
+                fixes.push_back(FixFace{b, f, q, F, qf});
+                c->fine_faces[b] |= (uint8_t)(1 << f);
+                amr[b] |= AMR_FLUX_N << f;
+                amr[F] |= AMR_FLUX_N << oppf[f];
+            }
+        }
+    }
+    nentries = 0;
+    for (int b = 0; b < nb; ++b) {
+        BlockMeta &m = c->meta_h[b];
+        m.amr = amr[b];
+        m.eoff = 0;
+        if (amr[b] & AMR_FLUX_ANY) {
+            m.eoff = nentries;
+            nentries += 2 * (int64_t)m.ni + 2 * (int64_t)m.nj;
+        }
+    }
+    ustore.assign((size_t)c->ncp, 0);
+    std::map<std::pair<int64_t, int>, int> pidx;   // (coarse cell, fine block) -> prolongation op
+    for (int b = 0; b < nb; ++b) {
+        const BlockMeta m = c->meta_h[b];
+        const int L = lev[b];
+        for (int j = 0; j < m.ncj; ++j)
+            for (int i = 0; i < m.nci; ++i) {
+                if (i >= 2 && i <= m.ni + 1 && j >= 2 && j <= m.nj + 1) continue;
+                const int64_t I = I0[b] + i - 2, J = J0[b] + j - 2;
+                const int64_t dst = cell(b, i, j);
+                const int s = locate(L, I, J);
+                if (s >= 0) {
+                    rops.push_back(RestrictOp{dst, cell(s, I - I0[s] + 2, J - J0[s] + 2), 0, 1});
+                    continue;
+                }
+                const int F = locate(L + 1, 2 * I, 2 * J);
+                if (F >= 0) {
+                    const BlockMeta &fm = c->meta_h[F];
+                    const int64_t s0 = cell(F, 2 * I - I0[F] + 2, 2 * J - J0[F] + 2);
+                    ustore[s0] = ustore[s0 + 1] = ustore[s0 + fm.pitch] = ustore[s0 + fm.pitch + 1] = 1;
+                    c->meta_h[F].amr |= AMR_USTORE;
+                    rops.push_back(RestrictOp{dst, s0, fm.pitch, 0});
+                    continue;
+                }
+                const int C = locate(L - 1, floor_div(I, 2), floor_div(J, 2));
+                if (C >= 0) {
+                    const int64_t Ic = floor_div(I, 2), Jc = floor_div(J, 2);
+                    const int q = (int)(I - 2 * Ic) + 2 * (int)(J - 2 * Jc);
+                    const int ic = (int)(Ic - I0[C] + 2), jc = (int)(Jc - J0[C] + 2);
+                    const int64_t src = cell(C, ic, jc);
+                    auto it = pidx.find({src, b});
+                    if (it == pidx.end()) {
+                        ProlongOp op{};
+                        op.src = src;
+                        op.blk = C;
+                        for (int k = 0; k < 4; ++k) {
+                            op.dst[k] = -1;
+                            host_quarter_centroid(md, c->meta_h[C], ic, jc, k, &op.xq[k], &op.yq[k]);
+                        }
+                        it = pidx.emplace(std::make_pair(src, b), (int)pops.size()).first;
+                        pops.push_back(op);
+                    }
+                    pops[it->second].dst[q] = dst;
+                    continue;
+                }
+                if (locate(L - 2, floor_div(I, 4), floor_div(J, 4)) >= 0 || locate(L + 2, 4 * I, 4 * J) >= 0)
+                    return fail(CFFC_E_TOPOLOGY, "block %d: a ghost cell two levels away", b);
+                // no block holds this ghost cell: a physical boundary, whose BC must set it
+                const bool gi = i < 2 || i > m.ni + 1, gj = j < 2 || j > m.nj + 1;
+                const int32_t *bcb = md->bc + m.bcoff;
+                const int bcv = gi && !gj ? bcb[(i < 2 ? 0 : m.ncj) + j]
+                               : gj && !gi ? bcb[2 * m.ncj + (j < 2 ? 0 : m.nci) + i] : -1;
+                if (bcv == CFFC_BC_NONE)
+                    return fail(CFFC_E_TOPOLOGY, "block %d ghost (%d, %d): no block and no boundary condition", b, i, j);
+            }
+    }
+    for (const ProlongOp &op : pops)
+        for (int k = 0; k < 4; ++k)
+            if (op.dst[k] < 0) return fail(CFFC_E_TOPOLOGY, "block %d: a coarse cell's quarters split", op.blk);
+    // flux corrections in the order N, S, E, W per cell
+    std::map<int64_t, int> at;
+    static const int face_off_n[4] = {0, 1, 2, 2};   // N, S, E, W entries at 0, ni, 2ni, 2ni+nj
+    auto off = [&](const BlockMeta &x, int face) -> int64_t {
+        return x.eoff + (int64_t)face_off_n[face] * x.ni + (face == 3 ? x.nj : 0);
+    };
+    for (const FixFace &fx : fixes) {
+        const BlockMeta &m = c->meta_h[fx.b], &fm = c->meta_h[fx.F];
+        int64_t cc;
+        if (fx.f == 0) cc = cell(fx.b, 2 + fx.q, m.nj + 1);
+        else if (fx.f == 1) cc = cell(fx.b, 2 + fx.q, 2);
+        else if (fx.f == 2) cc = cell(fx.b, m.ni + 1, 2 + fx.q);
+        else cc = cell(fx.b, 2, 2 + fx.q);
+        auto it = at.find(cc);
+        if (it == at.end()) {
+            it = at.emplace(cc, (int)fops.size()).first;
+            fops.push_back(FluxFixOp{cc, 0, fx.b, {0, 0}, {0, 0}});
+        }
+        FluxFixOp &op = fops[it->second];
+        if (op.n >= 2) return fail(CFFC_E_TOPOLOGY, "block %d: a cell with three corrected faces", fx.b);
+        op.own[op.n] = (int32_t)(off(m, fx.f) + fx.q);
+        op.fine[op.n] = (int32_t)(off(fm, oppf[fx.f]) + fx.qf);
+        op.n += 1;
+    }
+    if (nentries > INT32_MAX) return fail(CFFC_E_ARG, "too many edge-flux entries");
+    return 0;
+}
+
 // Block metadata (padded layout, offsets) and the same-level topology and BC
 // checks; host only.
 static int build_meta(cffc_ctx *c, const cffc_mesh_desc *md, int32_t &bcoff, int64_t &noff) {
@@ -642,7 +830,8 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
     std::vector<FluxFixOp> fops;
     std::vector<uint8_t> ustore;
     int64_t nentries = 0;
-    if (int ea = setup_levels(c, md, same, rops, pops, fops, ustore, nentries)) {
+    if (int ea = md->lattice ? setup_lattice(c, md, same, rops, pops, fops, ustore, nentries)
+                             : setup_levels(c, md, same, rops, pops, fops, ustore, nentries)) {
         free_ctx(c);
         return ea;
     }
@@ -796,7 +985,8 @@ extern "C" int cffc_check_mesh(const cffc_mesh_desc *md, int64_t counts[4]) {
     std::vector<uint8_t> ustore;
     int64_t nentries = 0;
     int e = build_meta(c, md, bcoff, noff);
-    if (!e) e = setup_levels(c, md, same, rops, pops, fops, ustore, nentries);
+    if (!e) e = md->lattice ? setup_lattice(c, md, same, rops, pops, fops, ustore, nentries)
+                            : setup_levels(c, md, same, rops, pops, fops, ustore, nentries);
     if (!e && c->amr) {
         static const int opp[8] = {1, 0, 3, 2, 7, 6, 5, 4};
         for (int b = 0; b < md->nblk && !e; ++b)
@@ -813,6 +1003,44 @@ extern "C" int cffc_check_mesh(const cffc_mesh_desc *md, int64_t counts[4]) {
         counts[1] = (int64_t)pops.size();
         counts[2] = (int64_t)fops.size();
         counts[3] = c->amr ? 1 : 0;
+    }
+    delete c;
+    return e;
+}
+
+// The operation lists cffc_create would build for a mesh, host only:
+// kind 0 restrictions, 1 same-level copies (RestrictOp, 24 bytes each),
+// 2 prolongations (ProlongOp), 3 flux corrections (FluxFixOp), as raw structs.
+extern "C" int cffc_mesh_ops(const cffc_mesh_desc *md, int kind, void *out, int64_t cap, int64_t *n) {
+    if (!md || !n || kind < 0 || kind > 3) return fail(CFFC_E_ARG, "bad argument");
+    if (md->nblk <= 0 || md->nghost != 2) return fail(CFFC_E_ARG, "nblk must be > 0 and nghost == 2");
+    cffc_ctx *c = new cffc_ctx();
+    c->nblk = md->nblk;
+    c->s.n_stage = 1;
+    int32_t bcoff = 0;
+    int64_t noff = 0, nentries = 0;
+    std::vector<int32_t> same;
+    std::vector<RestrictOp> rops, sel;
+    std::vector<ProlongOp> pops;
+    std::vector<FluxFixOp> fops;
+    std::vector<uint8_t> ustore;
+    int e = build_meta(c, md, bcoff, noff);
+    if (!e) e = md->lattice ? setup_lattice(c, md, same, rops, pops, fops, ustore, nentries)
+                            : setup_levels(c, md, same, rops, pops, fops, ustore, nentries);
+    if (!e) {
+        const void *src = nullptr;
+        size_t sz = 0, cnt = 0;
+        if (kind <= 1) {
+            for (const RestrictOp &o : rops)
+                if (o.kind == kind) sel.push_back(o);
+            src = sel.data(); sz = sizeof(RestrictOp); cnt = sel.size();
+        } else if (kind == 2) {
+            src = pops.data(); sz = sizeof(ProlongOp); cnt = pops.size();
+        } else {
+            src = fops.data(); sz = sizeof(FluxFixOp); cnt = fops.size();
+        }
+        *n = (int64_t)cnt;
+        if (out && cap > 0) memcpy(out, src, sz * std::min<size_t>(cnt, (size_t)cap));
     }
     delete c;
     return e;
@@ -900,6 +1128,8 @@ static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long lon
     p.no_bcs = 0;
     p.store_U = 0;
     p.Wf = nullptr;
+    p.W2 = nullptr;
+    p.Wf2 = nullptr;
     return p;
 }
 
@@ -912,7 +1142,10 @@ static int ensure_phi(cffc_ctx *c);
 static int amr_legs(cffc_ctx *c, const double *U, double *W, float *Wf, const int32_t *part_state,
                     const int32_t *stop_flag) {
     if (!c->amr) return 0;
-    HIPCHK(cffc_launch_restrict(c->rops, c->nrops, U, c->geom, c->ncp, Wf ? nullptr : W, Wf, stop_flag, c->stream));
+    // same-level copies (lattice meshes) take W(U) where U is the current
+    // state: the first stage and the granular path
+    HIPCHK(cffc_launch_restrict(c->rops, c->nrops, U, c->geom, c->ncp, Wf ? nullptr : W, Wf, stop_flag, U == c->U,
+                                c->stream));
     HIPCHK(cffc_launch_prolong(c->pops, c->npops, c->meta, c->geom, c->phi, c->freeze, part_state, c->ncp,
                                Wf ? nullptr : W, Wf, stop_flag, c->stream));
     return 0;
@@ -1178,6 +1411,12 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         if (int e = amr_legs(c, st == 1 ? c->U : c->Ustage, c->W[c->wcur], c->Wf[c->wcur],
                              parts ? c->part_state : nullptr, c->stop_flag))
             return e;
+        if (c->amr) {
+            // the next stage's coarse-to-fine messages read BC ghosts of this
+            // stage's BCs call, as in the reference's single W array
+            if (c->Wf[0]) ex.Wf2 = c->Wf[1 - c->wcur];
+            else ex.W2 = c->W[1 - c->wcur];
+        }
         if (c->push_ok) {
             // same-level ghosts were pushed by the previous stage kernel (or
             // filled by begin_run); only the physical BCs remain

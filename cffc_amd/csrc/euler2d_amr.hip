@@ -46,10 +46,22 @@ template <typename ST> __device__ __forceinline__ void amr_stW(ST *W, int64_t nc
 template <typename ST>
 __global__ __launch_bounds__(64) void restrict_kernel(const RestrictOp *ops, int nops, const double *U,
                                                        const double *geom, int64_t ncp, ST *W,
-                                                       const int32_t *stop_flag) {
+                                                       const int32_t *stop_flag, int copy_from_U) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nops || (stop_flag && *stop_flag)) return;
     const RestrictOp o = ops[q];
+    if (o.kind == 1) {
+        // same-level ghost of a lattice mesh: the neighbour cell's W, from its
+        // U where the reference transfers U (the first stage: W = W(U))
+        if (copy_from_U) {
+            const CS<double> Us = {U[o.src], U[ncp + o.src], U[2 * ncp + o.src], U[3 * ncp + o.src]};
+            amr_stW(W, ncp, o.dst, W_of_U(Us));
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) W[v * ncp + o.dst] = W[v * ncp + o.src];
+        }
+        return;
+    }
     const int64_t c00 = o.src, c10 = o.src + 1, c01 = o.src + o.pitch, c11 = o.src + o.pitch + 1;
     const double *A = geom + GEO_A * ncp;
     const double A00 = A[c00], A10 = A[c10], A01 = A[c01], A11 = A[c11];
@@ -176,16 +188,16 @@ using namespace cffc;
 
 extern "C" hipError_t cffc_launch_restrict(const RestrictOp *ops, int nops, const double *U, const double *geom,
                                            int64_t ncp, double *W, float *Wf, const int32_t *stop_flag,
-                                           hipStream_t st) {
+                                           int copy_from_U, hipStream_t st) {
     if (nops <= 0) return hipSuccess;
     // one wavefront per workgroup: short lists spread over more CUs
     const int nb = (nops + 63) / 64;
     if (Wf)
         hipLaunchKernelGGL((restrict_kernel<float>), dim3(nb), dim3(64), 0, st, ops, nops, U, geom, ncp, Wf,
-                           stop_flag);
+                           stop_flag, copy_from_U);
     else
         hipLaunchKernelGGL((restrict_kernel<double>), dim3(nb), dim3(64), 0, st, ops, nops, U, geom, ncp, W,
-                           stop_flag);
+                           stop_flag, copy_from_U);
     return hipGetLastError();
 }
 

@@ -59,10 +59,11 @@ struct TileDesc {
 };
 
 // Resolution-change operations (euler2d_amr.hip), built by the host per mesh.
-// restriction: coarse ghost dst <- fine cells src, src+1, src+pitch, src+pitch+1
+// kind 0, restriction: coarse ghost dst <- fine cells src, src+1, src+pitch,
+// src+pitch+1; kind 1 (lattice meshes): same-level ghost dst <- cell src
 struct RestrictOp {
     int64_t dst, src;
-    int32_t pitch, pad;
+    int32_t pitch, kind;
 };
 // prolongation: coarse cell src of block blk -> its quarters SW, SE, NW, NE,
 // centred at (xq, yq), into fine ghost cells dst

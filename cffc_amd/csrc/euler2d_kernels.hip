@@ -204,6 +204,18 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     }
     if (p.no_bcs) return;   // uniform over the workgroup
     __syncthreads();
+    // With double-buffered W (the fused path) the next stage's buffer also
+    // receives the BC ghost states: the coarse-to-fine messages of the next
+    // stage run before its BCs and read these ghosts, which in the
+    // reference hold the states of the previous BCs call
+    // (Euler2DQuadSolvers.cc:807-826).
+    ST *W2;
+    if constexpr (sizeof(ST) == 4) W2 = p.Wf2;
+    else W2 = p.W2;
+    auto stWb = [&](int64_t c, const PS<double> &r) {
+        stW(W, ncp, c, r);
+        if (W2) stW(W2, ncp, c, r);
+    };
     const int32_t *bcW = p.bc + m.bcoff, *bcE = bcW + m.ncj, *bcS = bcE + m.ncj, *bcN = bcS + m.nci;
     const double *woW = p.wo + 4 * m.bcoff, *woE = woW + 4 * m.ncj, *woS = woE + 4 * m.ncj, *woN = woS + 4 * m.nci;
     auto cidx = [&](int i, int j) { return m.coff + (int64_t)j * m.pitch + i; };
@@ -240,29 +252,29 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
         switch (bcv) {
         case 11000: {  // FIXED
             const PS<double> w0 = wo(west ? woW : woE, j);
-            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            stWb(g1c, w0); stWb(g2c, w0);
             putU(g1c, w0); putU(g2c, w0);
         } break;
         case 7001: {  // REFLECTION: ghost g mirrors cell in-(g-1)*sg
             const PS<double> r1 = reflect(ldW(W, ncp, cidx(in, j)), nx, ny);
             const PS<double> r2 = reflect(ldW(W, ncp, cidx(in - sg, j)), nx, ny);
-            stW(W, ncp, g1c, r1); stW(W, ncp, g2c, r2);
+            stWb(g1c, r1); stWb(g2c, r2);
             putU(g1c, r1); putU(g2c, r2);
         } break;
         case 6000: {  // PERIODIC: W ghosts take ICu-g+1 (E ghosts take ICl+g-1)
             const int64_t s1 = cidx(west ? ICu : ICl, j), s2 = cidx(west ? ICu - 1 : ICl + 1, j);
-            stW(W, ncp, g1c, ldW(W, ncp, s1));
-            stW(W, ncp, g2c, ldW(W, ncp, s2));
+            stWb(g1c, ldW(W, ncp, s1));
+            stWb(g2c, ldW(W, ncp, s2));
             cpyU(g1c, s1); cpyU(g2c, s2);
         } break;
         case 8001: {  // CHARACTERISTIC: first layer from BC_Characteristic_Pressure, second copied
             const PS<double> g1 = bc_characteristic_pressure(ldW(W, ncp, cidx(in, j)), wo(west ? woW : woE, j), nx, ny);
-            stW(W, ncp, g1c, g1); stW(W, ncp, g2c, g1);
+            stWb(g1c, g1); stWb(g2c, g1);
             putU(g1c, g1); cpyU(g2c, g1c);
         } break;
         default: {  // CONSTANT_EXTRAPOLATION (and the reference's default)
             const PS<double> w0 = ldW(W, ncp, cidx(in, j));
-            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            stWb(g1c, w0); stWb(g2c, w0);
             cpyU(g1c, cidx(in, j)); cpyU(g2c, cidx(in, j));
         } break;
         }
@@ -283,29 +295,29 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
         switch (bcv) {
         case 11000: {
             const PS<double> w0 = wo(south ? woS : woN, i);
-            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            stWb(g1c, w0); stWb(g2c, w0);
             putU(g1c, w0); putU(g2c, w0);
         } break;
         case 7001: {
             const PS<double> r1 = reflect(ldW(W, ncp, cidx(i, in)), nx, ny);
             const PS<double> r2 = reflect(ldW(W, ncp, cidx(i, in - sg)), nx, ny);
-            stW(W, ncp, g1c, r1); stW(W, ncp, g2c, r2);
+            stWb(g1c, r1); stWb(g2c, r2);
             putU(g1c, r1); putU(g2c, r2);
         } break;
         case 6000: {
             const int64_t s1 = cidx(i, south ? JCu : JCl), s2 = cidx(i, south ? JCu - 1 : JCl + 1);
-            stW(W, ncp, g1c, ldW(W, ncp, s1));
-            stW(W, ncp, g2c, ldW(W, ncp, s2));
+            stWb(g1c, ldW(W, ncp, s1));
+            stWb(g2c, ldW(W, ncp, s2));
             cpyU(g1c, s1); cpyU(g2c, s2);
         } break;
         case 8001: {
             const PS<double> g1 = bc_characteristic_pressure(ldW(W, ncp, cidx(i, in)), wo(south ? woS : woN, i), nx, ny);
-            stW(W, ncp, g1c, g1); stW(W, ncp, g2c, g1);
+            stWb(g1c, g1); stWb(g2c, g1);
             putU(g1c, g1); cpyU(g2c, g1c);
         } break;
         default: {
             const PS<double> w0 = ldW(W, ncp, cidx(i, in));
-            stW(W, ncp, g1c, w0); stW(W, ncp, g2c, w0);
+            stWb(g1c, w0); stWb(g2c, w0);
             cpyU(g1c, cidx(i, in)); cpyU(g2c, cidx(i, in));
         } break;
         }

@@ -27,6 +27,8 @@ struct ExchangeParams {
     const int32_t *blocks;     // blocks to process (workgroup k -> blocks[k]), or null: workgroup = block
     int32_t bcs_only;          // skip the same-level exchange (ghosts already pushed by the stage kernel)
     int32_t no_bcs;            // same-level exchange only (the resolution-change legs and BCs follow)
+    double *W2;                // if not null, the BC ghost states are also written here (FP64 storage) ...
+    float *Wf2;                // ... or here (FP32 storage): the next stage's W buffer, see exchange_bc_kernel
 };
 
 
@@ -98,7 +100,8 @@ hipError_t cffc_launch_update_tile_mixed(const cffc::StageParams *p, int ti, int
 hipError_t cffc_launch_w_from_u(const cffc::BlockMeta *meta, int nblk, int max_int, const double *U, double *W,
                                 float *Wf, int64_t ncp, hipStream_t st);
 hipError_t cffc_launch_restrict(const cffc::RestrictOp *ops, int nops, const double *U, const double *geom,
-                                int64_t ncp, double *W, float *Wf, const int32_t *stop_flag, hipStream_t st);
+                                int64_t ncp, double *W, float *Wf, const int32_t *stop_flag, int copy_from_U,
+                                hipStream_t st);
 hipError_t cffc_launch_prolong(const cffc::ProlongOp *ops, int nops, const cffc::BlockMeta *meta, const double *geom,
                                const double *phi, int frozen, const int32_t *part_state, int64_t ncp, double *W,
                                float *Wf, const int32_t *stop_flag, hipStream_t st);

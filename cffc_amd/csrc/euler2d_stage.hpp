@@ -188,7 +188,6 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
     gy = gx;
     ph = gx;
     if (!FULL && mask == 0u) return;
-    const double *G = p.geom;
     PS<FT> Dx = {FT(0), FT(0), FT(0), FT(0)}, Dy = Dx;
     FT DxDx = FT(0), DxDy = FT(0), DyDy = FT(0);
     const XT xc0 = sX[jl][il], yc0 = sY[jl][il];

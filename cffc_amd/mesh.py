@@ -35,6 +35,7 @@ class Mesh:
     info: dict = field(default_factory=dict)
     level: np.ndarray = None  # [nblk] int32 quadtree level, or None (one level)
     nbr2: np.ndarray = None   # [nblk, 4] int32 second finer face neighbour, or None
+    lattice: np.ndarray = None  # [nblk, 2] int32 first interior cell on its level's lattice, or None
 
     # --- offsets -----------------------------------------------------------
     def node_offsets(self):
@@ -343,12 +344,16 @@ def quadtree_box(nroot=4, n=8, max_level=2, refine=None, x0=0.0, x1=1.0, y0=0.0,
     N = n + 2 * NG + 1
     X = np.empty((nblk, N, N))
     Y = np.empty((nblk, N, N))
+    # lattice nodes from one integer lattice at the finest level, so that a
+    # node shared by several blocks has the same coordinates in all of them
+    K = (nroot * n) << max_level
     for b, k in enumerate(order):
-        xa, xb, ya, yb = extent(k)
-        hx, hy = (xb - xa) / n, (yb - ya) / n
-        idx = np.arange(-NG, n + NG + 1, dtype=np.float64)
-        X[b] = (xa + hx * idx)[None, :]
-        Y[b] = (ya + hy * idx)[:, None]
+        lv, ix, iy = k
+        sc = 1 << (max_level - lv)
+        hx, hy = Lx / ((nroot * n) << lv), Ly / ((nroot * n) << lv)
+        idx = np.arange(-NG, n + NG + 1, dtype=np.int64)
+        X[b] = (x0 + Lx * (((ix * n + idx) * sc) / K))[None, :]
+        Y[b] = (y0 + Ly * (((iy * n + idx) * sc) / K))[:, None]
         if jitter > 0:
             th = rng.uniform(0.0, 2 * np.pi, (n - 1, n - 1))
             d = jitter * min(hx, hy)
@@ -414,10 +419,15 @@ def quadtree_box(nroot=4, n=8, max_level=2, refine=None, x0=0.0, x1=1.0, y0=0.0,
                 if slot >= 4 and slot in covered:
                     continue
                 if stage == "same" and rel == 0:
-                    for i, j in cells(r):
-                        si, sj = i - dx * n, j - dy * n
-                        put(b, i, j, {(a, c): (X0[o, sj + c, si + a], Y0[o, sj + c, si + a])
-                                      for a in (0, 1) for c in (0, 1)})
+                    # the region's nodes, the block's own nodes excepted
+                    (ia, ib), (ja, jb) = r
+                    ii = np.arange(ia, ib + 2)
+                    jj = np.arange(ja, jb + 2)
+                    own = ((ii >= NG) & (ii <= n + NG))[None, :] & ((jj >= NG) & (jj <= n + NG))[:, None]
+                    src = np.ix_(jj - dy * n, ii - dx * n)
+                    dst = np.ix_(jj, ii)
+                    X[b][dst] = np.where(own, X[b][dst], X0[o][src])
+                    Y[b][dst] = np.where(own, Y[b][dst], Y0[o][src])
                 elif stage == "finer" and rel == 1:
                     for i, j in cells(r):
                         if slot < 4:
@@ -485,8 +495,105 @@ def quadtree_box(nroot=4, n=8, max_level=2, refine=None, x0=0.0, x1=1.0, y0=0.0,
                 xnode=X.reshape(-1), ynode=Y.reshape(-1), nbr=nbr, bc=bc, wo=np.zeros((bc.size, 4)),
                 info={"kind": "quadtree", "nroot": nroot, "n": n, "max_level": max_level, "jitter": jitter,
                       "seed": seed, "box": (x0, x1, y0, y1), "keys": order},
-                level=level, nbr2=nbr2)
+                level=level, nbr2=nbr2,
+                lattice=np.array([(x * n, y * n) for _, x, y in order], np.int32).reshape(-1, 2))
     return mesh
+
+
+def merge_quadtree(mesh, depth=2):
+    """Blocks of uneven size from a quadtree_box mesh: every aligned group of
+    2^k x 2^k leaves of one level (k = depth, then smaller; root blocks
+    group the same way over the root grid) becomes one block of (2^k n)^2
+    cells covering the same cells with the same nodes.  The result has
+    blocks of n^2, (2n)^2, ... (2^depth n)^2 cells at every level and the
+    resolution changes of the quadtree between them; the library derives
+    its messages from the blocks' lattice positions (cffc_mesh_desc.lattice).
+    Returns (merged mesh, cell_map): merged cell k is cell cell_map[k] of
+    the quadtree mesh (state-array order)."""
+    if mesh.info.get("kind") != "quadtree":
+        raise ValueError("merge_quadtree takes a quadtree_box mesh")
+    n = mesh.info["n"]
+    keys = mesh.info["keys"]
+    bid = {k: b for b, k in enumerate(keys)}
+    bo = mesh.bc_offsets()
+    co = mesh.cell_offsets()
+    no = mesh.node_offsets()
+    nc0 = n + 2 * NG
+    Xs = [(mesh.xnode[no[b]:no[b + 1]].reshape(nc0 + 1, nc0 + 1), mesh.ynode[no[b]:no[b + 1]].reshape(nc0 + 1, nc0 + 1))
+          for b in range(mesh.nblk)]
+    bcs = [{"W": mesh.bc[bo[b]:bo[b] + nc0], "E": mesh.bc[bo[b] + nc0:bo[b] + 2 * nc0],
+            "S": mesh.bc[bo[b] + 2 * nc0:bo[b] + 3 * nc0], "N": mesh.bc[bo[b] + 3 * nc0:bo[b] + 4 * nc0]}
+           for b in range(mesh.nblk)]
+    used = set()
+    groups = []
+    for k in range(depth, -1, -1):
+        g = 1 << k
+        for key in keys:
+            lv, x, y = key
+            if key in used or x % g or y % g:
+                continue
+            members = [(lv, x + a, y + c) for c in range(g) for a in range(g)]
+            if all(m in bid and m not in used for m in members):
+                used.update(members)
+                groups.append((lv, x, y, g))
+    # blocks in Morton order of their lower-left corner at the finest level
+    ml = mesh.info["max_level"]
+    groups.sort(key=lambda t: _morton(t[1] << (ml + 1 - t[0]), t[2] << (ml + 1 - t[0])))
+    xs, ys, bc, wo, ni, lat, lev, cmap = [], [], [], [], [], [], [], []
+    for lv, x, y, g in groups:
+        N = g * n
+        M = N + 2 * NG + 1
+        X = np.empty((M, M))
+        Y = np.empty((M, M))
+        for cj in range(g):
+            for ci in range(g):
+                b = bid[(lv, x + ci, y + cj)]
+                win = (slice(cj * n, cj * n + n + 2 * NG + 1), slice(ci * n, ci * n + n + 2 * NG + 1))
+                X[win] = Xs[b][0]
+                Y[win] = Xs[b][1]
+        # every leaf's nodes, its ghost nodes included, are the merged
+        # block's nodes at the same place: the merged block has the same
+        # cells, ghost cells included, as each of its leaves
+        for cj in range(g):
+            for ci in range(g):
+                b = bid[(lv, x + ci, y + cj)]
+                win = (slice(cj * n, cj * n + n + 2 * NG + 1), slice(ci * n, ci * n + n + 2 * NG + 1))
+                if not (np.array_equal(X[win], Xs[b][0]) and np.array_equal(Y[win], Xs[b][1])):
+                    raise ValueError("leaves of a merged block disagree on shared nodes")
+        xs.append(X.ravel())
+        ys.append(Y.ravel())
+        nc = N + 2 * NG
+        # boundary condition codes and reference states along each side
+        for side in ("W", "E", "S", "N"):
+            arr = np.empty(nc, np.int32)
+            st = np.empty((nc, 4))
+            for t in range(nc):
+                cc = min(max((t - NG) // n, 0), g - 1)
+                if side in ("W", "E"):
+                    b = bid[(lv, x + (0 if side == "W" else g - 1), y + cc)]
+                else:
+                    b = bid[(lv, x + cc, y + (0 if side == "S" else g - 1))]
+                k0 = {"W": 0, "E": n + 2 * NG, "S": 2 * (n + 2 * NG), "N": 3 * (n + 2 * NG)}[side]
+                arr[t] = bcs[b][side][t - cc * n]
+                st[t] = mesh.wo[bo[b] + k0 + t - cc * n]
+            bc.append(arr)
+            wo.append(st)
+        ni.append(N)
+        lat.append((x * n, y * n))
+        lev.append(lv)
+        # cells: merged (i, j) <- leaf (ci, cj) cell (i mod n, j mod n)
+        jj, ii = np.meshgrid(np.arange(N), np.arange(N), indexing="ij")
+        leaf = np.array([[bid[(lv, x + a, y + c)] for a in range(g)] for c in range(g)])
+        b_of = leaf[jj // n, ii // n]
+        cmap.append((co[b_of] + (jj % n) * n + (ii % n)).ravel())
+    nb = len(groups)
+    bc = np.concatenate(bc)
+    out = Mesh(nblk=nb, ni=np.array(ni, np.int32), nj=np.array(ni, np.int32), xnode=np.concatenate(xs),
+               ynode=np.concatenate(ys), nbr=np.full((nb, 8), -1, np.int32), bc=bc, wo=np.concatenate(wo),
+               info={"kind": "merged_quadtree", "n": n, "depth": depth, "max_level": ml,
+                     "groups": groups, "from": mesh.info},
+               level=np.array(lev, np.int32), lattice=np.array(lat, np.int32).reshape(-1, 2))
+    return out, np.concatenate(cmap)
 
 
 def circle_refinement(radius=0.3, centre=(0.5, 0.5), band=0.0):
@@ -521,6 +628,8 @@ def concat(meshes):
     """One multi-block mesh holding several disjoint meshes (cases): block
     indices of each part are offset so that neighbour tables stay inside the
     part.  Returns the mesh and the first block of every part."""
+    if any(m.lattice is not None and m.info.get("kind") != "quadtree" for m in meshes):
+        raise ValueError("meshes placed by lattice position cannot be concatenated")
     starts, nb = [], 0
     nbrs = []
     for m in meshes:

@@ -107,7 +107,20 @@ enum { CFFC_NBR_N = 0, CFFC_NBR_S, CFFC_NBR_E, CFFC_NBR_W,
  * prolonged from the coarse block, the coarse side's restricted from the fine
  * blocks, and the coarse face fluxes corrected to the fine ones
  * (AdaptiveBlock2D_MessagePassing.h:1185-2398, 2399-5679, 6284-7882, 8328).
- * Blocks next to a resolution change may not carry physical boundaries.
+ * The coarse-to-fine messages read the coarse block's BC ghosts as the
+ * previous BCs call left them, as in the reference.
+ *   lattice     : 2 ints per block, or NULL (no counterpart in the reference,
+ *                 whose blocks all have the same cell counts): the block's
+ *                 first interior cell (I0, J0) on the cell lattice of its
+ *                 level, where cell (I, J) of level l+1 lies in cell
+ *                 (I>>1, J>>1) of level l (level required).  With it,
+ *                 blocks of any size and position are allowed, and every
+ *                 ghost cell is filled from the block that holds its place:
+ *                 copied from the same level, restricted from the finer
+ *                 level, prolonged from the coarser level (the reference's
+ *                 message rules restated by position), or set by the BCs
+ *                 where no block holds it; nbr and nbr2 are then unused
+ *                 (all -1).  Levels of touching blocks differ by at most 1.
  */
 typedef struct cffc_mesh_desc {
     int32_t        nblk;
@@ -121,6 +134,7 @@ typedef struct cffc_mesh_desc {
     const double  *wo;
     const int32_t *level;
     const int32_t *nbr2;
+    const int32_t *lattice;
 } cffc_mesh_desc;
 
 /* The scheme fields of Euler2D_Input_Parameters read on this path
@@ -179,6 +193,14 @@ void cffc_destroy(cffc_ctx *ctx);
    counts[0..3] receive the numbers of restriction, prolongation and
    flux-correction operations and 1 if the mesh has resolution changes. */
 int  cffc_check_mesh(const cffc_mesh_desc *mesh, int64_t counts[4]);
+
+/* Diagnostic, no reference counterpart: the operation lists cffc_create
+   builds for a mesh, host only.  kind 0: restrictions, 1: same-level copies
+   (lattice meshes), 2: prolongations, 3: flux corrections, as the library's
+   packed records (24, 112 and 32 bytes; layout in
+   cffc_amd/csrc/euler2d_device.hpp).  *n receives the count; at most cap
+   records are copied to out (may be NULL). */
+int  cffc_mesh_ops(const cffc_mesh_desc *mesh, int kind, void *out, int64_t cap, int64_t *n);
 
 /* Number of interior cells over all blocks (the unit of the state arrays). */
 int64_t cffc_num_cells(const cffc_ctx *ctx);
