@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -39,6 +40,8 @@ static int fail(int code, const char *fmt, ...) {
         hipError_t e_ = (x);                                                                   \
         if (e_ != hipSuccess) return fail(CFFC_E_HIP, "%s: %s", #x, hipGetErrorString(e_));    \
     } while (0)
+
+static int resolve_scratch(int dev);
 
 static const unsigned long long MILLION_BITS = 0x412E848000000000ull;
 static const unsigned long long ERR_NONE = ~0ull;
@@ -811,6 +814,10 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
     c->s = *sd;
     c->nblk = md->nblk;
     HIPCHK(hipGetDevice(&c->device));
+    if (int e = resolve_scratch(c->device)) {
+        free_ctx(c);
+        return e;
+    }
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
         free_ctx(c);
         return fail(CFFC_E_HIP, "hipStreamCreate failed");
@@ -1611,6 +1618,79 @@ extern "C" int cffc_run_steps(cffc_ctx *c, int nsteps, double tol, double *norms
     collect_timing(c);
     (void)hipHostFree(hflag);
     return end_run(c, wstart, nsteps, norms, dt_out, steps_done);
+}
+
+// ---- the fused loop without allocations -------------------------------------
+// cffc_run_steps grows per-context device buffers for its per-step records
+// (norms, time steps, step clock) and polls through pinned host memory.  The
+// variant below keeps those records in `scratch`, a file-scope device array
+// of fixed size, and does not poll, so that a caller that may not allocate
+// (graph capture, a serving loop, a harness that forbids it) can advance an
+// existing context.  Every device holds its own copy of the array for the
+// life of the process; host code uses the address hipGetSymbolAddress gave
+// for that device (g_scratch), never the symbol.  Calls that overlap on one
+// device share the array and race.
+static constexpr int NOALLOC_MAX_STEPS = 4096;
+// norms [3][max], dt [max], then the step clock [max + 1] (64-bit words)
+static constexpr size_t NOALLOC_WORDS = 3 * (size_t)NOALLOC_MAX_STEPS + NOALLOC_MAX_STEPS + NOALLOC_MAX_STEPS + 1;
+static __device__ __align__(256) double scratch[NOALLOC_WORDS];
+static constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[NOALLOC_MAX_DEVICES];
+
+// Once per device, with that device current (cffc_create calls it).
+static int resolve_scratch(int dev) {
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return fail(CFFC_E_ARG, "device %d out of range", dev);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[dev]) return 0;
+    void *p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(CFFC_E_HIP, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch[dev] = static_cast<double *>(p);
+    return 0;
+}
+
+extern "C" int cffc_run_steps_noalloc(cffc_ctx *c, int nsteps, double tol, double *norms, double *dt_out,
+                                      int *steps_done) {
+    if (!c || nsteps < 0) return fail(CFFC_E_ARG, "bad argument");
+    if (nsteps > NOALLOC_MAX_STEPS)
+        return fail(CFFC_E_ARG, "%d steps exceed the %d-step record of cffc_run_steps_noalloc", nsteps,
+                    NOALLOC_MAX_STEPS);
+    double *sp = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        if (c->device >= 0 && c->device < NOALLOC_MAX_DEVICES) sp = g_scratch[c->device];
+    }
+    if (!sp) return fail(CFFC_E_HIP, "no scratch address for device %d", c->device);
+    // buffers the run needs must already exist (they are allocated by the
+    // set-up calls); kernel timing records events and is refused
+    if (c->timing) return fail(CFFC_E_ARG, "kernel timing is on: cffc_run_steps_noalloc records no events");
+    if (c->rs_iters > 0 && !c->granular_ready)
+        return fail(CFFC_E_ARG, "residual smoothing buffers are not allocated");
+    if ((c->store_phi || c->freeze || c->amr) && !c->phi)
+        return fail(CFFC_E_ARG, "limiter buffer is not allocated");
+    double *own_norm = c->norm_out, *own_dt = c->dt_out;
+    unsigned long long *own_clock = c->step_clock;
+    const int own_max = c->max_steps;
+    c->norm_out = sp;
+    c->dt_out = sp + 3 * (size_t)NOALLOC_MAX_STEPS;
+    c->step_clock = reinterpret_cast<unsigned long long *>(sp + 4 * (size_t)NOALLOC_MAX_STEPS);
+    c->max_steps = NOALLOC_MAX_STEPS;
+    int rc = begin_run(c, nsteps);
+    const int wstart = c->wcur;
+    for (int n = 0; !rc && n < nsteps; ++n) rc = enqueue_step(c, n & 1, tol);
+    if (!rc) {
+        // converged or failed parts skip their kernels; no host polling
+        (void)hipStreamSynchronize(c->stream);
+        rc = end_run(c, wstart, nsteps, norms, dt_out, steps_done);
+    } else {
+        (void)hipStreamSynchronize(c->stream);
+    }
+    c->norm_out = own_norm;
+    c->dt_out = own_dt;
+    c->step_clock = own_clock;
+    c->max_steps = own_max;
+    return rc;
 }
 
 extern "C" int cffc_run_steps_graph(cffc_ctx *c, int nsteps, double *norms, double *dt_out) {

@@ -65,6 +65,7 @@ def load():
     L.cffc_set_store_limiters.argtypes = [vp, i32]
     L.cffc_run_steps.argtypes = [vp, i32, d, pd, pd, ctypes.POINTER(ctypes.c_int)]
     L.cffc_run_steps_graph.argtypes = [vp, i32, pd, pd]
+    L.cffc_run_steps_noalloc.argtypes = [vp, i32, d, pd, pd, ctypes.POINTER(ctypes.c_int)]
     L.cffc_synchronize.argtypes = [vp]
     L.cffc_stream.argtypes = [vp]
     L.cffc_stream.restype = vp
@@ -251,6 +252,18 @@ class Euler2DQuadBlocks:
             rc = _check(L.cffc_run_steps(self.h, nsteps, tol, _p(norms), _p(dts), ctypes.byref(done)),
                         "cffc_run_steps")
             n = done.value
+        return rc, n, norms[:n], dts[:n]
+
+    def run_steps_noalloc(self, nsteps, tol=0.0):
+        """cffc_run_steps_noalloc: the same steps as run_steps without any
+        allocation in the library (at most 4096 steps).  Returns (rc,
+        steps_done, norms[steps,3], dt[steps])."""
+        norms = np.zeros((max(nsteps, 1), 3))
+        dts = np.zeros(max(nsteps, 1))
+        done = ctypes.c_int(0)
+        rc = _check(load().cffc_run_steps_noalloc(self.h, nsteps, tol, _p(norms), _p(dts), ctypes.byref(done)),
+                    "cffc_run_steps_noalloc")
+        n = done.value
         return rc, n, norms[:n], dts[:n]
 
     # ---- batched steady runs ------------------------------------------------------

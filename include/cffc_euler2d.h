@@ -303,6 +303,20 @@ int  cffc_run_steps(cffc_ctx *ctx, int nsteps, double tol, double *norms,
 int  cffc_run_steps_graph(cffc_ctx *ctx, int nsteps, double *norms,
                           double *dt_out);
 
+/* No counterpart in the reference: cffc_run_steps for callers that cannot
+   allocate (code under HIP graph capture, tight serving loops, harnesses
+   that forbid hipMalloc).  Same steps, results and arguments as
+   cffc_run_steps, but it calls no hipMalloc / hipFree / hipHostMalloc: the
+   per-step records live in a fixed device array of the library (one per
+   device, kept until the process ends), and the host does not poll (steps
+   after convergence or a failure are skipped by the kernels themselves).
+   At most 4096 steps per call (CFFC_E_ARG beyond).  Refused (CFFC_E_ARG)
+   while kernel timing is on, or when the context still lacks a buffer the
+   run needs (limiters, smoothing), which the set-up calls allocate.  Calls
+   that overlap on one device share that array and race. */
+int  cffc_run_steps_noalloc(cffc_ctx *ctx, int nsteps, double tol, double *norms,
+                            double *dt_out, int *steps_done);
+
 /* Implicit residual smoothing (Residual_Smoothing, Euler2DQuadSingleBlock.cc:
    4521-4564, applied between dUdt_Multistage_Explicit and the update as in
    Euler2DQuadSolvers.cc:861-867).  Inputs Residual_Smoothing_Epsilon and
