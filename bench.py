@@ -6,9 +6,11 @@ with a seeded 1% i.i.d. density perturbation, Roe flux, least-squares
 reconstruction, Venkatakrishnan limiter, RK4, CFL 0.5, global time step,
 constant-extrapolation boundaries.  Both precisions are run from the same
 initial state:
+  * fp64:  the reference's arithmetic, bit for bit with the CPU oracle
+    (reported as `value`, `dtype` f64, and `roofline`);
   * mixed: FP32 reconstruction/limiter/Roe flux, FP64 state/update/dt/norms
-    (reported as `value`);
-  * fp64:  the path that agrees with the CPU oracle bit for bit.
+    (sub-record `mixed`, with its own roofline), and its L1/Linf error
+    against FP64 after 500 steps (`mixed_vs_fp64`).
 A "step" is one RK4 time step (4 stages, each an exchange/BC kernel plus the
 fused residual/update kernel); cell-stage updates = cells x 4 x steps.  The
 timed steps are replayed from a captured hipGraph with the state resident in
@@ -17,13 +19,16 @@ HBM.
 roofline: 162 algorithmic bytes per cell-stage (FP64 state read/written once
 per stage, averaged over the four RK4 stages) x cells per launch / the fused
 stage kernel's average duration, measured with HIP event pairs on the
-context stream in a separate run of the same workload.
+context stream in a separate run of the same workload.  traffic: HBM bytes
+per launch of that kernel and tile shape from rocprofv3 FETCH_SIZE /
+WRITE_SIZE runs (profiles/), or null for a tile that was not profiled.
 
-cpu_baseline: the CPU oracle (a C restatement of the reference algorithm,
--O2, no FMA), one pthread per core over a contiguous block partition like
-the reference's one-MPI-rank-per-core decomposition, timed on a bounded
-sample of the same workload (a sub-domain of 32 x 32 blocks of 64^2 cells,
-32 steps, about 10 s on 16 threads).
+Further legs: the quadtree workload (resolution changes), the steady bump
+cases cold vs warm (single grid and FAS multigrid), batched serving of 256
+cases, and cpu_baseline: the CPU oracle (a C restatement of the reference
+algorithm, -O2, no FMA), one pthread per core over a contiguous block
+partition like the reference's one-MPI-rank-per-core decomposition, timed
+on bounded samples of the same workload on 1 core and on N cores.
 
 python bench.py [--gpus 1] [--steps 500] [--warmup 20]
 """
@@ -98,21 +103,84 @@ def kernel_time(m, W0, precision, steps, tile=None):
     return t
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(threads):
+    """The CPU oracle on bounded samples of the workload: 1 thread on 8 x 8
+    blocks of 64^2 cells (262,144 cells, 8 steps), and `threads` threads on
+    32 x 32 blocks of 64^2 (4,194,304 cells, 32 steps)."""
     import oracle
     from cffc_amd import abi
     oracle.build()
-    m, W0 = make_case(32, 64)
-    o = oracle.Oracle(m, abi.Scheme())
-    o.set_W(W0)
-    nsteps = 32
-    t0 = time.perf_counter()
-    rc, n, _, _ = o.run(nsteps, nthreads=threads)
-    t1 = time.perf_counter()
-    rate = m.ncell * 4 * n / (t1 - t0)
-    return dict(value=rate, unit="cell-stage updates/s", cores=threads, kind="port",
-                sample="CPU oracle, 32x32 blocks of 64^2 cells (4,194,304 cells, same IC family), "
-                       "%d RK4 steps, %.1f s wall on %d threads" % (n, t1 - t0, threads))
+
+    def rate(nb, nsteps, nth):
+        m, W0 = make_case(nb, 64)
+        o = oracle.Oracle(m, abi.Scheme())
+        o.set_W(W0)
+        t0 = time.perf_counter()
+        rc, n, _, _ = o.run(nsteps, nthreads=nth)
+        t1 = time.perf_counter()
+        del o
+        return m.ncell * 4 * n / (t1 - t0), t1 - t0, n, m.ncell
+
+    r1, t1, n1, c1 = rate(8, 8, 1)
+    rn, tn, nn, cn = rate(32, 32, threads)
+    return dict(value=rn, unit="cell-stage updates/s", cores=threads, kind="port",
+                cpu_model=cpu_model(), one_core=r1, n_core=rn, n_core_speedup=rn / r1,
+                sample="CPU oracle, same IC family and scheme: %d threads on 32x32 blocks of 64^2 cells "
+                       "(%d cells), %d RK4 steps, %.1f s wall; 1 thread on 8x8 blocks of 64^2 (%d cells), "
+                       "%d steps, %.1f s" % (threads, cn, nn, tn, c1, n1, t1))
+
+
+def error_leg(m, W0, steps, tile64, tile32):
+    """L1 / Linf of the mixed path against FP64 after `steps` graph-replayed
+    steps from the same state (independent of the timed run's length)."""
+    Ws = {}
+    for prec, tile in (("fp64", tile64), ("mixed", tile32)):
+        g, rc, dt, norms, W = gpu_run(m, W0, prec, steps, 0, tile)
+        g.close()
+        Ws[prec] = W
+    d = np.abs(Ws["mixed"] - Ws["fp64"])
+    print("[mixed vs fp64] %d steps: L1 %s, Linf %s" % (steps, d.mean(axis=0), d.max(axis=0)),
+          file=sys.stderr, flush=True)
+    return {"steps": steps, "L1": d.mean(axis=0).tolist(), "Linf": d.max(axis=0).tolist(),
+            "vars": ["rho", "u", "v", "p"]}
+
+
+def serving_leg(ncases, seed=5):
+    """Batched steady serving: `ncases` bump cases of uneven size (96x32 to
+    384x128 cells in blocks of 32^2) solved as one batch by the FAS
+    multigrid solver to the single-grid criterion 1e-10, cold (freestream)
+    and warm (the U-Net's prediction, resampled to each mesh).  Multigrid
+    stops on the batch's norm, so every case finishes with the batch."""
+    from cffc_amd import serving, warmstart
+    cases = serving.serving_cases(ncases, seed)
+    log = lambda s: print("[serving] " + s, file=sys.stderr, flush=True)
+    out = {"cases": ncases, "seed": seed, "solver": "FAS multigrid, 4 levels, V cycle, CFL 1.0, FP64",
+           "meshes": "96x32 .. 384x128 cells, blocks of 32^2", "tol": 1e-10}
+    out["cold"] = serving.serve_multigrid(cases, log=log)
+    model = os.path.join(ROOT, "cffc_amd", "data", "unet_bump.npz")
+    if os.path.exists(model):
+        net, norm = warmstart.load_model(model)
+        t0 = time.perf_counter()
+        init = serving.warm_states(net, norm, cases)
+        t_inf = time.perf_counter() - t0
+        out["warm"] = serving.serve_multigrid(cases, initial=init, log=log)
+        out["warm"]["inference_seconds"] = t_inf
+        out["warm"]["wall_seconds_with_inference"] = out["warm"]["wall_seconds"] + t_inf
+    for k in ("cold", "warm"):
+        if k in out:
+            out[k].pop("latency_s", None)
+    return out
 
 
 def steady_leg(ncases, seed=7):
@@ -138,10 +206,12 @@ def steady_leg(ncases, seed=7):
             "cold_batch_s": cold.seconds, "warm_batch_s": warm.seconds + t_inf, "inference_s": t_inf,
             "step_ratio_total": float(cs.sum() / max(ws.sum(), 1)),
             "time_ratio_batch": float(cold.seconds / (warm.seconds + t_inf)),
+            "median_step_ratio": float(np.median(cs / np.maximum(ws, 1))),
+            "target_ratio": 3.0,
             "all_converged": bool(np.all(cs > 0) and np.all(ws > 0))}
 
 
-def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=True, fmg_cycles=50):
+def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=True, fmg_cycles=0):
     """The same held-out steady bump cases, cold, solved together (one mesh
     of disjoint parts) by the FAS multigrid solver (FASMultigrid2D: V cycle,
     HLLE first-order coarse levels, 5-stage optimal smoothing).  Multigrid
@@ -214,15 +284,18 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=Tr
 
 
 def amr_case(seed=3):
-    """The quadtree workload: [0,1]^2 in 32 x 32 root blocks of 16^2 cells,
-    refined twice where blocks come within 0.02 of the circle r = 0.3 about
-    (0.5, 0.5), 2:1 balanced (3,088 blocks at levels 0/1/2, 790,528 cells),
-    interior nodes disturbed by 0.1 h; reflecting walls; the cylindrical
-    explosion (density ratio 8, pressure ratio 10) across that circle."""
-    from cffc_amd import abi
+    """The uneven-block quadtree workload: [0,1]^2 in 128 x 128 root blocks
+    of 16^2 cells, refined twice where blocks meet the circle r = 0.3 about
+    (0.5, 0.5), 2:1 balanced across faces and corners (20,044 equal blocks
+    at levels 0/1/2), interior nodes disturbed by 0.1 h; then every aligned
+    2x2 or 4x4 group of same-level blocks merged into one block (2,956 blocks
+    of 16^2, 32^2 and 64^2 cells, 5,128,192 cells); reflecting walls; the
+    cylindrical explosion (density ratio 8, pressure ratio 10) across that
+    circle."""
     from cffc_amd import mesh as M
-    m = M.quadtree_box(nroot=32, n=16, max_level=2, refine=M.circle_refinement(0.3, band=0.02),
+    q = M.quadtree_box(nroot=128, n=16, max_level=2, refine=M.circle_refinement(0.3, band=0.0),
                        jitter=0.1, seed=seed)
+    m, cmap = M.merge_quadtree(q)
     W0 = M.cylindrical_explosion(m, radius=0.3)
     return m, W0
 
@@ -253,8 +326,12 @@ def amr_leg(steps, warmup=10, autotune_tiles=True, fixed_tile=(16, 8)):
     m, W0 = amr_case()
     tmesh = time.perf_counter() - t0
     A = cell_areas(m)
-    out = {"workload": "euler2d_cylindrical_explosion_quadtree_3088_blocks_of_16x16", "blocks": int(m.nblk),
-           "blocks_per_level": np.bincount(m.level).tolist(), "cells": int(m.ncell), "steps": steps,
+    sizes = {"%dx%d" % (k, k): int(v) for k, v in zip(*np.unique(m.ni, return_counts=True))}
+    per_level = {str(l): {"%dx%d" % (k, k): int(v) for k, v in zip(*np.unique(m.ni[m.level == l], return_counts=True))}
+                 for l in range(int(m.level.max()) + 1)}
+    out = {"workload": "euler2d_cylindrical_explosion_uneven_quadtree_%d_blocks_16_32_64" % m.nblk,
+           "blocks": int(m.nblk), "blocks_by_size": sizes, "blocks_by_level_and_size": per_level,
+           "cells": int(m.ncell), "steps": steps, "launches_per_stage": "1 residual/update kernel for all blocks",
            "scheme": "RK4/Roe/LSQ/Venkatakrishnan/CFL0.5/global-dt", "mesh_seconds": tmesh}
     Ws = {}
     for prec in ("mixed", "fp64"):
@@ -317,14 +394,18 @@ def main():
     ap.add_argument("--precision", default="both", choices=["both", "mixed", "fp64"])
     ap.add_argument("--no-autotune", action="store_true", help="keep the default 16x8 tile")
     ap.add_argument("--tile", default=None, metavar="TIxTJ", help="use this tile shape (no autotuning)")
-    ap.add_argument("--steady-cases", type=int, default=8,
+    ap.add_argument("--error-steps", type=int, default=500,
+                    help="steps of the mixed-vs-FP64 error runs (0: skip)")
+    ap.add_argument("--c5-cases", type=int, default=256,
+                    help="serving cases of uneven size solved by FAS multigrid, cold and warm (0: skip)")
+    ap.add_argument("--steady-cases", type=int, default=64,
                     help="held-out steady bump cases solved cold and warm (0: skip)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES bytes (counter calibration)")
-    ap.add_argument("--amr-steps", type=int, default=1000,
-                    help="steps of the quadtree (resolution-change) workload (0: skip)")
+    ap.add_argument("--amr-steps", type=int, default=500,
+                    help="steps of the uneven-block quadtree (resolution-change) workload (0: skip)")
     ap.add_argument("--only-amr", action="store_true", help="run only the quadtree workload and print it")
-    ap.add_argument("--mg-cases", type=int, default=8, help="steady cases solved by FAS multigrid (0: skip)")
+    ap.add_argument("--mg-cases", type=int, default=64, help="steady cases solved by FAS multigrid (0: skip)")
     ap.add_argument("--mg-eager", action="store_true",
                     help="multigrid cycles launched one by one instead of replayed as a graph (profiling)")
     ap.add_argument("--only-steady", action="store_true",
@@ -371,14 +452,20 @@ def main():
         print("[%s] %.3f s for %d steps: %.3e cell-stage/s; stage kernel %.3f ms, exchange %.3f ms"
               % (prec, dt, args.steps, res[prec]["rate"], stage_ms, exch_ms), file=sys.stderr, flush=True)
 
-    head = res["mixed"] if "mixed" in res else res["fp64"]
-    head_prec = "mixed" if "mixed" in res else "fp64"
-    achieved = BYTES_PER_CELL_STAGE * ncell / (head["stage_ms"] * 1e-3) / 1e9
-    traffic = load_traffic("stage_kernel_%s_riemann4q_%dx%d_blocks_of_%dsq_traffic.json"
-                           % (head_prec, args.blocks, args.blocks, args.cells))
-    kname = "stage_kernel<%s,%d,%d,fused>" % ("float" if head_prec == "mixed" else "double", *head["tile"])
-    # per-launch HBM bytes measured for this kernel and tile shape (None: tile not profiled)
-    traffic = traffic.get("kernels", {}).get(kname) if traffic else None
+    def roofline(prec):
+        r = res[prec]
+        achieved = BYTES_PER_CELL_STAGE * ncell / (r["stage_ms"] * 1e-3) / 1e9
+        kname = "stage_kernel<%s,%d,%d,fused>" % ("float" if prec == "mixed" else "double", *r["tile"])
+        t = load_traffic("stage_kernel_%s_riemann4q_%dx%d_blocks_of_%dsq_traffic.json"
+                         % (prec, args.blocks, args.blocks, args.cells))
+        # per-launch HBM bytes measured for this kernel and tile shape (None: tile not profiled)
+        t = t.get("kernels", {}).get(kname) if t else None
+        return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": t.get("bytes_per_launch") if t else None,
+                "kernel": kname, "kernel_avg_ms": r["stage_ms"]}
+
+    head_prec = "fp64" if "fp64" in res else "mixed"
+    head = res[head_prec]
     out = {
         "metric": "cell-stage updates/s",
         "value": head["rate"],
@@ -390,29 +477,22 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32-flux/fp64-state" if head_prec == "mixed" else "f64",
+        "dtype": "f64" if head_prec == "fp64" else "fp32-flux/fp64-state",
         "data": "synthetic: 4-quadrant Riemann IC + seeded 1% density perturbation",
         "config": {"workload": "euler2d_riemann4q_64x64_blocks_of_64x64", "cells": ncell,
                    "tile": list(head["tile"]),
                    "blocks": args.blocks * args.blocks, "block_cells": [args.cells, args.cells],
                    "scheme": "RK4/Roe/LSQ/Venkatakrishnan/CFL0.5/global-dt",
                    "parallelism": "single-gpu"},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic.get("bytes_per_launch") if traffic else None,
-                     "kernel": kname,
-                     "kernel_avg_ms": head["stage_ms"]},
+        "roofline": roofline(head_prec),
     }
-    if "fp64" in res:
-        f = res["fp64"]
-        ach64 = BYTES_PER_CELL_STAGE * ncell / (f["stage_ms"] * 1e-3) / 1e9
-        out["fp64"] = {"value": f["rate"], "ms_per_step": f["seconds"] / args.steps * 1e3,
-                       "stage_kernel_ms": f["stage_ms"], "achieved_GBs": ach64, "frac": ach64 / HBM_PEAK_GBS}
-    if "fp64" in res and "mixed" in res:
-        d = np.abs(res["mixed"]["W"] - res["fp64"]["W"])
-        out["mixed_vs_fp64"] = {"steps": args.warmup + args.steps,
-                                "L1": d.mean(axis=0).tolist(), "Linf": d.max(axis=0).tolist(),
-                                "vars": ["rho", "u", "v", "p"]}
+    if "mixed" in res and head_prec != "mixed":
+        r = res["mixed"]
+        out["mixed"] = {"value": r["rate"], "unit": "cell-stage updates/s", "dtype": "fp32-flux/fp64-state",
+                        "ms_per_step": r["seconds"] / args.steps * 1e3, "tile": list(r["tile"]),
+                        "roofline": roofline("mixed")}
+    if "fp64" in res and "mixed" in res and args.error_steps > 0:
+        out["mixed_vs_fp64"] = error_leg(m, W0, args.error_steps, res["fp64"]["tile"], res["mixed"]["tile"])
     if args.amr_steps > 0:
         try:
             out["amr"] = amr_leg(args.amr_steps)
@@ -429,6 +509,11 @@ def main():
                 "steady_warm_vs_cold", {}).get("cold_batch_s"), graph=not args.mg_eager)
         except Exception as e:  # reported, never required for the throughput line
             out["steady_multigrid"] = {"error": str(e)}
+    if args.c5_cases > 0:
+        try:
+            out["serving"] = serving_leg(args.c5_cases)
+        except Exception as e:  # reported, never required for the throughput line
+            out["serving"] = {"error": str(e)}
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.cpu_threads)
