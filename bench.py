@@ -368,6 +368,32 @@ def load_traffic(name):
     return None
 
 
+def unet_probe(nsteps, batch=16):
+    """Training steps of the warm-start U-Net (BF16 MFMA forward, backward and
+    Adam kernels) on random 192 x 64 inputs: steps/s and the convolutions'
+    algorithmic FLOP rate (3 x forward: forward, data and weight gradients)
+    against MI355X's dense BF16 MFMA peak."""
+    from cffc_amd import unet
+    net = unet.UNet(64, 192, max_batch=batch, seed=1)
+    rng = np.random.default_rng(2)
+    X = rng.standard_normal((4 * batch, 64, 192, unet.NIN)).astype(np.float32)
+    Y = rng.standard_normal((4 * batch, 64, 192, unet.NOUT)).astype(np.float32)
+    net.set_dataset(X, Y)
+    fwd = 0.0
+    for l in range(net.nlayers):
+        ca, cb, co, lv = net.layer_shape(l)
+        fwd += 2.0 * 9 * (ca + cb) * co * (64 >> lv) * (192 >> lv)
+    net.train(3, batch)   # warm-up
+    t0 = time.perf_counter()
+    net.train(nsteps, batch)
+    dt = time.perf_counter() - t0
+    net.close()
+    flops = 3.0 * fwd * batch * nsteps / dt
+    print(json.dumps({"unet_probe_steps": nsteps, "batch": batch, "seconds": dt, "steps_per_s": nsteps / dt,
+                      "conv_tflops": flops / 1e12, "bf16_peak_tflops": 2500.0, "frac": flops / 2.5e15,
+                      "forward_gflop_per_sample": fwd / 1e9}))
+
+
 def copy_probe(nbytes):
     """A plain device copy of a known size (torch's copy kernel), run under
     rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE to calibrate those counters."""
@@ -400,6 +426,8 @@ def main():
                     help="serving cases of uneven size solved by FAS multigrid, cold and warm (0: skip)")
     ap.add_argument("--steady-cases", type=int, default=64,
                     help="held-out steady bump cases solved cold and warm (0: skip)")
+    ap.add_argument("--unet-probe", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS training steps of the warm-start U-Net (MFMA profiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain copy kernel of BYTES bytes (counter calibration)")
     ap.add_argument("--amr-steps", type=int, default=500,
@@ -417,6 +445,9 @@ def main():
         sys.exit(2)
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.unet_probe:
+        unet_probe(args.unet_probe)
         return
     if args.only_amr:
         fixed = tuple(int(x) for x in args.tile.lower().split("x")) if args.tile else (16, 8)

@@ -214,9 +214,17 @@ template <typename T> CFFC_HD CS<T> flux_roe_x(const PS<T> &Wl, const PS<T> &Wr)
     const PS<T> Wu = left ? Wl : Wr;
     CS<T> F = W_Fx(Wu);
     const T sg = left ? T(1) : T(-1);   // Flux += (left) / Flux -= (right): exact sign flip
+    // waves 1 and 4 share lp4 = HALF/c^2 and, up to sign, lp2 = -+HALF*d/c:
+    // (-HALF*d)/c is -(HALF*d/c) bit for bit (a negated product, then a
+    // negated quotient), so each is divided once
+    T lp2w4 = K::ZERO, lp4w14 = K::ZERO;
+    if (w1 != K::ZERO || w4 != K::ZERO) {
+        lp2w4 = qdiv(K::HALF * Wa.d, ca);
+        lp4w14 = qdiv(K::HALF, sqr(ca));
+    }
     // wave 1: lp = (0, -HALF*d/c, 0, HALF/c^2), rc = (1, u-c, v, h-u*c)
     if (Fast<T>::on || w1 != K::ZERO) {
-        const T lp2 = qdiv(-K::HALF * Wa.d, ca), lp4 = qdiv(K::HALF, sqr(ca));
+        const T lp2 = -lp2w4, lp4 = lp4w14;
         const T a = sg * (w1 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p));
         F.d += a * K::ONE;
         F.mx += a * (Wa.vx - ca);
@@ -240,7 +248,7 @@ template <typename T> CFFC_HD CS<T> flux_roe_x(const PS<T> &Wl, const PS<T> &Wr)
     }
     // wave 4: lp = (0, HALF*d/c, 0, HALF/c^2), rc = (1, u+c, v, h+u*c)
     if (Fast<T>::on || w4 != K::ZERO) {
-        const T lp2 = qdiv(K::HALF * Wa.d, ca), lp4 = qdiv(K::HALF, sqr(ca));
+        const T lp2 = lp2w4, lp4 = lp4w14;
         const T a = sg * (w4 * (K::ZERO * dW.d + (lp2 * dW.vx + K::ZERO * dW.vy) + lp4 * dW.p));
         F.d += a * K::ONE;
         F.mx += a * (Wa.vx + ca);
