@@ -256,7 +256,12 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
             const FT v = sW[n][jl + (kk / 3) - 1][il + (kk % 3) - 1];
             if (FULL || (mask & (1u << s))) {
                 if constexpr (Fast<FT>::on) { umin = fminf(umin, v); umax = fmaxf(umax, v); }
-                else { umin = dmin(umin, v); umax = dmax(umax, v); }
+                // fmin/fmax (one instruction) equal the reference's
+                // compare-and-select min/max here: they differ only in the sign
+                // of a zero result, and the limiter uses u_max/u_min only
+                // through u_max - u_0 and u_min - u_0, whose zero sign can only
+                // reach y = +-0, where f(y) = (y^2 + 2y)/(y^2 + y + 2) is +0
+                else { umin = fmin(umin, v); umax = fmax(umax, v); }
             }
         }
         const FT gxn = n == 0 ? gx.d : n == 1 ? gx.vx : n == 2 ? gx.vy : gx.p;

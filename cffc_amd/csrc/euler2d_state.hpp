@@ -38,6 +38,12 @@ template <typename T> CFFC_HD T gm1i() { return T(1.0 / (1.40 - 1.0)); }
 
 template <typename T> CFFC_HD T dmax(T x, T y) { return (x > y) ? x : y; }
 template <typename T> CFFC_HD T dmin(T x, T y) { return (x < y) ? x : y; }
+// IEEE minimum (v_min_f64 / v_min_f32): dmin's value wherever neither
+// argument is NaN and the arguments are not zeros of opposite sign
+template <typename T> CFFC_HD T tmin(T x, T y) { return fmin(x, y); }
+template <> CFFC_HD float tmin<float>(float x, float y) { return fminf(x, y); }
+template <typename T> CFFC_HD T tmax(T x, T y) { return fmax(x, y); }
+template <> CFFC_HD float tmax<float>(float x, float y) { return fmaxf(x, y); }
 template <typename T> CFFC_HD T sqr(T x) { return x * x; }
 template <typename T> CFFC_HD T tabs(T x) { return fabs(x); }
 template <typename T> CFFC_HD T tsqrt(T x) { return sqrt(x); }
@@ -365,10 +371,12 @@ template <typename T> CFFC_HD T venkat_fold(T phi, T uq, T u0, T u0Min, T u0Max)
         num = u0Min - u0;  // y = num/du, both <= 0
         if (num <= T(2.002) * du && num >= T(1.0e12) * du) return phi;
     } else {
-        return dmin(phi, K::ONE);
+        return tmin(phi, K::ONE);
     }
+    // phi and f(y) are never -0 or NaN (phi starts at 1, f(+-0) = +0), so
+    // the one-instruction minimum equals the reference's compare-and-select
     const T y = qdiv(num, du);
-    return dmin(phi, qdiv(sqr(y) + K::TWO * y, sqr(y) + y + K::TWO));
+    return tmin(phi, qdiv(sqr(y) + K::TWO * y, sqr(y) + y + K::TWO));
 }
 
 }  // namespace cffc

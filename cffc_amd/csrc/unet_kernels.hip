@@ -18,6 +18,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t bf16_t;
@@ -183,12 +185,16 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_data_kernel(const float *__re
 // backward wrt the weights: dW[co][k] += sum over a pixel chunk of
 // G[p,co] * X[p+off(k), ci(k)].  Workgroup tile: 64 output channels x 64 k,
 // pixels in sub-chunks of 32 staged in LDS (G as [co][p], X as [k][p]).
+// Staging walks co (resp. k) fastest across the threads, so the NHWC loads of
+// dY, Y and X are contiguous; each thread keeps one co and one k for the whole
+// chunk.  The workgroups of the first k tile also sum G in FP32 over their
+// pixels for the bias gradient db[co] (one atomic per co and thread).
 __global__ __launch_bounds__(256) void conv3x3_bwd_weight_kernel(const float *__restrict__ dy,
                                                                  const bf16_t *__restrict__ yact, int Cout,
                                                                  int relu, const bf16_t *__restrict__ xA, int CA,
                                                                  const bf16_t *__restrict__ xB, int CB, int B,
                                                                  int H, int W, float *__restrict__ dw,
-                                                                 int pix_per_wg) {
+                                                                 float *__restrict__ db, int pix_per_wg) {
     __shared__ bf16_t sG[64][32 + 8];
     __shared__ bf16_t sX[64][32 + 8];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, tid = threadIdx.x;
@@ -197,36 +203,42 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_weight_kernel(const float *__
     const int co0 = blockIdx.y * 64, k0 = blockIdx.z * 64;
     const int pstart = blockIdx.x * pix_per_wg;
     const int pend = min(P, pstart + pix_per_wg);
+    // this thread's staging row: channel co = co0 + r of G, tap/ci of k = k0 + r
+    const int r = tid & 63, pp0 = tid >> 6;
+    const int co = co0 + r, k = k0 + r;
+    const bool co_ok = co < Cout, k_ok = k < K;
+    const int tap = k_ok ? k / Cin : 0, ci = k_ok ? k - tap * Cin : 0;
+    const int dyy = tap / 3 - 1, dxx = tap % 3 - 1;
+    const bool useA = ci < CA;
+    const bf16_t *xsrc = useA ? xA : xB;
+    const int xc = useA ? CA : CB, xci = useA ? ci : ci - CA;
+    const bool bias = db != nullptr && blockIdx.z == 0;
+    float bsum = 0.f;
     // wave w owns output rows co0 + 16*w .. +16 and all 64 k (4 blocks)
     f32x4 acc[4];
     for (int nb = 0; nb < 4; ++nb) acc[nb] = (f32x4){0.f, 0.f, 0.f, 0.f};
     for (int ps = pstart; ps < pend; ps += 32) {
         // stage G[64 co][32 p] and X[64 k][32 p]
-        for (int q = tid; q < 64 * 32; q += 256) {
-            const int r = q / 32, pp = q % 32;
+        for (int pp = pp0; pp < 32; pp += 4) {
             const int p = ps + pp;
             float g = 0.f;
             bf16_t xv = 0;
             if (p < pend) {
-                const int co = co0 + r;
-                if (co < Cout) {
+                if (co_ok) {
                     g = dy[(int64_t)p * Cout + co];
                     if (relu) {
                         const bf16_t ya = yact[(int64_t)p * Cout + co];
                         if (!(ya != 0 && !(ya & 0x8000u))) g = 0.f;
                     }
                 }
-                const int k = k0 + r;
-                if (k < K) {
-                    const int tap = k / Cin, ci = k - tap * Cin;
+                if (k_ok) {
                     const int px = p % W, py = (p / W) % H, pb = p / (W * H);
-                    const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
-                    if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
-                        const int64_t qq = ((int64_t)pb * H + yy) * W + xx;
-                        xv = (ci < CA) ? xA[qq * CA + ci] : xB[qq * CB + (ci - CA)];
-                    }
+                    const int yy = py + dyy, xx = px + dxx;
+                    if (yy >= 0 && yy < H && xx >= 0 && xx < W)
+                        xv = xsrc[(((int64_t)pb * H + yy) * W + xx) * xc + xci];
                 }
             }
+            bsum += g;
             sG[r][pp] = f2bf(g);
             sX[r][pp] = xv;
         }
@@ -243,35 +255,14 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_weight_kernel(const float *__
         }
         __syncthreads();
     }
+    if (bias && co_ok) atomicAdd(&db[co], bsum);
     for (int nb = 0; nb < 4; ++nb) {
-        const int k = k0 + nb * 16 + (lane & 15);
+        const int kq = k0 + nb * 16 + (lane & 15);
         for (int i = 0; i < 4; ++i) {
-            const int co = co0 + 16 * wave + 4 * (lane >> 4) + i;
-            if (co < Cout && k < K) atomicAdd(&dw[(int64_t)co * K + k], acc[nb][i]);
+            const int cq = co0 + 16 * wave + 4 * (lane >> 4) + i;
+            if (cq < Cout && kq < K) atomicAdd(&dw[(int64_t)cq * K + kq], acc[nb][i]);
         }
     }
-}
-
-// bias gradient: db[co] = sum_p G[p,co]
-__global__ void bias_grad_kernel(const float *dy, const bf16_t *yact, int Cout, int relu, int P, float *db) {
-    __shared__ float sm[256];
-    const int co = blockIdx.x;
-    float s = 0.f;
-    for (int p = threadIdx.x; p < P; p += 256) {
-        float g = dy[(int64_t)p * Cout + co];
-        if (relu) {
-            const bf16_t ya = yact[(int64_t)p * Cout + co];
-            if (!(ya != 0 && !(ya & 0x8000u))) g = 0.f;
-        }
-        s += g;
-    }
-    sm[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = 128; o > 0; o >>= 1) {
-        if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) db[co] = sm[0];
 }
 
 // 2x2 average pooling (NHWC, bf16) and its backward (fp32, +=)
@@ -436,12 +427,16 @@ hipError_t cffc_unet_launch_conv_bwd_weight(const float *dy, const bf16_t *yact,
                                             float *db, hipStream_t st) {
     const int P = B * H * W;
     const int K = 9 * (CA + CB);
-    const int pix = 1024;
+    // pixel chunk per workgroup: about 1024 workgroups over the (co, k) tiles,
+    // chunks of 128..4096 pixels (whole 32-pixel sub-chunks)
+    const int64_t tiles = (int64_t)((Cout + 63) / 64) * ((K + 63) / 64);
+    const int pix = (int)std::min<int64_t>(4096, std::max<int64_t>(128, ((int64_t)P * tiles / 1024 + 31) / 32 * 32));
     hipError_t e = hipMemsetAsync(dw, 0, (size_t)Cout * K * sizeof(float), st);
+    if (e == hipSuccess && db) e = hipMemsetAsync(db, 0, (size_t)Cout * sizeof(float), st);
     if (e != hipSuccess) return e;
     dim3 grid((P + pix - 1) / pix, (Cout + 63) / 64, (K + 63) / 64);
-    hipLaunchKernelGGL(conv3x3_bwd_weight_kernel, grid, dim3(256), 0, st, dy, yact, Cout, relu, xA, CA, xB, CB, B, H, W, dw, pix);
-    hipLaunchKernelGGL(bias_grad_kernel, dim3(Cout), dim3(256), 0, st, dy, yact, Cout, relu, P, db);
+    hipLaunchKernelGGL(conv3x3_bwd_weight_kernel, grid, dim3(256), 0, st, dy, yact, Cout, relu, xA, CA, xB, CB, B, H, W,
+                       dw, db, pix);
     return hipGetLastError();
 }
 
@@ -463,7 +458,9 @@ hipError_t cffc_unet_launch_upsample_bwd(const float *dy, int B, int H, int W, i
 }
 hipError_t cffc_unet_launch_mse(const float *y32, int Cout, int nout, const float *t, int64_t P, float *dy,
                                 float *loss, hipStream_t st) {
-    hipLaunchKernelGGL(mse_kernel, dim3(grid_1d(P * Cout)), dim3(256), 0, st, y32, Cout, nout, t, P, dy, loss);
+    // grid-stride: 512 workgroups at most, so at most 512 atomics meet on the loss
+    hipLaunchKernelGGL(mse_kernel, dim3(std::min(grid_1d(P * Cout), 512)), dim3(256), 0, st, y32, Cout, nout, t, P, dy,
+                       loss);
     return hipGetLastError();
 }
 hipError_t cffc_unet_launch_adam(float *w, const float *g, float *m, float *v, int Cout, int Cin, int Kpad, int Kpad2,
