@@ -283,7 +283,7 @@ def multigrid_leg(ncases, seed=7, levels=4, cfl=1.0, cold_batch_s=None, graph=Tr
     return out
 
 
-def amr_case(seed=3):
+def amr_case(seed=3, with_quadtree=False):
     """The uneven-block quadtree workload: [0,1]^2 in 128 x 128 root blocks
     of 16^2 cells, refined twice where blocks meet the circle r = 0.3 about
     (0.5, 0.5), 2:1 balanced across faces and corners (20,044 equal blocks
@@ -291,12 +291,15 @@ def amr_case(seed=3):
     2x2 or 4x4 group of same-level blocks merged into one block (2,956 blocks
     of 16^2, 32^2 and 64^2 cells, 5,128,192 cells); reflecting walls; the
     cylindrical explosion (density ratio 8, pressure ratio 10) across that
-    circle."""
+    circle.  with_quadtree: also the equal-block quadtree and the merge's
+    cell map (merged cell k is quadtree cell cmap[k])."""
     from cffc_amd import mesh as M
     q = M.quadtree_box(nroot=128, n=16, max_level=2, refine=M.circle_refinement(0.3, band=0.0),
                        jitter=0.1, seed=seed)
     m, cmap = M.merge_quadtree(q)
     W0 = M.cylindrical_explosion(m, radius=0.3)
+    if with_quadtree:
+        return m, W0, q, cmap
     return m, W0
 
 

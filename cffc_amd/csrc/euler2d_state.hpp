@@ -121,7 +121,9 @@ template <typename T> CFFC_HD CS<T> W_Fx(const PS<T> &W) {
 // HALF*(la+|la|+delta) (positive part) or HALF*(la-|la|-delta) (negative).
 template <typename T> CFFC_HD T harten_delta(T la, T ll, T lr) {
     typedef Consts<T> K;
-    T delta = dmax(K::ZERO, K::TWO * (lr - ll));
+    // only compared with |la| and TOLER, or divided by when >= TOLER: the
+    // sign of a zero delta is immaterial, so the IEEE maximum serves
+    T delta = tmax(K::ZERO, K::TWO * (lr - ll));
     if constexpr (Fast<T>::on) {
         const T fixd = K::HALF * (qdiv(sqr(la), delta) + delta) - tabs(la);
         return (tabs(la) > delta || delta < K::TOLER) ? K::ZERO : fixd;

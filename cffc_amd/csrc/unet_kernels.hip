@@ -83,11 +83,12 @@ __global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16_t *__restri
     f32x4 acc[NB];
     for (int nb = 0; nb < NB; ++nb) acc[nb] = (f32x4){0.f, 0.f, 0.f, 0.f};
     const int kl = 8 * (lane >> 4);
+    // k = tap*Cin + ci, stepped by 32 along with k (no division per step)
+    int tap = kl / Cin, ci = kl - tap * Cin;
     for (int k0 = 0; k0 < Kpad; k0 += 32) {
         const int k = k0 + kl;
         Frag a = zero_frag();
         if (pvalid && k < K) {
-            const int tap = k / Cin, ci = k - tap * Cin;
             const int yy = py + tap / 3 - 1, xx = px + tap % 3 - 1;
             if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
                 const int64_t q = ((int64_t)pb * H + yy) * W + xx;
@@ -100,6 +101,8 @@ __global__ __launch_bounds__(256) void conv3x3_fwd_kernel(const bf16_t *__restri
             b.u = *(const uint4 *)(wt + (int64_t)n * Kpad + k);
             acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[nb], 0, 0, 0);
         }
+        ci += 32;
+        while (ci >= Cin) { ci -= Cin; ++tap; }
     }
     const int pr0 = blockIdx.x * 64 + wave * 16 + 4 * (lane >> 4);
     for (int nb = 0; nb < NB; ++nb) {
@@ -136,11 +139,12 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_data_kernel(const float *__re
     f32x4 acc[NB];
     for (int nb = 0; nb < NB; ++nb) acc[nb] = (f32x4){0.f, 0.f, 0.f, 0.f};
     const int kl = 8 * (lane >> 4);
+    // k = tap*Cout + co, stepped by 32 along with k (no division per step)
+    int tap = kl / Cout, co = kl - tap * Cout;
     for (int k0 = 0; k0 < Kpad; k0 += 32) {
         const int k = k0 + kl;
         Frag a = zero_frag();
         if (pvalid && k < K) {
-            const int tap = k / Cout, co = k - tap * Cout;
             // dX[p] gathers G at p - off(tap)
             const int yy = py - (tap / 3 - 1), xx = px - (tap % 3 - 1);
             if (yy >= 0 && yy < H && xx >= 0 && xx < W) {
@@ -162,6 +166,8 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_data_kernel(const float *__re
             b.u = *(const uint4 *)(wd + (int64_t)n * Kpad + k);
             acc[nb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a.v, b.v, acc[nb], 0, 0, 0);
         }
+        co += 32;
+        while (co >= Cout) { co -= Cout; ++tap; }
     }
     const int pr0 = blockIdx.x * 64 + wave * 16 + 4 * (lane >> 4);
     for (int nb = 0; nb < NB; ++nb) {
@@ -218,9 +224,11 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_weight_kernel(const float *__
     f32x4 acc[4];
     for (int nb = 0; nb < 4; ++nb) acc[nb] = (f32x4){0.f, 0.f, 0.f, 0.f};
     for (int ps = pstart; ps < pend; ps += 32) {
-        // stage G[64 co][32 p] and X[64 k][32 p]
-        for (int pp = pp0; pp < 32; pp += 4) {
-            const int p = ps + pp;
+        // stage G[64 co][32 p] and X[64 k][32 p]; the pixel's (x, y, image)
+        // is divided out once per sub-chunk, then stepped by 4
+        int p = ps + pp0;
+        int px = p % W, py = (p / W) % H, pb = p / (W * H);
+        for (int pp = pp0; pp < 32; pp += 4, p += 4) {
             float g = 0.f;
             bf16_t xv = 0;
             if (p < pend) {
@@ -232,7 +240,6 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_weight_kernel(const float *__
                     }
                 }
                 if (k_ok) {
-                    const int px = p % W, py = (p / W) % H, pb = p / (W * H);
                     const int yy = py + dyy, xx = px + dxx;
                     if (yy >= 0 && yy < H && xx >= 0 && xx < W)
                         xv = xsrc[(((int64_t)pb * H + yy) * W + xx) * xc + xci];
@@ -241,6 +248,11 @@ __global__ __launch_bounds__(256) void conv3x3_bwd_weight_kernel(const float *__
             bsum += g;
             sG[r][pp] = f2bf(g);
             sX[r][pp] = xv;
+            px += 4;
+            while (px >= W) {
+                px -= W;
+                if (++py == H) { py = 0; ++pb; }
+            }
         }
         __syncthreads();
         // A = G (16 co x 32 p), B = X^T (32 p x 16 k)

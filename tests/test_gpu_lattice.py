@@ -150,3 +150,36 @@ def test_merged_mass_conserved_and_mixed_bounded(E):
           flush=True)
     assert drift < 1e-14
     assert np.all(d.mean(axis=0) < 1e-5) and np.all(d.max(axis=0) < 1e-2)
+
+
+def test_c3_full_size_uneven_blocks_match_the_oracle(E):
+    """The bench's C3 workload at full size: 2,956 blocks of 16^2, 32^2 and
+    64^2 cells from three levels (5,131,264 cells), one stage launch for all
+    of them, 3 graph-replayed FP64 steps, against the oracle's run of the
+    20,044-block equal quadtree it was merged from: bit for bit through the
+    cell map, and the same time steps."""
+    import os
+    import bench
+    mm, W0m, q, cmap = bench.amr_case(with_quadtree=True)
+    print("C3 mesh: %d blocks (%s), %d cells; quadtree %d blocks" % (
+        mm.nblk, dict(zip(*[a.tolist() for a in np.unique(mm.ni, return_counts=True)])), mm.ncell, q.nblk), flush=True)
+    assert mm.nblk == 2956 and mm.ncell == 5131264 and sorted(set(mm.ni.tolist())) == [16, 32, 64]
+    W0 = np.empty_like(W0m)
+    W0[cmap] = W0m
+    s = abi.Scheme()
+    orc = oracle.Oracle(q, s)
+    orc.set_W(W0)
+    rc, n, norms_o, dt_o = orc.run(3, nthreads=min(16, os.cpu_count() or 1))
+    assert rc == 0 and n == 3
+    Wo = orc.get_W()
+    g = E.Euler2DQuadBlocks(mm, s)
+    g.set_W(W0m)
+    rc, n, norms, dts = g.run_steps(3, graph=True)
+    assert rc == 0 and n == 3
+    Wg = g.get_W()
+    g.close()
+    nb = nonbitwise(Wg, Wo[cmap])
+    print("C3 full size, 3 FP64 steps: %d of %d values not bit-identical" % (nb, Wg.size), flush=True)
+    assert nb == 0
+    assert np.array_equal(dts, dt_o)
+    assert np.allclose(norms, norms_o, rtol=1e-12, atol=0.0)
