@@ -312,11 +312,13 @@ template <typename FT> __device__ __forceinline__ bool positivity_fails(const CS
 // local-time-step halving, W^(s+1) (and its pushed ghost copies), the
 // accumulator, and at the last stage U^(n+1), the norm terms and the next
 // step's CFL time step (SingleBlock.cc:3036-3053).
+// lW, lE, lS, lN: the cell's face lengths as the stage read them (FP64
+// planes in the FP64 path, FP32 planes in the mixed path).
 template <typename FT>
 __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMeta &m, const TileDesc &t, int64_t c,
-                                             int ip, int jp, double A, double dtc, double (&dU)[4], double &l1,
-                                             double &l2, double &mx, double &dtn) {
-    const double *G = p.geom;
+                                             int ip, int jp, double A, double dtc, double lW, double lE, double lS,
+                                             double lN, double (&dU)[4], double &l1, double &l2, double &mx,
+                                             double &dtn) {
     const double om = p.omega;
     CS<double> Un;
     const double U0[4] = {p.U[c], p.U[p.ncp + c], p.U[2 * p.ncp + c], p.U[3 * p.ncp + c]};
@@ -377,16 +379,19 @@ __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMe
         l1 = fabs(dU[0]);
         l2 = dU[0] * dU[0];
         mx = l1;
-        // CFL of the next step, SingleBlock.cc:3036-3053, from the
-        // FP64 geometry in both paths
-        const double gE = G[GEO_LE * p.ncp + c], gW = G[GEO_LE * p.ncp + c - 1];
-        const double gN = G[GEO_LN * p.ncp + c], gS = G[GEO_LN * p.ncp + c - m.pitch];
-        const double d_i = 2.0 * (A / (gE + gW));
-        const double d_j = 2.0 * (A / (gN + gS));
-        const double nWx = -G[GEO_NEX * p.ncp + c - 1], nWy = -G[GEO_NEY * p.ncp + c - 1];
-        const double nSx = -G[GEO_NNX * p.ncp + c - m.pitch], nSy = -G[GEO_NNY * p.ncp + c - m.pitch];
-        const double v_i = 0.5 * (Wn.vx * (G[GEO_NEX * p.ncp + c] - nWx) + Wn.vy * (G[GEO_NEY * p.ncp + c] - nWy));
-        const double v_j = 0.5 * (Wn.vx * (G[GEO_NNX * p.ncp + c] - nSx) + Wn.vy * (G[GEO_NNY * p.ncp + c] - nSy));
+        // CFL of the next step, SingleBlock.cc:3036-3053, in FP64 from the
+        // geometry planes the stage reads (the mixed path's FP32 normals and
+        // lengths are still in cache from its flux phase)
+        const double d_i = 2.0 * (A / (lE + lW));
+        const double d_j = 2.0 * (A / (lN + lS));
+        const double nEx = rdgeo<FT>(p, GEO_NEX, GF_NEX, c), nEy = rdgeo<FT>(p, GEO_NEY, GF_NEY, c);
+        const double nNx = rdgeo<FT>(p, GEO_NNX, GF_NNX, c), nNy = rdgeo<FT>(p, GEO_NNY, GF_NNY, c);
+        const double nWx = -(double)rdgeo<FT>(p, GEO_NEX, GF_NEX, c - 1);
+        const double nWy = -(double)rdgeo<FT>(p, GEO_NEY, GF_NEY, c - 1);
+        const double nSx = -(double)rdgeo<FT>(p, GEO_NNX, GF_NNX, c - m.pitch);
+        const double nSy = -(double)rdgeo<FT>(p, GEO_NNY, GF_NNY, c - m.pitch);
+        const double v_i = 0.5 * (Wn.vx * (nEx - nWx) + Wn.vy * (nEy - nWy));
+        const double v_j = 0.5 * (Wn.vx * (nNx - nSx) + Wn.vy * (nNy - nSy));
         const double a = W_a(Wn);
         dtn = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
         if (p.lts) p.dt[c] = dtn;
@@ -794,7 +799,7 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
 #pragma unroll
             for (int v = 0; v < 4; ++v) Sx[v * ncp + c] = dU[v];
         } else {
-            fused_update<FT>(p, m, t, c, ip, jp, A, dtc, dU, l1, l2, mx, dtn);
+            fused_update<FT>(p, m, t, c, ip, jp, A, dtc, lW, lE, lS, lN, dU, l1, l2, mx, dtn);
         }
     }
     if (!(FUSED) || !p.last_stage || defer) return;
@@ -870,7 +875,9 @@ __global__ __launch_bounds__(UPD_NT) void update_tile_kernel(StageParams p, int 
         const double dtc = p.lts ? p.dt[c] : bitsd(*p.dtmin_in);
         double dU[4] = {p.Sd[c], p.Sd[ncp + c], p.Sd[2 * ncp + c], p.Sd[3 * ncp + c]};
         double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = MILLION;
-        fused_update<FT>(p, m, t, c, ip, jp, A, dtc, dU, a1, a2, a3, a4);
+        const double lW = rdgeo<FT>(p, GEO_LE, GF_LE, c - 1), lE = rdgeo<FT>(p, GEO_LE, GF_LE, c);
+        const double lS = rdgeo<FT>(p, GEO_LN, GF_LN, c - m.pitch), lN = rdgeo<FT>(p, GEO_LN, GF_LN, c);
+        fused_update<FT>(p, m, t, c, ip, jp, A, dtc, lW, lE, lS, lN, dU, a1, a2, a3, a4);
         l1 += a1; l2 += a2; mx = dmax(mx, a3); dtn = dmin(dtn, a4);
     }
     if (!p.last_stage) return;
