@@ -12,8 +12,9 @@
 //                            1383-1465, 1872-1905, 2311-2353, 2446-2489
 //   HartenFixPos/Neg, Limiter_Venkatakrishnan
 //                            src_2D/CFD/CFD.cc:605-643, 728-752
-// min/max are the reference's ternaries (Math.h:73-92), not fmin/fmax, so
-// signed zeros and NaNs propagate the same way.
+// min/max are the reference's ternaries (Math.h:73-92), so signed zeros and
+// NaNs propagate the same way, except where a comment shows that the IEEE
+// minimum or maximum (tmin/tmax, one instruction) gives the same bits.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -375,8 +376,9 @@ template <typename T> CFFC_HD T venkat_fold(T phi, T uq, T u0, T u0Min, T u0Max)
     } else {
         return tmin(phi, K::ONE);
     }
-    // phi and f(y) are never -0 or NaN (phi starts at 1, f(+-0) = +0), so
-    // the one-instruction minimum equals the reference's compare-and-select
+    // phi and f(y) are never -0 or NaN (phi starts at 1, f(+-0) = +0; y is
+    // finite since |du| > TOLER and |num| << 1e147), so the one-instruction
+    // minimum equals the reference's compare-and-select
     const T y = qdiv(num, du);
     return tmin(phi, qdiv(sqr(y) + K::TWO * y, sqr(y) + y + K::TWO));
 }
