@@ -80,6 +80,14 @@ struct cffc_ctx {
     int32_t *part_tile0 = nullptr, *part_state = nullptr, *part_steps = nullptr;
     double *part_l2 = nullptr, *part_last = nullptr;
     int part_l2_steps = 0;
+    // fused mixed path: FP32 RK accumulator; sf_live: it, not dUdt[0],
+    // holds the last fused run's residual
+    float *Sf = nullptr;
+    bool sf_live = false;
+    // stall freeze (serving policy): a running part's best L2 and its step
+    int stall_window = 0;
+    double *part_best = nullptr;
+    int32_t *part_best_step = nullptr;
     int steady_done = 0;   // steps of the current steady run before this segment
     unsigned long long *step_clock = nullptr;  // [max_steps + 1]: start, then end of every step
     double *chunk_part = nullptr;               // [4][256] first-level step reduction (large meshes)
@@ -128,7 +136,8 @@ static void free_ctx(cffc_ctx *c) {
                     c->counter, c->step_ctr, c->stop_flag, c->norm_out, c->dt_out, c->staging,
                     c->part_tile0, c->part_state, c->part_steps, c->part_l2, c->part_last, c->step_clock,
                     c->bc_blocks, c->Wf[0], c->Wf[1], c->chunk_part, c->rops, c->pops, c->fops, c->eflux,
-                    c->Ustage, c->ustore, c->Sd, c->tile_defer, c->defer_list};
+                    c->Ustage, c->ustore, c->Sd, c->tile_defer, c->defer_list, c->part_best,
+                    c->part_best_step, c->Sf};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
@@ -961,6 +970,10 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
                 free_ctx(c);
                 return e;
             }
+        if ((e = dalloc(&c->Sf, 4 * (size_t)ncp))) {
+            free_ctx(c);
+            return e;
+        }
     }
     // default tile 16 x 8 (the autotuner's pick at C2 size in both
     // precisions: 192 threads, three wavefronts, more resident workgroups per
@@ -1108,6 +1121,11 @@ extern "C" int cffc_get_dt(cffc_ctx *c, double *dt, int64_t n) { return c ? get_
 extern "C" int cffc_get_dUdt(cffc_ctx *c, int k, double *d, int64_t n) {
     if (!c || k < 0 || k > 2) return fail(CFFC_E_ARG, "bad residual index");
     if (!c->dUdt[k]) return fail(CFFC_E_ARG, "dUdt[%d] has not been computed", k);
+    if (k == 0 && c->sf_live) {
+        // the last fused mixed run kept its residual in FP32
+        HIPCHK(cffc_launch_widen(c->Sf, c->dUdt[0], 4 * c->ncp, c->stream));
+        c->sf_live = false;
+    }
     return get_field(c, c->dUdt[k], d, n, 4);
 }
 
@@ -1309,6 +1327,7 @@ extern "C" int cffc_freeze_limiters(cffc_ctx *c, int freeze) {
 extern "C" int cffc_dudt_stage(cffc_ctx *c, int i_stage) {
     if (!c) return fail(CFFC_E_ARG, "null context");
     if (i_stage < 1 || i_stage > c->s.n_stage) return fail(CFFC_E_ARG, "bad stage %d", i_stage);
+    c->sf_live = false;   // dUdt[k] is written in FP64 here
     if (int e = ensure_granular(c)) return e;
     if (int e = ensure_phi(c)) return e;
     if (i_stage == 1)
@@ -1440,9 +1459,15 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         p.Winf = c->Wf[c->wcur];
         p.Woutf = c->Wf[1 - c->wcur];
         p.S = c->dUdt[0];
+        // the FP32 accumulator, except where the residual smoothing shares
+        // dUdt[0] between the stored residual and the update
+        p.Sf = c->rs_iters > 0 ? nullptr : c->Sf;
         p.mode = fused_mode(c->s, st);
         p.omega = stage_omega(c->s, st);
         p.last_stage = (st == N);
+        // S is dead after this stage when the next one starts from zero
+        // (multistage optimal smoothing); the last stage always stores it
+        p.skip_S = (st < N && fused_mode(c->s, st + 1) == MODE_ZERO) ? 1 : 0;
         p.dtmin_in = c->scal + par;
         p.stop_flag = c->stop_flag;
         p.push = c->push_ok ? 1 : 0;
@@ -1513,6 +1538,9 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
         f.part_last = c->part_last;
         f.freeze_level = freeze_level;
         f.step_offset = c->steady_done;
+        f.stall_window = c->stall_window;
+        f.part_best = c->part_best;
+        f.part_best_step = c->part_best_step;
     }
     f.step_clock = c->step_clock ? c->step_clock + 1 : nullptr;
     // meshes of many tiles reduce their partials in two levels (256 chunks)
@@ -1523,6 +1551,7 @@ static int enqueue_step(cffc_ctx *c, int par, double tol, bool parts = false, do
 
 static int begin_run(cffc_ctx *c, int nsteps) {
     if (int e = ensure_steps(c, nsteps)) return e;
+    c->sf_live = c->Sf && c->rs_iters == 0;
     if (c->rs_iters > 0)
         if (int e = ensure_granular(c)) return e;   // dUdt[1] is the smoothing scratch
     if (c->store_phi || c->freeze)
@@ -1747,6 +1776,8 @@ extern "C" int cffc_set_parts(cffc_ctx *c, int npart, const int32_t *first_block
     for (int32_t **q : {&c->part_state, &c->part_steps})
         if (*q) { (void)hipFree(*q); *q = nullptr; }
     if (c->part_last) { (void)hipFree(c->part_last); c->part_last = nullptr; }
+    if (c->part_best) { (void)hipFree(c->part_best); c->part_best = nullptr; }
+    if (c->part_best_step) { (void)hipFree(c->part_best_step); c->part_best_step = nullptr; }
     c->npart = npart;
     c->part_of_block = pob;
     for (int b = 0; b < c->nblk; ++b) c->meta_h[b].spare = pob[b];
@@ -1754,6 +1785,8 @@ extern "C" int cffc_set_parts(cffc_ctx *c, int npart, const int32_t *first_block
     if (int e = dalloc(&c->part_state, npart)) return e;
     if (int e = dalloc(&c->part_steps, npart)) return e;
     if (int e = dalloc(&c->part_last, npart)) return e;
+    if (int e = dalloc(&c->part_best, npart)) return e;
+    if (int e = dalloc(&c->part_best_step, npart)) return e;
     return build_tiles(c, c->ti, c->tj);
 }
 
@@ -1780,6 +1813,9 @@ extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double fr
         HIPCHK(hipMemsetAsync(c->part_state, 0, c->npart * sizeof(int32_t), c->stream));       // PART_RUNNING
         HIPCHK(hipMemsetAsync(c->part_steps, 0xFF, c->npart * sizeof(int32_t), c->stream));    // -1
         HIPCHK(hipMemsetAsync(c->part_last, 0, c->npart * sizeof(double), c->stream));
+        // best L2 so far: 0x7F7F...7F = 1.4e306, above any norm
+        HIPCHK(hipMemsetAsync(c->part_best, 0x7F, c->npart * sizeof(double), c->stream));
+        HIPCHK(hipMemsetAsync(c->part_best_step, 0, c->npart * sizeof(int32_t), c->stream));
         c->steady_done = 0;
     } else {
         // a resumed run whose parts have all converged has nothing to do
@@ -1847,6 +1883,12 @@ extern "C" int cffc_run_steady(cffc_ctx *c, int max_steps, double tol, double fr
 }
 
 extern "C" int cffc_num_parts(const cffc_ctx *c) { return c ? c->npart : CFFC_E_ARG; }
+
+extern "C" int cffc_set_stall_freeze(cffc_ctx *c, int window) {
+    if (!c || window < 0) return fail(CFFC_E_ARG, "bad argument");
+    c->stall_window = window;
+    return 0;
+}
 
 extern "C" int cffc_step_times(cffc_ctx *c, int nsteps, double *seconds) {
     if (!c || !seconds || nsteps < 0) return fail(CFFC_E_ARG, "bad argument");

@@ -519,6 +519,14 @@ __global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) 
                         p.part_steps[pt] = p.step_offset + s + 1;
                     } else if (st == PART_RUNNING && p.freeze_level > 0.0 && l2p <= p.freeze_level) {
                         st = PART_FROZEN;
+                    } else if (st == PART_RUNNING && p.stall_window > 0) {
+                        const int gs = p.step_offset + s;
+                        if (l2p < p.part_best[pt]) {
+                            p.part_best[pt] = l2p;
+                            p.part_best_step[pt] = gs;
+                        } else if (gs - p.part_best_step[pt] >= p.stall_window) {
+                            st = PART_FROZEN;
+                        }
                     }
                     p.part_state[pt] = st;
                 } else if (p.part_l2) {
@@ -546,6 +554,12 @@ __global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) 
         *p.step_ctr = s + 1;
         if (p.npart == 0 ? (p.tol > 0.0 && l2n <= p.tol) : nd == p.npart) *p.stop_flag = 1;
     }
+}
+
+// FP32 -> FP64 copy (the mixed path's residual accumulator into dUdt[0])
+__global__ void widen_kernel(const float *src, double *dst, int64_t n) {
+    const int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (q < n) dst[q] = (double)src[q];
 }
 
 // interior W = W(U) (FP64 arithmetic), stored as ST
@@ -708,6 +722,10 @@ hipError_t cffc_launch_gather(const BlockMeta *meta, int nblk, int max_int, cons
 hipError_t cffc_launch_exchange(const ExchangeParams *p, int nblk, hipStream_t st) {
     if (p->Wf) hipLaunchKernelGGL(exchange_bc_kernel<float>, dim3(nblk), dim3(256), 0, st, *p);
     else hipLaunchKernelGGL(exchange_bc_kernel<double>, dim3(nblk), dim3(256), 0, st, *p);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream_t st) {
+    hipLaunchKernelGGL(widen_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
     return hipGetLastError();
 }
 hipError_t cffc_launch_cfl(const BlockMeta *meta, const TileDesc *tiles, int ntiles, int ti, int tj, const double *W,

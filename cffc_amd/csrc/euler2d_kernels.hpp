@@ -59,6 +59,11 @@ struct FinishParams {
     unsigned long long *step_clock;         // [max_steps] device wall clock at the end of each step, or null
     const double *chunk_part;               // [4][nchunk] chunk partials of norm_part (nchunk > 0, no parts)
     int32_t nchunk;
+    // stall freeze: a running part whose L2 has not set a new minimum for
+    // stall_window steps freezes its limiters (0: never)
+    int32_t stall_window;
+    double *part_best;                      // [npart] smallest L2 so far
+    int32_t *part_best_step;                // [npart] step of that L2
 };
 
 }  // namespace cffc
@@ -67,6 +72,7 @@ struct FinishParams {
 #define CFFC_TILE_LIST(X) X(16, 16) X(32, 8) X(8, 32) X(16, 8) X(8, 8) X(32, 4) X(32, 16) X(16, 4) X(8, 16)
 
 extern "C" {
+hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream_t st);
 hipError_t cffc_launch_geometry(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *xn,
                                 const double *yn, double *geom, int64_t ncp, unsigned int *ndeg, hipStream_t st);
 hipError_t cffc_launch_geometry_f32(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *geom,

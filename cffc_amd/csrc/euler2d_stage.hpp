@@ -38,6 +38,7 @@ struct StageParams {
     float *Woutf;
     double *U;              // conserved U^n (fused path)
     double *S;              // residual accumulator dUdt[k]
+    float *Sf;              // fused mixed path: the accumulator in FP32 (null: S)
     const double *S1, *S2;  // dUdt[1], dUdt[2] (granular RK4 stage 4)
     double *dt;             // per-cell time step (padded)
     const unsigned long long *dtmin_in;  // global dt (bit pattern) of this step
@@ -56,6 +57,7 @@ struct StageParams {
     const int32_t *nbr;     // 8 same-level neighbours per block (push exchange)
     int32_t push;           // fused: also write W^(s+1) into the neighbours' ghost cells
     int32_t defer_update;   // fused: store the residual only; update_tile_kernel follows
+    int32_t skip_S;         // fused: no stage after this one reads S (multistage smoothing), so it is not stored
     // resolution changes
     const uint8_t *tile_defer;  // fused: per tile, nonzero = store the residual in Sd (flux correction follows)
     double *Sd;             // residual of deferred tiles; update_tile_kernel reads it
@@ -367,7 +369,19 @@ __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMe
         // a coarser neighbour restricts U^(s+1) of this cell into its ghosts
         p.Ustage[c] = Un.d; p.Ustage[p.ncp + c] = Un.mx; p.Ustage[2 * p.ncp + c] = Un.my; p.Ustage[3 * p.ncp + c] = Un.E;
     }
-    if (p.mode == MODE_ZERO_ACC2) {
+    if (p.skip_S) {
+        // the next stage starts from zero and the last stage stores dUdt[0]
+    } else if (Fast<FT>::on && p.Sf) {
+        // mixed path: the RK accumulator holds FP32-flux residuals, so it
+        // is kept in FP32 (half the bytes); the stage's own dU stays FP64
+        if (p.mode == MODE_ZERO_ACC2) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) p.Sf[v * p.ncp + c] = (float)((double)p.Sf[v * p.ncp + c] + 2.0 * dU[v]);
+        } else {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) p.Sf[v * p.ncp + c] = (float)dU[v];
+        }
+    } else if (p.mode == MODE_ZERO_ACC2) {
 #pragma unroll
         for (int v = 0; v < 4; ++v) p.S[v * p.ncp + c] = p.S[v * p.ncp + c] + 2.0 * dU[v];
     } else {
@@ -677,7 +691,10 @@ __global__ __launch_bounds__((TileThreads<TI, TJ>::value)) void stage_kernel(Sta
         const double cdt = p.cfl * dtc;
         double dU[4];
         if (FUSED) {
-            if (p.mode == MODE_FROM_S) {
+            if (p.mode == MODE_FROM_S && Fast<FT>::on && p.Sf) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) dU[v] = (double)p.Sf[v * ncp + c];
+            } else if (p.mode == MODE_FROM_S) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) dU[v] = p.S[v * ncp + c];
             } else {

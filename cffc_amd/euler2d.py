@@ -76,6 +76,7 @@ def load():
     pi32 = ctypes.POINTER(ctypes.c_int32)
     L.cffc_set_parts.argtypes = [vp, i32, pi32]
     L.cffc_num_parts.argtypes = [vp]
+    L.cffc_set_stall_freeze.argtypes = [vp, i32]
     L.cffc_run_steady.argtypes = [vp, i32, d, d, i32, pi32, pd, ctypes.POINTER(ctypes.c_int)]
     L.cffc_get_part_history.argtypes = [vp, i32, pd]
     L.cffc_step_times.argtypes = [vp, i32, pd]
@@ -273,6 +274,11 @@ class Euler2DQuadBlocks:
         _check(load().cffc_set_parts(self.h, len(fb) - 1, fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
                "cffc_set_parts")
         self.npart = len(fb) - 1
+
+    def set_stall_freeze(self, window):
+        """Freeze a running part's limiters once its L2 has not set a new
+        minimum for `window` steps (no counterpart in the reference; 0: off)."""
+        _check(load().cffc_set_stall_freeze(self.h, int(window)), "cffc_set_stall_freeze")
 
     def run_steady(self, max_steps, tol, freeze_level=0.0, history=False, resume=False):
         """Every part to its own L2 <= tol (limiters frozen at freeze_level).

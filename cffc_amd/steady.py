@@ -100,13 +100,15 @@ class BatchResult:
 
 
 def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=STEADY_FREEZE_LEVEL, history=False,
-                segment=0, log=None):
+                segment=0, log=None, stall_window=0):
     """Solve the cases [(mesh, W0), ...] together: one mesh of disjoint
     parts, one launch per stage for all of them, each part converging (and
     freezing its limiters) on its own.  segment > 0 drives the run in
     resumed segments of that many steps (same result), logging progress
-    after each.  Returns a BatchResult with the steps of every case and the
-    final state arrays."""
+    after each.  stall_window > 0 also freezes a part's limiters after that
+    many steps without a new L2 minimum (a serving policy; the reference
+    freezes at freeze_level only).  Returns a BatchResult with the steps of
+    every case and the final state arrays."""
     from . import euler2d
     meshes = [m for m, _ in cases]
     mc, starts = M.concat(meshes)
@@ -114,6 +116,8 @@ def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=ST
     g = euler2d.Euler2DQuadBlocks(mc, scheme or bump_scheme())
     try:
         g.set_parts(starts + [mc.nblk])
+        if stall_window:
+            g.set_stall_freeze(stall_window)
         g.set_W(W0)
         g.synchronize()
         t0 = time.perf_counter()

@@ -113,3 +113,41 @@ def test_fp32_flux_then_fp64_finish():
           "fp64 only", ref.part_steps.tolist())
     for a, b in zip(second.states, ref.states):
         assert np.max(np.abs(a - b)) < 1e-3
+
+
+def _stall_step(hist, window, freeze_level):
+    """The step at which the stall rule fires on a part's L2 history (the
+    finish kernel's rule restated): the first step, while the part runs,
+    that comes window steps after its last new minimum; None if never."""
+    best, best_step = np.inf, 0
+    for s, l2 in enumerate(hist):
+        if freeze_level > 0.0 and l2 <= freeze_level:
+            return None
+        if l2 < best:
+            best, best_step = l2, s
+        elif s - best_step >= window:
+            return s
+    return None
+
+
+def test_stall_freeze_fires_where_the_rule_says():
+    """cffc_set_stall_freeze: up to and including a part's stall step the
+    run is the reference's bit for bit (equal L2 histories); the next step
+    loads the frozen limiters, so the part's history leaves the
+    reference's; parts that never stall are untouched."""
+    cases = [_case(0.08, 0.5), _case(0.05, 0.3), _case(0.09, 0.58)]
+    window, n = 40, 1500
+    a = steady.solve_batch(cases, tol=1e-14, max_steps=n, freeze_level=0.0, history=True)
+    b = steady.solve_batch(cases, tol=1e-14, max_steps=n, freeze_level=0.0, history=True, stall_window=window)
+    assert a.rc == 0 and b.rc == 0 and a.history.shape == b.history.shape == (n, 3)
+    fired = 0
+    for q in range(3):
+        f = _stall_step(a.history[:, q], window, 0.0)
+        print("part %d: stall step %s" % (q, f), flush=True)
+        if f is None:
+            np.testing.assert_array_equal(b.history[:, q], a.history[:, q])
+            continue
+        fired += 1
+        np.testing.assert_array_equal(b.history[:f + 1, q], a.history[:f + 1, q])
+        assert np.any(b.history[f + 1:, q] != a.history[f + 1:, q])
+    assert fired >= 1
