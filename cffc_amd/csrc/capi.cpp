@@ -892,11 +892,6 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
     HIPCHK(hipMemcpy(c->meta, c->meta_h.data(), md->nblk * sizeof(BlockMeta), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->nbr, same.data(), 8 * md->nblk * sizeof(int32_t), hipMemcpyHostToDevice));
     if (c->amr) {
-        // copies first, then restrictions, each in ghost order: a wavefront
-        // then runs one kind of operation (the ops are independent: every
-        // one reads interior cells and writes one ghost cell)
-        std::stable_sort(rops.begin(), rops.end(),
-                         [](const RestrictOp &a, const RestrictOp &b) { return a.kind > b.kind; });
         c->nrops = (int)rops.size();
         c->npops = (int)pops.size();
         c->nfops = (int)fops.size();

@@ -44,7 +44,7 @@ template <typename ST> __device__ __forceinline__ void amr_stW(ST *W, int64_t nc
 }
 
 template <typename ST>
-__global__ __launch_bounds__(256) void restrict_kernel(const RestrictOp *ops, int nops, const double *U,
+__global__ __launch_bounds__(64) void restrict_kernel(const RestrictOp *ops, int nops, const double *U,
                                                        const double *geom, int64_t ncp, ST *W,
                                                        const int32_t *stop_flag, int copy_from_U) {
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
@@ -190,9 +190,8 @@ extern "C" hipError_t cffc_launch_restrict(const RestrictOp *ops, int nops, cons
                                            int64_t ncp, double *W, float *Wf, const int32_t *stop_flag,
                                            int copy_from_U, hipStream_t st) {
     if (nops <= 0) return hipSuccess;
-    // one wavefront per workgroup for short lists (spread over more CUs),
-    // four for long ones (the same-level copies of a lattice mesh)
-    const int nt = nops >= 64 * 1024 ? 256 : 64;
+    // one wavefront per workgroup: short lists spread over more CUs
+    const int nt = 64;
     const int nb = (nops + nt - 1) / nt;
     if (Wf)
         hipLaunchKernelGGL((restrict_kernel<float>), dim3(nb), dim3(nt), 0, st, ops, nops, U, geom, ncp, Wf,
