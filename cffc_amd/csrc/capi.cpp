@@ -53,6 +53,7 @@ struct cffc_ctx {
     int nblk = 0;
     int64_t ncp = 0, ncell = 0;
     int max_int = 0, max_cells = 0;
+    int max_edge = 0;   // largest padded block edge (nci or ncj): sizes the exchange workgroups
     std::vector<BlockMeta> meta_h;
     BlockMeta *meta = nullptr;
     int32_t *nbr = nullptr, *bc = nullptr;
@@ -778,6 +779,7 @@ static int build_meta(cffc_ctx *c, const cffc_mesh_desc *md, int32_t &bcoff, int
         bcoff += 2 * m.ncj + 2 * m.nci;
         c->max_int = std::max(c->max_int, m.ni * m.nj);
         c->max_cells = std::max(c->max_cells, m.nci * m.ncj);
+        c->max_edge = std::max(c->max_edge, std::max(m.nci, m.ncj));
     }
     for (int b = 0; b < md->nblk; ++b) {
         const BlockMeta &m = c->meta_h[b];
@@ -1155,6 +1157,10 @@ static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long lon
     p.Wf = nullptr;
     p.W2 = nullptr;
     p.Wf2 = nullptr;
+    // every phase of the exchange has at most 2 x (padded edge) items per
+    // block: small blocks (the coarse multigrid levels) get one wavefront
+    const int items = 2 * c->max_edge;
+    p.nthreads = items <= 64 ? 64 : items <= 128 ? 128 : 256;
     return p;
 }
 
@@ -2202,8 +2208,9 @@ static int mg_stage(cffc_ctx *c, int i_stage, int residual, const double *P) {
 // Smooth :2865-3110
 static int mg_smooth(cffc_mg *mg, int top, int l) {
     cffc_ctx *c = mg->lv[l];
-    HIPCHK(cffc_launch_set_u64(c->scal, MILLION_BITS, 0, 1, c->stream));
-    HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, c->scal,
+    // local time steps only (the levels' stages read dt per cell): no global
+    // minimum is formed
+    HIPCHK(cffc_launch_cfl(c->meta, c->tiles, c->ntiles, c->ti, c->tj, c->W[c->wcur], c->geom, c->ncp, c->dt, nullptr,
                            c->stream));
     if (l > top) {
         cffc_ctx *f = mg->lv[l - 1];

@@ -326,7 +326,8 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
 
 // ---------------------------------------------------------------------------
 // CFL over the interior cells of each tile; global minimum with atomicMin on
-// the bit pattern (positive doubles order like their bits).
+// the bit pattern (positive doubles order like their bits), unless dtmin is
+// null (local time steps only).
 __global__ __launch_bounds__(256) void cfl_kernel(const BlockMeta *meta, const TileDesc *tiles, int ti, int tj,
                                                   const double *W, const double *geom, int64_t ncp, double *dt,
                                                   unsigned long long *dtmin) {
@@ -357,7 +358,7 @@ __global__ __launch_bounds__(256) void cfl_kernel(const BlockMeta *meta, const T
     for (int off = 32; off > 0; off >>= 1) dmn = dmin(dmn, __shfl_xor(dmn, off));
     if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = dmn;
     __syncthreads();
-    if (threadIdx.x == 0) {
+    if (threadIdx.x == 0 && dtmin) {
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) dmn = dmin(dmn, sm[w]);
         dmn = dmin(sm[0], dmn);
         atomicMin(dtmin, dbits(dmn));
@@ -720,8 +721,9 @@ hipError_t cffc_launch_gather(const BlockMeta *meta, int nblk, int max_int, cons
     return hipGetLastError();
 }
 hipError_t cffc_launch_exchange(const ExchangeParams *p, int nblk, hipStream_t st) {
-    if (p->Wf) hipLaunchKernelGGL(exchange_bc_kernel<float>, dim3(nblk), dim3(256), 0, st, *p);
-    else hipLaunchKernelGGL(exchange_bc_kernel<double>, dim3(nblk), dim3(256), 0, st, *p);
+    const int nt = (p->nthreads == 64 || p->nthreads == 128) ? p->nthreads : 256;
+    if (p->Wf) hipLaunchKernelGGL(exchange_bc_kernel<float>, dim3(nblk), dim3(nt), 0, st, *p);
+    else hipLaunchKernelGGL(exchange_bc_kernel<double>, dim3(nblk), dim3(nt), 0, st, *p);
     return hipGetLastError();
 }
 hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream_t st) {
@@ -730,7 +732,9 @@ hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream
 }
 hipError_t cffc_launch_cfl(const BlockMeta *meta, const TileDesc *tiles, int ntiles, int ti, int tj, const double *W,
                            const double *geom, int64_t ncp, double *dt, unsigned long long *dtmin, hipStream_t st) {
-    hipLaunchKernelGGL(cfl_kernel, dim3(ntiles), dim3(256), 0, st, meta, tiles, ti, tj, W, geom, ncp, dt, dtmin);
+    // one thread per tile cell, in whole wavefronts up to 256
+    const int nt = ti * tj <= 64 ? 64 : ti * tj <= 128 ? 128 : 256;
+    hipLaunchKernelGGL(cfl_kernel, dim3(ntiles), dim3(nt), 0, st, meta, tiles, ti, tj, W, geom, ncp, dt, dtmin);
     return hipGetLastError();
 }
 hipError_t cffc_launch_set_global_dt(const BlockMeta *meta, int nblk, int max_int, double *dt,

@@ -29,6 +29,7 @@ struct ExchangeParams {
     int32_t no_bcs;            // same-level exchange only (the resolution-change legs and BCs follow)
     double *W2;                // if not null, the BC ghost states are also written here (FP64 storage) ...
     float *Wf2;                // ... or here (FP32 storage): the next stage's W buffer, see exchange_bc_kernel
+    int32_t nthreads;          // workgroup size (64, 128 or 256; 0: 256)
 };
 
 
