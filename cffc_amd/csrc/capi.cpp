@@ -217,9 +217,12 @@ static int build_tiles(cffc_ctx *c, int ti, int tj) {
     return 0;
 }
 
-static bool tile_supported(int ti, int tj) {
+static bool tile_supported(int ti, int tj, bool fp64) {
 #define T(A, B) if (ti == A && tj == B) return true;
     CFFC_TILE_LIST(T)
+    if (fp64) {
+        CFFC_SMALL_TILE_LIST(T)
+    }
 #undef T
     return false;
 }
@@ -979,11 +982,13 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
     }
     // default tile 16 x 8 (the autotuner's pick at C2 size in both
     // precisions: 192 threads, three wavefronts, more resident workgroups per
-    // CU than 16 x 16); 8 x 8 for blocks narrower than 16 cells
+    // CU than 16 x 16); 8 x 8 for blocks narrower than 16 cells; in FP64, 4 x 4
+    // (one wavefront per block) for blocks narrower than 8 cells
     int ti = 16, tj = 8;
     int mini = 1 << 30;
     for (auto &m : c->meta_h) mini = std::min(mini, m.ni);
     if (mini < 16) ti = tj = 8;
+    if (mini < 8 && c->s.flux_precision == CFFC_FLUX_FP64) ti = tj = 4;
     if ((e = build_tiles(c, ti, tj))) {
         free_ctx(c);
         return e;
@@ -1078,7 +1083,8 @@ extern "C" void *cffc_stream(cffc_ctx *c) { return c ? (void *)c->stream : nullp
 
 extern "C" int cffc_set_tile(cffc_ctx *c, int ti, int tj) {
     if (!c) return fail(CFFC_E_ARG, "null context");
-    if (!tile_supported(ti, tj)) return fail(CFFC_E_ARG, "tile %dx%d is not instantiated", ti, tj);
+    if (!tile_supported(ti, tj, c->s.flux_precision == CFFC_FLUX_FP64))
+        return fail(CFFC_E_ARG, "tile %dx%d is not instantiated", ti, tj);
     HIPCHK(hipStreamSynchronize(c->stream));
     return build_tiles(c, ti, tj);
 }

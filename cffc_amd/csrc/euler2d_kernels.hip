@@ -813,6 +813,7 @@ hipError_t cffc_launch_stage_fp64(const StageParams *p, int ti, int tj, int fuse
         return hipGetLastError();                                                                          \
     }
     CFFC_TILE_LIST(L)
+    CFFC_SMALL_TILE_LIST(L)
 #undef L
     return hipErrorInvalidValue;
 }

@@ -71,6 +71,9 @@ struct FinishParams {
 
 // tile shapes the stage kernel is instantiated for (TI x TJ interior cells)
 #define CFFC_TILE_LIST(X) X(16, 16) X(32, 8) X(8, 32) X(16, 8) X(8, 8) X(32, 4) X(32, 16) X(16, 4) X(8, 16)
+// FP64 only, not autotuned: the default tile of meshes whose blocks are
+// narrower than 8 cells (the coarse multigrid levels), one wavefront each
+#define CFFC_SMALL_TILE_LIST(X) X(4, 4)
 
 extern "C" {
 hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream_t st);
