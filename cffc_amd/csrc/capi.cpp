@@ -1163,9 +1163,10 @@ static ExchangeParams exchange_params(cffc_ctx *c, int from_U, unsigned long lon
     p.Wf = nullptr;
     p.W2 = nullptr;
     p.Wf2 = nullptr;
-    // every phase of the exchange has at most 2 x (padded edge) items per
-    // block: small blocks (the coarse multigrid levels) get one wavefront
-    const int items = 2 * c->max_edge;
+    // the same-level phase has 4 (ni + nj) + 16 ghost cells per block, about
+    // 8 x (padded edge) - 16, the BC phases 2 x (padded edge): small blocks
+    // (the coarse multigrid levels) get one wavefront
+    const int items = 8 * c->max_edge - 16;
     p.nthreads = items <= 64 ? 64 : items <= 128 ? 128 : 256;
     return p;
 }

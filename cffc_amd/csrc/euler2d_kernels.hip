@@ -169,23 +169,39 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     // phase 0: same-level neighbours, faces (2 layers over the interior span)
     // and corners (2 x 2); U is transferred and W = W(U) recomputed on stage 1
     // (Euler2DQuad.h:1643-1647); later stages copy W, which equals W(U).
-    for (int s = 0; s < (p.bcs_only ? 0 : 8); ++s) {
-        const int nb = p.nbr[8 * b + s];
-        if (nb < 0) continue;
-        const BlockMeta S = p.meta[nb];
-        int i0, i1, j0, j1, di, dj;
-        switch (s) {
-        case 0: i0 = ICl; i1 = ICu; j0 = JCu + 1; j1 = JCu + 2; di = 2 - ICl; dj = 2 - (JCu + 1); break;
-        case 1: i0 = ICl; i1 = ICu; j0 = JCl - 2; j1 = JCl - 1; di = 2 - ICl; dj = (S.nj + 1) - (JCl - 1); break;
-        case 2: i0 = ICu + 1; i1 = ICu + 2; j0 = JCl; j1 = JCu; di = 2 - (ICu + 1); dj = 2 - JCl; break;
-        case 3: i0 = ICl - 2; i1 = ICl - 1; j0 = JCl; j1 = JCu; di = (S.ni + 1) - (ICl - 1); dj = 2 - JCl; break;
-        case 4: i0 = ICu + 1; i1 = ICu + 2; j0 = JCu + 1; j1 = JCu + 2; di = 2 - (ICu + 1); dj = 2 - (JCu + 1); break;
-        case 5: i0 = ICl - 2; i1 = ICl - 1; j0 = JCu + 1; j1 = JCu + 2; di = (S.ni + 1) - (ICl - 1); dj = 2 - (JCu + 1); break;
-        case 6: i0 = ICu + 1; i1 = ICu + 2; j0 = JCl - 2; j1 = JCl - 1; di = 2 - (ICu + 1); dj = (S.nj + 1) - (JCl - 1); break;
-        default: i0 = ICl - 2; i1 = ICl - 1; j0 = JCl - 2; j1 = JCl - 1; di = (S.ni + 1) - (ICl - 1); dj = (S.nj + 1) - (JCl - 1); break;
+    // The eight neighbours' tables are read once into LDS and the ghost
+    // cells of all eight segments form one index space, so that every
+    // thread's loads are independent of the other segments' stores.
+    __shared__ int32_t sNb[8];
+    __shared__ BlockMeta sMeta[8];
+    if (!p.bcs_only) {
+        if (tid < 8) {
+            const int nb = p.nbr[8 * b + tid];
+            sNb[tid] = nb;
+            if (nb >= 0) sMeta[tid] = p.meta[nb];
         }
-        const int w = i1 - i0 + 1, h = j1 - j0 + 1;
-        for (int q = tid; q < w * h; q += NT) {
+        __syncthreads();
+        int start[9];
+        start[0] = 0;
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+            start[s + 1] = start[s] + (sNb[s] < 0 ? 0 : s < 2 ? 2 * m.ni : s < 4 ? 2 * m.nj : 4);
+        for (int k = tid; k < start[8]; k += NT) {
+            int s = 0;
+            while (k >= start[s + 1]) ++s;
+            const int q = k - start[s];
+            const BlockMeta &S = sMeta[s];
+            int i0, j0, w, di, dj;
+            switch (s) {
+            case 0: i0 = ICl; j0 = JCu + 1; w = m.ni; di = 2 - ICl; dj = 2 - (JCu + 1); break;
+            case 1: i0 = ICl; j0 = JCl - 2; w = m.ni; di = 2 - ICl; dj = (S.nj + 1) - (JCl - 1); break;
+            case 2: i0 = ICu + 1; j0 = JCl; w = 2; di = 2 - (ICu + 1); dj = 2 - JCl; break;
+            case 3: i0 = ICl - 2; j0 = JCl; w = 2; di = (S.ni + 1) - (ICl - 1); dj = 2 - JCl; break;
+            case 4: i0 = ICu + 1; j0 = JCu + 1; w = 2; di = 2 - (ICu + 1); dj = 2 - (JCu + 1); break;
+            case 5: i0 = ICl - 2; j0 = JCu + 1; w = 2; di = (S.ni + 1) - (ICl - 1); dj = 2 - (JCu + 1); break;
+            case 6: i0 = ICu + 1; j0 = JCl - 2; w = 2; di = 2 - (ICu + 1); dj = (S.nj + 1) - (JCl - 1); break;
+            default: i0 = ICl - 2; j0 = JCl - 2; w = 2; di = (S.ni + 1) - (ICl - 1); dj = (S.nj + 1) - (JCl - 1); break;
+            }
             const int i = i0 + q % w, j = j0 + q / w;
             const int64_t cd = m.coff + (int64_t)j * m.pitch + i;
             const int64_t cs = S.coff + (int64_t)(j + dj) * S.pitch + (i + di);
@@ -521,8 +537,10 @@ __global__ __launch_bounds__(FINISH_NT) void step_finish_kernel(FinishParams p) 
                     } else if (st == PART_RUNNING && p.freeze_level > 0.0 && l2p <= p.freeze_level) {
                         st = PART_FROZEN;
                     } else if (st == PART_RUNNING && p.stall_window > 0) {
+                        // a new minimum must be 1% below the best so far, so
+                        // that a cycling norm's small dips do not count
                         const int gs = p.step_offset + s;
-                        if (l2p < p.part_best[pt]) {
+                        if (l2p < 0.99 * p.part_best[pt]) {
                             p.part_best[pt] = l2p;
                             p.part_best_step[pt] = gs;
                         } else if (gs - p.part_best_step[pt] >= p.stall_window) {

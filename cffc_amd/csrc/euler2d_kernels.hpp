@@ -60,8 +60,8 @@ struct FinishParams {
     unsigned long long *step_clock;         // [max_steps] device wall clock at the end of each step, or null
     const double *chunk_part;               // [4][nchunk] chunk partials of norm_part (nchunk > 0, no parts)
     int32_t nchunk;
-    // stall freeze: a running part whose L2 has not set a new minimum for
-    // stall_window steps freezes its limiters (0: never)
+    // stall freeze: a running part whose L2 has not gone 1% below its best
+    // for stall_window steps freezes its limiters (0: never)
     int32_t stall_window;
     double *part_best;                      // [npart] smallest L2 so far
     int32_t *part_best_step;                // [npart] step of that L2

@@ -276,8 +276,8 @@ class Euler2DQuadBlocks:
         self.npart = len(fb) - 1
 
     def set_stall_freeze(self, window):
-        """Freeze a running part's limiters once its L2 has not set a new
-        minimum for `window` steps (no counterpart in the reference; 0: off)."""
+        """Freeze a running part's limiters once its L2 has not gone 1% below
+        its best for `window` steps (no counterpart in the reference; 0: off)."""
         _check(load().cffc_set_stall_freeze(self.h, int(window)), "cffc_set_stall_freeze")
 
     def run_steady(self, max_steps, tol, freeze_level=0.0, history=False, resume=False):

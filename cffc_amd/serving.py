@@ -44,10 +44,12 @@ def warm_states(net, norm, cases):
 
 
 def serve(cases, initial=None, tol=1e-10, max_steps=400000, freeze_level=steady.STEADY_FREEZE_LEVEL, mixed=True,
-          switch_level=3e-4, segment=20000, log=None):
+          switch_level=3e-4, segment=20000, log=None, stall_window=0):
     """Solve all cases in one batch.  mixed: FP32 flux down to switch_level,
     then FP64 to tol (steady.solve_batch_fp32_then_fp64); else FP64 only.
-    Returns a dict of the batch figures."""
+    stall_window > 0: the FP64 phase also freezes a case's limiters after
+    that many steps without its L2 going 1% below its best.  Returns a dict
+    of the batch figures."""
     batch = []
     for q, (h, m, k) in enumerate(cases):
         mesh, W0 = case_mesh(h, m, k)
@@ -55,7 +57,8 @@ def serve(cases, initial=None, tol=1e-10, max_steps=400000, freeze_level=steady.
     t0 = time.perf_counter()
     if mixed:
         first, r = steady.solve_batch_fp32_then_fp64(batch, tol=tol, switch_level=switch_level, max_steps=max_steps,
-                                                     freeze_level=freeze_level, segment=segment, log=log)
+                                                     freeze_level=freeze_level, segment=segment, log=log,
+                                                     stall_window=stall_window)
         if r is None:
             return dict(rc=first.rc, n=len(cases), error="first phase failed")
         s1 = np.where(first.part_steps > 0, first.part_steps, first.steps_done)
@@ -65,7 +68,7 @@ def serve(cases, initial=None, tol=1e-10, max_steps=400000, freeze_level=steady.
                       fp64_steps=r.part_steps.tolist(), fp64_seconds=r.seconds)
     else:
         r = steady.solve_batch(batch, tol=tol, max_steps=max_steps, freeze_level=freeze_level, segment=segment,
-                               log=log)
+                               log=log, stall_window=stall_window)
         steps, lat, phases = r.part_steps, r.part_seconds, {}
     wall = time.perf_counter() - t0
     ok = np.isfinite(lat) & (r.part_steps > 0)
@@ -160,9 +163,15 @@ def main(argv=None):
     ap.add_argument("--solver", choices=["single", "multigrid"], default="single",
                     help="single-grid batched steady runs, or FAS multigrid over the batch")
     ap.add_argument("--levels", type=int, default=4, help="multigrid levels")
+    ap.add_argument("--pick", default=None, metavar="I,J,...",
+                    help="serve only these cases of the n drawn (indices into the seed's list)")
+    ap.add_argument("--stall-window", type=int, default=0,
+                    help="single grid: freeze a case's limiters after this many FP64 steps without a 1%% lower L2")
     a = ap.parse_args(argv)
     log = lambda s: print(s, flush=True)
     cases = serving_cases(a.n, a.seed)
+    if a.pick:
+        cases = [cases[int(i)] for i in a.pick.split(",")]
     log("%d cases, %d cells" % (len(cases), sum(96 * k * 32 * k for _, _, k in cases)))
     if a.solver == "multigrid":
         out = {}
@@ -182,7 +191,7 @@ def main(argv=None):
             with open(a.out, "w") as fo:
                 json.dump({"cases": cases, **out}, fo)
         return
-    kw = dict(mixed=not a.fp64, max_steps=a.max_steps, log=log)
+    kw = dict(mixed=not a.fp64, max_steps=a.max_steps, log=log, stall_window=a.stall_window)
     cold = warm = None
     if a.only == "warm" or (a.only == "both" and a.model):
         if not a.model:

@@ -106,7 +106,7 @@ def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=ST
     freezing its limiters) on its own.  segment > 0 drives the run in
     resumed segments of that many steps (same result), logging progress
     after each.  stall_window > 0 also freezes a part's limiters after that
-    many steps without a new L2 minimum (a serving policy; the reference
+    many steps without a new L2 minimum 1% below the best (a serving policy; the reference
     freezes at freeze_level only).  Returns a BatchResult with the steps of
     every case and the final state arrays."""
     from . import euler2d
@@ -154,14 +154,14 @@ def solve_batch(cases, scheme=None, tol=1e-10, max_steps=200000, freeze_level=ST
 
 
 def solve_batch_fp32_then_fp64(cases, tol=1e-10, switch_level=1e-5, max_steps=200000,
-                               freeze_level=STEADY_FREEZE_LEVEL, segment=0, log=None):
+                               freeze_level=STEADY_FREEZE_LEVEL, segment=0, log=None, stall_window=0):
     """Mixed-precision steady batch: every part first marches with FP32
     reconstruction/limiter/Roe flux (FP64 state and update) down to
     switch_level, above the FP32 round-off floor of the residual, then the
     FP64 path takes the conserved state and finishes to tol (limiters frozen
     at freeze_level as usual).  Steps and latencies of a part add over the
     two phases; the second phase starts when the first has ended for all
-    parts."""
+    parts.  stall_window applies to the FP64 phase (solve_batch)."""
     first = solve_batch(cases, scheme=bump_scheme(flux_precision=abi.FLUX_FP32), tol=switch_level,
                         max_steps=max_steps, freeze_level=0.0, segment=segment, log=log)
     if first.rc != 0:
@@ -169,7 +169,8 @@ def solve_batch_fp32_then_fp64(cases, tol=1e-10, switch_level=1e-5, max_steps=20
     # the FP64 phase restarts from the FP64 state the first phase ended in
     meshes = [m for m, _ in cases]
     second = solve_batch([(m, W) for m, W in zip(meshes, first.states)], tol=tol,
-                         max_steps=max_steps, freeze_level=freeze_level, segment=segment, log=log)
+                         max_steps=max_steps, freeze_level=freeze_level, segment=segment, log=log,
+                         stall_window=stall_window)
     return first, second
 
 

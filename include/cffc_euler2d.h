@@ -342,8 +342,8 @@ int  cffc_num_parts(const cffc_ctx *ctx);
 /* Stall freeze for batched steady runs: no counterpart in the reference,
    whose Freeze_Limiters waits for Freeze_Limiter_Residual_Level alone.  With
    window > 0, cffc_run_steady also freezes a running part's limiters once
-   its L2 norm has not set a new minimum for window consecutive steps (the
-   limiter cycling that keeps a case above the freeze level); window = 0
+   its L2 norm has not gone 1% below its best for window consecutive steps
+   (the limiter cycling that keeps a case above the freeze level); window = 0
    (the default) keeps the reference's rule only.  Counted from the start of
    the run, across resumed segments. */
 int  cffc_set_stall_freeze(cffc_ctx *ctx, int window);

@@ -118,12 +118,13 @@ def test_fp32_flux_then_fp64_finish():
 def _stall_step(hist, window, freeze_level):
     """The step at which the stall rule fires on a part's L2 history (the
     finish kernel's rule restated): the first step, while the part runs,
-    that comes window steps after its last new minimum; None if never."""
+    that comes window steps after its last new minimum (1% below the best
+    before it); None if never."""
     best, best_step = np.inf, 0
     for s, l2 in enumerate(hist):
         if freeze_level > 0.0 and l2 <= freeze_level:
             return None
-        if l2 < best:
+        if l2 < 0.99 * best:
             best, best_step = l2, s
         elif s - best_step >= window:
             return s
