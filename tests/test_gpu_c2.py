@@ -4,7 +4,7 @@ density perturbation, RK4 / Roe / least squares / Venkatakrishnan, CFL 0.5,
 global time step, constant extrapolation at the domain boundary.
 
 * FP64 for 5 steps against the CPU oracle on the host's cores: every value
-  bit for bit (the north star's bar is 1e-12 relative per cell).
+  bit for bit (the required bar is 1e-12 relative per cell).
 * The mixed path (FP32 reconstruction, limiter and Roe flux over the FP64
   state) for 500 steps against FP64 for 500 steps: the L1 / Linf error of
   each primitive variable is printed and bounded."""
