@@ -315,15 +315,17 @@ template <typename FT> __device__ __forceinline__ bool positivity_fails(const CS
 // accumulator, and at the last stage U^(n+1), the norm terms and the next
 // step's CFL time step (SingleBlock.cc:3036-3053).
 // lW, lE, lS, lN: the cell's face lengths as the stage read them (FP64
-// planes in the FP64 path, FP32 planes in the mixed path).
+// planes in the FP64 path, FP32 planes in the mixed path); U0: U^n of the
+// cell; Sacc: the FP64 accumulator's value as the caller loaded it, if
+// have_Sacc.
 template <typename FT>
 __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMeta &m, const TileDesc &t, int64_t c,
                                              int ip, int jp, double A, double dtc, double lW, double lE, double lS,
-                                             double lN, double (&dU)[4], double &l1, double &l2, double &mx,
-                                             double &dtn) {
+                                             double lN, const double (&U0)[4], bool have_Sacc,
+                                             const double (&Sacc)[4], double (&dU)[4], double &l1, double &l2,
+                                             double &mx, double &dtn) {
     const double om = p.omega;
     CS<double> Un;
-    const double U0[4] = {p.U[c], p.U[p.ncp + c], p.U[2 * p.ncp + c], p.U[3 * p.ncp + c]};
     Un.d = U0[0] + om * dU[0]; Un.mx = U0[1] + om * dU[1];
     Un.my = U0[2] + om * dU[2]; Un.E = U0[3] + om * dU[3];
     bool bad = positivity_fails<FT>(Un);
@@ -383,7 +385,7 @@ __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMe
         }
     } else if (p.mode == MODE_ZERO_ACC2) {
 #pragma unroll
-        for (int v = 0; v < 4; ++v) p.S[v * p.ncp + c] = p.S[v * p.ncp + c] + 2.0 * dU[v];
+        for (int v = 0; v < 4; ++v) p.S[v * p.ncp + c] = (have_Sacc ? Sacc[v] : p.S[v * p.ncp + c]) + 2.0 * dU[v];
     } else {
 #pragma unroll
         for (int v = 0; v < 4; ++v) p.S[v * p.ncp + c] = dU[v];
@@ -576,13 +578,52 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
             sPy[2][jr][ir] = ph.vy * gy.vy; sPy[3][jr][ir] = ph.p * gy.p;
         }
     }
+
+    // Faces of phase 3: a tile thread (tc,tr) takes its E face (i-face
+    // tc+1) and N face (j-face tr+1); W-ring thread k the i-face 0 of row k,
+    // S-ring thread k the j-face 0 of column k (E/N-ring threads have none);
+    // in the ring pass the first TI + TJ threads also take the W ring face
+    // of row k = tid or the S ring face of column k = tid - TJ.
+    // A face's normal and midpoint are read from the cell on its low side.
+    struct FaceGeo { XT nx, ny, xf, yf; };
+    auto face_cell = [&](bool idir, int f, int g, int64_t &cL) -> bool {
+        const int ipL = idir ? pi0 + 1 + f : pi0 + 2 + g;
+        const int jpL = idir ? pj0 + 2 + g : pj0 + 1 + f;
+        cL = m.coff + (int64_t)jpL * m.pitch + ipL;
+        return jpL <= JCu && ipL <= ICu;
+    };
+    auto face_geo = [&](bool idir, int64_t cL) -> FaceGeo {
+        FaceGeo q;
+        q.nx = idir ? rdgeo<FT>(p, GEO_NEX, GF_NEX, cL) : rdgeo<FT>(p, GEO_NNX, GF_NNX, cL);
+        q.ny = idir ? rdgeo<FT>(p, GEO_NEY, GF_NEY, cL) : rdgeo<FT>(p, GEO_NNY, GF_NNY, cL);
+        q.xf = idir ? rdgeo<FT>(p, GEO_XFE, GF_XFE, cL) : rdgeo<FT>(p, GEO_XFN, GF_XFN, cL);
+        q.yf = idir ? rdgeo<FT>(p, GEO_YFE, GF_YFE, cL) : rdgeo<FT>(p, GEO_YFN, GF_YFN, cL);
+        return q;
+    };
+    const bool ring_face = RP && tid < TI + TJ;
+    const bool ring_w = tid < TJ;
+    const int ring_fk = ring_w ? tid : tid - TJ;
+    int64_t cF1 = 0, cF2 = 0, cFR = 0;
+    bool v1 = false, v2 = false, vR = false;
+    if (is_tile) {
+        v1 = face_cell(true, tc + 1, tr, cF1);
+        v2 = face_cell(false, tr + 1, tc, cF2);
+    } else if (ring_side == 0) {
+        v1 = face_cell(true, 0, ring_k, cF1);
+    } else if (ring_side == 2) {
+        v2 = face_cell(false, 0, ring_k, cF2);
+    }
+    if (ring_face) vR = face_cell(ring_w, 0, ring_fk, cFR);
+    // issued before the barrier: the loads are in flight while the
+    // workgroup's last reconstructions finish
+    FaceGeo g1{}, g2{}, gR{};
+    if (v1) g1 = face_geo(true, cF1);
+    if (v2) g2 = face_geo(false, cF2);
+    if (vR) gR = face_geo(ring_w, cFR);
     __syncthreads();
 
     // ---- phase 3: Roe fluxes ------------------------------------------------
-    // Tile thread (tc,tr): its E face (i-face tc+1) and N face (j-face tr+1);
-    // W-ring thread k: i-face 0 of row k; S-ring thread k: j-face 0 of
-    // column k.  E/N-ring threads have no face.
-    auto face = [&](bool idir, int f, int g, CS<FT> &Fout) -> bool {
+    auto face = [&](bool idir, int f, int g, const FaceGeo &geo, CS<FT> &Fout) -> bool {
         // idir: i-face f in row g; else j-face f in column g.  L/R recon coords:
         int ipL, jpL, rl_i, rl_j, rr_i, rr_j;
         if (idir) {
@@ -593,12 +634,8 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
             rl_i = g + 1; rl_j = f; rr_i = g + 1; rr_j = f + 1;
         }
         if (jpL > JCu || ipL > ICu) return false;
-        const int64_t cL = m.coff + (int64_t)jpL * m.pitch + ipL;
-        const XT nxd = idir ? rdgeo<FT>(p, GEO_NEX, GF_NEX, cL) : rdgeo<FT>(p, GEO_NNX, GF_NNX, cL);
-        const XT nyd = idir ? rdgeo<FT>(p, GEO_NEY, GF_NEY, cL) : rdgeo<FT>(p, GEO_NNY, GF_NNY, cL);
-        const XT xf = idir ? rdgeo<FT>(p, GEO_XFE, GF_XFE, cL) : rdgeo<FT>(p, GEO_XFN, GF_XFN, cL);
-        const XT yf = idir ? rdgeo<FT>(p, GEO_YFE, GF_YFE, cL) : rdgeo<FT>(p, GEO_YFN, GF_YFN, cL);
-        const FT nx = (FT)nxd, ny = (FT)nyd;
+        const XT xf = geo.xf, yf = geo.yf;
+        const FT nx = (FT)geo.nx, ny = (FT)geo.ny;
         auto load_state = [&](int ri, int rj, PS<FT> &W) {
             const int il = ri + 1, jl = rj + 1;
             W.d = sW[0][jl][il]; W.vx = sW[1][jl][il]; W.vy = sW[2][jl][il]; W.p = sW[3][jl][il];
@@ -655,26 +692,47 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
     // up to two faces per thread: (idir, f, g) of each
     CS<FT> F1, F2;
     bool h1 = false, h2 = false;
-    if (RP && tid < TI + TJ) {
+    if (vR) {
         // ring pass: W ring face (row tid) or S ring face (column tid - TJ),
         // stored at once (first, so no flux is held while computing it)
         CS<FT> Fr;
-        const bool wface = tid < TJ;
-        const int k = wface ? tid : tid - TJ;
-        if (face(wface, 0, k, Fr)) {
-            FT *dst = wface ? sRing + k : sRing + 4 * TJ + k;
-            const int st = wface ? TJ : TI;
-            dst[0] = Fr.d; dst[st] = Fr.mx; dst[2 * st] = Fr.my; dst[3 * st] = Fr.E;
+        face(ring_w, 0, ring_fk, gR, Fr);
+        FT *dst = ring_w ? sRing + ring_fk : sRing + 4 * TJ + ring_fk;
+        const int st = ring_w ? TJ : TI;
+        dst[0] = Fr.d; dst[st] = Fr.mx; dst[2 * st] = Fr.my; dst[3 * st] = Fr.E;
+    }
+    if (v1) h1 = face(true, is_tile ? tc + 1 : 0, is_tile ? tr : ring_k, g1, F1);
+    if (v2) h2 = face(false, is_tile ? tr + 1 : 0, is_tile ? tc : ring_k, g2, F2);
+
+    // phase-4 operands of the thread's cell: area, face lengths, time step,
+    // U^n and the FP64 accumulator where the stage reads it.  Only this
+    // thread writes U, S and dt of its cell in this launch.
+    const int ip = pi0 + 2 + tc, jp = pj0 + 2 + tr;
+    const bool valid = is_tile && (ip <= ICu) && (jp <= JCu);
+    const int64_t c4 = m.coff + (int64_t)jp * m.pitch + ip;
+    double A4 = 0.0, lW4 = 0.0, lE4 = 0.0, lS4 = 0.0, lN4 = 0.0, dt4 = 0.0;
+    double U4[4] = {0.0, 0.0, 0.0, 0.0}, S4[4] = {0.0, 0.0, 0.0, 0.0};
+    const bool fastS = Fast<FT>::on && p.Sf;
+    const bool readS = FUSED && !fastS &&
+                       (p.mode == MODE_FROM_S || (p.mode == MODE_ZERO_ACC2 && !defer && !p.skip_S));
+    auto load4 = [&]() {
+        A4 = G[GEO_A * ncp + c4];
+        lW4 = rdgeo<FT>(p, GEO_LE, GF_LE, c4 - 1);
+        lE4 = rdgeo<FT>(p, GEO_LE, GF_LE, c4);
+        lS4 = rdgeo<FT>(p, GEO_LN, GF_LN, c4 - m.pitch);
+        lN4 = rdgeo<FT>(p, GEO_LN, GF_LN, c4);
+        dt4 = p.lts ? p.dt[c4] : bitsd(*p.dtmin_in);
+        if (FUSED && !defer) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) U4[v] = p.U[v * ncp + c4];
         }
-    }
-    if (is_tile) {
-        h1 = face(true, tc + 1, tr, F1);
-        h2 = face(false, tr + 1, tc, F2);
-    } else if (ring_side == 0) {
-        h1 = face(true, 0, ring_k, F1);
-    } else if (ring_side == 2) {
-        h2 = face(false, 0, ring_k, F2);
-    }
+        if (readS) {
+#pragma unroll
+            for (int v = 0; v < 4; ++v) S4[v] = p.S[v * ncp + c4];
+        }
+    };
+    // issued before the two barriers of the flux exchange, which they overlap
+    if (valid) load4();
     __syncthreads();  // all gradient reads done: the flux arrays may now overwrite them
     if (h1) {
         const int f = is_tile ? tc + 1 : 0, g = is_tile ? tr : ring_k;
@@ -687,24 +745,19 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
     __syncthreads();
 
     // ---- phase 4: accumulate (+W -E +S -N) and update -------------------------
-    const int ip = pi0 + 2 + tc, jp = pj0 + 2 + tr;
-    const bool valid = is_tile && (ip <= ICu) && (jp <= JCu);
     double l1 = 0.0, l2 = 0.0, mx = 0.0, dtn = MILLION;
     if (valid) {
-        const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
-        const double A = G[GEO_A * ncp + c];
-        const double lW = rdgeo<FT>(p, GEO_LE, GF_LE, c - 1), lE = rdgeo<FT>(p, GEO_LE, GF_LE, c);
-        const double lS = rdgeo<FT>(p, GEO_LN, GF_LN, c - m.pitch), lN = rdgeo<FT>(p, GEO_LN, GF_LN, c);
-        const double dtc = p.lts ? p.dt[c] : bitsd(*p.dtmin_in);
+        const int64_t c = c4;
+        const double A = A4, lW = lW4, lE = lE4, lS = lS4, lN = lN4, dtc = dt4;
         const double cdt = p.cfl * dtc;
         double dU[4];
         if (FUSED) {
-            if (p.mode == MODE_FROM_S && Fast<FT>::on && p.Sf) {
+            if (p.mode == MODE_FROM_S && fastS) {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) dU[v] = (double)p.Sf[v * ncp + c];
             } else if (p.mode == MODE_FROM_S) {
 #pragma unroll
-                for (int v = 0; v < 4; ++v) dU[v] = p.S[v * ncp + c];
+                for (int v = 0; v < 4; ++v) dU[v] = S4[v];
             } else {
 #pragma unroll
                 for (int v = 0; v < 4; ++v) dU[v] = 0.0;
@@ -824,7 +877,7 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
 #pragma unroll
             for (int v = 0; v < 4; ++v) Sx[v * ncp + c] = dU[v];
         } else {
-            fused_update<FT>(p, m, t, c, ip, jp, A, dtc, lW, lE, lS, lN, dU, l1, l2, mx, dtn);
+            fused_update<FT>(p, m, t, c, ip, jp, A, dtc, lW, lE, lS, lN, U4, readS, S4, dU, l1, l2, mx, dtn);
         }
     }
     if (!(FUSED) || !p.last_stage || defer) return;
@@ -902,7 +955,8 @@ __global__ __launch_bounds__(UPD_NT) void update_tile_kernel(StageParams p, int 
         double a1 = 0.0, a2 = 0.0, a3 = 0.0, a4 = MILLION;
         const double lW = rdgeo<FT>(p, GEO_LE, GF_LE, c - 1), lE = rdgeo<FT>(p, GEO_LE, GF_LE, c);
         const double lS = rdgeo<FT>(p, GEO_LN, GF_LN, c - m.pitch), lN = rdgeo<FT>(p, GEO_LN, GF_LN, c);
-        fused_update<FT>(p, m, t, c, ip, jp, A, dtc, lW, lE, lS, lN, dU, a1, a2, a3, a4);
+        const double U0[4] = {p.U[c], p.U[ncp + c], p.U[2 * ncp + c], p.U[3 * ncp + c]};
+        fused_update<FT>(p, m, t, c, ip, jp, A, dtc, lW, lE, lS, lN, U0, false, U0, dU, a1, a2, a3, a4);
         l1 += a1; l2 += a2; mx = dmax(mx, a3); dtn = dmin(dtn, a4);
     }
     if (!p.last_stage) return;
