@@ -359,15 +359,15 @@ __global__ __launch_bounds__(256) void cfl_kernel(const BlockMeta *meta, const T
         const double A = G[GEO_A * ncp + c];
         const double lW = G[GEO_LE * ncp + c - 1], lE = G[GEO_LE * ncp + c];
         const double lS = G[GEO_LN * ncp + c - m.pitch], lN = G[GEO_LN * ncp + c];
-        const double d_i = 2.0 * (A / (lE + lW));
-        const double d_j = 2.0 * (A / (lN + lS));
+        const double d_i = 2.0 * qdiv(A, lE + lW);
+        const double d_j = 2.0 * qdiv(A, lN + lS);
         const PS<double> Wc = ldW(W, ncp, c);
         const double nWx = -G[GEO_NEX * ncp + c - 1], nWy = -G[GEO_NEY * ncp + c - 1];
         const double nSx = -G[GEO_NNX * ncp + c - m.pitch], nSy = -G[GEO_NNY * ncp + c - m.pitch];
         const double v_i = 0.5 * (Wc.vx * (G[GEO_NEX * ncp + c] - nWx) + Wc.vy * (G[GEO_NEY * ncp + c] - nWy));
         const double v_j = 0.5 * (Wc.vx * (G[GEO_NNX * ncp + c] - nSx) + Wc.vy * (G[GEO_NNY * ncp + c] - nSy));
         const double a = W_a(Wc);
-        const double d = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
+        const double d = dmin(qdiv(d_i, a + fabs(v_i)), qdiv(d_j, a + fabs(v_j)));
         dt[c] = d;
         dmn = dmin(dmn, d);
     }

@@ -217,10 +217,10 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
         Dy.d *= rn; Dy.vx *= rn; Dy.vy *= rn; Dy.p *= rn;
         DxDx *= rn; DxDy *= rn; DyDy *= rn;
     } else {
-        const FT np = (FT)n_pts;
-        Dx.d /= np; Dx.vx /= np; Dx.vy /= np; Dx.p /= np;
-        Dy.d /= np; Dy.vx /= np; Dy.vy /= np; Dy.p /= np;
-        DxDx /= np; DxDy /= np; DyDy /= np;
+        const Recip<FT> rn = recip((FT)n_pts);
+        Dx.d = rdiv(Dx.d, rn); Dx.vx = rdiv(Dx.vx, rn); Dx.vy = rdiv(Dx.vy, rn); Dx.p = rdiv(Dx.p, rn);
+        Dy.d = rdiv(Dy.d, rn); Dy.vx = rdiv(Dy.vx, rn); Dy.vy = rdiv(Dy.vy, rn); Dy.p = rdiv(Dy.p, rn);
+        DxDx = rdiv(DxDx, rn); DxDy = rdiv(DxDy, rn); DyDy = rdiv(DyDy, rn);
     }
     const FT det = DxDx * DyDy - DxDy * DxDy;
     if constexpr (Fast<FT>::on) {
@@ -230,10 +230,12 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
         gy.d = (Dy.d * DxDx - Dx.d * DxDy) * rd; gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) * rd;
         gy.vy = (Dy.vy * DxDx - Dx.vy * DxDy) * rd; gy.p = (Dy.p * DxDx - Dx.p * DxDy) * rd;
     } else {
-        gx.d = (Dx.d * DyDy - Dy.d * DxDy) / det; gx.vx = (Dx.vx * DyDy - Dy.vx * DxDy) / det;
-        gx.vy = (Dx.vy * DyDy - Dy.vy * DxDy) / det; gx.p = (Dx.p * DyDy - Dy.p * DxDy) / det;
-        gy.d = (Dy.d * DxDx - Dx.d * DxDy) / det; gy.vx = (Dy.vx * DxDx - Dx.vx * DxDy) / det;
-        gy.vy = (Dy.vy * DxDx - Dx.vy * DxDy) / det; gy.p = (Dy.p * DxDx - Dx.p * DxDy) / det;
+        // eight correctly rounded quotients over one shared reciprocal of det
+        const Recip<FT> rd = recip(det);
+        gx.d = rdiv(Dx.d * DyDy - Dy.d * DxDy, rd); gx.vx = rdiv(Dx.vx * DyDy - Dy.vx * DxDy, rd);
+        gx.vy = rdiv(Dx.vy * DyDy - Dy.vy * DxDy, rd); gx.p = rdiv(Dx.p * DyDy - Dy.p * DxDy, rd);
+        gy.d = rdiv(Dy.d * DxDx - Dx.d * DxDy, rd); gy.vx = rdiv(Dy.vx * DxDx - Dx.vx * DxDy, rd);
+        gy.vy = rdiv(Dy.vy * DxDx - Dx.vy * DxDy, rd); gy.p = rdiv(Dy.p * DxDx - Dx.p * DxDy, rd);
     }
     if (phim == PHI_LOAD) {
         ph.d = (FT)p.phi[c]; ph.vx = (FT)p.phi[ncp + c]; ph.vy = (FT)p.phi[2 * ncp + c]; ph.p = (FT)p.phi[3 * ncp + c];
@@ -398,8 +400,8 @@ __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMe
         // CFL of the next step, SingleBlock.cc:3036-3053, in FP64 from the
         // geometry planes the stage reads (the mixed path's FP32 normals and
         // lengths are still in cache from its flux phase)
-        const double d_i = 2.0 * (A / (lE + lW));
-        const double d_j = 2.0 * (A / (lN + lS));
+        const double d_i = 2.0 * qdiv(A, lE + lW);
+        const double d_j = 2.0 * qdiv(A, lN + lS);
         const double nEx = rdgeo<FT>(p, GEO_NEX, GF_NEX, c), nEy = rdgeo<FT>(p, GEO_NEY, GF_NEY, c);
         const double nNx = rdgeo<FT>(p, GEO_NNX, GF_NNX, c), nNy = rdgeo<FT>(p, GEO_NNY, GF_NNY, c);
         const double nWx = -(double)rdgeo<FT>(p, GEO_NEX, GF_NEX, c - 1);
@@ -409,7 +411,7 @@ __device__ __forceinline__ void fused_update(const StageParams &p, const BlockMe
         const double v_i = 0.5 * (Wn.vx * (nEx - nWx) + Wn.vy * (nEy - nWy));
         const double v_j = 0.5 * (Wn.vx * (nNx - nSx) + Wn.vy * (nNy - nSy));
         const double a = W_a(Wn);
-        dtn = dmin(d_i / (a + fabs(v_i)), d_j / (a + fabs(v_j)));
+        dtn = dmin(qdiv(d_i, a + fabs(v_i)), qdiv(d_j, a + fabs(v_j)));
         if (p.lts) p.dt[c] = dtn;
     }
 }
@@ -817,6 +819,8 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
             // ((CFL*dt)*F*l)/A per face, +W -E +S -N (Euler2DQuad.cc:1502-1509)
             bool res = false;
             if constexpr (!FUSED) res = p.residual != 0;
+            // sixteen correctly rounded quotients over one reciprocal of A
+            const Recip<double> rA = recip(A);
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const double FW = (double)fW(v), FE = (double)sFi[v][tr][tc + 1];
@@ -824,17 +828,17 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
                 double d = dU[v];
                 if (res) {
                     // (Flux*l)/A of dUdt_Residual_Evaluation, Euler2DQuad.cc:1001-1006
-                    d += FW * lW / A;
-                    d -= FE * lE / A;
+                    d += rdiv(FW * lW, rA);
+                    d -= rdiv(FE * lE, rA);
                     if (axi) d += Sx[v];
-                    d += FS * lS / A;
-                    d -= FN * lN / A;
+                    d += rdiv(FS * lS, rA);
+                    d -= rdiv(FN * lN, rA);
                 } else {
-                    d += cdt * FW * lW / A;
-                    d -= cdt * FE * lE / A;
+                    d += rdiv(cdt * FW * lW, rA);
+                    d -= rdiv(cdt * FE * lE, rA);
                     if (axi) d += cdt * Sx[v];
-                    d += cdt * FS * lS / A;
-                    d -= cdt * FN * lN / A;
+                    d += rdiv(cdt * FS * lS, rA);
+                    d -= rdiv(cdt * FN * lN, rA);
                 }
                 dU[v] = d;
             }

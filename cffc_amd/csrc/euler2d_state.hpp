@@ -56,9 +56,47 @@ template <> CFFC_HD float tabs<float>(float x) { return fabsf(x); }
 // requirement): hardware reciprocal / square-root approximations.
 template <typename T> struct Fast { static constexpr bool on = false; };
 template <> struct Fast<float> { static constexpr bool on = true; };
+
+// Correctly rounded FP64 division with the divisor's reciprocal shared by
+// every quotient over the same divisor.  The device steps are gfx950's own
+// IEEE a / b: r = v_rcp_f64(b) refined by two Newton steps
+// r += r(1 - b r), then q = a r, the exact FMA remainder a - b q, the
+// corrected quotient q + (a - b q) r and v_div_fixup_f64, which gives zeros,
+// infinities and NaNs their IEEE results.  Only the operand scaling
+// (v_div_scale_f64 and the scale applied by v_div_fmas_f64) is left out: it
+// acts when a/b or 1/b would be subnormal, |a/b| >= 2^767 or a nonzero
+// |a| < 2^-969, so for every other operand pair the result is a / b bit for
+// bit.  The Newton steps depend on b alone, which is what makes the
+// reciprocal shareable.  float: plain division (callers choose their own
+// approximations on the mixed path).
+template <typename T> struct Recip { T b, r; };
+template <typename T> CFFC_HD Recip<T> recip(T b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (!Fast<T>::on) {
+        double r = __builtin_amdgcn_rcp(b);
+        double e = fma(-b, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-b, r, 1.0);
+        r = fma(r, e, r);
+        return {b, r};
+    }
+#endif
+    return {b, T(0)};
+}
+template <typename T> CFFC_HD T rdiv(T a, const Recip<T> &d) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (!Fast<T>::on) {
+        double q = a * d.r;
+        const double rem = fma(-d.b, q, a);
+        q = fma(rem, d.r, q);
+        return __builtin_amdgcn_div_fixup(q, d.b, a);
+    }
+#endif
+    return a / d.b;
+}
 template <typename T> CFFC_HD T qdiv(T a, T b) {
     if constexpr (Fast<T>::on) return a * __builtin_amdgcn_rcpf(b);
-    else return a / b;
+    else return rdiv(a, recip(b));
 }
 template <typename T> CFFC_HD T qsqrt(T a) {
     if constexpr (Fast<T>::on) return __builtin_amdgcn_sqrtf(a);
@@ -74,10 +112,11 @@ template <typename T> CFFC_HD T get(const PS<T> &w, int n) {
 
 template <typename T> CFFC_HD PS<T> W_of_U(const CS<T> &U) {
     PS<T> W;
+    const Recip<T> rd = recip(U.d);
     W.d = U.d;
-    W.vx = U.mx / U.d;
-    W.vy = U.my / U.d;
-    W.p = gm1<T>() * (U.E - Consts<T>::HALF * (U.mx * U.mx + U.my * U.my) / U.d);
+    W.vx = rdiv(U.mx, rd);
+    W.vy = rdiv(U.my, rd);
+    W.p = gm1<T>() * (U.E - rdiv(Consts<T>::HALF * (U.mx * U.mx + U.my * U.my), rd));
     return W;
 }
 template <typename T> CFFC_HD CS<T> U_of_W(const PS<T> &W) {
@@ -98,15 +137,16 @@ template <typename T> CFFC_HD T W_a(const PS<T> &W) { return qsqrt(qdiv(gam<T>()
 // axisymmetric source S(W, X), Euler2DState.h:3179-3182 (y the radius)
 template <typename T> CFFC_HD CS<T> axisym_source(const PS<T> &W, T y) {
     CS<T> S;
-    S.d = -W.d * W.vy / y;
-    S.mx = -W.d * W.vx * W.vy / y;
-    S.my = -W.d * (W.vy * W.vy) / y;
-    S.E = -W.vy * W_H(W) / y;
+    const Recip<T> ry = recip(y);
+    S.d = rdiv(-W.d * W.vy, ry);
+    S.mx = rdiv(-W.d * W.vx * W.vy, ry);
+    S.my = rdiv(-W.d * (W.vy * W.vy), ry);
+    S.E = rdiv(-W.vy * W_H(W), ry);
     return S;
 }
 template <typename T> CFFC_HD T W_a2(const PS<T> &W) { return qdiv(gam<T>() * W.p, W.d); }
 template <typename T> CFFC_HD T U_e(const CS<T> &U) {
-    return U.E / U.d - Consts<T>::HALF * (U.mx * U.mx + U.my * U.my) / sqr(U.d);
+    return qdiv(U.E, U.d) - qdiv(Consts<T>::HALF * (U.mx * U.mx + U.my * U.my), sqr(U.d));
 }
 template <typename T> CFFC_HD CS<T> W_Fx(const PS<T> &W) {
     CS<T> F;
@@ -130,7 +170,7 @@ template <typename T> CFFC_HD T harten_delta(T la, T ll, T lr) {
         return (tabs(la) > delta || delta < K::TOLER) ? K::ZERO : fixd;
     } else {
         if (tabs(la) > delta || delta < K::TOLER) return K::ZERO;
-        return K::HALF * (sqr(la) / delta + delta) - tabs(la);
+        return K::HALF * (qdiv(sqr(la), delta) + delta) - tabs(la);
     }
 }
 template <typename T> CFFC_HD T harten_pos(T la, T ll, T lr) {
@@ -155,11 +195,12 @@ template <typename T> CFFC_HD PS<T> roe_average(const PS<T> &Wl, const PS<T> &Wr
         const T aa2 = gm1<T>() * (ha - K::HALF * (sqr(Wa.vx) + sqr(Wa.vy)));
         Wa.p = Wa.d * aa2 * T(1.0 / 1.4);
     } else {
-        Wa.vx = (sdl * Wl.vx + sdr * Wr.vx) / (sdl + sdr);
-        Wa.vy = (sdl * Wl.vy + sdr * Wr.vy) / (sdl + sdr);
-        T ha = (sdl * hl + sdr * hr) / (sdl + sdr);
+        const Recip<T> rs = recip(sdl + sdr);
+        Wa.vx = rdiv(sdl * Wl.vx + sdr * Wr.vx, rs);
+        Wa.vy = rdiv(sdl * Wl.vy + sdr * Wr.vy, rs);
+        T ha = rdiv(sdl * hl + sdr * hr, rs);
         T aa2 = gm1<T>() * (ha - K::HALF * (sqr(Wa.vx) + sqr(Wa.vy)));
-        Wa.p = Wa.d * aa2 / gam<T>();
+        Wa.p = qdiv(Wa.d * aa2, gam<T>());
     }
     return Wa;
 }
@@ -288,12 +329,13 @@ template <typename T> CFFC_HD CS<T> flux_hlle_x(const PS<T> &Wl, const PS<T> &Wr
     if (sl >= T(0)) return W_Fx(Wl);
     if (sr <= T(0)) return W_Fx(Wr);
     const CS<T> Fl = W_Fx(Wl), Fr = W_Fx(Wr);
-    const T slr = sl * sr, den = sr - sl;
+    const T slr = sl * sr;
+    const Recip<T> rden = recip(sr - sl);
     CS<T> F;
-    F.d = ((sr * Fl.d - sl * Fr.d) + slr * (Ur.d - Ul.d)) / den;
-    F.mx = ((sr * Fl.mx - sl * Fr.mx) + slr * (Ur.mx - Ul.mx)) / den;
-    F.my = ((sr * Fl.my - sl * Fr.my) + slr * (Ur.my - Ul.my)) / den;
-    F.E = ((sr * Fl.E - sl * Fr.E) + slr * (Ur.E - Ul.E)) / den;
+    F.d = rdiv((sr * Fl.d - sl * Fr.d) + slr * (Ur.d - Ul.d), rden);
+    F.mx = rdiv((sr * Fl.mx - sl * Fr.mx) + slr * (Ur.mx - Ul.mx), rden);
+    F.my = rdiv((sr * Fl.my - sl * Fr.my) + slr * (Ur.my - Ul.my), rden);
+    F.E = rdiv((sr * Fl.E - sl * Fr.E) + slr * (Ur.E - Ul.E), rden);
     return F;
 }
 template <typename T> CFFC_HD CS<T> flux_hlle_n(const PS<T> &Wl, const PS<T> &Wr, T nx, T ny) {
