@@ -265,7 +265,7 @@ __device__ __forceinline__ void reconstruct(const StageParams &p, int phim, int6
                 // of a zero result, and the limiter uses u_max/u_min only
                 // through u_max - u_0 and u_min - u_0, whose zero sign can only
                 // reach y = +-0, where f(y) = (y^2 + 2y)/(y^2 + y + 2) is +0
-                else { umin = fmin(umin, v); umax = fmax(umax, v); }
+                else { umin = vmin64(umin, v); umax = vmax64(umax, v); }
             }
         }
         const FT gxn = n == 0 ? gx.d : n == 1 ? gx.vx : n == 2 ? gx.vy : gx.p;

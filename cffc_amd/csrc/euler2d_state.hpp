@@ -45,6 +45,28 @@ template <typename T> CFFC_HD T tmin(T x, T y) { return fmin(x, y); }
 template <> CFFC_HD float tmin<float>(float x, float y) { return fminf(x, y); }
 template <typename T> CFFC_HD T tmax(T x, T y) { return fmax(x, y); }
 template <> CFFC_HD float tmax<float>(float x, float y) { return fmaxf(x, y); }
+// The same IEEE minimum and maximum of two FP64 values as one v_min_f64 /
+// v_max_f64 each, for operands read from LDS: fmin/fmax first quiet
+// signalling NaNs with an extra instruction per operand, which finite
+// operands do not need.
+CFFC_HD double vmin64(double x, double y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+#else
+    return fmin(x, y);
+#endif
+}
+CFFC_HD double vmax64(double x, double y) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(x), "v"(y));
+    return r;
+#else
+    return fmax(x, y);
+#endif
+}
 template <typename T> CFFC_HD T sqr(T x) { return x * x; }
 template <typename T> CFFC_HD T tabs(T x) { return fabs(x); }
 template <typename T> CFFC_HD T tsqrt(T x) { return sqrt(x); }
@@ -98,9 +120,30 @@ template <typename T> CFFC_HD T qdiv(T a, T b) {
     if constexpr (Fast<T>::on) return a * __builtin_amdgcn_rcpf(b);
     else return rdiv(a, recip(b));
 }
+// Correctly rounded FP64 square root: gfx950's own IEEE sqrt(x) (v_rsq_f64,
+// then Goldschmidt's coupled iteration for g ~ sqrt(x) and h ~ 1/(2 sqrt(x))
+// and two FMA corrections g += (x - g^2) h; zeros and +inf returned as
+// they are) without the input scaling by 2^256 that only acts below
+// x = 2^-767, so for x >= 2^-767 the result is sqrt(x) bit for bit.
 template <typename T> CFFC_HD T qsqrt(T a) {
     if constexpr (Fast<T>::on) return __builtin_amdgcn_sqrtf(a);
-    else return sqrt(a);
+    else {
+#if defined(__HIP_DEVICE_COMPILE__)
+        const double y = __builtin_amdgcn_rsq(a);
+        double g = a * y, h = y * 0.5;
+        const double r = fma(-h, g, 0.5);
+        g = fma(g, r, g);
+        double d = fma(-g, g, a);
+        h = fma(h, r, h);
+        g = fma(d, h, g);
+        d = fma(-g, g, a);
+        g = fma(d, h, g);
+        // 0x260: -0, +0, +inf
+        return __builtin_amdgcn_class(a, 0x260) ? a : g;
+#else
+        return sqrt(a);
+#endif
+    }
 }
 
 template <typename T> struct PS { T d, vx, vy, p; };   // Euler2D_pState
@@ -416,13 +459,13 @@ template <typename T> CFFC_HD T venkat_fold(T phi, T uq, T u0, T u0Min, T u0Max)
         num = u0Min - u0;  // y = num/du, both <= 0
         if (num <= T(2.002) * du && num >= T(1.0e12) * du) return phi;
     } else {
-        return tmin(phi, K::ONE);
+        return vmin64(phi, K::ONE);
     }
     // phi and f(y) are never -0 or NaN (phi starts at 1, f(+-0) = +0; y is
     // finite since |du| > TOLER and |num| << 1e147), so the one-instruction
     // minimum equals the reference's compare-and-select
     const T y = qdiv(num, du);
-    return tmin(phi, qdiv(sqr(y) + K::TWO * y, sqr(y) + y + K::TWO));
+    return vmin64(phi, qdiv(sqr(y) + K::TWO * y, sqr(y) + y + K::TWO));
 }
 
 }  // namespace cffc
