@@ -517,18 +517,55 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
     }
 
     // ---- phase 1: W and centroids of the window ---------------------------
-    for (int q = tid; q < LI * LJ; q += NT) {
-        const int il = q % LI, jl = q / LI;
-        const int ip = pi0 + il, jp = pj0 + jl;
-        if (ip < m.nci && jp < m.ncj) {
-            const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
+    if constexpr (!RP) {
+        // tiles with ring threads: every thread issues the loads of all its
+        // window cells before its first LDS store, without branches.  A
+        // thread past the window's end repeats the window's last cell, and
+        // a window cell outside the block takes the nearest cell of the
+        // block: such slots are never read (every reconstructed cell's
+        // stencil lies inside its block), and a slot written twice gets the
+        // same value twice.
+        constexpr int NWIN = LI * LJ, NIT = (NWIN + NT - 1) / NT;
+        FT wv[NIT][4];
+        XT xw[NIT], yw[NIT];
+        int slot[NIT];
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int q = tid + k * NT, qq = q < NWIN ? q : NWIN - 1;
+            slot[k] = qq;
+            const int ip = pi0 + qq % LI, jp = pj0 + qq / LI;
+            const int ipc = ip < m.nci ? ip : m.nci - 1, jpc = jp < m.ncj ? jp : m.ncj - 1;
+            const int64_t c = m.coff + (int64_t)jpc * m.pitch + ipc;
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                if constexpr (FUSED && Fast<FT>::on) sW[v][jl][il] = p.Winf[v * ncp + c];
-                else sW[v][jl][il] = (FT)p.Win[v * ncp + c];
+                if constexpr (FUSED && Fast<FT>::on) wv[k][v] = p.Winf[v * ncp + c];
+                else wv[k][v] = (FT)p.Win[v * ncp + c];
             }
-            sX[jl][il] = rdgeo<FT>(p, GEO_XC, GF_XC, c);
-            sY[jl][il] = rdgeo<FT>(p, GEO_YC, GF_YC, c);
+            xw[k] = rdgeo<FT>(p, GEO_XC, GF_XC, c);
+            yw[k] = rdgeo<FT>(p, GEO_YC, GF_YC, c);
+        }
+#pragma unroll
+        for (int k = 0; k < NIT; ++k) {
+            const int il = slot[k] % LI, jl = slot[k] / LI;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sW[v][jl][il] = wv[k][v];
+            sX[jl][il] = xw[k];
+            sY[jl][il] = yw[k];
+        }
+    } else {
+        for (int q = tid; q < LI * LJ; q += NT) {
+            const int il = q % LI, jl = q / LI;
+            const int ip = pi0 + il, jp = pj0 + jl;
+            if (ip < m.nci && jp < m.ncj) {
+                const int64_t c = m.coff + (int64_t)jp * m.pitch + ip;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    if constexpr (FUSED && Fast<FT>::on) sW[v][jl][il] = p.Winf[v * ncp + c];
+                    else sW[v][jl][il] = (FT)p.Win[v * ncp + c];
+                }
+                sX[jl][il] = rdgeo<FT>(p, GEO_XC, GF_XC, c);
+                sY[jl][il] = rdgeo<FT>(p, GEO_YC, GF_YC, c);
+            }
         }
     }
     __syncthreads();
