@@ -997,6 +997,27 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
     return 0;
 }
 
+extern "C" int cffc_check_arith(const double *a, const double *b, int64_t n, double *out) {
+    if (!a || !b || !out || n < 0) return fail(CFFC_E_ARG, "null argument or negative count");
+    if (n == 0) return 0;
+    double *da = nullptr, *dout = nullptr;
+    if (hipMalloc((void **)&da, 2 * n * sizeof(double)) != hipSuccess)
+        return fail(CFFC_E_NOMEM, "hipMalloc of %zu bytes failed", (size_t)(2 * n * sizeof(double)));
+    if (hipMalloc((void **)&dout, 8 * n * sizeof(double)) != hipSuccess) {
+        (void)hipFree(da);
+        return fail(CFFC_E_NOMEM, "hipMalloc of %zu bytes failed", (size_t)(8 * n * sizeof(double)));
+    }
+    hipError_t e = hipMemcpy(da, a, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(da + n, b, n * sizeof(double), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = cffc_launch_arith_check(da, da + n, n, dout, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dout, 8 * n * sizeof(double), hipMemcpyDeviceToHost);
+    (void)hipFree(da);
+    (void)hipFree(dout);
+    if (e != hipSuccess) return fail(CFFC_E_HIP, "cffc_check_arith: %s", hipGetErrorString(e));
+    return 0;
+}
+
 extern "C" int cffc_check_mesh(const cffc_mesh_desc *md, int64_t counts[4]) {
     if (!md) return fail(CFFC_E_ARG, "null argument");
     if (md->nblk <= 0 || md->nghost != 2) return fail(CFFC_E_ARG, "nblk must be > 0 and nghost == 2");

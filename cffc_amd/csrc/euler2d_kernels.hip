@@ -712,6 +712,24 @@ __global__ void set_u64_kernel(unsigned long long *dst, unsigned long long v0, u
 
 using namespace cffc;
 
+// The stage kernels' FP64 arithmetic beside the compiler's own, per operand
+// pair: rdiv(a, recip(b)) and a / b, qsqrt(|a|) and sqrt(|a|), vmin64 and
+// fmin, vmax64 and fmax (out: 8 values per pair).
+__global__ void arith_check_kernel(const double *a, const double *b, int64_t n, double *out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double x = a[i], y = b[i];
+    double *o = out + 8 * i;
+    o[0] = rdiv(x, recip(y));
+    o[1] = x / y;
+    o[2] = qsqrt(fabs(x));
+    o[3] = sqrt(fabs(x));
+    o[4] = vmin64(x, y);
+    o[5] = fmin(x, y);
+    o[6] = vmax64(x, y);
+    o[7] = fmax(x, y);
+}
+
 extern "C" {
 
 hipError_t cffc_launch_geometry(const BlockMeta *meta, int nblk, int max_cells, const double *xn, const double *yn,
@@ -742,6 +760,10 @@ hipError_t cffc_launch_exchange(const ExchangeParams *p, int nblk, hipStream_t s
     const int nt = (p->nthreads == 64 || p->nthreads == 128) ? p->nthreads : 256;
     if (p->Wf) hipLaunchKernelGGL(exchange_bc_kernel<float>, dim3(nblk), dim3(nt), 0, st, *p);
     else hipLaunchKernelGGL(exchange_bc_kernel<double>, dim3(nblk), dim3(nt), 0, st, *p);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_arith_check(const double *a, const double *b, int64_t n, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(arith_check_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, n, out);
     return hipGetLastError();
 }
 hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream_t st) {

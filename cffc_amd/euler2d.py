@@ -94,6 +94,7 @@ def load():
     L.cffc_mg_execute.argtypes = [vp, i32, d, pd, pint, pint]
     L.cffc_mg_get_P.argtypes = [vp, i32, pd, ctypes.c_int64]
     L.cffc_mg_set_graph.argtypes = [vp, i32]
+    L.cffc_check_arith.argtypes = [pd, pd, i64, pd]
     L.cffc_last_error.argtypes = []
     L.cffc_last_error.restype = ctypes.c_char_p
     _lib = L
@@ -108,6 +109,19 @@ def _check(rc, what):
     if rc < 0:
         raise CffcError("%s failed (%d): %s" % (what, rc, load().cffc_last_error().decode()))
     return rc
+
+
+def check_arith(a, b):
+    """The stage kernels' FP64 division, square root, minimum and maximum
+    beside the compiler's, on the device, for operand arrays a and b:
+    returns an (n, 8) array (see cffc_check_arith)."""
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    if a.shape != b.shape or a.ndim != 1:
+        raise ValueError("a and b must be 1-D arrays of one length")
+    out = np.empty((a.size, 8), dtype=np.float64)
+    _check(load().cffc_check_arith(_p(a), _p(b), a.size, _p(out)), "cffc_check_arith")
+    return out
 
 
 class Euler2DQuadBlocks:

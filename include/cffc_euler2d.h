@@ -194,6 +194,14 @@ void cffc_destroy(cffc_ctx *ctx);
    flux-correction operations and 1 if the mesh has resolution changes. */
 int  cffc_check_mesh(const cffc_mesh_desc *mesh, int64_t counts[4]);
 
+/* Diagnostic, no reference counterpart: the device FP64 arithmetic of the
+   stage kernels beside the compiler's, for n operand pairs (a[k], b[k]).
+   out[8k..8k+7] = the kernels' correctly rounded quotient a/b (shared-
+   reciprocal form), the compiler's a / b, the kernels' sqrt(|a|), the
+   compiler's sqrt(|a|), the kernels' single-instruction min(a, b), fmin,
+   the kernels' max(a, b), fmax.  Runs on the current device. */
+int  cffc_check_arith(const double *a, const double *b, int64_t n, double *out);
+
 /* Diagnostic, no reference counterpart: the operation lists cffc_create
    builds for a mesh, host only.  kind 0: restrictions, 1: same-level copies
    (lattice meshes), 2: prolongations, 3: flux corrections, as the library's
