@@ -65,12 +65,13 @@ __global__ __launch_bounds__(64) void restrict_kernel(const RestrictOp *ops, int
     const int64_t c00 = o.src, c10 = o.src + 1, c01 = o.src + o.pitch, c11 = o.src + o.pitch + 1;
     const double *A = geom + GEO_A * ncp;
     const double A00 = A[c00], A10 = A[c10], A01 = A[c01], A11 = A[c11];
-    const double den = A00 + A10 + A01 + A11;
+    // four correctly rounded quotients over one reciprocal of the area sum
+    const Recip<double> rden = recip(A00 + A10 + A01 + A11);
     double r[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double *u = U + v * ncp;
-        r[v] = (A00 * u[c00] + A10 * u[c10] + A01 * u[c01] + A11 * u[c11]) / den;
+        r[v] = rdiv(A00 * u[c00] + A10 * u[c10] + A01 * u[c01] + A11 * u[c11], rden);
     }
     const CS<double> Ur = {r[0], r[1], r[2], r[3]};
     amr_stW(W, ncp, o.dst, W_of_U(Ur));
@@ -119,12 +120,12 @@ __global__ __launch_bounds__(64) void prolong_kernel(const ProlongOp *ops, int n
 #pragma unroll
     for (int v = 0; v < 4; ++v) { Dx[v] /= 8.0; Dy[v] /= 8.0; }
     DxDx /= 8.0; DxDy /= 8.0; DyDy /= 8.0;
-    const double det = DxDx * DyDy - DxDy * DxDy;
+    const Recip<double> rdet = recip(DxDx * DyDy - DxDy * DxDy);
     double gx[4], gy[4], ph[4];
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        gx[v] = (Dx[v] * DyDy - Dy[v] * DxDy) / det;
-        gy[v] = (Dy[v] * DxDx - Dx[v] * DxDy) / det;
+        gx[v] = rdiv(Dx[v] * DyDy - Dy[v] * DxDy, rdet);
+        gy[v] = rdiv(Dy[v] * DxDx - Dx[v] * DxDy, rdet);
     }
     if (frozen) {
 #pragma unroll
@@ -166,7 +167,8 @@ __global__ __launch_bounds__(256) void flux_fix_kernel(const FluxFixOp *ops, int
     const FluxFixOp o = ops[q];
     const int64_t c = o.cell;
     const double dtc = lts ? dt[c] : __longlong_as_double((long long)*dtmin);
-    const double cdt = cfl * dtc, A = geom[GEO_A * ncp + c];
+    const double cdt = cfl * dtc;
+    const Recip<double> rA = recip(geom[GEO_A * ncp + c]);
     double d[4] = {S[c], S[ncp + c], S[2 * ncp + c], S[3 * ncp + c]};
     for (int k = 0; k < o.n; ++k) {
         const double *own = eflux + 4 * (int64_t)o.own[k];
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void flux_fix_kernel(const FluxFixOp *ops, int
         for (int v = 0; v < 4; ++v) {
             // FluxX = -(fine pair) - own; dUdt -= (CFL*dt)*FluxX/A
             const double F = -(fa[v] + fa[4 + v]) - own[v];
-            d[v] -= cdt * F / A;
+            d[v] -= rdiv(cdt * F, rA);
         }
     }
 #pragma unroll

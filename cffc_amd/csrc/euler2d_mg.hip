@@ -42,11 +42,11 @@ __global__ void mg_restrict_kernel(const BlockMeta *mf, const BlockMeta *mc, con
     const int64_t q0 = cell(f, i0, j0), q1 = q0 + 1, q2 = q0 + f.pitch, q3 = q2 + 1;
     const double A0 = Af[q0], A1 = Af[q1], A2 = Af[q2], A3 = Af[q3];
     const int64_t cc = cell(c, ic, jc);
-    const double A = Ac[cc];
+    const Recip<double> rA = recip(Ac[cc]);
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const double *s = src + v * ncpf;
-        dst[v * ncpc + cc] = (s[q0] * A0 + s[q1] * A1 + s[q2] * A2 + s[q3] * A3) / A;
+        dst[v * ncpc + cc] = rdiv(s[q0] * A0 + s[q1] * A1 + s[q2] * A2 + s[q3] * A3, rA);
     }
 }
 
@@ -81,10 +81,10 @@ __global__ void mg_cfl_kernel(const BlockMeta *mf, const BlockMeta *mc, const do
     const int64_t sw = cell(f, i0, j0), se = sw + 1, nw = sw + f.pitch, ne = nw + 1;
     const int64_t cc = cell(c, ic, jc);
     const double A = Ac[cc];
-    const double dt_NE = sqrt(A / Af[ne]) * dtf[ne];
-    const double dt_SE = sqrt(A / Af[se]) * dtf[se];
-    const double dt_NW = sqrt(A / Af[nw]) * dtf[nw];
-    const double dt_SW = sqrt(A / Af[sw]) * dtf[sw];
+    const double dt_NE = qsqrt(qdiv(A, Af[ne])) * dtf[ne];
+    const double dt_SE = qsqrt(qdiv(A, Af[se])) * dtf[se];
+    const double dt_NW = qsqrt(qdiv(A, Af[nw])) * dtf[nw];
+    const double dt_SW = qsqrt(qdiv(A, Af[sw])) * dtf[sw];
     dtc[cc] = dmin(dtc[cc], dmin(dt_NE, dmin(dt_SE, dmin(dt_NW, dt_SW))));
 }
 
