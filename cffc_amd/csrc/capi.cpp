@@ -79,6 +79,9 @@ struct cffc_ctx {
     int npart = 0;
     std::vector<int32_t> part_of_block;
     int32_t *part_tile0 = nullptr, *part_state = nullptr, *part_steps = nullptr;
+    // a multigrid level of a batch with per-case limiter freezing (owned by
+    // the cffc_mg): the part of every block, each part's frozen flag
+    const int32_t *mg_part_of_block = nullptr, *mg_part_frozen = nullptr;
     double *part_l2 = nullptr, *part_last = nullptr;
     int part_l2_steps = 0;
     // fused mixed path: FP32 RK accumulator; sf_live: it, not dUdt[0],
@@ -2087,6 +2090,17 @@ struct cffc_mg {
     hipGraph_t graph = nullptr;
     hipGraphExec_t gexec = nullptr;
     int graph_frozen = -1, graph_top = -1;
+    // per-case stopping (cffc_mg_set_parts): part q owns level-0 blocks
+    // part_block0[q] .. part_block0[q+1]-1; per part its tolerance, freeze
+    // level, last checked L2 norm and the regular cycles done when its norm
+    // first met the tolerance (-1 before); cycles counts the regular cycles
+    // since cffc_mg_set_parts
+    int npart = 0;
+    std::vector<int32_t> part_block0, part_done, part_frozen;
+    std::vector<double> part_tol, part_freeze, part_l2;
+    int32_t *d_tile0 = nullptr, *d_part_of_block = nullptr, *d_part_frozen = nullptr;
+    double *d_part_l2 = nullptr;
+    long long cycles = 0;
 };
 
 // the flux function and limiter of every level for a top level: the
@@ -2103,6 +2117,9 @@ static void mg_set_top(cffc_mg *mg, int top) {
 
 static void mg_free(cffc_mg *mg) {
     if (!mg) return;
+    for (int32_t *q : {mg->d_tile0, mg->d_part_of_block, mg->d_part_frozen})
+        if (q) (void)hipFree(q);
+    if (mg->d_part_l2) (void)hipFree(mg->d_part_l2);
     if (mg->gexec) (void)hipGraphExecDestroy(mg->gexec);
     if (mg->graph) (void)hipGraphDestroy(mg->graph);
     for (size_t l = 0; l < mg->lv.size(); ++l) {
@@ -2226,6 +2243,8 @@ static int mg_stage(cffc_ctx *c, int i_stage, int residual, const double *P) {
     p.dtmin_in = c->scal;
     p.lts = 1;
     p.phi_mode = c->freeze ? PHI_LOAD : PHI_COMPUTE_STORE;
+    p.part_of_block = c->mg_part_of_block;
+    p.part_frozen = c->mg_part_frozen;
     p.hlle = c->s.i_flux_function == CFFC_FLUX_FUNCTION_HLLE;
     p.residual = residual;
     p.P = P;
@@ -2461,6 +2480,88 @@ extern "C" int cffc_mg_get_P(cffc_mg *mg, int level, double *P, int64_t ncell) {
 // absolute / relative tolerance (the FMG ones above level 0).  A spent
 // budget still prolongs through the remaining start-up levels, each
 // stopping at its first check.  norms: 3 per check.
+extern "C" int cffc_mg_set_parts(cffc_mg *mg, int npart, const int32_t *first_block, const double *tol,
+                                 const double *freeze) {
+    if (!mg) return fail(CFFC_E_ARG, "null argument");
+    cffc_ctx *f = mg->lv[0];
+    for (cffc_ctx *c : mg->lv) {
+        c->mg_part_of_block = nullptr;
+        c->mg_part_frozen = nullptr;
+    }
+    if (npart == 0) {
+        mg->npart = 0;
+        return 0;
+    }
+    if (npart < 0 || npart > f->nblk || !first_block || !tol) return fail(CFFC_E_ARG, "bad argument");
+    if (first_block[0] != 0 || first_block[npart] != f->nblk) return fail(CFFC_E_ARG, "parts must cover all blocks");
+    std::vector<int32_t> pob(f->nblk);
+    for (int q = 0; q < npart; ++q) {
+        if (first_block[q + 1] <= first_block[q]) return fail(CFFC_E_ARG, "part %d is empty", q);
+        if (!(tol[q] > 0.0) || (freeze && !(freeze[q] >= 0.0))) return fail(CFFC_E_ARG, "bad tolerance of part %d", q);
+        for (int b = first_block[q]; b < first_block[q + 1]; ++b) pob[b] = q;
+    }
+    // parts must not exchange ghost cells with each other
+    std::vector<int32_t> nbr_h(8 * (size_t)f->nblk);
+    HIPCHK(hipMemcpy(nbr_h.data(), f->nbr, nbr_h.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+    for (int b = 0; b < f->nblk; ++b)
+        for (int k = 0; k < 8; ++k) {
+            const int nb = nbr_h[8 * (size_t)b + k];
+            if (nb >= 0 && pob[nb] != pob[b])
+                return fail(CFFC_E_TOPOLOGY, "block %d neighbours block %d of another part", b, nb);
+        }
+    for (int32_t **q : {&mg->d_tile0, &mg->d_part_of_block, &mg->d_part_frozen})
+        if (*q) { (void)hipFree(*q); *q = nullptr; }
+    if (mg->d_part_l2) (void)hipFree(mg->d_part_l2);
+    mg->d_part_l2 = nullptr;
+    if (int e = dalloc(&mg->d_tile0, npart + 1)) return e;
+    if (int e = dalloc(&mg->d_part_l2, npart)) return e;
+    if (int e = dalloc(&mg->d_part_of_block, f->nblk)) return e;
+    if (int e = dalloc(&mg->d_part_frozen, npart)) return e;
+    HIPCHK(hipMemcpy(mg->d_part_of_block, pob.data(), pob.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    mg->part_frozen.assign(npart, 0);
+    HIPCHK(hipMemcpy(mg->d_part_frozen, mg->part_frozen.data(), npart * sizeof(int32_t), hipMemcpyHostToDevice));
+    // every level has the finest level's blocks in the same order
+    for (cffc_ctx *c : mg->lv) {
+        c->mg_part_of_block = mg->d_part_of_block;
+        c->mg_part_frozen = mg->d_part_frozen;
+    }
+    mg->npart = npart;
+    mg->part_block0.assign(first_block, first_block + npart + 1);
+    mg->part_tol.assign(tol, tol + npart);
+    mg->part_freeze.assign(npart, 0.0);
+    if (freeze) mg->part_freeze.assign(freeze, freeze + npart);
+    mg->part_l2.assign(npart, -1.0);
+    mg->part_done.assign(npart, -1);
+    mg->cycles = 0;
+    return 0;
+}
+
+extern "C" int cffc_mg_part_status(cffc_mg *mg, double *l2, int32_t *done_cycles) {
+    if (!mg) return fail(CFFC_E_ARG, "null argument");
+    if (!mg->npart) return fail(CFFC_E_ARG, "no parts set");
+    if (l2) memcpy(l2, mg->part_l2.data(), mg->npart * sizeof(double));
+    if (done_cycles) memcpy(done_cycles, mg->part_done.data(), mg->npart * sizeof(int32_t));
+    return mg->npart;
+}
+
+// first level-0 tile of every part (tiles are in block order), uploaded for
+// the per-part norms
+static int mg_part_tiles(cffc_mg *mg) {
+    const cffc_ctx *f = mg->lv[0];
+    std::vector<int32_t> t0(mg->npart + 1);
+    int32_t nt = 0;
+    int q = 0;
+    for (int b = 0; b < f->nblk; ++b) {
+        while (q < mg->npart && mg->part_block0[q] == b) t0[q++] = nt;
+        const BlockMeta &m = f->meta_h[b];
+        nt += ((m.ni + f->ti - 1) / f->ti) * ((m.nj + f->tj - 1) / f->tj);
+    }
+    t0[mg->npart] = nt;
+    if (nt != f->ntiles) return fail(CFFC_E_ARG, "tile count %d, expected %d", nt, f->ntiles);
+    HIPCHK(hipMemcpy(mg->d_tile0, t0.data(), t0.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done,
                                int *checks) {
     if (!mg || max_cycles < 0) return fail(CFFC_E_ARG, "bad argument");
@@ -2469,6 +2570,8 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
     int n = 0, nchk = 0, rc = 0;
     std::vector<unsigned long long> keys(mg->lv.size());
     if (int e = mg_reset_errors(mg)) return e;
+    if (mg->npart)
+        if (int e = mg_part_tiles(mg)) return e;
     const int top0 = mg->d.fmg_cycles > 0 ? L - 2 : 0;
     for (int top = top0; top >= 0 && !rc; --top) {
         mg_set_top(mg, top);
@@ -2484,6 +2587,12 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
             HIPCHK(cffc_launch_norms(t->meta, t->tiles, t->ntiles, t->ti, t->tj, t->dUdt[0], t->ncp, 1,
                                      t->norm_part, t->norm_small, t->stream));
             HIPCHK(hipMemcpyAsync(nr, t->norm_small, 3 * sizeof(double), hipMemcpyDeviceToHost, t->stream));
+            const bool parts = mg->npart && top == 0;
+            if (parts) {
+                HIPCHK(cffc_launch_part_norms(t->norm_part, mg->d_tile0, mg->npart, mg->d_part_l2, t->stream));
+                HIPCHK(hipMemcpyAsync(mg->part_l2.data(), mg->d_part_l2, mg->npart * sizeof(double),
+                                      hipMemcpyDeviceToHost, t->stream));
+            }
             for (size_t l = 0; l < mg->lv.size(); ++l)
                 HIPCHK(hipMemcpyAsync(&keys[l], mg->lv[l]->scal + 2, sizeof(unsigned long long),
                                       hipMemcpyDeviceToHost, t->stream));
@@ -2496,20 +2605,43 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
             if (cycles == 2) init_l2 = nr[1];
             else if (cycles > 2 && fabs(init_l2) > 1e-9) ratio = nr[1] / init_l2;
             bool stop = n >= max_cycles;
-            if (cycles > 2) {
+            bool freeze_now = nr[1] <= freeze_level;
+            if (parts) {
+                // per-case stopping: every part's own norm has met its
+                // tolerance at a check after the second cycle (the single
+                // norm's rule); each part's limiters freeze once its own norm
+                // is at or below its freeze level (the single norm's rule)
+                bool all = true, newly = false;
+                freeze_now = false;
+                for (int q = 0; q < mg->npart; ++q) {
+                    if (cycles > 2 && mg->part_done[q] < 0 && mg->part_l2[q] < mg->part_tol[q])
+                        mg->part_done[q] = (int32_t)mg->cycles;
+                    if (mg->part_done[q] < 0) all = false;
+                    if (!first_step && cycles > 1 && freeze_level > 0 && !mg->part_frozen[q] &&
+                        mg->fine_s.i_limiter != CFFC_LIMITER_ZERO && mg->part_l2[q] <= mg->part_freeze[q]) {
+                        mg->part_frozen[q] = 1;
+                        newly = true;
+                    }
+                }
+                if (all) stop = true;
+                if (newly && !stop)
+                    HIPCHK(hipMemcpyAsync(mg->d_part_frozen, mg->part_frozen.data(), mg->npart * sizeof(int32_t),
+                                          hipMemcpyHostToDevice, t->stream));
+            } else if (cycles > 2) {
                 const double at = top == 0 ? mg->d.absolute_tolerance : mg->d.fmg_absolute_tolerance;
                 const double rt = top == 0 ? mg->d.relative_tolerance : mg->d.fmg_relative_tolerance;
                 if (nr[1] < at || ratio < rt) stop = true;
             }
             if (stop) break;
             if (!first_step && cycles > 1 && freeze_level > 0 && !mg->frozen &&
-                mg->fine_s.i_limiter != CFFC_LIMITER_ZERO && nr[1] <= freeze_level) {
+                mg->fine_s.i_limiter != CFFC_LIMITER_ZERO && freeze_now) {
                 for (int l = top; l < L; ++l) mg->lv[l]->freeze = 1;
                 mg->frozen = true;
             }
             if (int e = mg_enqueue_cycle(mg, top)) return e;
             first_step = false;
             n++;
+            if (top == 0) mg->cycles++;
         }
         if (top != 0 && !rc) {
             cffc_ctx *f = mg->lv[top - 1];

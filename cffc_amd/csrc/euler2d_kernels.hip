@@ -480,6 +480,20 @@ __global__ __launch_bounds__(256) void norm_final_kernel(const double *part, int
     }
 }
 
+// The L2 norm of each part's cells from the per-tile partials of
+// norm_partial_kernel (sum of squares at 3q + 1): one wavefront per part, the
+// lanes over the part's tiles from its first, so a part's norm is the same
+// alone and in a batch.
+__global__ __launch_bounds__(256) void part_norm_kernel(const double *part, const int32_t *tile0, int npart,
+                                                        double *out) {
+    const int pt = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (pt >= npart) return;
+    double b = 0.0;
+    for (int q = tile0[pt] + lane; q < tile0[pt + 1]; q += 64) b += part[3 * q + 1];
+    for (int off = 32; off > 0; off >>= 1) b += __shfl_xor(b, off);
+    if (lane == 0) out[pt] = sqrt(b);
+}
+
 // One workgroup closes a fused time step.  Partials are summed in tile order
 // with a fixed thread assignment, so the norms are reproducible run to run.
 // Skipped when a stage of this step (or an earlier step) raised the stop
@@ -764,6 +778,12 @@ hipError_t cffc_launch_exchange(const ExchangeParams *p, int nblk, hipStream_t s
 }
 hipError_t cffc_launch_arith_check(const double *a, const double *b, int64_t n, double *out, hipStream_t st) {
     hipLaunchKernelGGL(arith_check_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, a, b, n, out);
+    return hipGetLastError();
+}
+hipError_t cffc_launch_part_norms(const double *part, const int32_t *tile0, int npart, double *out,
+                                  hipStream_t st) {
+    hipLaunchKernelGGL(part_norm_kernel, dim3((unsigned)((npart + 3) / 4)), dim3(256), 0, st, part, tile0, npart,
+                       out);
     return hipGetLastError();
 }
 hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream_t st) {

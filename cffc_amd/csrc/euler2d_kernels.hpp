@@ -78,6 +78,8 @@ struct FinishParams {
 extern "C" {
 hipError_t cffc_launch_widen(const float *src, double *dst, int64_t n, hipStream_t st);
 hipError_t cffc_launch_arith_check(const double *a, const double *b, int64_t n, double *out, hipStream_t st);
+hipError_t cffc_launch_part_norms(const double *part, const int32_t *tile0, int npart, double *out,
+                                  hipStream_t st);
 hipError_t cffc_launch_geometry(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *xn,
                                 const double *yn, double *geom, int64_t ncp, unsigned int *ndeg, hipStream_t st);
 hipError_t cffc_launch_geometry_f32(const cffc::BlockMeta *meta, int nblk, int max_cells, const double *geom,

@@ -60,6 +60,11 @@ struct StageParams {
     int32_t skip_S;         // fused: no stage after this one reads S (multistage smoothing), so it is not stored
     // resolution changes
     const uint8_t *tile_defer;  // fused: per tile, nonzero = store the residual in Sd (flux correction follows)
+    // granular: per-case limiter freezing of a multigrid batch (null: the
+    // phi_mode of the launch); the part of every block and whether each
+    // part's limiters are frozen
+    const int32_t *part_of_block;
+    const int32_t *part_frozen;
     double *Sd;             // residual of deferred tiles; update_tile_kernel reads it
     const int32_t *tile_list;   // update_tile_kernel: the tiles to update (null: all)
     int32_t nlist;
@@ -496,6 +501,7 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
         }
         phim = ps == PART_FROZEN ? PHI_LOAD : p.last_stage ? PHI_COMPUTE_STORE : PHI_COMPUTE;
     }
+    if (!FUSED && p.part_frozen) phim = p.part_frozen[p.part_of_block[t.b]] ? PHI_LOAD : PHI_COMPUTE_STORE;
     const double *G = p.geom;
     const int ICu = m.ni + 1, JCu = m.nj + 1;
     const int pi0 = t.i0, pj0 = t.j0;   // padded index of the LDS window origin

@@ -94,6 +94,9 @@ def load():
     L.cffc_mg_execute.argtypes = [vp, i32, d, pd, pint, pint]
     L.cffc_mg_get_P.argtypes = [vp, i32, pd, ctypes.c_int64]
     L.cffc_mg_set_graph.argtypes = [vp, i32]
+    pi32 = ctypes.POINTER(ctypes.c_int32)
+    L.cffc_mg_set_parts.argtypes = [vp, i32, pi32, pd, pd]
+    L.cffc_mg_part_status.argtypes = [vp, pd, pi32]
     L.cffc_check_arith.argtypes = [pd, pd, i64, pd]
     L.cffc_last_error.argtypes = []
     L.cffc_last_error.restype = ctypes.c_char_p

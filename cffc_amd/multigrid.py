@@ -368,6 +368,33 @@ class FASMultigrid:
         _check(_lib().cffc_mg_get_P(self.h, level, _p(out), self.levels[level].ncell), "cffc_mg_get_P")
         return out
 
+    def set_parts(self, first_block, tol, freeze=None):
+        """Per-case stopping (cffc_mg_set_parts): part q owns level-0 blocks
+        first_block[q] .. first_block[q+1]-1; execute's regular cycles stop
+        once every part's own L2 norm has met tol[q], and the limiters
+        freeze once every part's norm is at or below freeze[q]."""
+        from .euler2d import _check, _p
+        fb = np.ascontiguousarray(first_block, dtype=np.int32)
+        self._part_tol = np.ascontiguousarray(tol, dtype=np.float64)
+        self._part_frz = None if freeze is None else np.ascontiguousarray(freeze, dtype=np.float64)
+        npart = fb.size - 1
+        if self._part_tol.size != npart or (self._part_frz is not None and self._part_frz.size != npart):
+            raise ValueError("one tolerance (and freeze level) per part")
+        _check(_lib().cffc_mg_set_parts(self.h, npart, fb.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                        _p(self._part_tol), None if self._part_frz is None else _p(self._part_frz)),
+               "cffc_mg_set_parts")
+        self.npart = npart
+
+    def part_status(self):
+        """(last L2 norm per part, regular cycles done when each part first
+        met its tolerance, -1 before)"""
+        from .euler2d import _check, _p
+        l2 = np.empty(self.npart)
+        done = np.empty(self.npart, dtype=np.int32)
+        _check(_lib().cffc_mg_part_status(self.h, _p(l2), done.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))),
+               "cffc_mg_part_status")
+        return l2, done
+
     def execute(self, max_cycles, freeze_level=0.0):
         """Execute (full-multigrid start-up when fmg_cycles > 0, then the
         regular cycles); returns (rc, cycles, norms[checks, 3]), the norms

@@ -82,13 +82,19 @@ def serve(cases, initial=None, tol=1e-10, max_steps=400000, freeze_level=steady.
 
 
 def serve_multigrid(cases, initial=None, tol=1e-10, max_cycles=20000, freeze_level=steady.STEADY_FREEZE_LEVEL,
-                    levels=4, cfl=1.0, segment=500, log=None):
+                    levels=4, cfl=1.0, segment=500, per_case=True, log=None):
     """Solve all cases in one batch with the FAS multigrid solver (one
-    mesh of disjoint parts; FP64).  Multigrid stops on the L2 norm of the
-    unscaled residual R over the whole batch, so every case finishes with
-    the batch.  The single-grid criterion (L2 of (CFL*dt)*R <= tol per case)
-    is converted with the largest per-case median of CFL*dt (the strictest
-    case), and so is the freeze level."""
+    mesh of disjoint parts; FP64).  The single-grid criterion (L2 of
+    (CFL*dt)*R <= tol per case) is converted to the unscaled residual R
+    with the case's median of CFL*dt, and so is the freeze level.
+
+    per_case: every case is held to its own converted tolerance on the L2
+    norm of R over its own cells (cffc_mg_set_parts); the batch stops when
+    the last case has met its tolerance, and a case's time to convergence is
+    the batch's wall clock at the cycle where it first met it.  Otherwise
+    the batch stops on one L2 norm over all cases against the strictest
+    case's conversion (the largest median of CFL*dt), and every case
+    finishes with the batch."""
     from . import euler2d, multigrid
     batch = []
     for q, (h, m, k) in enumerate(cases):
@@ -104,37 +110,64 @@ def serve_multigrid(cases, initial=None, tol=1e-10, max_cycles=20000, freeze_lev
     dt = f.get_dt()
     f.close()
     offs = np.concatenate([[0], np.cumsum([b[0].ncell for b in batch])])
-    scale = max(float(np.median(sg.cfl_number * dt[offs[q]:offs[q + 1]])) for q in range(len(batch)))
+    scales = np.array([float(np.median(sg.cfl_number * dt[offs[q]:offs[q + 1]])) for q in range(len(batch))])
+    scale = float(scales.max())
     t0 = time.perf_counter()
     g = multigrid.FASMultigrid(mesh, steady.bump_scheme(cfl=cfl),
                                multigrid.MGParams(levels=levels, absolute_tolerance=tol / scale))
     g.set_W(W0)
     g.initialize()
+    if per_case:
+        g.set_parts(list(starts) + [mesh.nblk], tol / scales, freeze_level / scales)
     t_setup = time.perf_counter() - t0
-    # Execute in segments, one progress line each; a diverged batch stops
-    cycles, parts, rc = 0, [], 0
+    # Execute in segments, one progress line each; a diverged batch stops.
+    # clock: (regular cycles done, seconds since t0) after each segment
+    cycles, parts, rc, clock = 0, [], 0, [(0, t_setup)]
     while cycles < max_cycles:
         rc, n, nr = g.execute(min(segment, max_cycles - cycles), freeze_level=freeze_level / scale)
         cycles += n
+        clock.append((cycles, time.perf_counter() - t0))
         parts.append(nr[1:] if parts else nr)
         last = nr[-1, 1] if len(nr) else float("nan")
-        if log:
+        if per_case:
+            l2, done = g.part_status()
+            if log:
+                log("  multigrid: %d cycles, %d/%d cases converged, R %.3e, %.1f s" %
+                    (cycles, int((done >= 0).sum()), len(cases), last, time.perf_counter() - t0))
+        elif log:
             log("  multigrid: %d cycles, R %.3e, %.1f s" % (cycles, last, time.perf_counter() - t0))
         if rc != 0 or n < segment or not np.isfinite(last) or last > 1e12:
             break
     norms = np.concatenate(parts) if parts else np.zeros((0, 3))
     wall = time.perf_counter() - t0
-    g.close()
-    ok = rc == 0 and len(norms) and norms[-1, 1] < tol / scale
-    if log:
-        log("multigrid: %d cycles, R %.3e (tol %.3e), %.2f s (set-up %.2f s)" %
-            (cycles, norms[-1, 1] if len(norms) else -1, tol / scale, wall, t_setup))
-    lat = [wall] * len(cases)
-    return dict(rc=rc, n=len(cases), converged=len(cases) if ok else 0, wall_seconds=wall,
-                solves_per_s=len(cases) / wall if wall > 0 else None, p50_s=wall if ok else None,
-                p99_s=wall if ok else None, cycles=cycles, setup_seconds=t_setup, levels=levels, cfl=cfl,
-                tol_R=tol / scale, final_R=float(norms[-1, 1]) if len(norms) else None,
-                latency_s=lat, cells=int(mesh.ncell), precision="fp64", solver="FAS multigrid, V cycle")
+    out = dict(rc=rc, n=len(cases), wall_seconds=wall, solves_per_s=len(cases) / wall if wall > 0 else None,
+               cycles=cycles, setup_seconds=t_setup, levels=levels, cfl=cfl,
+               final_R=float(norms[-1, 1]) if len(norms) else None, cells=int(mesh.ncell), precision="fp64",
+               solver="FAS multigrid, V cycle")
+    if per_case:
+        l2, done = g.part_status()
+        g.close()
+        conv = (done >= 0) & (rc == 0)
+        cy, tt = np.array([c for c, _ in clock], float), np.array([t for _, t in clock])
+        # wall clock at the cycle where each case first met its tolerance,
+        # along the batch's cycle clock (linear within a segment)
+        lat = np.where(conv, np.interp(np.maximum(done, 0), cy, tt), np.inf)
+        out.update(converged=int(conv.sum()), stopping="per case", case_cycles=done.tolist(),
+                   tol_R=(tol / scales).tolist(), case_final_R=l2.tolist(),
+                   p50_s=float(np.percentile(lat, 50)) if conv.all() else None,
+                   p99_s=float(np.percentile(lat, 99)) if conv.all() else None, latency_s=lat.tolist())
+        if log:
+            log("multigrid: %d cycles, %d/%d cases converged (cycles to convergence %d .. %d), %.2f s "
+                "(set-up %.2f s)" % (cycles, int(conv.sum()), len(cases), done.min(), done.max(), wall, t_setup))
+    else:
+        g.close()
+        ok = rc == 0 and len(norms) and norms[-1, 1] < tol / scale
+        if log:
+            log("multigrid: %d cycles, R %.3e (tol %.3e), %.2f s (set-up %.2f s)" %
+                (cycles, norms[-1, 1] if len(norms) else -1, tol / scale, wall, t_setup))
+        out.update(converged=len(cases) if ok else 0, stopping="batch norm", tol_R=tol / scale,
+                   p50_s=wall if ok else None, p99_s=wall if ok else None, latency_s=[wall] * len(cases))
+    return out
 
 
 def serve_cold_warm(net, norm, cases, **kw):

@@ -449,6 +449,22 @@ int  cffc_mg_cycle(cffc_mg *mg, int64_t *bad_cell);
    exits on max_cycles cycles in all or the tolerances of cffc_mg_desc */
 int  cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *norms, int *cycles_done,
                      int *checks);
+/* Per-case stopping for a batch of independent cases in one mesh (no
+   counterpart in the reference, whose Execute stops on one norm over all
+   blocks): part q owns level-0 blocks first_block[q] .. first_block[q+1]-1
+   (npart+1 entries, first 0, last the level's block count; no block may
+   neighbour a block of another part, CFFC_E_TOPOLOGY otherwise).  With
+   parts set, cffc_mg_execute's regular cycles (top level 0) stop once every
+   part's L2 norm of the density residual over its own cells has been below
+   tol[q] at a check after the second cycle (the rule of the single norm),
+   and with freeze_level > 0 the limiters freeze once every part's norm is
+   at or below freeze[q] (freeze may be NULL: 0).  A part's norm is the
+   same alone and in a batch.  npart = 0 clears the parts. */
+int  cffc_mg_set_parts(cffc_mg *mg, int npart, const int32_t *first_block, const double *tol, const double *freeze);
+/* per part: the last checked L2 norm (l2, may be NULL) and the regular
+   cycles done since cffc_mg_set_parts when its norm first met its
+   tolerance (done_cycles, -1: not yet; may be NULL); returns npart */
+int  cffc_mg_part_status(cffc_mg *mg, double *l2, int32_t *done_cycles);
 /* cffc_mg_execute replays one captured cycle as a HIP graph (on by
    default; recaptured once when the limiters freeze); off: the launches
    are issued one by one, with the same arithmetic.  No reference

@@ -261,10 +261,51 @@ def test_full_multigrid_four_levels_w_cycle_matches_oracle(budget):
 def test_serving_batch_with_multigrid_converges():
     """serving.serve_multigrid on 4 cases of two mesh sizes (96 x 32 and
     192 x 64 cells in blocks of 32^2): one multigrid batch, run in segments,
-    reaches every case's single-grid-equivalent tolerance."""
+    reaches every case's single-grid-equivalent tolerance, per case and with
+    the batch norm against the strictest case."""
     from cffc_amd import serving
     cases = [(0.05, 0.4, 1), (0.08, 0.55, 2), (0.06, 0.35, 1), (0.09, 0.5, 2)]
     r = serving.serve_multigrid(cases, levels=3, segment=200, log=lambda s: print(s, flush=True))
-    print("batch: %d cycles, %.2f s, R %.3e (tol %.3e)" % (r["cycles"], r["wall_seconds"], r["final_R"], r["tol_R"]),
-          flush=True)
-    assert r["rc"] == 0 and r["converged"] == 4 and r["final_R"] < r["tol_R"]
+    print("per case: %d cycles, %.2f s, cycles to convergence %s" % (r["cycles"], r["wall_seconds"],
+                                                                      r["case_cycles"]), flush=True)
+    assert r["rc"] == 0 and r["converged"] == 4 and r["cycles"] == max(r["case_cycles"])
+    assert np.all(np.array(r["case_final_R"]) < np.array(r["tol_R"]))
+    b = serving.serve_multigrid(cases, levels=3, segment=200, per_case=False)
+    print("batch norm: %d cycles, %.2f s, R %.3e (tol %.3e)" % (b["cycles"], b["wall_seconds"], b["final_R"],
+                                                                 b["tol_R"]), flush=True)
+    assert b["rc"] == 0 and b["converged"] == 4 and b["final_R"] < b["tol_R"]
+
+
+@pytest.mark.parametrize("limiter", [abi.LIMITER_ZERO, abi.LIMITER_VENKATAKRISHNAN])
+def test_per_case_stopping_equals_each_case_alone(limiter):
+    """cffc_mg_set_parts: a batch of three bump cases of two mesh sizes,
+    each with its own tolerance (and, with the Venkatakrishnan limiter, its
+    own limiter freeze level), reaches each case's tolerance after exactly
+    the cycles the case takes alone, and stops with the last of them; that
+    case's norm at the stop equals its norm alone bit for bit."""
+    from cffc_amd import serving
+    cases = [(0.05, 0.4, 1), (0.08, 0.55, 2), (0.06, 0.35, 1)]
+    zero = limiter == abi.LIMITER_ZERO
+    tols = np.array([3e-8, 1e-8, 2e-8]) if zero else np.array([3e-9, 1e-9, 2e-9])
+    freeze = None if zero else 1e5 * tols
+
+    def run(parts):
+        meshes = [serving.case_mesh(*cases[q]) for q in parts]
+        m, starts = M.concat([mm[0] for mm in meshes])
+        g = MG.FASMultigrid(m, _scheme(limiter), MG.MGParams(levels=3, absolute_tolerance=1e-30))
+        g.set_W(np.concatenate([mm[1] for mm in meshes]))
+        g.initialize()
+        g.set_parts(list(starts) + [m.nblk], tols[parts], None if zero else freeze[parts])
+        rc, n, norms = g.execute(6000, freeze_level=0.0 if zero else 1.0)
+        l2, done = g.part_status()
+        g.close()
+        assert rc == 0
+        return n, l2, done
+
+    alone = [run([q]) for q in range(3)]
+    n, l2, done = run([0, 1, 2])
+    print("alone:", [(a[0], a[2][0]) for a in alone], "batch:", n, done.tolist(), flush=True)
+    assert [a[2][0] for a in alone] == done.tolist()
+    assert n == max(a[0] for a in alone) == done.max()
+    last = int(np.argmax(done))
+    assert l2[last] == alone[last][1][0]
