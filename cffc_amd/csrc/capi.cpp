@@ -775,7 +775,7 @@ static int build_meta(cffc_ctx *c, const cffc_mesh_desc *md, int32_t &bcoff, int
         m.bcoff = bcoff;
         m.has_bc = 0;
         m.amr = 0;
-        m.pad = 0;
+        m.skip = 0;
         m.eoff = 0;
         for (int q = 0; q < 2 * (m.ni + 4) + 2 * (m.nj + 4); ++q)
             if (md->bc[bcoff + q] != CFFC_BC_NONE) m.has_bc = 1;
@@ -2480,6 +2480,19 @@ extern "C" int cffc_mg_get_P(cffc_mg *mg, int level, double *P, int64_t ncell) {
 // absolute / relative tolerance (the FMG ones above level 0).  A spent
 // budget still prolongs through the remaining start-up levels, each
 // stopping at its first check.  norms: 3 per check.
+// the blocks of part q on every level: skipped by the cycle's kernels (a
+// converged case) or not; the changed metadata go to the device in stream
+// order
+static int mg_mark_part(cffc_mg *mg, int q, int skip) {
+    const int b0 = mg->part_block0[q], nb = mg->part_block0[q + 1] - b0;
+    for (cffc_ctx *c : mg->lv) {
+        for (int b = b0; b < b0 + nb; ++b) c->meta_h[b].skip = skip;
+        HIPCHK(hipMemcpyAsync(c->meta + b0, c->meta_h.data() + b0, nb * sizeof(BlockMeta), hipMemcpyHostToDevice,
+                              c->stream));
+    }
+    return 0;
+}
+
 extern "C" int cffc_mg_set_parts(cffc_mg *mg, int npart, const int32_t *first_block, const double *tol,
                                  const double *freeze) {
     if (!mg) return fail(CFFC_E_ARG, "null argument");
@@ -2487,6 +2500,12 @@ extern "C" int cffc_mg_set_parts(cffc_mg *mg, int npart, const int32_t *first_bl
     for (cffc_ctx *c : mg->lv) {
         c->mg_part_of_block = nullptr;
         c->mg_part_frozen = nullptr;
+        bool any = false;
+        for (BlockMeta &m : c->meta_h) {
+            any = any || m.skip;
+            m.skip = 0;
+        }
+        if (any) HIPCHK(hipMemcpy(c->meta, c->meta_h.data(), c->nblk * sizeof(BlockMeta), hipMemcpyHostToDevice));
     }
     if (npart == 0) {
         mg->npart = 0;
@@ -2614,8 +2633,12 @@ extern "C" int cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level,
                 bool all = true, newly = false;
                 freeze_now = false;
                 for (int q = 0; q < mg->npart; ++q) {
-                    if (cycles > 2 && mg->part_done[q] < 0 && mg->part_l2[q] < mg->part_tol[q])
+                    if (cycles > 2 && mg->part_done[q] < 0 && mg->part_l2[q] < mg->part_tol[q]) {
+                        // converged: the cycle's kernels leave the case alone
+                        // from here on, so it keeps the state it converged in
                         mg->part_done[q] = (int32_t)mg->cycles;
+                        if (int e = mg_mark_part(mg, q, 1)) return e;
+                    }
                     if (mg->part_done[q] < 0) all = false;
                     if (!first_step && cycles > 1 && freeze_level > 0 && !mg->part_frozen[q] &&
                         mg->fine_s.i_limiter != CFFC_LIMITER_ZERO && mg->part_l2[q] <= mg->part_freeze[q]) {

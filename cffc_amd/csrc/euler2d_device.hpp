@@ -45,7 +45,7 @@ struct BlockMeta {
     int32_t bcoff;   // first entry of the block in the bc / wo arrays
     int32_t has_bc;  // nonzero if any BCtypeN/S/E/W entry is not BC_NONE
     int32_t amr;     // AMR_* flags (resolution changes)
-    int32_t pad;
+    int32_t skip;    // nonzero: a converged case of a multigrid batch, left alone by the cycle's kernels
     int64_t eoff;    // first entry of the block's saved face fluxes (N [ni], S [ni], E [nj], W [nj])
 };
 

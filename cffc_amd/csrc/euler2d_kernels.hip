@@ -160,6 +160,7 @@ __global__ __launch_bounds__(256) void exchange_bc_kernel(ExchangeParams p) {
     if (blockIdx.x == 0 && threadIdx.x == 0 && p.reset_slot) *p.reset_slot = 0x412E848000000000ull;  // MILLION
     const int b = p.blocks ? p.blocks[blockIdx.x] : (int)blockIdx.x;
     const BlockMeta m = p.meta[b];
+    if (m.skip) return;   // uniform over the workgroup
     const int64_t ncp = p.ncp;
     ST *W;
     if constexpr (sizeof(ST) == 4) W = p.Wf;
@@ -350,6 +351,7 @@ __global__ __launch_bounds__(256) void cfl_kernel(const BlockMeta *meta, const T
     __shared__ double sm[4];
     const TileDesc t = tiles[blockIdx.x];
     const BlockMeta m = meta[t.b];
+    if (m.skip) return;   // uniform over the workgroup
     double dmn = MILLION;
     for (int q = threadIdx.x; q < ti * tj; q += blockDim.x) {
         const int ip = t.i0 + 2 + q % ti, jp = t.j0 + 2 + q / ti;
@@ -396,7 +398,7 @@ __global__ void update_kernel(const BlockMeta *meta, double *W, double *U, const
                               double *dt, int64_t ncp, double omega, int lts, unsigned long long *err) {
     const BlockMeta m = meta[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m.ni * m.nj) return;
+    if (q >= m.ni * m.nj || m.skip) return;
     const int i = q % m.ni + 2, j = q / m.ni + 2;
     const int64_t c = m.coff + (int64_t)j * m.pitch + i;
     double d[4] = {dU_k[c], dU_k[ncp + c], dU_k[2 * ncp + c], dU_k[3 * ncp + c]};
@@ -430,7 +432,7 @@ __global__ void update_kernel(const BlockMeta *meta, double *W, double *U, const
 __global__ void copy_interior_kernel(const BlockMeta *meta, const double *src, double *dst, int64_t ncp) {
     const BlockMeta m = meta[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m.ni * m.nj) return;
+    if (q >= m.ni * m.nj || m.skip) return;
     const int i = q % m.ni + 2, j = q / m.ni + 2;
     const int64_t c = m.coff + (int64_t)j * m.pitch + i;
     for (int v = 0; v < 4; ++v) dst[v * ncp + c] = src[v * ncp + c];
@@ -443,6 +445,7 @@ __global__ __launch_bounds__(256) void norm_partial_kernel(const BlockMeta *meta
     __shared__ double sm[4][3];
     const TileDesc t = tiles[blockIdx.x];
     const BlockMeta m = meta[t.b];
+    if (m.skip) return;   // a converged case keeps its last partials
     double a = 0.0, b = 0.0, mx = 0.0;
     for (int q = threadIdx.x; q < ti * tj; q += blockDim.x) {
         const int ip = t.i0 + 2 + q % ti, jp = t.j0 + 2 + q / ti;

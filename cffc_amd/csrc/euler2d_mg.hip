@@ -36,7 +36,7 @@ __global__ void mg_restrict_kernel(const BlockMeta *mf, const BlockMeta *mc, con
                                    int64_t ncpf, double *dst, const double *Ac, int64_t ncpc) {
     const BlockMeta f = mf[blockIdx.y], c = mc[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= c.ni * c.nj) return;
+    if (q >= c.ni * c.nj || c.skip) return;
     const int ic = q % c.ni + 2, jc = q / c.ni + 2;
     const int i0 = 2 * (ic - 2) + 2, j0 = 2 * (jc - 2) + 2;
     const int64_t q0 = cell(f, i0, j0), q1 = q0 + 1, q2 = q0 + f.pitch, q3 = q2 + 1;
@@ -54,7 +54,7 @@ __global__ void mg_restrict_kernel(const BlockMeta *mf, const BlockMeta *mc, con
 __global__ void mg_primitive_kernel(const BlockMeta *meta, const double *U, double *W, int64_t ncp) {
     const BlockMeta m = meta[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m.ni * m.nj) return;
+    if (q >= m.ni * m.nj || m.skip) return;
     const int64_t c = cell(m, q % m.ni + 2, q / m.ni + 2);
     const PS<double> w = W_of_U(ldU(U, ncp, c));
     W[c] = w.d; W[ncp + c] = w.vx; W[2 * ncp + c] = w.vy; W[3 * ncp + c] = w.p;
@@ -64,7 +64,7 @@ __global__ void mg_primitive_kernel(const BlockMeta *meta, const double *U, doub
 __global__ void mg_subtract_kernel(const BlockMeta *meta, double *P, const double *R, int64_t ncp) {
     const BlockMeta m = meta[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= m.ni * m.nj) return;
+    if (q >= m.ni * m.nj || m.skip) return;
     const int64_t c = cell(m, q % m.ni + 2, q / m.ni + 2);
 #pragma unroll
     for (int v = 0; v < 4; ++v) P[v * ncp + c] -= R[v * ncp + c];
@@ -75,7 +75,7 @@ __global__ void mg_cfl_kernel(const BlockMeta *mf, const BlockMeta *mc, const do
                               double *dtc, const double *Ac) {
     const BlockMeta f = mf[blockIdx.y], c = mc[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= c.ni * c.nj) return;
+    if (q >= c.ni * c.nj || c.skip) return;
     const int ic = q % c.ni + 2, jc = q / c.ni + 2;
     const int i0 = 2 * (ic - 2) + 2, j0 = 2 * (jc - 2) + 2;
     const int64_t sw = cell(f, i0, j0), se = sw + 1, nw = sw + f.pitch, ne = nw + 1;
@@ -106,6 +106,7 @@ __global__ void mg_copy_kernel(double *dst, const double *src, int64_t n) {
 // boundaries; one thread per boundary cell of the interior span
 __global__ void mg_neumann_kernel(const BlockMeta *meta, const int32_t *bc, double *U, int64_t ncp) {
     const BlockMeta m = meta[blockIdx.y];
+    if (m.skip) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     const int32_t *bW = bc + m.bcoff, *bE = bW + m.ncj, *bS = bE + m.ncj, *bN = bS + m.nci;
     if (t < m.nj) {
@@ -180,7 +181,7 @@ __global__ void mg_prolong_kernel(const BlockMeta *mc, const BlockMeta *mf, cons
                                   int inj_dirichlet, int inj_always, int stability, int max_red, int assign) {
     const BlockMeta c = mc[blockIdx.y], f = mf[blockIdx.y];
     const int q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= c.ni * c.nj) return;
+    if (q >= c.ni * c.nj || c.skip) return;
     const int ic = q % c.ni + 2, jc = q / c.ni + 2;
     const int i0 = 2 * (ic - 2) + 2, j0 = 2 * (jc - 2) + 2;
     const int32_t *bW = bc + c.bcoff, *bE = bW + c.ncj, *bS = bE + c.ncj, *bN = bS + c.nci;

@@ -470,6 +470,7 @@ __attribute__((amdgpu_waves_per_eu(StageWaves<FT, TI, TJ>::value))) void stage_k
     const int tix = xcd_tile(blockIdx.x, p.ntiles);
     const TileDesc t = p.tiles[tix];
     const BlockMeta m = p.meta[t.b];
+    if (!FUSED && m.skip) return;   // a converged case of a multigrid batch
     const int tid = threadIdx.x;
     const int64_t ncp = p.ncp;
     int phim = p.phi_mode;

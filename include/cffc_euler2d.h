@@ -458,8 +458,10 @@ int  cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *n
    part's L2 norm of the density residual over its own cells has been below
    tol[q] at a check after the second cycle (the rule of the single norm),
    and with freeze_level > 0 the limiters freeze once every part's norm is
-   at or below freeze[q] (freeze may be NULL: 0).  A part's norm is the
-   same alone and in a batch.  npart = 0 clears the parts. */
+   at or below freeze[q] (freeze may be NULL: 0).  From the check at which
+   a part has met its tolerance the cycles leave its blocks alone on every
+   level, so it keeps the state it converged in: every part ends in the
+   state it reaches alone.  npart = 0 clears the parts. */
 int  cffc_mg_set_parts(cffc_mg *mg, int npart, const int32_t *first_block, const double *tol, const double *freeze);
 /* per part: the last checked L2 norm (l2, may be NULL) and the regular
    cycles done since cffc_mg_set_parts when its norm first met its
