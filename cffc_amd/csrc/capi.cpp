@@ -983,15 +983,19 @@ extern "C" int cffc_create(const cffc_mesh_desc *md, const cffc_scheme_desc *sd,
             return e;
         }
     }
-    // default tile 16 x 8 (the autotuner's pick at C2 size in both
-    // precisions: 192 threads, three wavefronts, more resident workgroups per
-    // CU than 16 x 16); 8 x 8 for blocks narrower than 16 cells; in FP64, 4 x 4
-    // (one wavefront per block) for blocks narrower than 8 cells
+    // default tile: for blocks at least 32 cells wide the autotuner's picks
+    // on the 16.8 M-cell and uneven quadtree workloads, 32 x 16 (FP64) and
+    // 32 x 8 (mixed), both ring-pass tiles; 16 x 8 for blocks 16 to 31
+    // cells wide (192 threads, three wavefronts); 8 x 8 for blocks narrower
+    // than 16 cells; in FP64, 4 x 4 (one wavefront per block) for blocks
+    // narrower than 8 cells
+    const bool fp64 = c->s.flux_precision == CFFC_FLUX_FP64;
     int ti = 16, tj = 8;
     int mini = 1 << 30;
     for (auto &m : c->meta_h) mini = std::min(mini, m.ni);
+    if (mini >= 32) { ti = 32; tj = fp64 ? 16 : 8; }
     if (mini < 16) ti = tj = 8;
-    if (mini < 8 && c->s.flux_precision == CFFC_FLUX_FP64) ti = tj = 4;
+    if (mini < 8 && fp64) ti = tj = 4;
     if ((e = build_tiles(c, ti, tj))) {
         free_ctx(c);
         return e;

@@ -33,8 +33,10 @@ def nonbitwise(a, b):
     return int(np.sum(np.ascontiguousarray(a).view(np.uint64) != np.ascontiguousarray(b).view(np.uint64)))
 
 
-def run_both(euler2d, mesh, W0, scheme, nsteps, nthreads=8):
+def run_both(euler2d, mesh, W0, scheme, nsteps, nthreads=8, tile=None):
     gpu = euler2d.Euler2DQuadBlocks(mesh, scheme)
+    if tile:
+        gpu.set_tile(*tile)
     gpu.set_W(W0)
     rc, n, norms, dts = gpu.run_steps(nsteps)
     orc = oracle.Oracle(mesh, scheme)
@@ -258,7 +260,7 @@ def test_two_level_step_reduction_large_mesh(euler2d):
     W0 = M.riemann_4quadrant(m)
     W0 = M.density_perturbation(m, W0, amplitude=0.01, seed=3)
     m.set_wo_from_initial(W0)
-    gpu, orc, g, o = run_both(euler2d, m, W0, abi.Scheme(), 2)
+    gpu, orc, g, o = run_both(euler2d, m, W0, abi.Scheme(), 2, tile=(16, 8))
     assert gpu.get_tile() == (16, 8) and m.ncell // 128 >= 16384
     assert g[0] == o[0] == 0
     np.testing.assert_array_equal(g[3], o[3])
