@@ -2388,8 +2388,28 @@ static int mg_restrict_wo(cffc_mg *mg, int lf) {
 // The start of Execute :3395-3420 for a state set on level 0: coarse levels
 // restricted from the level above (the IC_RESTART path of Apply_ICs
 // :719-733) with their boundary reference states, then messages and BCs.
+// a new start of a batch with parts: no case converged, frozen or skipped
+static int mg_restart_parts(cffc_mg *mg) {
+    for (cffc_ctx *c : mg->lv) {
+        bool any = false;
+        for (BlockMeta &m : c->meta_h) {
+            any = any || m.skip;
+            m.skip = 0;
+        }
+        if (any) HIPCHK(hipMemcpy(c->meta, c->meta_h.data(), c->nblk * sizeof(BlockMeta), hipMemcpyHostToDevice));
+    }
+    if (!mg->npart) return 0;
+    mg->part_done.assign(mg->npart, -1);
+    mg->part_l2.assign(mg->npart, -1.0);
+    mg->part_frozen.assign(mg->npart, 0);
+    HIPCHK(hipMemcpy(mg->d_part_frozen, mg->part_frozen.data(), mg->npart * sizeof(int32_t), hipMemcpyHostToDevice));
+    mg->cycles = 0;
+    return 0;
+}
+
 extern "C" int cffc_mg_initialize(cffc_mg *mg) {
     if (!mg) return fail(CFFC_E_ARG, "null multigrid solver");
+    if (int e = mg_restart_parts(mg)) return e;
     if (int e = mg_exchange(mg->lv[0])) return e;
     for (size_t l = 1; l < mg->lv.size(); ++l) {
         cffc_ctx *f = mg->lv[l - 1], *c = mg->lv[l];

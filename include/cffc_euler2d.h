@@ -461,7 +461,9 @@ int  cffc_mg_execute(cffc_mg *mg, int max_cycles, double freeze_level, double *n
    at or below freeze[q] (freeze may be NULL: 0).  From the check at which
    a part has met its tolerance the cycles leave its blocks alone on every
    level, so it keeps the state it converged in: every part ends in the
-   state it reaches alone.  npart = 0 clears the parts. */
+   state it reaches alone.  cffc_mg_initialize starts the parts afresh
+   (none converged, frozen or skipped; the tolerances are kept).  npart = 0
+   clears the parts. */
 int  cffc_mg_set_parts(cffc_mg *mg, int npart, const int32_t *first_block, const double *tol, const double *freeze);
 /* per part: the last checked L2 norm (l2, may be NULL) and the regular
    cycles done since cffc_mg_set_parts when its norm first met its

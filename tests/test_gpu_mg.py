@@ -313,3 +313,29 @@ def test_per_case_stopping_equals_each_case_alone(limiter):
     for q in range(3):
         assert l2[q] == alone[q][1][0]
         assert np.array_equal(Ws[q].view(np.uint64), alone[q][3][0].view(np.uint64)), "case %d" % q
+
+
+def test_initialize_restarts_the_cases():
+    """cffc_mg_initialize after a batch run with parts starts every case
+    afresh (none converged or skipped): the same state run again takes the
+    same cycles per case and ends in the same states."""
+    from cffc_amd import serving
+    cases = [(0.05, 0.4, 1), (0.08, 0.55, 2)]
+    meshes = [serving.case_mesh(*c) for c in cases]
+    m, starts = M.concat([mm[0] for mm in meshes])
+    W0 = np.concatenate([mm[1] for mm in meshes])
+    g = MG.FASMultigrid(m, _scheme(abi.LIMITER_ZERO), MG.MGParams(levels=3, absolute_tolerance=1e-30))
+    out = []
+    for rep in range(2):
+        g.set_W(W0)
+        g.initialize()
+        if rep == 0:
+            g.set_parts(list(starts) + [m.nblk], [3e-8, 1e-8])
+        rc, n, _ = g.execute(4000)
+        l2, done = g.part_status()
+        assert rc == 0 and (done >= 0).all()
+        out.append((n, done.tolist(), g.get_W()))
+    g.close()
+    print("runs:", [(o[0], o[1]) for o in out], flush=True)
+    assert out[0][:2] == out[1][:2]
+    assert np.array_equal(out[0][2].view(np.uint64), out[1][2].view(np.uint64))
