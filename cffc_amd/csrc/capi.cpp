@@ -2288,7 +2288,7 @@ static int mg_smooth(cffc_mg *mg, int top, int l) {
 
 static int mg_store_uo(cffc_mg *mg, int l) {
     cffc_ctx *c = mg->lv[l];
-    HIPCHK(cffc_launch_mg_copy(mg->uo[l], c->U, 4 * c->ncp, c->stream));
+    HIPCHK(cffc_launch_mg_copy(c->meta, c->nblk, c->max_cells, mg->uo[l], c->U, c->ncp, c->stream));
     return 0;
 }
 
@@ -2336,7 +2336,7 @@ static int mg_cgc(cffc_mg *mg, int top, int l) {
                 if ((e = mg_smooth(mg, top, l))) return e;
         }
     }
-    if (l != top) HIPCHK(cffc_launch_mg_changes(c->U, mg->uo[l], 4 * c->ncp, c->stream));
+    if (l != top) HIPCHK(cffc_launch_mg_changes(c->meta, c->nblk, c->max_cells, c->U, mg->uo[l], c->ncp, c->stream));
     return 0;
 }
 

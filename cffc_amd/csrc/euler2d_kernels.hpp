@@ -132,8 +132,10 @@ hipError_t cffc_launch_mg_subtract(const cffc::BlockMeta *meta, int nblk, int ma
                                    int64_t ncp, hipStream_t st);
 hipError_t cffc_launch_mg_cfl(const cffc::BlockMeta *mf, const cffc::BlockMeta *mc, int nblk, int max_int_c,
                               const double *dtf, const double *Af, double *dtc, const double *Ac, hipStream_t st);
-hipError_t cffc_launch_mg_changes(double *U, const double *uo, int64_t n, hipStream_t st);
-hipError_t cffc_launch_mg_copy(double *dst, const double *src, int64_t n, hipStream_t st);
+hipError_t cffc_launch_mg_changes(const cffc::BlockMeta *meta, int nblk, int max_cells, double *U, const double *uo,
+                                  int64_t ncp, hipStream_t st);
+hipError_t cffc_launch_mg_copy(const cffc::BlockMeta *meta, int nblk, int max_cells, double *dst, const double *src,
+                               int64_t ncp, hipStream_t st);
 hipError_t cffc_launch_mg_prolong(const cffc::BlockMeta *mc, const cffc::BlockMeta *mf, int nblk, int max_int_c,
                                   int max_edge_c, const int32_t *bc, double *Uc, const double *gc, int64_t ncpc,
                                   double *Uf, const double *gf, int64_t ncpf, int neumann, int inj_dirichlet,

@@ -88,17 +88,27 @@ __global__ void mg_cfl_kernel(const BlockMeta *mf, const BlockMeta *mc, const do
     dtc[cc] = dmin(dtc[cc], dmin(dt_NE, dmin(dt_SE, dmin(dt_NW, dt_SW))));
 }
 
-// Evaluate_Solution_Changes :2454-2478 over every padded cell (U -= uo)
-__global__ void mg_changes_kernel(double *U, const double *uo, int64_t n) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) U[k] -= uo[k];
+// Evaluate_Solution_Changes :2454-2478 over every cell of each block,
+// ghosts included (U -= uo); a converged case of a batch is skipped
+__global__ void mg_changes_kernel(const BlockMeta *meta, double *U, const double *uo, int64_t ncp) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.nci * m.ncj || m.skip) return;
+    const int64_t c = cell(m, q % m.nci, q / m.nci);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) U[v * ncp + c] -= uo[v * ncp + c];
 }
 
-// uo = U over every padded cell (Store_Current_Solution_in_uo_MG); a kernel
-// rather than a copy node, so that a captured cycle holds kernels only
-__global__ void mg_copy_kernel(double *dst, const double *src, int64_t n) {
-    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < n) dst[k] = src[k];
+// uo = U over every cell of each block, ghosts included
+// (Store_Current_Solution_in_uo_MG); a kernel rather than a copy node, so
+// that a captured cycle holds kernels only
+__global__ void mg_copy_kernel(const BlockMeta *meta, double *dst, const double *src, int64_t ncp) {
+    const BlockMeta m = meta[blockIdx.y];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= m.nci * m.ncj || m.skip) return;
+    const int64_t c = cell(m, q % m.nci, q / m.nci);
+#pragma unroll
+    for (int v = 0; v < 4; ++v) dst[v * ncp + c] = src[v * ncp + c];
 }
 
 // Neumann conditions on the coarse solution change before bilinear
@@ -273,12 +283,14 @@ extern "C" hipError_t cffc_launch_mg_cfl(const BlockMeta *mf, const BlockMeta *m
     hipLaunchKernelGGL(mg_cfl_kernel, grid, dim3(256), 0, st, mf, mc, dtf, Af, dtc, Ac);
     return hipGetLastError();
 }
-extern "C" hipError_t cffc_launch_mg_changes(double *U, const double *uo, int64_t n, hipStream_t st) {
-    hipLaunchKernelGGL(mg_changes_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, U, uo, n);
+extern "C" hipError_t cffc_launch_mg_changes(const BlockMeta *meta, int nblk, int max_cells, double *U,
+                                             const double *uo, int64_t ncp, hipStream_t st) {
+    hipLaunchKernelGGL(mg_changes_kernel, dim3((max_cells + 255) / 256, nblk), dim3(256), 0, st, meta, U, uo, ncp);
     return hipGetLastError();
 }
-extern "C" hipError_t cffc_launch_mg_copy(double *dst, const double *src, int64_t n, hipStream_t st) {
-    hipLaunchKernelGGL(mg_copy_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, dst, src, n);
+extern "C" hipError_t cffc_launch_mg_copy(const BlockMeta *meta, int nblk, int max_cells, double *dst,
+                                          const double *src, int64_t ncp, hipStream_t st) {
+    hipLaunchKernelGGL(mg_copy_kernel, dim3((max_cells + 255) / 256, nblk), dim3(256), 0, st, meta, dst, src, ncp);
     return hipGetLastError();
 }
 extern "C" hipError_t cffc_launch_mg_prolong(const BlockMeta *mc, const BlockMeta *mf, int nblk, int max_int_c, int max_edge_c,
